@@ -1,0 +1,144 @@
+"""Throughput of the FDTD hot path on one MI355X.
+
+A step is one full leapfrog step of the 256^3 scene (32 seeded dielectric spheres, 10-cell
+absorber, z thin-wire half-wave dipole behind a 50 ohm resistive feed): fused B+E update,
+source injection, port V/I accumulation with series recording, and the near-to-far-field
+phasor accumulation on the Huygens box. Prints one JSON line.
+
+value    = whole-job Mcell-updates/s (cells x timed steps / wall time between synchronisations)
+roofline = the fused Yee kernel: algorithmic 48 B per cell-update (E and B read once, written
+           once, FP32) / its average launch time from HIP events on the library's stream
+cpu_baseline = the CPU oracle (scalar C, OpenMP over z-planes) on a bounded number of steps
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mcell-updates/s & %HBM peak"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = 48  # FP32 E and B (6 components), each read once and written once
+
+
+def cpu_baseline(scene, budget_s: float = 12.0, max_steps: int = 40):
+    """Time the oracle's step loop (update_B, update_E, sources) on the host cores."""
+    import numpy as np
+    from oracle import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    cfg = scene.grid_config()
+    sim = O.OracleSim(cfg.size, cfg.courant, cfg.pml_thickness, cfg.pml_sigma_max, cfg.pml_grading)
+    s, e = scene.flat_materials()
+    sim.upload_materials(s, e, scene.subcell.edges, scene.subcell.wire_terms)
+    sim.set_sources(scene.sources)
+    sim.step(1)  # first touch of the arrays
+    n = int(np.prod(cfg.size))
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < max_steps:
+        sim.step(1)
+        steps += 1
+        if time.perf_counter() - t0 > budget_s and steps >= 3:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(n * steps / dt / 1e6, 3), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{cfg.size[0]}x{cfg.size[1]}x{cfg.size[2]} same scene, {steps} steps of the oracle's "
+                      f"update_B/update_E/apply_sources (OpenMP {threads} threads), {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--kernel", type=int, default=0, help="EMFD_KERNEL_* variant (| tile << 8 | zchunk << 16)")
+    ap.add_argument("--no-monitors", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the single-GPU figure is the per-GPU share; see the sharding note in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    from em_lab_amd import emfdtd as E
+    from em_lab_amd.scene import spheres_dipole
+
+    scene = spheres_dipole(n=args.size, pml=10)
+    g = E.SimulationGrid(0)
+    g.configure(scene.grid_config())
+    s, e = scene.flat_materials()
+    g.upload_materials(s, e, scene.subcell)
+    g.set_sources(scene.sources)
+    g.set_kernel(args.kernel)
+    if not args.no_monitors:
+        pm = E.PortMonitor(g)
+        pm.configure(scene.ports, scene.far_field.averaging_periods, scene.sweep, scene.cell_size)
+        ff = E.FarFieldMonitor(g)
+        ff.configure(scene.far_field)
+    nx, ny, nz = g.size
+    cells = nx * ny * nz
+
+    g.step(args.warmup)
+    g.synchronize()
+    g.kernel_timing(1)
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.synchronize()
+    t1 = time.perf_counter()
+    yee_ms, launches = g.kernel_timing(0)
+    wall = t1 - t0
+    Eg, _ = g.fields(E.LAYOUT_SOA)
+    import numpy as np
+    if not np.all(np.isfinite(Eg)):
+        print("bench.py: non-finite fields", file=sys.stderr)
+        return 2
+
+    value = cells * args.steps / wall / 1e6
+    avg_ms = yee_ms / max(launches, 1)
+    achieved = BYTES_PER_CELL * cells / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tf = ROOT / "profiles" / "yee_fused_256_traffic.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{nx}^3 Yee leapfrog, 10-cell absorber, 32 seeded dielectric spheres, z thin-wire "
+                               f"half-wave dipole (lambda 20) on a 50 ohm resistive feed, port + far-field "
+                               f"accumulation every step" + (" (monitors off)" if args.no_monitors else ""),
+                   "grid": [nx, ny, nz], "courant": round(g.dt, 6), "material_classes": g.material_classes(),
+                   "kernel": args.kernel},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_yee_fused", "avg_launch_ms": round(avg_ms, 5), "launches": launches,
+                     "bytes_per_cell": BYTES_PER_CELL},
+        "cpu_baseline": None,
+    }
+    g.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(scene)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,116 @@
+// Context state of libemfdtd.so (host side). Not part of the public ABI.
+#pragma once
+
+#include "../../include/emfdtd.h"
+#include "internal.h"
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace emfd {
+
+// Thread-local error reporting for the C ABI.
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define EMFD_HIP(call)                                      \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return ::emfd::hip_fail(e_, #call); \
+    } while (0)
+
+// Device buffer with capacity tracking.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);  // grows (contents not kept); returns EMFD_OK or an error code
+    void release();
+    template <typename T>
+    T *as() const { return static_cast<T *>(ptr); }
+};
+
+struct PortsState {
+    std::vector<emfd_port_setup> ports;
+    float averaging = 4.0f;
+    float wavelength = 16.0f;
+    bool has_sweep = false;
+    emfd_sweep_setup sweep{};
+    float cell_size = 1.0e-3f;
+    int64_t record_steps = 0;
+    int64_t capacity_steps = 0;
+    int64_t recorded = 0;
+    double start_time = 0.0;
+    DevBuf d_ports, d_phasors, d_series;
+    bool enabled() const { return !ports.empty(); }
+    bool recording() const { return has_sweep || record_steps > 0; }
+};
+
+struct NtffState {
+    bool enabled = false;
+    emfd_far_field_setup setup{};
+    NtffBox box{};
+    float map_normal[3] = {0, 0, 1};
+    std::vector<float> dirs;   // 4 per direction
+    std::vector<int> map_index;
+    float frame[9] = {0, 0, 1, 1, 0, 0, 0, 1, 0};
+    double start_time = 0.0;
+    DevBuf d_phasors;  // SoA: 24 planes of n_points floats
+    DevBuf d_dirs, d_partials, d_out, d_power;
+};
+
+}  // namespace emfd
+
+struct emfd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool configured = false;
+    emfd_grid_config cfg{};
+    size_t n = 0;  // cells
+
+    // fields: two sets (ping-pong for the fused kernel); cur = the one holding the state
+    emfd::DevBuf field_buf[2];
+    int cur = 0;
+    emfd::DevBuf sigma_cell, eps_cell;  // EM_Lab's alpha channels (probes)
+    emfd::DevBuf mat_id, table, prof;
+    int id_bytes = 0;
+    int classes = 0;
+    int has_wire = 0;
+    int prof_stride = 0;
+
+    // sources
+    std::vector<emfd_source_desc> sources;  // kept sources, in order
+    emfd::DevBuf d_packed, d_cells, d_lists;
+    int n_cells = 0;
+
+    // per-call step constants
+    emfd::DevBuf d_consts;
+    std::vector<float> h_consts;
+
+    uint64_t step_count = 0;
+    int kernel_variant = EMFD_KERNEL_AUTO;
+    int tile_cfg = 0;
+
+    // kernel timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    double timing_ms = 0.0;
+    int64_t timing_launches = 0;
+
+    emfd::PortsState ports;
+    emfd::NtffState ntff;
+
+    emfd::DevBuf scratch_partials, scratch_misc;
+
+    emfd::FieldPtrs fields(int which) const;
+    emfd::LatticeArgs lattice() const;
+    double time() const { return static_cast<double>(step_count) * static_cast<double>(cfg.courant); }
+};
+
+namespace emfd {
+// Per-step monitor hooks used by emfd_grid_step (monitors.cpp).
+int monitors_consts_floats(emfd_ctx *ctx);
+int monitors_fill_consts(emfd_ctx *ctx, float *consts_step, uint64_t step_after);
+int monitors_after_step(emfd_ctx *ctx, const float *consts_step, int64_t k);
+}  // namespace emfd

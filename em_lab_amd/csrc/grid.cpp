@@ -1,0 +1,635 @@
+// libemfdtd.so host side: context, SimulationGrid entry points, materials and sources.
+//
+// Host logic restates EM_Lab's SimulationGrid (src/SimulationGrid.cpp) where the result must
+// match bit for bit (clamps, edge averaging, binary16 quantisation, source lists, FP64 signals);
+// the GPU state and launch sequence are this library's own design.
+#include "context.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+
+namespace emfd {
+
+namespace {
+thread_local std::string g_error;
+constexpr float kMaxCourant = 0.99f / 1.7320508f;  // src/SimulationGrid.cpp:24
+constexpr float kPerfectEdgeSigma = 6.0e4f;        // src/SimulationGrid.hpp:103
+constexpr int kMaxSources = 256;                   // src/SimulationGrid.hpp:157
+}  // namespace
+
+int fail(int code, const std::string &msg) {
+    g_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(e == hipErrorOutOfMemory ? EMFD_ERR_NOMEM : EMFD_ERR_HIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int DevBuf::ensure(size_t need) {
+    if (need <= bytes && ptr) return EMFD_OK;
+    release();
+    if (need == 0) need = 16;
+    hipError_t e = hipMalloc(&ptr, need);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        return hip_fail(e, "hipMalloc");
+    }
+    bytes = need;
+    return EMFD_OK;
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+uint16_t f32_to_f16(float f) {
+    // binary16, round to nearest even: glm::packHalf2x16 (src/SimulationGrid.cpp:260-261)
+    const _Float16 h = static_cast<_Float16>(f);
+    uint16_t u;
+    std::memcpy(&u, &h, 2);
+    return u;
+}
+
+// SourceDesc::signal (src/SimulationGrid.cpp:99-130), double precision
+double source_signal(const emfd_source_desc &s, double tAbs) {
+    const double wl = std::max(static_cast<double>(s.wavelength), 1e-3);
+    const double t = tAbs - std::max(static_cast<double>(s.delay_periods), 0.0) * wl;
+    if (!s.enabled || t < 0.0) return 0.0;
+    const double period = wl;
+    const double w = 6.283185307179586476925286766559 / wl;
+    if (s.waveform == 0) {
+        const double rampT = std::max(static_cast<double>(s.ramp_periods), 0.0) * period;
+        double ramp = 1.0;
+        if (rampT > 0.0 && t < rampT) {
+            const double r = t / rampT;
+            ramp = r * r * r * (10.0 + r * (-15.0 + 6.0 * r));
+        }
+        return ramp * std::sin(w * t + s.phase);
+    }
+    const double tau = std::max(static_cast<double>(s.ramp_periods), 0.1) * period;
+    const double t0 = 3.0 * tau;
+    double tl = t;
+    if (s.repeat_periods > 0.0f) {
+        const double rep = std::max(static_cast<double>(s.repeat_periods) * period, 2.0 * t0);
+        tl = std::fmod(t, rep);
+    }
+    const double u = (tl - t0) / tau;
+    if (std::abs(u) > 6.0) return 0.0;
+    return std::exp(-u * u) * std::sin(w * (tl - t0) + s.phase);
+}
+
+}  // namespace emfd
+
+using namespace emfd;
+
+emfd::FieldPtrs emfd_ctx::fields(int which) const {
+    FieldPtrs f;
+    float *base = field_buf[which].as<float>();
+    for (int c = 0; c < 3; ++c) {
+        f.e[c] = base + static_cast<size_t>(c) * n;
+        f.b[c] = base + static_cast<size_t>(3 + c) * n;
+    }
+    return f;
+}
+
+emfd::LatticeArgs emfd_ctx::lattice() const {
+    LatticeArgs a{};
+    a.nx = cfg.size[0];
+    a.ny = cfg.size[1];
+    a.nz = cfg.size[2];
+    a.pml = cfg.pml_thickness;
+    a.dt = cfg.courant;
+    a.prof_stride = prof_stride;
+    a.prof = prof.as<float>();
+    a.mat_id = classes > 0 ? mat_id.ptr : nullptr;
+    a.id_bytes = id_bytes;
+    a.table = table.as<float>();
+    a.has_wire = has_wire;
+    return a;
+}
+
+extern "C" {
+
+const char *emfd_last_error(void) { return g_error.c_str(); }
+
+const char *emfd_version(void) {
+    return "emfdtd 0.1 gfx950: fused z-march Yee (LDS tiles, ping-pong), two-pass Yee, sources, ports, ntff, "
+           "flux, probes";
+}
+
+int emfd_create(int device, emfd_ctx **out) {
+    if (!out) return fail(EMFD_ERR_INVALID, "emfd_create: out is null");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(EMFD_ERR_HIP, "emfd_create: no HIP device available");
+    if (device < 0 || device >= count) return fail(EMFD_ERR_INVALID, "emfd_create: device index out of range");
+    EMFD_HIP(hipSetDevice(device));
+    auto *ctx = new emfd_ctx();
+    ctx->device = device;
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return hip_fail(e, "hipStreamCreateWithFlags");
+    }
+    *out = ctx;
+    return EMFD_OK;
+}
+
+void emfd_destroy(emfd_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &b : ctx->field_buf) b.release();
+    DevBuf *bufs[] = {&ctx->sigma_cell, &ctx->eps_cell, &ctx->mat_id, &ctx->table, &ctx->prof, &ctx->d_packed,
+                      &ctx->d_cells, &ctx->d_lists, &ctx->d_consts, &ctx->ports.d_ports, &ctx->ports.d_phasors,
+                      &ctx->ports.d_series, &ctx->ntff.d_phasors, &ctx->ntff.d_dirs, &ctx->ntff.d_partials,
+                      &ctx->ntff.d_out, &ctx->ntff.d_power, &ctx->scratch_partials, &ctx->scratch_misc};
+    for (DevBuf *b : bufs) b->release();
+    for (hipEvent_t ev : ctx->ev_pool) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int emfd_grid_configure(emfd_ctx *ctx, const emfd_grid_config *requested, emfd_grid_config *applied) {
+    if (!ctx || !requested) return fail(EMFD_ERR_INVALID, "emfd_grid_configure: null argument");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));  // buffers may be reallocated below
+    // SimulationGrid::configure clamps (src/SimulationGrid.cpp:150-156)
+    emfd_grid_config c = *requested;
+    c.pml_thickness = std::clamp(c.pml_thickness, 0, 64);
+    const int min_size = 2 * c.pml_thickness + 8;
+    for (int a = 0; a < 3; ++a) c.size[a] = std::max(c.size[a], min_size);
+    c.courant = std::clamp(c.courant, 0.05f, kMaxCourant);
+    c.pml_sigma_max = std::max(c.pml_sigma_max, 0.0f);
+    c.pml_grading = std::clamp(c.pml_grading, 0.25f, 12.0f);
+    // The reference rejects grids beyond GL_MAX_3D_TEXTURE_SIZE; here the limit is the 32-bit
+    // plane index used by the kernels for one component plane.
+    const size_t n = static_cast<size_t>(c.size[0]) * c.size[1] * c.size[2];
+    if (n > (size_t(1) << 34)) return fail(EMFD_ERR_INVALID, "SimulationGrid: grid too large");
+
+    if (n != ctx->n || !ctx->field_buf[0].ptr) {
+        for (int k = 0; k < 2; ++k) {
+            int rc = ctx->field_buf[k].ensure(n * 6 * sizeof(float));
+            if (rc) return rc;
+        }
+        if (int rc = ctx->sigma_cell.ensure(n * sizeof(float))) return rc;
+        if (int rc = ctx->eps_cell.ensure(n * sizeof(float))) return rc;
+    }
+    ctx->n = n;
+    ctx->cfg = c;
+    ctx->configured = true;
+    ctx->cur = 0;
+
+    // absorber profiles pmlProfile(depth) per axis at i and i + 1/2 (shaders/common.glsl:23-36),
+    // evaluated in float exactly as the shader does
+    const int stride = std::max({c.size[0], c.size[1], c.size[2]});
+    std::vector<float> prof(static_cast<size_t>(6) * stride, 0.0f);
+    const float L = static_cast<float>(c.pml_thickness);
+    auto profile = [&](float depth) -> float {
+        if (depth <= 0.0f) return 0.0f;
+        const float u = std::min(depth / static_cast<float>(c.pml_thickness), 1.0f);
+        return c.pml_sigma_max * (std::exp(c.pml_grading * u) - 1.0f) / (std::exp(c.pml_grading) - 1.0f);
+    };
+    for (int a = 0; a < 3; ++a) {
+        const float hi = static_cast<float>(c.size[a] - 1) - L;
+        for (int h = 0; h < 2; ++h)
+            for (int i = 0; i < c.size[a]; ++i) {
+                const float x = static_cast<float>(i) + (h ? 0.5f : 0.0f);
+                prof[static_cast<size_t>(2 * a + h) * stride + i] = profile(std::max(L - x, x - hi));
+            }
+    }
+    ctx->prof_stride = stride;
+    if (int rc = ctx->prof.ensure(prof.size() * sizeof(float))) return rc;
+    EMFD_HIP(hipMemcpyAsync(ctx->prof.ptr, prof.data(), prof.size() * sizeof(float), hipMemcpyHostToDevice,
+                            ctx->stream));
+
+    // vacuum everywhere: E = (0,0,0, sigma 0), B = (0,0,0, eps 1) (src/SimulationGrid.cpp:185-192)
+    ctx->classes = 0;
+    ctx->has_wire = 0;
+    ctx->id_bytes = 0;
+    EMFD_HIP(hipMemsetAsync(ctx->field_buf[0].ptr, 0, n * 6 * sizeof(float), ctx->stream));
+    EMFD_HIP(hipMemsetAsync(ctx->sigma_cell.ptr, 0, n * sizeof(float), ctx->stream));
+    EMFD_HIP(launch_fill(ctx->eps_cell.as<float>(), n, 1.0f, ctx->stream));
+    ctx->step_count = 0;
+    if (applied) *applied = c;
+    return EMFD_OK;
+}
+
+int emfd_grid_get_config(emfd_ctx *ctx, emfd_grid_config *out) {
+    if (!ctx || !out) return fail(EMFD_ERR_INVALID, "null argument");
+    if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
+    *out = ctx->cfg;
+    return EMFD_OK;
+}
+
+int emfd_grid_upload_materials(emfd_ctx *ctx, const float *sigma, const float *eps_r, int64_t n_cells,
+                               const emfd_edge_override *edges, int64_t n_edges, const emfd_wire_coefficient *wires,
+                               int64_t n_wires) {
+    if (!ctx || !sigma || !eps_r) return fail(EMFD_ERR_INVALID, "emfd_grid_upload_materials: null argument");
+    if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
+    if (n_cells < 0 || static_cast<size_t>(n_cells) != ctx->n)
+        return fail(EMFD_ERR_INVALID, "SimulationGrid: material volume size does not match the grid");
+    if ((n_edges > 0 && !edges) || (n_wires > 0 && !wires))
+        return fail(EMFD_ERR_INVALID, "emfd_grid_upload_materials: null override array");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));  // buffers may be reallocated below
+    const int nx = ctx->cfg.size[0], ny = ctx->cfg.size[1], nz = ctx->cfg.size[2];
+    const size_t n = ctx->n;
+
+    // cell materials (alpha channels of the reference's field textures)
+    EMFD_HIP(hipMemcpyAsync(ctx->sigma_cell.ptr, sigma, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    EMFD_HIP(hipMemcpyAsync(ctx->eps_cell.ptr, eps_r, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+
+    // sub-cell overrides sorted by texel, keeping their order (later entries win, as the
+    // reference's sequential loop at src/SimulationGrid.cpp:251-254)
+    struct EdgeOv { size_t idx; int axis; float sigma; };
+    std::vector<EdgeOv> ov;
+    ov.reserve(static_cast<size_t>(std::max<int64_t>(n_edges, 0)));
+    for (int64_t k = 0; k < n_edges; ++k) {
+        const emfd_edge_override &o = edges[k];
+        if (o.texel[0] < 0 || o.texel[1] < 0 || o.texel[2] < 0 || o.texel[0] >= nx || o.texel[1] >= ny ||
+            o.texel[2] >= nz || o.axis < 0 || o.axis > 2)
+            continue;
+        ov.push_back({(static_cast<size_t>(o.texel[2]) * ny + o.texel[1]) * nx + o.texel[0], o.axis,
+                      std::max(o.sigma, 0.0f)});
+    }
+    std::stable_sort(ov.begin(), ov.end(), [](const EdgeOv &a, const EdgeOv &b) { return a.idx < b.idx; });
+    struct WireOv { size_t idx; int slot; float factor; };
+    std::vector<WireOv> wv;
+    for (int64_t k = 0; k < n_wires; ++k) {
+        const emfd_wire_coefficient &w = wires[k];
+        if (w.texel[0] < 0 || w.texel[1] < 0 || w.texel[2] < 0 || w.texel[0] >= nx || w.texel[1] >= ny ||
+            w.texel[2] >= nz || w.component < 0 || w.component > 2)
+            continue;
+        wv.push_back({(static_cast<size_t>(w.texel[2]) * ny + w.texel[1]) * nx + w.texel[0],
+                      2 * w.component + (w.term == 0 ? 0 : 1), w.factor});
+    }
+    std::stable_sort(wv.begin(), wv.end(), [](const WireOv &a, const WireOv &b) { return a.idx < b.idx; });
+    const bool thin = n_wires > 0;  // the reference keys thin-wire mode on a non-empty list
+
+    // edge averages (src/SimulationGrid.cpp:224-248), quantised to binary16, classified
+    using Key = std::array<uint16_t, 12>;
+    struct KeyHash {
+        size_t operator()(const Key &k) const noexcept {
+            uint64_t h = 1469598103934665603ull;
+            for (uint16_t v : k) h = (h ^ v) * 1099511628211ull;
+            return static_cast<size_t>(h);
+        }
+    };
+    const uint16_t h_one = f32_to_f16(1.0f);
+    const Key vacuum = {0, 0, 0, h_one, h_one, h_one, h_one, h_one, h_one, h_one, h_one, h_one};
+    std::unordered_map<Key, uint32_t, KeyHash> classes;
+    std::vector<Key> keys;
+    classes.emplace(vacuum, 0u);
+    keys.push_back(vacuum);
+    std::vector<uint32_t> ids(n);
+    auto idx = [&](int x, int y, int z) {
+        x = std::max(x, 0);
+        y = std::max(y, 0);
+        z = std::max(z, 0);
+        return (static_cast<size_t>(z) * ny + y) * nx + x;
+    };
+    size_t oi = 0, wi = 0;
+    Key last = vacuum;
+    uint32_t last_id = 0;
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const size_t c00 = idx(x, y, z);
+                const size_t ex[4] = {c00, idx(x, y - 1, z), idx(x, y, z - 1), idx(x, y - 1, z - 1)};
+                const size_t ey[4] = {c00, idx(x - 1, y, z), idx(x, y, z - 1), idx(x - 1, y, z - 1)};
+                const size_t ez[4] = {c00, idx(x - 1, y, z), idx(x, y - 1, z), idx(x - 1, y - 1, z)};
+                auto avg = [](const float *v, const size_t (&c)[4]) {
+                    return 0.25f * (v[c[0]] + v[c[1]] + v[c[2]] + v[c[3]]);
+                };
+                float es[3] = {avg(sigma, ex), avg(sigma, ey), avg(sigma, ez)};
+                const float ee[3] = {avg(eps_r, ex), avg(eps_r, ey), avg(eps_r, ez)};
+                for (; oi < ov.size() && ov[oi].idx == c00; ++oi) es[ov[oi].axis] = ov[oi].sigma;
+                float wf[6] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f};
+                for (; wi < wv.size() && wv[wi].idx == c00; ++wi) wf[wv[wi].slot] = wv[wi].factor;
+                Key k;
+                for (int c = 0; c < 3; ++c) {
+                    k[c] = f32_to_f16(std::min(es[c], kPerfectEdgeSigma));
+                    k[3 + c] = f32_to_f16(ee[c]);
+                }
+                for (int j = 0; j < 6; ++j) k[6 + j] = f32_to_f16(wf[j]);
+                uint32_t id;
+                if (k == last) {
+                    id = last_id;
+                } else {
+                    auto it = classes.find(k);
+                    if (it == classes.end()) {
+                        id = static_cast<uint32_t>(keys.size());
+                        classes.emplace(k, id);
+                        keys.push_back(k);
+                    } else {
+                        id = it->second;
+                    }
+                    last = k;
+                    last_id = id;
+                }
+                ids[c00] = id;
+            }
+
+    ctx->has_wire = thin ? 1 : 0;
+    if (keys.size() == 1 && !thin) {
+        ctx->classes = 0;  // uniform vacuum: kernels skip the class id stream
+        ctx->id_bytes = 0;
+    } else {
+        ctx->classes = static_cast<int>(keys.size());
+        ctx->id_bytes = keys.size() <= 65536 ? 2 : 4;
+        if (int rc = ctx->mat_id.ensure(n * static_cast<size_t>(ctx->id_bytes))) return rc;
+        if (ctx->id_bytes == 2) {
+            std::vector<uint16_t> id16(n);
+            for (size_t i = 0; i < n; ++i) id16[i] = static_cast<uint16_t>(ids[i]);
+            EMFD_HIP(hipMemcpy(ctx->mat_id.ptr, id16.data(), n * 2, hipMemcpyHostToDevice));
+        } else {
+            EMFD_HIP(hipMemcpy(ctx->mat_id.ptr, ids.data(), n * 4, hipMemcpyHostToDevice));
+        }
+        DevBuf d_keys;
+        if (int rc = d_keys.ensure(keys.size() * sizeof(Key))) return rc;
+        if (int rc = ctx->table.ensure(keys.size() * kTableStride * sizeof(float))) return rc;
+        EMFD_HIP(hipMemcpy(d_keys.ptr, keys.data(), keys.size() * sizeof(Key), hipMemcpyHostToDevice));
+        EMFD_HIP(launch_build_table(d_keys.as<uint16_t>(), ctx->classes, ctx->cfg.courant, ctx->table.as<float>(),
+                                    ctx->stream));
+        EMFD_HIP(hipStreamSynchronize(ctx->stream));
+        d_keys.release();
+    }
+    // uploadMaterials clears E and B and rewinds time (src/SimulationGrid.cpp:216-222,294)
+    ctx->cur = 0;
+    EMFD_HIP(hipMemsetAsync(ctx->field_buf[0].ptr, 0, n * 6 * sizeof(float), ctx->stream));
+    ctx->step_count = 0;
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    return EMFD_OK;
+}
+
+int emfd_grid_material_classes(emfd_ctx *ctx, int *classes) {
+    if (!ctx || !classes) return fail(EMFD_ERR_INVALID, "null argument");
+    *classes = ctx->classes;
+    return EMFD_OK;
+}
+
+int emfd_grid_set_sources(emfd_ctx *ctx, const emfd_source_desc *sources, int n, int *n_active) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
+    if (n > 0 && !sources) return fail(EMFD_ERR_INVALID, "null sources");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    // packing (src/SimulationGrid.cpp:297-318)
+    std::vector<float> packed;
+    std::vector<emfd_source_desc> kept;
+    for (int i = 0; i < n && static_cast<int>(kept.size()) < kMaxSources; ++i) {
+        const emfd_source_desc &s = sources[i];
+        const float len = std::sqrt(s.polarization[0] * s.polarization[0] + s.polarization[1] * s.polarization[1] +
+                                    s.polarization[2] * s.polarization[2]);
+        if (!s.enabled || !(len > 1e-6f) || !std::isfinite(len)) continue;
+        float g[16];
+        for (int a = 0; a < 3; ++a) {
+            g[a] = s.center[a];
+            g[4 + a] = std::clamp(s.half_extent[a], 0.0f, 1.0e5f);
+            g[8 + a] = s.polarization[a] / len;
+            g[12 + a] = std::max(s.softness[a], 0.25f);
+        }
+        g[3] = g[7] = g[15] = 0.0f;
+        g[11] = s.amplitude;
+        packed.insert(packed.end(), g, g + 16);
+        kept.push_back(s);
+    }
+    // per-texel source lists (src/SimulationGrid.cpp:320-349)
+    const int nx = ctx->cfg.size[0], ny = ctx->cfg.size[1], nz = ctx->cfg.size[2];
+    const int dims[3] = {nx, ny, nz};
+    std::vector<std::pair<uint32_t, uint32_t>> pairs;
+    for (size_t i = 0; i < kept.size(); ++i) {
+        const float *g = &packed[16 * i];
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::clamp(static_cast<int>(std::floor(g[a] - g[4 + a] - 3.5f * g[12 + a])) - 1, 0, dims[a] - 1);
+            hi[a] = std::clamp(static_cast<int>(std::ceil(g[a] + g[4 + a] + 3.5f * g[12 + a])), 0, dims[a] - 1);
+        }
+        for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int x = lo[0]; x <= hi[0]; ++x)
+                    pairs.emplace_back(static_cast<uint32_t>((z * ny + y) * nx + x), static_cast<uint32_t>(i));
+    }
+    std::sort(pairs.begin(), pairs.end());
+    std::vector<int> cells, lists;
+    for (size_t k = 0; k < pairs.size(); ++k) {
+        if (k == 0 || pairs[k].first != pairs[k - 1].first) {
+            const uint32_t id = pairs[k].first;
+            cells.push_back(static_cast<int>(id % static_cast<uint32_t>(nx)));
+            cells.push_back(static_cast<int>((id / static_cast<uint32_t>(nx)) % static_cast<uint32_t>(ny)));
+            cells.push_back(static_cast<int>(id / static_cast<uint32_t>(nx * ny)));
+            cells.push_back(static_cast<int>(lists.size()));
+        }
+        lists.push_back(static_cast<int>(pairs[k].second));
+    }
+    ctx->n_cells = static_cast<int>(cells.size() / 4);
+    cells.insert(cells.end(), {0, 0, 0, static_cast<int>(lists.size())});
+    if (lists.empty()) lists.push_back(0);
+    if (packed.empty()) packed.assign(16, 0.0f);
+    if (int rc = ctx->d_packed.ensure(packed.size() * sizeof(float))) return rc;
+    if (int rc = ctx->d_cells.ensure(cells.size() * sizeof(int))) return rc;
+    if (int rc = ctx->d_lists.ensure(lists.size() * sizeof(int))) return rc;
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    EMFD_HIP(hipMemcpy(ctx->d_packed.ptr, packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice));
+    EMFD_HIP(hipMemcpy(ctx->d_cells.ptr, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice));
+    EMFD_HIP(hipMemcpy(ctx->d_lists.ptr, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice));
+    ctx->sources = kept;
+    if (n_active) *n_active = static_cast<int>(kept.size());
+    return EMFD_OK;
+}
+
+int emfd_grid_reset_fields(emfd_ctx *ctx) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    ctx->cur = 0;
+    EMFD_HIP(launch_fill(ctx->field_buf[0].as<float>(), ctx->n * 6, 0.0f, ctx->stream));
+    ctx->step_count = 0;
+    return EMFD_OK;
+}
+
+int emfd_grid_set_kernel(emfd_ctx *ctx, int variant) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    const int v = variant & 0xff;
+    if (v < 0 || v > 2) return fail(EMFD_ERR_INVALID, "unknown kernel variant");
+    ctx->kernel_variant = v;
+    ctx->tile_cfg = variant >> 8;  // optional tile shape / z-chunk selector for the fused kernel
+    return EMFD_OK;
+}
+
+int emfd_grid_kernel_timing(emfd_ctx *ctx, int enable, double *total_ms, int64_t *launches) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    if (ctx->ev_used > 0) {
+        EMFD_HIP(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i + 1 < ctx->ev_used; i += 2) {
+            float ms = 0.0f;
+            EMFD_HIP(hipEventElapsedTime(&ms, ctx->ev_pool[i], ctx->ev_pool[i + 1]));
+            ctx->timing_ms += ms;
+            ctx->timing_launches += 1;
+        }
+        ctx->ev_used = 0;
+    }
+    if (total_ms) *total_ms = ctx->timing_ms;
+    if (launches) *launches = ctx->timing_launches;
+    if (enable >= 0) {
+        if (enable && !ctx->timing) {
+            ctx->timing_ms = 0.0;
+            ctx->timing_launches = 0;
+        }
+        ctx->timing = enable != 0;
+    }
+    return EMFD_OK;
+}
+
+int emfd_grid_step(emfd_ctx *ctx, int count) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
+    if (count <= 0) return EMFD_OK;
+    EMFD_HIP(hipSetDevice(ctx->device));
+
+    // step constants for this call, in one upload: signals at t = (n + 1/2) dt in FP64 cast to
+    // float (src/SimulationGrid.cpp:371-387), then the monitors' rotation factors
+    const int nsrc = static_cast<int>(ctx->sources.size());
+    const int mon = monitors_consts_floats(ctx);
+    const int stride = std::max(nsrc + mon, 1);
+    ctx->h_consts.assign(static_cast<size_t>(stride) * count, 0.0f);
+    const double dt = static_cast<double>(ctx->cfg.courant);
+    for (int k = 0; k < count; ++k) {
+        float *row = &ctx->h_consts[static_cast<size_t>(k) * stride];
+        const double t = (static_cast<double>(ctx->step_count + static_cast<uint64_t>(k)) + 0.5) * dt;
+        for (int s = 0; s < nsrc; ++s) row[s] = static_cast<float>(source_signal(ctx->sources[s], t));
+        monitors_fill_consts(ctx, row + nsrc, ctx->step_count + static_cast<uint64_t>(k) + 1);
+    }
+    if (int rc = ctx->d_consts.ensure(ctx->h_consts.size() * sizeof(float))) return rc;
+    EMFD_HIP(hipMemcpyAsync(ctx->d_consts.ptr, ctx->h_consts.data(), ctx->h_consts.size() * sizeof(float),
+                            hipMemcpyHostToDevice, ctx->stream));
+
+    const LatticeArgs lat = ctx->lattice();
+    for (int k = 0; k < count; ++k) {
+        const float *consts = ctx->d_consts.as<float>() + static_cast<size_t>(k) * stride;
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        if (ctx->timing) {
+            if (ctx->ev_used + 2 > ctx->ev_pool.size()) {
+                hipEvent_t a, b;
+                EMFD_HIP(hipEventCreate(&a));
+                EMFD_HIP(hipEventCreate(&b));
+                ctx->ev_pool.push_back(a);
+                ctx->ev_pool.push_back(b);
+            }
+            ev0 = ctx->ev_pool[ctx->ev_used];
+            ev1 = ctx->ev_pool[ctx->ev_used + 1];
+            ctx->ev_used += 2;
+            EMFD_HIP(hipEventRecord(ev0, ctx->stream));
+        }
+        if (ctx->kernel_variant == EMFD_KERNEL_TWO_PASS) {
+            EMFD_HIP(launch_yee_two_pass(lat, ctx->fields(ctx->cur), ctx->stream));
+        } else {
+            EMFD_HIP(launch_yee_fused(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->tile_cfg,
+                                      ctx->stream));
+            ctx->cur ^= 1;
+        }
+        if (ev1) EMFD_HIP(hipEventRecord(ev1, ctx->stream));
+        if (ctx->n_cells > 0 && nsrc > 0)
+            EMFD_HIP(launch_apply_sources(lat, ctx->fields(ctx->cur), ctx->d_cells.as<int>(), ctx->n_cells,
+                                          ctx->d_lists.as<int>(), ctx->d_packed.as<float>(), consts, ctx->stream));
+        ++ctx->step_count;  // observers see the time of the step that just completed
+        if (int rc = monitors_after_step(ctx, consts + nsrc, k)) return rc;
+        if (ctx->timing && ctx->ev_used >= 4096) {
+            double ms;
+            int64_t l;
+            if (int rc = emfd_grid_kernel_timing(ctx, -1, &ms, &l)) return rc;
+        }
+    }
+    return EMFD_OK;
+}
+
+int emfd_grid_synchronize(emfd_ctx *ctx) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    return EMFD_OK;
+}
+
+int emfd_grid_state(emfd_ctx *ctx, double *time, uint64_t *step_count, float *dt) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    if (time) *time = ctx->time();
+    if (step_count) *step_count = ctx->step_count;
+    if (dt) *dt = ctx->cfg.courant;
+    return EMFD_OK;
+}
+
+int emfd_grid_download_fields(emfd_ctx *ctx, float *E, float *B, int layout) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
+    if (layout != EMFD_LAYOUT_TEXEL && layout != EMFD_LAYOUT_SOA) return fail(EMFD_ERR_INVALID, "unknown layout");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    const size_t n = ctx->n;
+    const float *base = ctx->field_buf[ctx->cur].as<float>();
+    if (layout == EMFD_LAYOUT_SOA) {
+        if (E) EMFD_HIP(hipMemcpy(E, base, 3 * n * sizeof(float), hipMemcpyDeviceToHost));
+        if (B) EMFD_HIP(hipMemcpy(B, base + 3 * n, 3 * n * sizeof(float), hipMemcpyDeviceToHost));
+        return EMFD_OK;
+    }
+    std::vector<float> plane(3 * n), alpha(n);
+    for (int f = 0; f < 2; ++f) {
+        float *dst = f == 0 ? E : B;
+        if (!dst) continue;
+        EMFD_HIP(hipMemcpy(plane.data(), base + static_cast<size_t>(3 * f) * n, 3 * n * sizeof(float),
+                           hipMemcpyDeviceToHost));
+        EMFD_HIP(hipMemcpy(alpha.data(), f == 0 ? ctx->sigma_cell.ptr : ctx->eps_cell.ptr, n * sizeof(float),
+                           hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            dst[4 * i + 0] = plane[i];
+            dst[4 * i + 1] = plane[n + i];
+            dst[4 * i + 2] = plane[2 * n + i];
+            dst[4 * i + 3] = alpha[i];
+        }
+    }
+    return EMFD_OK;
+}
+
+int emfd_grid_upload_fields(emfd_ctx *ctx, const float *E, const float *B, int layout) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
+    if (layout != EMFD_LAYOUT_TEXEL && layout != EMFD_LAYOUT_SOA) return fail(EMFD_ERR_INVALID, "unknown layout");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    const size_t n = ctx->n;
+    float *base = ctx->field_buf[ctx->cur].as<float>();
+    for (int f = 0; f < 2; ++f) {
+        const float *srcp = f == 0 ? E : B;
+        if (!srcp) continue;
+        if (layout == EMFD_LAYOUT_SOA) {
+            EMFD_HIP(hipMemcpy(base + static_cast<size_t>(3 * f) * n, srcp, 3 * n * sizeof(float),
+                               hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> plane(3 * n);
+            for (size_t i = 0; i < n; ++i)
+                for (int c = 0; c < 3; ++c) plane[static_cast<size_t>(c) * n + i] = srcp[4 * i + c];
+            EMFD_HIP(hipMemcpy(base + static_cast<size_t>(3 * f) * n, plane.data(), 3 * n * sizeof(float),
+                               hipMemcpyHostToDevice));
+        }
+    }
+    return EMFD_OK;
+}
+
+int64_t emfd_grid_memory_bytes(emfd_ctx *ctx) {
+    if (!ctx) return -1;
+    int64_t total = 0;
+    const DevBuf *bufs[] = {&ctx->field_buf[0], &ctx->field_buf[1], &ctx->sigma_cell, &ctx->eps_cell, &ctx->mat_id,
+                            &ctx->table, &ctx->prof, &ctx->d_packed, &ctx->d_cells, &ctx->d_lists, &ctx->d_consts};
+    for (const DevBuf *b : bufs) total += static_cast<int64_t>(b->bytes);
+    return total;
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+// Internal declarations shared by the host side (grid.cpp, monitors.cpp) and the device side
+// (kernels.hip) of libemfdtd.so. Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace emfd {
+
+// Coefficient class table entry: one per distinct (edge sigma, edge eps_r, thin-wire factors)
+// combination of a lattice texel. Fields are float offsets inside a kTableStride-float record.
+constexpr int kTableStride = 20;
+enum TableField : int {
+    kSig = 0,    // 3: max(edge sigma, 0) per E component (from binary16)
+    kEps = 3,    // 3: max(edge eps_r, 1)
+    kCa = 6,     // 3: exp(-x) with x = sigma dt / eps   (interior, no absorber)
+    kCb = 9,     // 3: (dt / eps) phi(x)
+    kWire = 12,  // 6: thin-wire factors (Bx t0, Bx t1, By t0, By t1, Bz t0, Bz t1)
+};
+
+// Per-grid constants every lattice kernel needs.
+struct LatticeArgs {
+    int nx, ny, nz;
+    int pml;
+    float dt;
+    int prof_stride;            // >= max(nx, ny, nz)
+    const float *prof;          // absorber profile [axis][half][i]: sigma_pml contribution of axis
+                                // at coordinate i (half = 0) or i + 1/2 (half = 1)
+    const void *mat_id;         // per-texel class (uint16 or uint32), or null for uniform vacuum
+    int id_bytes;               // 2 or 4
+    const float *table;         // classes x kTableStride floats
+    int has_wire;               // scene has thin wires
+};
+
+struct FieldPtrs {
+    float *e[3];
+    float *b[3];
+};
+
+// Device-side step constants: per step k, nsrc signals, then port rotation (4), then the
+// far-field rotation (4). Stride per step = step_stride floats.
+struct StepConsts {
+    const float *base;
+    int step_stride;
+    int nsrc;
+    int port_off;
+    int ntff_off;
+};
+
+// ---- launchers (kernels.hip) ----
+hipError_t launch_build_table(const uint16_t *keys, int classes, float dt, float *table, hipStream_t s);
+hipError_t launch_yee_fused(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int tile_cfg,
+                            hipStream_t s);
+hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_apply_sources(const LatticeArgs &a, const FieldPtrs &f, const int *cells, int n_cells,
+                                const int *lists, const float *packed, const float *signals, hipStream_t s);
+hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
+
+struct PortDev {
+    int texel[3];
+    int axis;
+    int edges;
+    int sign;
+};
+hipError_t launch_port_accumulate(const LatticeArgs &a, const FieldPtrs &f, const PortDev *ports, int n_ports,
+                                  const float *rot, float blend, float *phasors, float *series, int64_t record_index,
+                                  hipStream_t s);
+
+struct NtffBox {
+    int bmin[3], bmax[3];
+    int face_offset[7];
+    int n_points;
+};
+hipError_t launch_ntff_accumulate(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, const float *rot,
+                                  float blend, float *phasors, hipStream_t s);
+hipError_t launch_ntff_pattern(const NtffBox &box, const float *phasors, const float *dirs, int n_dirs, float k,
+                               const float center[3], float *partials, int slices, float *out_f2,
+                               float *power_partials, hipStream_t s);
+int ntff_pattern_slices(int n_points);
+
+struct FluxPlane {
+    int axis, index;
+    int rmin[2], rsize[2];
+    int first_group;
+};
+hipError_t launch_flux(const LatticeArgs &a, const FieldPtrs &f, const FluxPlane *planes, int n_planes,
+                       int total_groups, float *partials, hipStream_t s);
+hipError_t launch_probe_gather(const LatticeArgs &a, const FieldPtrs &f, const int *origins, int n_probes,
+                               const float *sigma_cell, const float *eps_cell, float *blocks, hipStream_t s);
+
+}  // namespace emfd

@@ -1,0 +1,554 @@
+// HIP kernels of the FDTD hot path for gfx950 (MI355X).
+//
+// Field storage: six float planes (Ex Ey Ez Bx By Bz), each x-fastest with EM_Lab's lattice index
+// (z*ny + y)*nx + x, so a 64-lane wavefront reads 256 contiguous bytes per component. The cell's
+// material enters through a 2-byte class id that indexes a small coefficient table (absent for
+// uniform vacuum); the absorber conductivity is rebuilt from three 1-D profiles.
+//
+// Arithmetic follows the reference shaders operation by operation (shaders/common.glsl:21-54,
+// shaders/update_B.comp:31-70, shaders/update_E.comp:31-66, shaders/apply_sources.comp:37-79);
+// this file is compiled with -ffp-contract=off so no multiply-add is fused.
+#include "internal.h"
+
+#include <algorithm>
+
+namespace emfd {
+namespace {
+
+constexpr int kXcds = 8;  // MI355X: 8 accelerator dies, each with its own L2
+
+__device__ __forceinline__ float phi_f(float x) {
+    // phi(x) = (1 - e^{-x}) / x, Taylor series below 1e-2 (shaders/common.glsl:50-54)
+    const float series = 1.0f - x * (0.5f - x * (1.0f / 6.0f - x * (1.0f / 24.0f)));
+    const float exact = (1.0f - expf(-x)) / fmaxf(x, 1e-12f);
+    return x >= 1e-2f ? exact : series;
+}
+
+__device__ __forceinline__ bool in_pml(const LatticeArgs &a, int x, int y, int z) {
+    // shaders/common.glsl:40-43
+    const int L = a.pml;
+    return x < L || y < L || z < L || x >= a.nx - (L + 1) || y >= a.ny - (L + 1) || z >= a.nz - (L + 1);
+}
+
+// Absorber sigma at a staggered point: (profile_x + profile_y) + profile_z as pmlSigma sums it
+// (shaders/common.glsl:31-36). hx/hy/hz select the +1/2 samples.
+__device__ __forceinline__ float pml_sigma(const LatticeArgs &a, int x, int hx, int y, int hy, int z, int hz) {
+    const int s = a.prof_stride;
+    return a.prof[(2 * 0 + hx) * s + x] + a.prof[(2 * 1 + hy) * s + y] + a.prof[(2 * 2 + hz) * s + z];
+}
+
+__device__ __forceinline__ uint32_t load_id(const LatticeArgs &a, size_t i) {
+    if (a.id_bytes == 2) return static_cast<const uint16_t *>(a.mat_id)[i];
+    return static_cast<const uint32_t *>(a.mat_id)[i];
+}
+
+// Faraday update of one texel (shaders/update_B.comp:35-69). e0: E at p; exn/eyn/ezn: E at
+// p+x, p+y, p+z (0 beyond the upper faces). id: class of p (0 when no table).
+__device__ __forceinline__ void faraday(const LatticeArgs &a, int x, int y, int z, const float e0[3],
+                                        const float exn[3], const float eyn[3], const float ezn[3], uint32_t id,
+                                        float b[3]) {
+    float t0x = eyn[2] - e0[2], t0y = ezn[0] - e0[0], t0z = exn[1] - e0[1];
+    float t1x = ezn[1] - e0[1], t1y = exn[2] - e0[2], t1z = eyn[0] - e0[0];
+    if (a.has_wire) {
+        const float *w = a.table + static_cast<size_t>(id) * kTableStride + kWire;
+        t0x *= w[0];
+        t1x *= w[1];
+        t0y *= w[2];
+        t1y *= w[3];
+        t0z *= w[4];
+        t1z *= w[5];
+    }
+    const float cx = t0x - t1x, cy = t0y - t1y, cz = t0z - t1z;
+    const float dt = a.dt;
+    if (in_pml(a, x, y, z)) {
+        const float sx = pml_sigma(a, x, 0, y, 1, z, 1) * dt;
+        const float sy = pml_sigma(a, x, 1, y, 0, z, 1) * dt;
+        const float sz = pml_sigma(a, x, 1, y, 1, z, 0) * dt;
+        b[0] = expf(-sx) * b[0] - (dt * phi_f(sx)) * cx;
+        b[1] = expf(-sy) * b[1] - (dt * phi_f(sy)) * cy;
+        b[2] = expf(-sz) * b[2] - (dt * phi_f(sz)) * cz;
+    } else {
+        b[0] = b[0] - dt * cx;
+        b[1] = b[1] - dt * cy;
+        b[2] = b[2] - dt * cz;
+    }
+}
+
+// Update coefficients Ca, Cb of the three E edges of texel p (shaders/update_E.comp:46-61).
+__device__ __forceinline__ void edge_coefficients(const LatticeArgs &a, int x, int y, int z, uint32_t id,
+                                                  float ca[3], float cb[3]) {
+    const float dt = a.dt;
+    if (in_pml(a, x, y, z)) {
+        float sig[3] = {0.0f, 0.0f, 0.0f}, eps[3] = {1.0f, 1.0f, 1.0f};
+        if (a.mat_id) {
+            const float *t = a.table + static_cast<size_t>(id) * kTableStride;
+            sig[0] = t[kSig + 0]; sig[1] = t[kSig + 1]; sig[2] = t[kSig + 2];
+            eps[0] = t[kEps + 0]; eps[1] = t[kEps + 1]; eps[2] = t[kEps + 2];
+        }
+        sig[0] += pml_sigma(a, x, 1, y, 0, z, 0);
+        sig[1] += pml_sigma(a, x, 0, y, 1, z, 0);
+        sig[2] += pml_sigma(a, x, 0, y, 0, z, 1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float dtEps = dt / eps[c];
+            const float xx = sig[c] * dtEps;
+            ca[c] = expf(-xx);
+            cb[c] = dtEps * phi_f(xx);
+        }
+    } else if (a.mat_id) {
+        const float *t = a.table + static_cast<size_t>(id) * kTableStride;
+        ca[0] = t[kCa + 0]; ca[1] = t[kCa + 1]; ca[2] = t[kCa + 2];
+        cb[0] = t[kCb + 0]; cb[1] = t[kCb + 1]; cb[2] = t[kCb + 2];
+    } else {
+        // vacuum: x = 0 -> exp(-0) = 1, phi(0) = 1, dt / 1 = dt
+        ca[0] = ca[1] = ca[2] = 1.0f;
+        cb[0] = cb[1] = cb[2] = dt;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Coefficient table from binary16 keys: 12 halves per class
+// (sigma x y z, eps x y z, wire Bx t0 t1, By t0 t1, Bz t0 t1).
+// ---------------------------------------------------------------------------------------------
+__global__ void k_build_table(const uint16_t *keys, int classes, float dt, float *table) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= classes) return;
+    const uint16_t *k = keys + 12 * static_cast<size_t>(i);
+    float *t = table + static_cast<size_t>(i) * kTableStride;
+    float h[12];
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+        _Float16 v;
+        __builtin_memcpy(&v, &k[j], 2);
+        h[j] = static_cast<float>(v);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float sig = fmaxf(h[c], 0.0f);
+        const float eps = fmaxf(h[3 + c], 1.0f);
+        const float dtEps = dt / eps;
+        const float xx = sig * dtEps;
+        t[kSig + c] = sig;
+        t[kEps + c] = eps;
+        t[kCa + c] = expf(-xx);
+        t[kCb + c] = dtEps * phi_f(xx);
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) t[kWire + j] = h[6 + j];
+    t[18] = 0.0f;
+    t[19] = 0.0f;
+}
+
+__global__ void k_fill(float *p, size_t n, float v) {
+    size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Two-pass variant: update_B then update_E in place, one thread per texel, as the reference
+// dispatches them (src/SimulationGrid.cpp:393-408). Kept as a plain baseline.
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_update_b(LatticeArgs a, FieldPtrs f) {
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    if (x >= a.nx || y >= a.ny) return;
+    const size_t sx = 1, sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
+    const size_t p = z * sz + y * sy + x;
+    float e0[3], exn[3], eyn[3], ezn[3], b[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float *e = f.e[c];
+        e0[c] = e[p];
+        exn[c] = x + 1 < a.nx ? e[p + sx] : 0.0f;
+        eyn[c] = y + 1 < a.ny ? e[p + sy] : 0.0f;
+        ezn[c] = z + 1 < a.nz ? e[p + sz] : 0.0f;
+        b[c] = f.b[c][p];
+    }
+    const uint32_t id = a.mat_id ? load_id(a, p) : 0u;
+    faraday(a, x, y, z, e0, exn, eyn, ezn, id, b);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f.b[c][p] = b[c];
+}
+
+__global__ void __launch_bounds__(256) k_update_e(LatticeArgs a, FieldPtrs f) {
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    if (x >= a.nx || y >= a.ny) return;
+    const size_t sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
+    const size_t p = z * sz + y * sy + x;
+    float b0[3], bxm[3], bym[3], bzm[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float *b = f.b[c];
+        b0[c] = b[p];
+        bxm[c] = x > 0 ? b[p - 1] : 0.0f;
+        bym[c] = y > 0 ? b[p - sy] : 0.0f;
+        bzm[c] = z > 0 ? b[p - sz] : 0.0f;
+    }
+    const float curl[3] = {(b0[2] - bym[2]) - (b0[1] - bzm[1]), (b0[0] - bzm[0]) - (b0[2] - bxm[2]),
+                           (b0[1] - bxm[1]) - (b0[0] - bym[0])};
+    const uint32_t id = a.mat_id ? load_id(a, p) : 0u;
+    float ca[3], cb[3];
+    edge_coefficients(a, x, y, z, id, ca, cb);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f.e[c][p] = ca[c] * f.e[c][p] + cb[c] * curl[c];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fused leapfrog: B^{n+1/2} and E^{n+1} in one pass, ping-pong buffers.
+//
+// A workgroup owns a TX x (TY*RY) tile of (x, y) columns and marches up a z-chunk. Per plane z it
+//   - holds E^n(z) of the tile plus a one-cell halo in LDS (double-buffered: E^n(z+1) is staged
+//     while plane z is consumed),
+//   - computes B^{n+1/2}(z) on the tile and on the low-x column / low-y row halo (recomputed, not
+//     exchanged), keeps it in LDS for the -x/-y neighbours of the E update, and carries the own
+//     texel's B(z) in registers as the -z neighbour of plane z+1,
+//   - updates E^{n+1}(z) from LDS and registers.
+// HBM traffic per texel is E and B read once and written once (48 B) plus the class id.
+// A z-chunk that does not start at 0 recomputes B at z0-1 first (one extra plane).
+// ---------------------------------------------------------------------------------------------
+template <int TX, int TY, int RY>
+struct FusedTile {
+    static constexpr int RYT = TY * RY;            // rows per tile
+    static constexpr int EW = TX + 2, EH = RYT + 2;  // E plane with halo on every side
+    static constexpr int BW = TX + 1, BH = RYT + 1;  // B plane with low-side halo
+    static constexpr int NT = TX * TY;
+    static constexpr int EHALO = 2 * TX + 2 * EH;  // top/bottom rows + left/right columns
+    static constexpr int BHALO = TX + RYT;         // low-y row + low-x column
+    static constexpr int EH_SLOTS = (EHALO + NT - 1) / NT;
+    static constexpr int BH_SLOTS = (BHALO + NT - 1) / NT;
+};
+
+template <int TX, int TY, int RY>
+__global__ void __launch_bounds__(TX *TY) k_yee_fused(LatticeArgs a, FieldPtrs src, FieldPtrs dst, int tiles_x,
+                                                       int tiles_y, int zchunk, int n_tiles) {
+    using T = FusedTile<TX, TY, RY>;
+    constexpr int RYT = T::RYT, EW = T::EW, EH = T::EH, BW = T::BW, NT = T::NT;
+    __shared__ float sE[2][3][EH][EW];
+    __shared__ float sB[3][T::BH][BW];
+
+    // XCD-aware tile order: consecutive blocks land on different XCDs, so give each XCD a
+    // contiguous run of tiles (neighbours then share that XCD's L2 for the halo reads).
+    const int per = (n_tiles + kXcds - 1) / kXcds;
+    const int tile = (static_cast<int>(blockIdx.x) % kXcds) * per + static_cast<int>(blockIdx.x) / kXcds;
+    if (tile >= n_tiles) return;
+    const int ti_x = tile % tiles_x;
+    const int ti_y = (tile / tiles_x) % tiles_y;
+    const int ti_z = tile / (tiles_x * tiles_y);
+    const int x0 = ti_x * TX, y0 = ti_y * RYT;
+    const int z0 = ti_z * zchunk;
+    const int z1 = min(z0 + zchunk, a.nz);
+    const int zs = z0 > 0 ? z0 - 1 : 0;
+
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int tid = tx + TX * ty;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const size_t sy = static_cast<size_t>(nx), sz = static_cast<size_t>(nx) * ny;
+
+    // ---- E-region entry coordinates (own rows + halo) ----
+    // own: (ex, ey) = (tx+1, ty + TY r + 1); halo entries enumerated by h.
+    int ehx[T::EH_SLOTS], ehy[T::EH_SLOTS];
+#pragma unroll
+    for (int s = 0; s < T::EH_SLOTS; ++s) {
+        const int h = tid + s * NT;
+        int ex = -1, ey = -1;
+        if (h < TX) { ex = h + 1; ey = 0; }
+        else if (h < 2 * TX) { ex = h - TX + 1; ey = EH - 1; }
+        else if (h < T::EHALO) {
+            const int hh = h - 2 * TX;
+            ex = hh < EH ? 0 : EW - 1;
+            ey = hh % EH;
+        }
+        ehx[s] = ex;
+        ehy[s] = ey;
+    }
+    // ---- B halo entries: low-y row (bx in 1..TX, by 0), low-x column (bx 0, by 1..RYT) ----
+    int bhx[T::BH_SLOTS], bhy[T::BH_SLOTS];
+#pragma unroll
+    for (int s = 0; s < T::BH_SLOTS; ++s) {
+        const int h = tid + s * NT;
+        int bx = -1, by = -1;
+        if (h < TX) { bx = h + 1; by = 0; }
+        else if (h < T::BHALO) { bx = 0; by = h - TX + 1; }
+        bhx[s] = bx;
+        bhy[s] = by;
+    }
+
+    auto ld_e = [&](int c, int gx, int gy, int gz) -> float {
+        return (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz < nz) ? src.e[c][gz * sz + gy * sy + gx] : 0.0f;
+    };
+    auto ld_b = [&](int c, int gx, int gy, int gz) -> float {
+        return (gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? src.b[c][gz * sz + gy * sy + gx] : 0.0f;
+    };
+    auto ld_id = [&](int gx, int gy, int gz) -> uint32_t {
+        return (a.mat_id && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? load_id(a, gz * sz + gy * sy + gx) : 0u;
+    };
+
+    // registers: own E at z (current) and staged E at z+1, own B_old(z), halo E staged
+    float eo[RY][3], en[RY][3];
+    float bprev[RY][2];  // B^{n+1/2} at (own, z-1): x, y components (the -z neighbour)
+
+    // prologue: E plane zs into LDS buffer (zs & 1) and registers
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        const int gy = y0 + ty + TY * r;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            eo[r][c] = ld_e(c, x0 + tx, gy, zs);
+            sE[zs & 1][c][ty + TY * r + 1][tx + 1] = eo[r][c];
+        }
+        bprev[r][0] = bprev[r][1] = 0.0f;
+    }
+#pragma unroll
+    for (int s = 0; s < T::EH_SLOTS; ++s)
+        if (ehx[s] >= 0)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sE[zs & 1][c][ehy[s]][ehx[s]] = ld_e(c, x0 + ehx[s] - 1, y0 + ehy[s] - 1, zs);
+
+    for (int z = zs; z < z1; ++z) {
+        const int cur = z & 1, nxt = cur ^ 1;
+        // (a) stage E(z+1) into LDS (zero beyond the top face)
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const int gy = y0 + ty + TY * r;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                en[r][c] = ld_e(c, x0 + tx, gy, z + 1);
+                sE[nxt][c][ty + TY * r + 1][tx + 1] = en[r][c];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < T::EH_SLOTS; ++s)
+            if (ehx[s] >= 0)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) sE[nxt][c][ehy[s]][ehx[s]] = ld_e(c, x0 + ehx[s] - 1, y0 + ehy[s] - 1, z + 1);
+        // B_old(z) for own texels and B-halo texels, class ids
+        float bo[RY][3];
+        uint32_t ido[RY];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const int gy = y0 + ty + TY * r;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) bo[r][c] = ld_b(c, x0 + tx, gy, z);
+            ido[r] = ld_id(x0 + tx, gy, z);
+        }
+        float bho[T::BH_SLOTS][3];
+        uint32_t idh[T::BH_SLOTS];
+#pragma unroll
+        for (int s = 0; s < T::BH_SLOTS; ++s) {
+            const int gx = x0 + bhx[s] - 1, gy = y0 + bhy[s] - 1;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) bho[s][c] = bhx[s] >= 0 ? ld_b(c, gx, gy, z) : 0.0f;
+            idh[s] = bhx[s] >= 0 ? ld_id(gx, gy, z) : 0u;
+        }
+        __syncthreads();  // #1: E(z+1) staged
+
+        // (d) B^{n+1/2}(z): own texels
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const int ey = ty + TY * r + 1, ex = tx + 1;
+            const int gx = x0 + tx, gy = y0 + ty + TY * r;
+            float b[3] = {bo[r][0], bo[r][1], bo[r][2]};
+            if (gx < nx && gy < ny) {
+                float e0[3], exn[3], eyn[3], ezn[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    e0[c] = sE[cur][c][ey][ex];
+                    exn[c] = sE[cur][c][ey][ex + 1];
+                    eyn[c] = sE[cur][c][ey + 1][ex];
+                    ezn[c] = en[r][c];
+                }
+                faraday(a, gx, gy, z, e0, exn, eyn, ezn, ido[r], b);
+                if (z >= z0) {
+                    const size_t p = z * sz + gy * sy + gx;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) dst.b[c][p] = b[c];
+                }
+            } else {
+                b[0] = b[1] = b[2] = 0.0f;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sB[c][ey][ex] = b[c];
+            bo[r][0] = b[0];
+            bo[r][1] = b[1];
+            bo[r][2] = b[2];
+        }
+        // halo texels (recomputed; zero outside the domain = "B below the low faces is 0")
+#pragma unroll
+        for (int s = 0; s < T::BH_SLOTS; ++s) {
+            if (bhx[s] < 0) continue;
+            const int bx = bhx[s], by = bhy[s];
+            const int gx = x0 + bx - 1, gy = y0 + by - 1;
+            float b[3] = {bho[s][0], bho[s][1], bho[s][2]};
+            if (gx >= 0 && gy >= 0 && gx < nx && gy < ny) {
+                float e0[3], exn[3], eyn[3], ezn[3];
+                // E index of texel (gx, gy) is (bx, by) in the E region
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    e0[c] = sE[cur][c][by][bx];
+                    exn[c] = sE[cur][c][by][bx + 1];
+                    eyn[c] = sE[cur][c][by + 1][bx];
+                    ezn[c] = sE[nxt][c][by][bx];
+                }
+                faraday(a, gx, gy, z, e0, exn, eyn, ezn, idh[s], b);
+            } else {
+                b[0] = b[1] = b[2] = 0.0f;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sB[c][by][bx] = b[c];
+        }
+        __syncthreads();  // #2: B(z) visible
+
+        // (f) E^{n+1}(z) for own texels
+        if (z >= z0) {
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                const int gx = x0 + tx, gy = y0 + ty + TY * r;
+                if (gx < nx && gy < ny) {
+                    const int by = ty + TY * r + 1, bx = tx + 1;
+                    const float b0x = bo[r][0], b0y = bo[r][1], b0z = bo[r][2];
+                    const float bxm_y = sB[1][by][bx - 1], bxm_z = sB[2][by][bx - 1];
+                    const float bym_x = sB[0][by - 1][bx], bym_z = sB[2][by - 1][bx];
+                    const float bzm_x = bprev[r][0], bzm_y = bprev[r][1];
+                    const float curl[3] = {(b0z - bym_z) - (b0y - bzm_y), (b0x - bzm_x) - (b0z - bxm_z),
+                                           (b0y - bxm_y) - (b0x - bym_x)};
+                    float ca[3], cb[3];
+                    edge_coefficients(a, gx, gy, z, ido[r], ca, cb);
+                    const size_t p = z * sz + gy * sy + gx;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) dst.e[c][p] = ca[c] * eo[r][c] + cb[c] * curl[c];
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            bprev[r][0] = bo[r][0];
+            bprev[r][1] = bo[r][1];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) eo[r][c] = en[r][c];
+        }
+        // next iteration writes sE[cur] (read in (d) above, before barrier #2) and sB (read in
+        // (f) above, before the next barrier #1): no third barrier needed.
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// apply_sources.comp (shaders/apply_sources.comp:37-79): one thread per affected texel.
+// packed: 16 floats per source (center, halfExtent, polAmp, softness; vec4 each).
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ float footprint(const float *s, float x, float y, float z) {
+    const float dx = fmaxf(fabsf(x - s[0]) - s[4], 0.0f) / s[12];
+    const float dy = fmaxf(fabsf(y - s[1]) - s[5], 0.0f) / s[13];
+    const float dz = fmaxf(fabsf(z - s[2]) - s[6], 0.0f) / s[14];
+    return expf(-(dx * dx + dy * dy + dz * dz));
+}
+
+__global__ void __launch_bounds__(64) k_apply_sources(LatticeArgs a, FieldPtrs f, const int *cells, int n_cells,
+                                                      const int *lists, const float *packed, const float *signals) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n_cells) return;
+    const int x = cells[4 * id + 0], y = cells[4 * id + 1], z = cells[4 * id + 2];
+    const int first = cells[4 * id + 3], end = cells[4 * (id + 1) + 3];
+    const float cx = static_cast<float>(x), cy = static_cast<float>(y), cz = static_cast<float>(z);
+    float J[3] = {0.0f, 0.0f, 0.0f};
+    for (int k = first; k < end; ++k) {
+        const int i = lists[k];
+        const float signal = signals[i];
+        if (signal == 0.0f) continue;
+        const float *s = packed + 16 * i;
+        const float w[3] = {footprint(s, cx + 0.5f, cy + 0.0f, cz + 0.0f), footprint(s, cx + 0.0f, cy + 0.5f, cz + 0.0f),
+                            footprint(s, cx + 0.0f, cy + 0.0f, cz + 0.5f)};
+        const float as = s[11] * signal;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) J[c] += s[8 + c] * as * w[c];
+    }
+    if (J[0] == 0.0f && J[1] == 0.0f && J[2] == 0.0f) return;
+    const size_t p = (static_cast<size_t>(z) * a.ny + y) * a.nx + x;
+    const uint32_t cid = a.mat_id ? load_id(a, p) : 0u;
+    float ca[3], cb[3];
+    edge_coefficients(a, x, y, z, cid, ca, cb);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f.e[c][p] = f.e[c][p] - cb[c] * J[c];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------------------------
+hipError_t launch_build_table(const uint16_t *keys, int classes, float dt, float *table, hipStream_t s) {
+    if (classes <= 0) return hipSuccess;
+    k_build_table<<<(classes + 127) / 128, 128, 0, s>>>(keys, classes, dt, table);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+    k_fill<<<static_cast<unsigned>(blocks), 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s) {
+    dim3 block(64, 4, 1), grid((a.nx + 63) / 64, (a.ny + 3) / 4, a.nz);
+    k_update_b<<<grid, block, 0, s>>>(a, f);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_update_e<<<grid, block, 0, s>>>(a, f);
+    return hipGetLastError();
+}
+
+namespace {
+template <int TX, int TY, int RY>
+hipError_t launch_fused_cfg(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int zchunk,
+                            hipStream_t s) {
+    const int tiles_x = (a.nx + TX - 1) / TX;
+    const int tiles_y = (a.ny + TY * RY - 1) / (TY * RY);
+    const int tiles_z = (a.nz + zchunk - 1) / zchunk;
+    const int n_tiles = tiles_x * tiles_y * tiles_z;
+    const int blocks = (n_tiles + kXcds - 1) / kXcds * kXcds;
+    dim3 block(TX, TY, 1);
+    k_yee_fused<TX, TY, RY><<<blocks, block, 0, s>>>(a, src, dst, tiles_x, tiles_y, zchunk, n_tiles);
+    return hipGetLastError();
+}
+}  // namespace
+
+// tile_cfg: low byte = tile shape id, upper bits = z-chunk (0 = choose automatically)
+hipError_t launch_yee_fused(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int tile_cfg,
+                            hipStream_t s) {
+    const int shape = tile_cfg & 0xff;
+    int zchunk = tile_cfg >> 8;
+    auto auto_chunk = [&](int tx, int ty) {
+        // aim for >= 4 workgroups per CU over 256 CUs
+        const long cols = static_cast<long>((a.nx + tx - 1) / tx) * ((a.ny + ty - 1) / ty);
+        long chunks = (2048 + cols - 1) / cols;
+        if (chunks < 1) chunks = 1;
+        int zc = static_cast<int>((a.nz + chunks - 1) / chunks);
+        return std::max(zc, 8);
+    };
+    switch (shape) {
+    case 1:
+        if (!zchunk) zchunk = auto_chunk(64, 8);
+        return launch_fused_cfg<64, 4, 2>(a, src, dst, zchunk, s);
+    case 2:
+        if (!zchunk) zchunk = auto_chunk(64, 16);
+        return launch_fused_cfg<64, 4, 4>(a, src, dst, zchunk, s);
+    case 3:
+        if (!zchunk) zchunk = auto_chunk(64, 8);
+        return launch_fused_cfg<64, 8, 1>(a, src, dst, zchunk, s);
+    case 4:
+        if (!zchunk) zchunk = auto_chunk(128, 4);
+        return launch_fused_cfg<128, 2, 2>(a, src, dst, zchunk, s);
+    default:
+        if (!zchunk) zchunk = auto_chunk(64, 8);
+        return launch_fused_cfg<64, 4, 2>(a, src, dst, zchunk, s);
+    }
+}
+
+hipError_t launch_apply_sources(const LatticeArgs &a, const FieldPtrs &f, const int *cells, int n_cells,
+                                const int *lists, const float *packed, const float *signals, hipStream_t s) {
+    if (n_cells <= 0) return hipSuccess;
+    k_apply_sources<<<(n_cells + 63) / 64, 64, 0, s>>>(a, f, cells, n_cells, lists, packed, signals);
+    return hipGetLastError();
+}
+
+}  // namespace emfd

@@ -1,0 +1,391 @@
+// HIP kernels of the per-step and on-demand monitors: port gap V/I with the two-stage EMA,
+// near-to-far-field Huygens-box phasors, radiation integrals, Poynting flux, probe gather.
+// Compiled with -ffp-contract=off, like the Yee kernels, so each expression rounds as written.
+#include "internal.h"
+
+namespace emfd {
+namespace {
+
+__device__ __forceinline__ float fetch_c(const LatticeArgs &a, const float *plane, int x, int y, int z) {
+    // texelFetch with indices clamped to the grid (shaders/field_sampling.glsl:14-15,
+    // shaders/port_accumulate.comp:40-41)
+    x = min(max(x, 0), a.nx - 1);
+    y = min(max(y, 0), a.ny - 1);
+    z = min(max(z, 0), a.nz - 1);
+    return plane[(static_cast<size_t>(z) * a.ny + y) * a.nx + x];
+}
+
+// eAtNode / bAtNode (shaders/field_sampling.glsl:17-32)
+__device__ __forceinline__ void e_at_node(const LatticeArgs &a, const FieldPtrs &f, int x, int y, int z, float o[3]) {
+    o[0] = 0.5f * (fetch_c(a, f.e[0], x, y, z) + fetch_c(a, f.e[0], x - 1, y, z));
+    o[1] = 0.5f * (fetch_c(a, f.e[1], x, y, z) + fetch_c(a, f.e[1], x, y - 1, z));
+    o[2] = 0.5f * (fetch_c(a, f.e[2], x, y, z) + fetch_c(a, f.e[2], x, y, z - 1));
+}
+__device__ __forceinline__ void b_at_node(const LatticeArgs &a, const FieldPtrs &f, int x, int y, int z, float o[3]) {
+    const float bx = fetch_c(a, f.b[0], x, y, z) + fetch_c(a, f.b[0], x, y - 1, z) + fetch_c(a, f.b[0], x, y, z - 1) +
+                     fetch_c(a, f.b[0], x, y - 1, z - 1);
+    const float by = fetch_c(a, f.b[1], x, y, z) + fetch_c(a, f.b[1], x - 1, y, z) + fetch_c(a, f.b[1], x, y, z - 1) +
+                     fetch_c(a, f.b[1], x - 1, y, z - 1);
+    const float bz = fetch_c(a, f.b[2], x, y, z) + fetch_c(a, f.b[2], x - 1, y, z) + fetch_c(a, f.b[2], x, y - 1, z) +
+                     fetch_c(a, f.b[2], x - 1, y - 1, z);
+    o[0] = 0.25f * bx;
+    o[1] = 0.25f * by;
+    o[2] = 0.25f * bz;
+}
+
+// ---------------------------------------------------------------------------------------------
+// port_accumulate.comp (shaders/port_accumulate.comp:45-83)
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ float edge_current(const LatticeArgs &a, const FieldPtrs &f, int p[3], int ax) {
+    const int u = (ax + 1) % 3, v = (ax + 2) % 3;
+    int pu[3] = {p[0], p[1], p[2]}, pv[3] = {p[0], p[1], p[2]};
+    pu[u] -= 1;
+    pv[v] -= 1;
+    const float b0v = fetch_c(a, f.b[v], p[0], p[1], p[2]), buv = fetch_c(a, f.b[v], pu[0], pu[1], pu[2]);
+    const float b0u = fetch_c(a, f.b[u], p[0], p[1], p[2]), bvu = fetch_c(a, f.b[u], pv[0], pv[1], pv[2]);
+    return (b0v - buv) - (b0u - bvu);
+}
+
+__global__ void __launch_bounds__(64) k_port_accumulate(LatticeArgs a, FieldPtrs f, const PortDev *ports, int n_ports,
+                                                        const float *rot, float blend, float *ph, float *series,
+                                                        int64_t rec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_ports) return;
+    const PortDev pt = ports[i];
+    int p[3] = {pt.texel[0], pt.texel[1], pt.texel[2]};
+    const int ax = pt.axis;
+    float V = 0.0f, I = 0.0f;
+    if (pt.edges == 4) {
+        const int u = (ax + 1) % 3, v = (ax + 2) % 3;
+        for (int k = 0; k < 4; ++k) {
+            int q[3] = {p[0], p[1], p[2]};
+            if (k == 1 || k == 3) q[u] += 1;
+            if (k == 2 || k == 3) q[v] += 1;
+            V += 0.25f * fetch_c(a, f.e[ax], q[0], q[1], q[2]);
+            I += edge_current(a, f, q, ax);
+        }
+    } else {
+        V = fetch_c(a, f.e[ax], p[0], p[1], p[2]);
+        I = edge_current(a, f, p, ax);
+    }
+    const float s = static_cast<float>(pt.sign);
+    V *= s;
+    I *= s;
+    if (rec >= 0 && series) {
+        series[2 * (rec * n_ports + i) + 0] = V;
+        series[2 * (rec * n_ports + i) + 1] = I;
+    }
+    const float obs[4] = {2.0f * rot[0] * V, -2.0f * rot[1] * V, 2.0f * rot[2] * I, -2.0f * rot[3] * I};
+    float *s1 = ph + 8 * i, *s2 = s1 + 4;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s1[c] += blend * (obs[c] - s1[c]);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s2[c] += blend * (s1[c] - s2[c]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Huygens box point (shaders/ntff_common.glsl:30-53)
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ void ntff_point(const NtffBox &b, int index, int node[3], float n[3], float *w) {
+    int f = 0;
+#pragma unroll
+    for (int i = 1; i < 6; ++i)
+        if (index >= b.face_offset[i]) f = i;
+    const int ax = f >> 1, side = f & 1;
+    const int u = ax == 0 ? 1 : 0, v = ax == 2 ? 1 : 2;
+    const int nu = b.bmax[u] - b.bmin[u], nv = b.bmax[v] - b.bmin[v];
+    const int local = index - b.face_offset[f];
+    const int iu = local % (nu + 1), iv = local / (nu + 1);
+    node[0] = b.bmin[0];
+    node[1] = b.bmin[1];
+    node[2] = b.bmin[2];
+    node[ax] = side == 1 ? b.bmax[ax] : b.bmin[ax];
+    node[u] = b.bmin[u] + iu;
+    node[v] = b.bmin[v] + iv;
+    *w = ((iu == 0 || iu == nu) ? 0.5f : 1.0f) * ((iv == 0 || iv == nv) ? 0.5f : 1.0f);
+    n[0] = n[1] = n[2] = 0.0f;
+    n[ax] = side == 1 ? 1.0f : -1.0f;
+}
+
+// ntff_accumulate.comp (shaders/ntff_accumulate.comp:30-52). Phasors are stored as 24 planes of
+// n_points floats: stage s (0,1) x quantity q (eRe, eIm, hRe, hIm) x component c -> plane 12 s + 3 q + c.
+__global__ void __launch_bounds__(256) k_ntff_accumulate(LatticeArgs a, FieldPtrs f, NtffBox box, const float *rot,
+                                                         float blend, float *ph) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= box.n_points) return;
+    int node[3];
+    float n[3], w;
+    ntff_point(box, i, node, n, &w);
+    float E[3], H[3];
+    e_at_node(a, f, node[0], node[1], node[2], E);
+    b_at_node(a, f, node[0], node[1], node[2], H);
+    const size_t np = static_cast<size_t>(box.n_points);
+    const float co[4] = {2.0f * rot[0], -2.0f * rot[1], 2.0f * rot[2], -2.0f * rot[3]};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float obs = co[q] * (q < 2 ? E[c] : H[c]);
+            float *p1 = ph + (3 * q + c) * np + i, *p2 = ph + (12 + 3 * q + c) * np + i;
+            float s1 = *p1, s2 = *p2;
+            s1 += blend * (obs - s1);
+            s2 += blend * (s1 - s2);
+            *p1 = s1;
+            *p2 = s2;
+        }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Radiation integrals: stage 1 (per direction x surface slice partial N, L) and stage 2 (sum of
+// slices and |N_perp - r x L|^2), shaders/ntff_pattern.comp:45-110 and ntff_reduce.comp:19-39.
+// Each thread owns one direction; the slice's surface points stream through LDS in tiles.
+// ---------------------------------------------------------------------------------------------
+constexpr int kPatTile = 256;
+
+__global__ void __launch_bounds__(kPatTile) k_ntff_pattern(NtffBox box, const float *ph, const float *dirs, int n_dirs,
+                                                           float k, float cx, float cy, float cz, int slices,
+                                                           float *partials) {
+    __shared__ float sPos[3][kPatTile];
+    __shared__ float sCur[12][kPatTile];
+    const int lid = threadIdx.x;
+    const int d = blockIdx.x * kPatTile + lid;
+    const int slice = blockIdx.y;
+    const bool in_range = d < n_dirs;
+    const float rx = in_range ? dirs[4 * d + 0] : 0.0f, ry = in_range ? dirs[4 * d + 1] : 0.0f,
+                rz = in_range ? dirs[4 * d + 2] : 1.0f;
+    const int np = box.n_points;
+    const int per = (np + slices - 1) / slices;
+    const int first = slice * per, last = min(first + per, np);
+    const size_t stride = static_cast<size_t>(np);
+    float Nr[3] = {0, 0, 0}, Ni[3] = {0, 0, 0}, Lr[3] = {0, 0, 0}, Li[3] = {0, 0, 0};
+    for (int base = first; base < last; base += kPatTile) {
+        const int i = base + lid;
+        if (i < last) {
+            int node[3];
+            float n[3], w;
+            ntff_point(box, i, node, n, &w);
+            sPos[0][lid] = static_cast<float>(node[0]) - cx;
+            sPos[1][lid] = static_cast<float>(node[1]) - cy;
+            sPos[2][lid] = static_cast<float>(node[2]) - cz;
+            float v[4][3];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) v[q][c] = ph[(12 + 3 * q + c) * stride + i];
+            // J = n x H, M = -n x E, weighted (cross order as GLSL cross())
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float *h = v[q];
+                const float cxv = n[1] * h[2] - h[1] * n[2];
+                const float cyv = n[2] * h[0] - h[2] * n[0];
+                const float czv = n[0] * h[1] - h[0] * n[1];
+                const int slot = q < 2 ? 6 + 3 * q : 3 * (q - 2);  // eRe->Mr, eIm->Mi, hRe->Jr, hIm->Ji
+                const float sw = q < 2 ? -w : w;
+                sCur[slot + 0][lid] = sw * cxv;
+                sCur[slot + 1][lid] = sw * cyv;
+                sCur[slot + 2][lid] = sw * czv;
+            }
+        }
+        __syncthreads();
+        const int count = min(kPatTile, last - base);
+        for (int kk = 0; kk < count; ++kk) {
+            const float psi = k * (rx * sPos[0][kk] + ry * sPos[1][kk] + rz * sPos[2][kk]);
+            float s, c;
+            sincosf(psi, &s, &c);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const float jr = sCur[j][kk], ji = sCur[3 + j][kk], mr = sCur[6 + j][kk], mi = sCur[9 + j][kk];
+                Nr[j] += jr * c - ji * s;
+                Ni[j] += jr * s + ji * c;
+                Lr[j] += mr * c - mi * s;
+                Li[j] += mr * s + mi * c;
+            }
+        }
+        __syncthreads();
+    }
+    if (in_range) {
+        float *o = partials + (static_cast<size_t>(d) * slices + slice) * 12;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            o[j] = Nr[j];
+            o[3 + j] = Ni[j];
+            o[6 + j] = Lr[j];
+            o[9 + j] = Li[j];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) k_ntff_reduce(const float *dirs, int n_dirs, int slices, const float *partials,
+                                                    float *out) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n_dirs) return;
+    const float r[3] = {dirs[4 * d + 0], dirs[4 * d + 1], dirs[4 * d + 2]};
+    float acc[12];
+#pragma unroll
+    for (int j = 0; j < 12; ++j) acc[j] = 0.0f;
+    for (int s = 0; s < slices; ++s) {
+        const float *p = partials + (static_cast<size_t>(d) * slices + s) * 12;
+#pragma unroll
+        for (int j = 0; j < 12; ++j) acc[j] += p[j];
+    }
+    const float *Nr = acc, *Ni = acc + 3, *Lr = acc + 6, *Li = acc + 9;
+    const float dr = Nr[0] * r[0] + Nr[1] * r[1] + Nr[2] * r[2];
+    const float di = Ni[0] * r[0] + Ni[1] * r[1] + Ni[2] * r[2];
+    const float crr[3] = {r[1] * Lr[2] - Lr[1] * r[2], r[2] * Lr[0] - Lr[2] * r[0], r[0] * Lr[1] - Lr[0] * r[1]};
+    const float cri[3] = {r[1] * Li[2] - Li[1] * r[2], r[2] * Li[0] - Li[2] * r[0], r[0] * Li[1] - Li[0] * r[1]};
+    float fr[3], fi[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        fr[j] = (Nr[j] - dr * r[j]) - crr[j];
+        fi[j] = (Ni[j] - di * r[j]) - cri[j];
+    }
+    out[d] = (fr[0] * fr[0] + fr[1] * fr[1] + fr[2] * fr[2]) + (fi[0] * fi[0] + fi[1] * fi[1] + fi[2] * fi[2]);
+}
+
+// Radiated power through the box: P = 1/2 sum w Re(E x H*) . n; one float partial per block
+// (tree reduction), summed in double on the host.
+__global__ void __launch_bounds__(256) k_ntff_power(NtffBox box, const float *ph, float *partials) {
+    __shared__ float sP[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    float P = 0.0f;
+    if (i < box.n_points) {
+        int node[3];
+        float n[3], w;
+        ntff_point(box, i, node, n, &w);
+        const size_t np = static_cast<size_t>(box.n_points);
+        float v[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[q][c] = ph[(12 + 3 * q + c) * np + i];
+        const float *eR = v[0], *eI = v[1], *hR = v[2], *hI = v[3];
+        const float c1[3] = {eR[1] * hR[2] - hR[1] * eR[2], eR[2] * hR[0] - hR[2] * eR[0], eR[0] * hR[1] - hR[0] * eR[1]};
+        const float c2[3] = {eI[1] * hI[2] - hI[1] * eI[2], eI[2] * hI[0] - hI[2] * eI[0], eI[0] * hI[1] - hI[0] * eI[1]};
+        P = w * ((c1[0] + c2[0]) * n[0] + (c1[1] + c2[1]) * n[1] + (c1[2] + c2[2]) * n[2]);
+    }
+    sP[threadIdx.x] = P;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sP[threadIdx.x] += sP[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = sP[0];
+}
+
+// ---------------------------------------------------------------------------------------------
+// poynting_flux.comp (shaders/poynting_flux.comp:32-63): 16x16 tiles, shared-memory tree, one
+// float partial per tile. Several planes in one launch: blockIdx.y selects the plane.
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_flux(LatticeArgs a, FieldPtrs f, const FluxPlane *planes, float *partials) {
+    __shared__ float sPart[256];
+    const FluxPlane pl = planes[blockIdx.y];
+    const int gx = (pl.rsize[0] + 15) / 16, gy = (pl.rsize[1] + 15) / 16;
+    const int group = blockIdx.x;
+    const int lid = threadIdx.x;
+    float flux = 0.0f;
+    const bool active = group < gx * gy;
+    if (active) {
+        const int lu = (group % gx) * 16 + (lid & 15), lv = (group / gx) * 16 + (lid >> 4);
+        if (lu < pl.rsize[0] && lv < pl.rsize[1]) {
+            const int uu = pl.rmin[0] + lu, vv = pl.rmin[1] + lv;
+            int p[3];
+            if (pl.axis == 0) { p[0] = pl.index; p[1] = uu; p[2] = vv; }
+            else if (pl.axis == 1) { p[0] = uu; p[1] = pl.index; p[2] = vv; }
+            else { p[0] = uu; p[1] = vv; p[2] = pl.index; }
+            if (p[0] >= 0 && p[1] >= 0 && p[2] >= 0 && p[0] < a.nx && p[1] < a.ny && p[2] < a.nz) {
+                float e[3], b[3];
+                e_at_node(a, f, p[0], p[1], p[2], e);
+                b_at_node(a, f, p[0], p[1], p[2], b);
+                const float S[3] = {e[1] * b[2] - b[1] * e[2], e[2] * b[0] - b[2] * e[0], e[0] * b[1] - b[0] * e[1]};
+                flux = S[pl.axis];
+            }
+        }
+    }
+    sPart[lid] = flux;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (lid < s) sPart[lid] += sPart[lid + s];
+        __syncthreads();
+    }
+    if (lid == 0 && active) partials[pl.first_group + group] = sPart[0];
+}
+
+// probe_gather.comp (shaders/probe_gather.comp:20-26): 27 E texels then 27 B texels per probe,
+// as float4 (x, y, z, alpha) with alpha = cell sigma (E) or eps_r (B).
+__global__ void __launch_bounds__(64) k_probe_gather(LatticeArgs a, FieldPtrs f, const int *origins, int n_probes,
+                                                     const float *sigma_cell, const float *eps_cell, float *blocks) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n_probes * 54) return;
+    const int probe = id / 54, k = id % 54, t = k % 27;
+    const int x = min(max(origins[4 * probe + 0] + t % 3, 0), a.nx - 1);
+    const int y = min(max(origins[4 * probe + 1] + (t / 3) % 3, 0), a.ny - 1);
+    const int z = min(max(origins[4 * probe + 2] + t / 9, 0), a.nz - 1);
+    const size_t p = (static_cast<size_t>(z) * a.ny + y) * a.nx + x;
+    float *o = blocks + 4 * static_cast<size_t>(id);
+    if (k < 27) {
+        o[0] = f.e[0][p]; o[1] = f.e[1][p]; o[2] = f.e[2][p]; o[3] = sigma_cell[p];
+    } else {
+        o[0] = f.b[0][p]; o[1] = f.b[1][p]; o[2] = f.b[2][p]; o[3] = eps_cell[p];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_port_accumulate(const LatticeArgs &a, const FieldPtrs &f, const PortDev *ports, int n_ports,
+                                  const float *rot, float blend, float *phasors, float *series, int64_t record_index,
+                                  hipStream_t s) {
+    if (n_ports <= 0) return hipSuccess;
+    k_port_accumulate<<<(n_ports + 63) / 64, 64, 0, s>>>(a, f, ports, n_ports, rot, blend, phasors, series,
+                                                          record_index);
+    return hipGetLastError();
+}
+
+hipError_t launch_ntff_accumulate(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, const float *rot,
+                                  float blend, float *phasors, hipStream_t s) {
+    if (box.n_points <= 0) return hipSuccess;
+    k_ntff_accumulate<<<(box.n_points + 255) / 256, 256, 0, s>>>(a, f, box, rot, blend, phasors);
+    return hipGetLastError();
+}
+
+int ntff_pattern_slices(int n_points) {
+    // enough slices that even one tile of directions spreads over the CUs
+    int slices = (n_points + 2047) / 2048;
+    return slices < 1 ? 1 : (slices > 256 ? 256 : slices);
+}
+
+hipError_t launch_ntff_pattern(const NtffBox &box, const float *phasors, const float *dirs, int n_dirs, float k,
+                               const float center[3], float *partials, int slices, float *out_f2,
+                               float *power_partials, hipStream_t s) {
+    if (n_dirs > 0) {
+        dim3 grid((n_dirs + kPatTile - 1) / kPatTile, slices, 1);
+        k_ntff_pattern<<<grid, kPatTile, 0, s>>>(box, phasors, dirs, n_dirs, k, center[0], center[1], center[2],
+                                                 slices, partials);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        k_ntff_reduce<<<(n_dirs + 63) / 64, 64, 0, s>>>(dirs, n_dirs, slices, partials, out_f2);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (power_partials && box.n_points > 0) {
+        k_ntff_power<<<(box.n_points + 255) / 256, 256, 0, s>>>(box, phasors, power_partials);
+        return hipGetLastError();
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_flux(const LatticeArgs &a, const FieldPtrs &f, const FluxPlane *planes, int n_planes,
+                       int max_groups, float *partials, hipStream_t s) {
+    if (n_planes <= 0 || max_groups <= 0) return hipSuccess;
+    dim3 grid(max_groups, n_planes, 1);
+    k_flux<<<grid, 256, 0, s>>>(a, f, planes, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_gather(const LatticeArgs &a, const FieldPtrs &f, const int *origins, int n_probes,
+                               const float *sigma_cell, const float *eps_cell, float *blocks, hipStream_t s) {
+    if (n_probes <= 0) return hipSuccess;
+    k_probe_gather<<<(n_probes * 54 + 63) / 64, 64, 0, s>>>(a, f, origins, n_probes, sigma_cell, eps_cell, blocks);
+    return hipGetLastError();
+}
+
+}  // namespace emfd

@@ -1,0 +1,615 @@
+"""Python host binding of libemfdtd.so (ctypes over the C ABI in include/emfdtd.h).
+
+The classes keep EM_Lab's names and call order (src/SimulationGrid.hpp:155-216,
+src/Ports.hpp:84-140, src/FarField.hpp:75-133, src/FieldAnalyzer.hpp:73-130). Every call
+goes to the HIP library; there is no CPU fallback. If the library is missing or no GPU is
+present, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Optional, Sequence
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libemfdtd.so"
+_lib = None
+
+EMFD_OK = 0
+LAYOUT_TEXEL = 0
+LAYOUT_SOA = 1
+KERNEL_AUTO = 0
+KERNEL_FUSED = 1
+KERNEL_TWO_PASS = 2
+
+
+class EmfdError(RuntimeError):
+    """Raised for any non-zero status of the C ABI (message from emfd_last_error)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"emfdtd error {code}: {msg}")
+        self.code = code
+
+
+# ---------------------------------------------------------------------------------------------
+# C structures (must match include/emfdtd.h)
+# ---------------------------------------------------------------------------------------------
+class GridConfigC(C.Structure):
+    _fields_ = [("size", C.c_int * 3), ("courant", C.c_float), ("pml_thickness", C.c_int),
+                ("pml_sigma_max", C.c_float), ("pml_grading", C.c_float)]
+
+
+class SourceDescC(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("half_extent", C.c_float * 3), ("softness", C.c_float * 3),
+                ("polarization", C.c_float * 3), ("amplitude", C.c_float), ("wavelength", C.c_float),
+                ("phase", C.c_float), ("waveform", C.c_int), ("ramp_periods", C.c_float),
+                ("repeat_periods", C.c_float), ("delay_periods", C.c_float), ("enabled", C.c_int)]
+
+
+class PortSetupC(C.Structure):
+    _fields_ = [("texel", C.c_int * 3), ("axis", C.c_int), ("sign", C.c_float), ("edges", C.c_int),
+                ("wavelength", C.c_float), ("voltage_feed", C.c_int), ("impedance_ohm", C.c_float),
+                ("drive", C.c_float)]
+
+
+class SweepSetupC(C.Structure):
+    _fields_ = [("wavelength_min", C.c_float), ("wavelength_max", C.c_float), ("points", C.c_int)]
+
+
+class PortReadingC(C.Structure):
+    _fields_ = [("valid", C.c_int)] + [(n, C.c_double) for n in (
+        "v_re", "v_im", "i_re", "i_im", "z_re", "z_im", "gamma_re", "gamma_im", "vswr", "return_loss_db",
+        "accepted_power", "available_power")]
+
+
+class SweepPointC(C.Structure):
+    _fields_ = [("wavelength", C.c_double), ("z_re", C.c_double), ("z_im", C.c_double), ("s11_db", C.c_double),
+                ("vswr", C.c_double)]
+
+
+class PortSweepC(C.Structure):
+    _fields_ = [("best", C.c_int64), ("best_wavelength", C.c_double), ("has_band", C.c_int),
+                ("band_low_wavelength", C.c_double), ("band_high_wavelength", C.c_double)]
+
+
+class FarFieldSetupC(C.Structure):
+    _fields_ = [("box_min", C.c_int * 3), ("box_max", C.c_int * 3), ("wavelength", C.c_float),
+                ("averaging_periods", C.c_float), ("map_axis", C.c_float * 3)]
+
+
+class FluxSurfaceC(C.Structure):
+    _fields_ = [("kind", C.c_int), ("axis", C.c_int), ("index", C.c_int), ("rect_min", C.c_int * 2),
+                ("rect_max", C.c_int * 2), ("box_min", C.c_int * 3), ("box_max", C.c_int * 3)]
+
+
+class ProbeSampleC(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("E", C.c_float * 3), ("B", C.c_float * 3), ("S", C.c_float * 3),
+                ("sigma", C.c_float), ("epsilon_r", C.c_float)]
+
+
+EDGE_DTYPE = np.dtype([("texel", "<i4", 3), ("axis", "<i4"), ("sigma", "<f4")])
+WIRE_DTYPE = np.dtype([("texel", "<i4", 3), ("component", "<i4"), ("term", "<i4"), ("factor", "<f4")])
+
+_P = C.c_void_p
+_SIGNATURES = {
+    "emfd_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "emfd_destroy": (None, [_P]),
+    "emfd_last_error": (C.c_char_p, []),
+    "emfd_version": (C.c_char_p, []),
+    "emfd_grid_configure": (C.c_int, [_P, C.POINTER(GridConfigC), C.POINTER(GridConfigC)]),
+    "emfd_grid_get_config": (C.c_int, [_P, C.POINTER(GridConfigC)]),
+    "emfd_grid_upload_materials": (C.c_int, [_P, _P, _P, C.c_int64, _P, C.c_int64, _P, C.c_int64]),
+    "emfd_grid_set_sources": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_int)]),
+    "emfd_grid_reset_fields": (C.c_int, [_P]),
+    "emfd_grid_step": (C.c_int, [_P, C.c_int]),
+    "emfd_grid_synchronize": (C.c_int, [_P]),
+    "emfd_grid_state": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.POINTER(C.c_float)]),
+    "emfd_grid_download_fields": (C.c_int, [_P, _P, _P, C.c_int]),
+    "emfd_grid_upload_fields": (C.c_int, [_P, _P, _P, C.c_int]),
+    "emfd_grid_memory_bytes": (C.c_int64, [_P]),
+    "emfd_grid_set_kernel": (C.c_int, [_P, C.c_int]),
+    "emfd_grid_material_classes": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "emfd_grid_kernel_timing": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "emfd_ports_configure": (C.c_int, [_P, _P, C.c_int, C.c_float, _P, C.c_float, C.c_int64]),
+    "emfd_ports_disable": (C.c_int, [_P]),
+    "emfd_ports_read_phasors": (C.c_int, [_P, _P, C.c_int]),
+    "emfd_ports_evaluate": (C.c_int, [_P, _P, C.c_int]),
+    "emfd_ports_recorded_steps": (C.c_int64, [_P]),
+    "emfd_ports_read_series": (C.c_int, [_P, _P, C.c_int64]),
+    "emfd_ports_evaluate_sweep": (C.c_int, [_P, C.c_double, _P, _P, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                            C.POINTER(C.c_int)]),
+    "emfd_ntff_configure": (C.c_int, [_P, C.POINTER(FarFieldSetupC), C.POINTER(FarFieldSetupC), C.POINTER(C.c_int)]),
+    "emfd_ntff_disable": (C.c_int, [_P]),
+    "emfd_ntff_reset": (C.c_int, [_P]),
+    "emfd_ntff_read_phasors": (C.c_int, [_P, _P, C.c_int64]),
+    "emfd_ntff_directions": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_int), _P, _P]),
+    "emfd_ntff_eval": (C.c_int, [_P, _P, C.c_int, _P, C.POINTER(C.c_double)]),
+    "emfd_ntff_evaluate": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_float), _P, _P,
+                                     _P]),
+    "emfd_flux_measure": (C.c_int, [_P, _P, C.c_int, _P]),
+    "emfd_probe_measure": (C.c_int, [_P, _P, C.c_int, _P]),
+}
+
+
+def library_path() -> Path:
+    return _LIB_PATH
+
+
+def load_library() -> C.CDLL:
+    """Load libemfdtd.so once. Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        raise RuntimeError(f"{_LIB_PATH} is missing: build it with `python -m em_lab_amd.build`")
+    lib = C.CDLL(str(_LIB_PATH))
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != EMFD_OK:
+        raise EmfdError(rc, _lib.emfd_last_error().decode(errors="replace"))
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------------------------------------
+# Plain data mirrors of the reference's structs
+# ---------------------------------------------------------------------------------------------
+@dataclass
+class GridConfig:
+    """GridConfig (src/SimulationGrid.hpp:25-31)."""
+    size: tuple = (128, 128, 128)
+    courant: float = 0.5
+    pml_thickness: int = 16
+    pml_sigma_max: float = 1.6
+    pml_grading: float = 3.0
+
+    def to_c(self) -> GridConfigC:
+        return GridConfigC((C.c_int * 3)(*[int(v) for v in self.size]), self.courant, int(self.pml_thickness),
+                           self.pml_sigma_max, self.pml_grading)
+
+    @staticmethod
+    def from_c(c: GridConfigC) -> "GridConfig":
+        return GridConfig(tuple(c.size), c.courant, c.pml_thickness, c.pml_sigma_max, c.pml_grading)
+
+
+WAVE_CW = 0
+WAVE_PULSE = 1
+
+
+@dataclass
+class SourceDesc:
+    """SourceDesc (src/SimulationGrid.hpp:113-150); factories as src/SimulationGrid.cpp:58-97."""
+    center: tuple = (0.0, 0.0, 0.0)
+    half_extent: tuple = (0.0, 0.0, 0.0)
+    softness: tuple = (0.75, 0.75, 0.75)
+    polarization: tuple = (0.0, 0.0, 1.0)
+    amplitude: float = 1.0
+    wavelength: float = 16.0
+    phase: float = 0.0
+    waveform: int = WAVE_CW
+    ramp_periods: float = 3.0
+    repeat_periods: float = 0.0
+    delay_periods: float = 0.0
+    enabled: bool = True
+    port: int = -1
+
+    def to_c(self) -> SourceDescC:
+        f3 = C.c_float * 3
+        return SourceDescC(f3(*self.center), f3(*self.half_extent), f3(*self.softness), f3(*self.polarization),
+                           self.amplitude, self.wavelength, self.phase, int(self.waveform), self.ramp_periods,
+                           self.repeat_periods, self.delay_periods, 1 if self.enabled else 0)
+
+
+@dataclass
+class PortSetup:
+    """PortSetup (src/Ports.hpp:30-40)."""
+    name: str = "port"
+    texel: tuple = (0, 0, 0)
+    axis: int = 2
+    sign: float = 1.0
+    edges: int = 1
+    wavelength: float = 16.0
+    voltage_feed: bool = True
+    impedance_ohm: float = 50.0
+    drive: float = 1.0
+
+    def to_c(self) -> PortSetupC:
+        return PortSetupC((C.c_int * 3)(*self.texel), self.axis, self.sign, self.edges, self.wavelength,
+                          1 if self.voltage_feed else 0, self.impedance_ohm, self.drive)
+
+
+@dataclass
+class SweepSetup:
+    """SweepSetup (src/Ports.hpp:42-47)."""
+    wavelength_min: float = 12.0
+    wavelength_max: float = 20.0
+    points: int = 81
+
+
+@dataclass
+class FarFieldSetup:
+    """FarFieldSetup (src/FarField.hpp:39-47)."""
+    box_min: tuple = (0, 0, 0)
+    box_max: tuple = (0, 0, 0)
+    wavelength: float = 16.0
+    averaging_periods: float = 4.0
+    map_axis: tuple = (0.0, 0.0, 1.0)
+    name: str = "far field"
+
+    def to_c(self) -> FarFieldSetupC:
+        return FarFieldSetupC((C.c_int * 3)(*self.box_min), (C.c_int * 3)(*self.box_max), self.wavelength,
+                              self.averaging_periods, (C.c_float * 3)(*self.map_axis))
+
+
+@dataclass
+class FluxSurface:
+    """FluxSurface (src/FieldAnalyzer.hpp:32-51)."""
+    kind: str = "plane"
+    axis: int = 0
+    index: int = 0
+    rect_min: tuple = (0, 0)
+    rect_max: tuple = (0, 0)
+    box_min: tuple = (0, 0, 0)
+    box_max: tuple = (0, 0, 0)
+    label: str = "flux"
+
+    @staticmethod
+    def plane(axis, index, rect_min, rect_max, label="flux"):
+        return FluxSurface("plane", max(0, min(2, axis)), index, tuple(rect_min), tuple(rect_max), label=label)
+
+    @staticmethod
+    def box(box_min, box_max, label="flux"):
+        lo = tuple(min(a, b) for a, b in zip(box_min, box_max))
+        hi = tuple(max(a, b) for a, b in zip(box_min, box_max))
+        return FluxSurface("box", box_min=lo, box_max=hi, label=label)
+
+    def to_c(self) -> FluxSurfaceC:
+        return FluxSurfaceC(0 if self.kind == "plane" else 1, self.axis, self.index, (C.c_int * 2)(*self.rect_min),
+                            (C.c_int * 2)(*self.rect_max), (C.c_int * 3)(*self.box_min), (C.c_int * 3)(*self.box_max))
+
+
+@dataclass
+class SubcellModel:
+    """SubcellModel (src/SimulationGrid.hpp:96-101): edge overrides and thin-wire terms."""
+    edges: list = field(default_factory=list)      # (texel(3), axis, sigma)
+    wire_terms: list = field(default_factory=list)  # (texel(3), component, term, factor)
+
+    def edge_array(self) -> np.ndarray:
+        a = np.zeros(len(self.edges), EDGE_DTYPE)
+        for i, (t, ax, s) in enumerate(self.edges):
+            a[i] = (t, ax, s)
+        return a
+
+    def wire_array(self) -> np.ndarray:
+        a = np.zeros(len(self.wire_terms), WIRE_DTYPE)
+        for i, (t, comp, term, f) in enumerate(self.wire_terms):
+            a[i] = (t, comp, term, f)
+        return a
+
+
+# ---------------------------------------------------------------------------------------------
+# SimulationGrid and monitors
+# ---------------------------------------------------------------------------------------------
+class SimulationGrid:
+    """EM_Lab's SimulationGrid on one HIP device (src/SimulationGrid.hpp:155-216)."""
+
+    kMaxSources = 256
+
+    def __init__(self, device: int = 0):
+        lib = load_library()
+        h = C.c_void_p()
+        _check(lib.emfd_create(device, C.byref(h)))
+        self._h = h
+        self._lib = lib
+        self._config: Optional[GridConfig] = None
+        self._sources: list = []
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.emfd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- SimulationGrid API --
+    def configure(self, config: GridConfig) -> GridConfig:
+        applied = GridConfigC()
+        _check(self._lib.emfd_grid_configure(self._h, C.byref(config.to_c()), C.byref(applied)))
+        self._config = GridConfig.from_c(applied)
+        return self._config
+
+    def upload_materials(self, sigma: np.ndarray, eps_r: np.ndarray, subcell: Optional[SubcellModel] = None) -> None:
+        s = np.ascontiguousarray(sigma, dtype=np.float32).ravel()
+        e = np.ascontiguousarray(eps_r, dtype=np.float32).ravel()
+        subcell = subcell or SubcellModel()
+        ed = subcell.edge_array()
+        wi = subcell.wire_array()
+        _check(self._lib.emfd_grid_upload_materials(self._h, _ptr(s), _ptr(e), s.size, _ptr(ed) if ed.size else None,
+                                                    ed.size, _ptr(wi) if wi.size else None, wi.size))
+
+    def set_sources(self, sources: Sequence[SourceDesc]) -> int:
+        arr = (SourceDescC * max(len(sources), 1))(*[s.to_c() for s in sources])
+        n = C.c_int(0)
+        _check(self._lib.emfd_grid_set_sources(self._h, C.cast(arr, C.c_void_p), len(sources), C.byref(n)))
+        self._sources = list(sources)
+        return n.value
+
+    def reset_fields(self) -> None:
+        _check(self._lib.emfd_grid_reset_fields(self._h))
+
+    def step(self, count: int) -> None:
+        _check(self._lib.emfd_grid_step(self._h, int(count)))
+
+    def synchronize(self) -> None:
+        _check(self._lib.emfd_grid_synchronize(self._h))
+
+    def set_kernel(self, variant: int) -> None:
+        _check(self._lib.emfd_grid_set_kernel(self._h, int(variant)))
+
+    def material_classes(self) -> int:
+        c = C.c_int(0)
+        _check(self._lib.emfd_grid_material_classes(self._h, C.byref(c)))
+        return c.value
+
+    def kernel_timing(self, enable: int = -1):
+        ms = C.c_double(0)
+        n = C.c_int64(0)
+        _check(self._lib.emfd_grid_kernel_timing(self._h, enable, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    @property
+    def config(self) -> GridConfig:
+        c = GridConfigC()
+        _check(self._lib.emfd_grid_get_config(self._h, C.byref(c)))
+        return GridConfig.from_c(c)
+
+    @property
+    def size(self) -> tuple:
+        return tuple(self.config.size)
+
+    def _state(self):
+        t = C.c_double()
+        n = C.c_uint64()
+        dt = C.c_float()
+        _check(self._lib.emfd_grid_state(self._h, C.byref(t), C.byref(n), C.byref(dt)))
+        return t.value, n.value, dt.value
+
+    @property
+    def dt(self) -> float:
+        return self._state()[2]
+
+    @property
+    def time(self) -> float:
+        return self._state()[0]
+
+    @property
+    def step_count(self) -> int:
+        return self._state()[1]
+
+    def gpu_memory_bytes(self) -> int:
+        return int(self._lib.emfd_grid_memory_bytes(self._h))
+
+    def fields(self, layout: int = LAYOUT_TEXEL):
+        """(E, B) as numpy arrays: (n, 4) texels (alpha = sigma / eps_r) or (3, n) planes."""
+        nx, ny, nz = self.size
+        n = nx * ny * nz
+        shape = (n, 4) if layout == LAYOUT_TEXEL else (3, n)
+        E = np.empty(shape, np.float32)
+        B = np.empty(shape, np.float32)
+        _check(self._lib.emfd_grid_download_fields(self._h, _ptr(E), _ptr(B), layout))
+        return E, B
+
+    def set_fields(self, E: Optional[np.ndarray], B: Optional[np.ndarray], layout: int = LAYOUT_TEXEL) -> None:
+        Ec = None if E is None else np.ascontiguousarray(E, np.float32)
+        Bc = None if B is None else np.ascontiguousarray(B, np.float32)
+        _check(self._lib.emfd_grid_upload_fields(self._h, _ptr(Ec), _ptr(Bc), layout))
+
+
+@dataclass
+class PortReading:
+    valid: bool
+    V: complex
+    I: complex
+    z_ohm: complex
+    gamma: complex
+    vswr: float
+    return_loss_db: float
+    accepted_power: float
+    available_power: float
+
+
+class PortMonitor:
+    """PortMonitor on a grid's context (src/Ports.hpp:84-140)."""
+
+    def __init__(self, grid: SimulationGrid):
+        self.grid = grid
+        self._lib = grid._lib
+        self.ports: list = []
+        self._sweep: Optional[SweepSetup] = None
+
+    def configure(self, ports: Sequence[PortSetup], averaging_periods: float, sweep: Optional[SweepSetup] = None,
+                  cell_size: float = 1.0e-3, record_steps: int = 0) -> None:
+        arr = (PortSetupC * max(len(ports), 1))(*[p.to_c() for p in ports])
+        sw = SweepSetupC(sweep.wavelength_min, sweep.wavelength_max, sweep.points) if sweep else None
+        _check(self._lib.emfd_ports_configure(self.grid.handle, C.cast(arr, C.c_void_p), len(ports),
+                                              averaging_periods, C.byref(sw) if sw else None, cell_size,
+                                              int(record_steps)))
+        self.ports = list(ports)[:256]
+        self._sweep = sweep
+
+    def disable(self) -> None:
+        _check(self._lib.emfd_ports_disable(self.grid.handle))
+        self.ports = []
+
+    def phasors(self) -> np.ndarray:
+        out = np.zeros((max(len(self.ports), 1), 8), np.float32)
+        _check(self._lib.emfd_ports_read_phasors(self.grid.handle, _ptr(out), out.shape[0]))
+        return out[: len(self.ports)]
+
+    def evaluate(self) -> list:
+        n = len(self.ports)
+        arr = (PortReadingC * max(n, 1))()
+        _check(self._lib.emfd_ports_evaluate(self.grid.handle, C.cast(arr, C.c_void_p), n))
+        res = []
+        for r in arr[:n]:
+            res.append(PortReading(bool(r.valid), complex(r.v_re, r.v_im), complex(r.i_re, r.i_im),
+                                   complex(r.z_re, r.z_im), complex(r.gamma_re, r.gamma_im), r.vswr,
+                                   r.return_loss_db, r.accepted_power, r.available_power))
+        return res
+
+    def recorded_steps(self) -> int:
+        return int(self._lib.emfd_ports_recorded_steps(self.grid.handle))
+
+    def read_series(self) -> np.ndarray:
+        n = self.recorded_steps()
+        out = np.zeros((n, len(self.ports), 2), np.float32)
+        _check(self._lib.emfd_ports_read_series(self.grid.handle, _ptr(out), out.size))
+        return out
+
+    def evaluate_sweep(self, dt: float) -> dict:
+        n = len(self.ports)
+        pts = max(self._sweep.points if self._sweep else 2, 2)
+        parr = (SweepPointC * (n * pts))()
+        sarr = (PortSweepC * max(n, 1))()
+        res = C.c_double()
+        samples = C.c_int64()
+        full = C.c_int()
+        _check(self._lib.emfd_ports_evaluate_sweep(self.grid.handle, dt, C.cast(parr, C.c_void_p),
+                                                   C.cast(sarr, C.c_void_p), C.byref(res), C.byref(samples),
+                                                   C.byref(full)))
+        ports = []
+        for p in range(n):
+            points = np.array([(q.wavelength, q.z_re, q.z_im, q.s11_db, q.vswr) for q in parr[p * pts:(p + 1) * pts]])
+            s = sarr[p]
+            ports.append(dict(points=points, best=s.best, best_wavelength=s.best_wavelength, has_band=bool(s.has_band),
+                              band_low_wavelength=s.band_low_wavelength, band_high_wavelength=s.band_high_wavelength))
+        return dict(residual=res.value, samples=samples.value, full=bool(full.value), ports=ports)
+
+
+class FarFieldMonitor:
+    """FarFieldMonitor on a grid's context (src/FarField.hpp:75-133)."""
+
+    def __init__(self, grid: SimulationGrid):
+        self.grid = grid
+        self._lib = grid._lib
+        self.setup: Optional[FarFieldSetup] = None
+        self.n_points = 0
+
+    def configure(self, setup: FarFieldSetup) -> FarFieldSetup:
+        applied = FarFieldSetupC()
+        npts = C.c_int()
+        _check(self._lib.emfd_ntff_configure(self.grid.handle, C.byref(setup.to_c()), C.byref(applied),
+                                             C.byref(npts)))
+        self.setup = FarFieldSetup(tuple(applied.box_min), tuple(applied.box_max), applied.wavelength,
+                                   applied.averaging_periods, tuple(applied.map_axis), setup.name)
+        self.n_points = npts.value
+        return self.setup
+
+    def disable(self) -> None:
+        _check(self._lib.emfd_ntff_disable(self.grid.handle))
+
+    def reset_accumulation(self) -> None:
+        _check(self._lib.emfd_ntff_reset(self.grid.handle))
+
+    def phasors(self) -> np.ndarray:
+        out = np.zeros((self.n_points, 32), np.float32)
+        _check(self._lib.emfd_ntff_read_phasors(self.grid.handle, _ptr(out), self.n_points))
+        return out
+
+    def directions(self):
+        n = C.c_int()
+        _check(self._lib.emfd_ntff_directions(self.grid.handle, None, 0, C.byref(n), None, None))
+        dirs = np.zeros((n.value, 4), np.float32)
+        mi = np.zeros(2 * 81 * 81, np.int32)
+        frame = np.zeros(9, np.float32)
+        _check(self._lib.emfd_ntff_directions(self.grid.handle, _ptr(dirs), n.value, C.byref(n), _ptr(mi),
+                                              _ptr(frame)))
+        return dirs, mi, frame
+
+    def radiation(self, dirs: np.ndarray):
+        """|N_perp - r x L|^2 per direction and the radiated power P."""
+        d = np.ascontiguousarray(dirs, np.float32)
+        if d.ndim == 2 and d.shape[1] == 3:
+            d = np.ascontiguousarray(np.concatenate([d, np.zeros((d.shape[0], 1), np.float32)], 1))
+        out = np.zeros(d.shape[0], np.float32)
+        P = C.c_double()
+        _check(self._lib.emfd_ntff_eval(self.grid.handle, _ptr(d), d.shape[0], _ptr(out), C.byref(P)))
+        return out, P.value
+
+    def evaluate(self) -> dict:
+        dirs, _, _ = self.directions()
+        valid = C.c_int()
+        P = C.c_double()
+        mx = C.c_float()
+        mdir = np.zeros(3, np.float32)
+        dbi = np.zeros(dirs.shape[0], np.float32)
+        cuts = np.zeros(18, np.float32)
+        _check(self._lib.emfd_ntff_evaluate(self.grid.handle, C.byref(valid), C.byref(P), C.byref(mx), _ptr(mdir),
+                                            _ptr(dbi), _ptr(cuts)))
+        names = ("xy", "xz", "yz")
+        return dict(valid=bool(valid.value), radiated_power=P.value, max_directivity_dbi=mx.value,
+                    max_direction=mdir, directivity_dbi=dbi,
+                    cuts={names[c]: dict(peak_dbi=float(cuts[6 * c]), peak_angle_deg=float(cuts[6 * c + 1]),
+                                         beamwidth_deg=float(cuts[6 * c + 2]), front_to_back_db=float(cuts[6 * c + 3]))
+                          for c in range(3)})
+
+
+class FieldAnalyzer:
+    """FieldAnalyzer probes and flux meters on a grid's context (src/FieldAnalyzer.hpp:73-130)."""
+
+    def __init__(self, grid: SimulationGrid):
+        self.grid = grid
+        self._lib = grid._lib
+
+    def measure_flux(self, surfaces) -> np.ndarray:
+        if isinstance(surfaces, FluxSurface):
+            surfaces = [surfaces]
+        arr = (FluxSurfaceC * max(len(surfaces), 1))(*[s.to_c() for s in surfaces])
+        out = np.zeros(max(len(surfaces), 1), np.float64)
+        _check(self._lib.emfd_flux_measure(self.grid.handle, C.cast(arr, C.c_void_p), len(surfaces), _ptr(out)))
+        return out[: len(surfaces)]
+
+    def probe(self, positions) -> list:
+        p = np.ascontiguousarray(np.atleast_2d(np.asarray(positions, np.float32)))
+        arr = (ProbeSampleC * max(p.shape[0], 1))()
+        _check(self._lib.emfd_probe_measure(self.grid.handle, _ptr(p), p.shape[0], C.cast(arr, C.c_void_p)))
+        return [dict(position=np.array(s.position), E=np.array(s.E), B=np.array(s.B), S=np.array(s.S),
+                     sigma=s.sigma, epsilon_r=s.epsilon_r) for s in arr[: p.shape[0]]]
+
+
+def version() -> str:
+    return load_library().emfd_version().decode()
+
+
+def dipole_source(center, axis, wavelength, current, length=0.0) -> SourceDesc:
+    """SourceDesc::dipole (src/SimulationGrid.cpp:70-82), float arithmetic as the reference."""
+    f = np.float32
+    soft = f(0.75)
+    amp = f(current) / (f(math.pi) * soft * soft)
+    ax = np.asarray(axis, np.float32)
+    len2 = f(np.dot(ax, ax))
+    n = ax / np.sqrt(len2) if len2 > 0 else np.array([0, 0, 1], np.float32)
+    half = np.abs(n) * (f(0.5) * f(max(length, 0.0)))
+    return SourceDesc(center=tuple(float(v) for v in center), half_extent=tuple(float(v) for v in half),
+                      softness=(0.75, 0.75, 0.75), polarization=tuple(float(v) for v in ax), amplitude=float(amp),
+                      wavelength=float(wavelength))

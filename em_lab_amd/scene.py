@@ -1,0 +1,433 @@
+"""Scene description -> solver inputs (materials, sub-cell model, sources, ports, monitors).
+
+A Python counterpart of the numerically relevant part of EM_Lab's Lua scene API
+(src/ScriptEngine.cpp) and EnvironmentBuilder (src/Environment.cpp): material resolution,
+voxel fill, thin-wire dipoles with resistive feeds, ideal (Hertzian) dipoles, thin wires with
+the free-end correction, the far-field box and the sweep band. The arithmetic keeps the
+reference's float/double boundaries so the sub-cell coefficients match bit for bit.
+
+Also holds the seeded generators of the benchmark scenes (SplitMix64 -> xoshiro256**).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from .emfdtd import (WAVE_CW, WAVE_PULSE, FarFieldSetup, FluxSurface, GridConfig, PortSetup, SourceDesc,
+                     SubcellModel, SweepSetup)
+
+f32 = np.float32
+ETA0 = 376.730313668            # src/ScriptEngine.cpp:27
+ETA0F = f32(376.730313668)      # src/ScriptEngine.cpp:28
+C0 = 299792458.0
+PERFECT_EDGE_SIGMA = f32(6.0e4)  # src/SimulationGrid.hpp:103
+MAX_COURANT = float(f32(0.99) / f32(1.7320508))  # src/SimulationGrid.cpp:24
+
+
+def round_half_away(x: float) -> int:
+    """glm::round / std::lround: halves away from zero."""
+    return int(math.floor(abs(x) + 0.5)) * (1 if x >= 0 else -1)
+
+
+# ---------------------------------------------------------------------------------------------
+# PRNG: SplitMix64 seeding xoshiro256** (Blackman & Vigna)
+# ---------------------------------------------------------------------------------------------
+_M64 = (1 << 64) - 1
+
+
+class Xoshiro256ss:
+    def __init__(self, seed: int):
+        s = seed & _M64
+        self.s = []
+        for _ in range(4):
+            s = (s + 0x9E3779B97F4A7C15) & _M64
+            z = s
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+            self.s.append(z ^ (z >> 31))
+
+    @staticmethod
+    def _rotl(x, k):
+        return ((x << k) | (x >> (64 - k))) & _M64
+
+    def next(self) -> int:
+        s = self.s
+        result = (self._rotl((s[1] * 5) & _M64, 7) * 9) & _M64
+        t = (s[1] << 17) & _M64
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = self._rotl(s[3], 45)
+        return result
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        return lo + (hi - lo) * ((self.next() >> 11) * (1.0 / (1 << 53)))
+
+    def log_uniform(self, lo: float, hi: float) -> float:
+        return math.exp(self.uniform(math.log(lo), math.log(hi)))
+
+
+# ---------------------------------------------------------------------------------------------
+# Scene
+# ---------------------------------------------------------------------------------------------
+@dataclass
+class ThinWireSpec:
+    node: tuple
+    axis: int
+    i0: int
+    i1: int
+    radius: float
+    gap_edge: Optional[int] = None
+
+
+@dataclass
+class Scene:
+    """Settings as scene{} (src/ScriptEngine.cpp:605-650) and the EnvironmentBuilder voxels."""
+    size: tuple = (128, 128, 128)
+    pml: int = 16
+    courant: float = 0.5
+    cell_size: float = 1.0e-3
+    wavelength: float = 16.0
+    pml_sigma_max: float = 1.6
+    pml_grading: float = 3.0
+    sigma: np.ndarray = None
+    eps: np.ndarray = None
+    subcell: SubcellModel = field(default_factory=SubcellModel)
+    sources: list = field(default_factory=list)
+    ports: list = field(default_factory=list)
+    thin_wires: list = field(default_factory=list)
+    sweep: Optional[SweepSetup] = None
+    far_field: Optional[FarFieldSetup] = None
+    fluxes: list = field(default_factory=list)
+    warnings: list = field(default_factory=list)
+
+    def __post_init__(self):
+        nx, ny, nz = self.size
+        if self.sigma is None:
+            self.sigma = np.zeros((nz, ny, nx), np.float32)
+        if self.eps is None:
+            self.eps = np.ones((nz, ny, nx), np.float32)
+
+    # -- grid --
+    def grid_config(self) -> GridConfig:
+        return GridConfig(tuple(self.size), float(self.courant), int(self.pml), float(self.pml_sigma_max),
+                          float(self.pml_grading))
+
+    # -- materials (src/ScriptEngine.cpp:458-465) --
+    def resolve(self, eps_r: float = 1.0, sigma: Optional[float] = None, conductivity: Optional[float] = None,
+                loss_tangent: Optional[float] = None):
+        s = 0.0
+        if sigma is not None:
+            s = sigma
+        elif conductivity is not None:
+            s = conductivity * float(f32(self.cell_size)) * ETA0
+        elif loss_tangent is not None:
+            s = loss_tangent * (2.0 * math.pi / float(f32(self.wavelength))) * float(f32(eps_r))
+        return f32(s), max(f32(eps_r), f32(1.0))
+
+    def _set(self, x, y, z, sig, eps):
+        # MaterialVolume::set (src/SimulationGrid.cpp:42-47)
+        self.sigma[z, y, x] = max(sig, f32(0.0))
+        self.eps[z, y, x] = max(eps, f32(1.0))
+
+    def sphere(self, center, radius, eps_r=1.0, sigma=None, conductivity=None, loss_tangent=None) -> int:
+        """EnvironmentBuilder::sphere -> fill(EllipsoidShape) (src/Environment.cpp:82-105)."""
+        sig, eps = self.resolve(eps_r, sigma, conductivity, loss_tangent)
+        c = np.asarray(center, np.float32)
+        r = max(f32(radius), f32(1e-3))
+        nx, ny, nz = self.size
+        lo = [min(max(int(math.floor(float(c[a] - r - f32(0.5)))), 0), self.size[a]) for a in range(3)]
+        hi = [min(max(int(math.ceil(float(c[a] + r + f32(0.5)))), 0), self.size[a]) for a in range(3)]
+        zz, yy, xx = np.meshgrid(np.arange(lo[2], hi[2]), np.arange(lo[1], hi[1]), np.arange(lo[0], hi[0]),
+                                 indexing="ij")
+        px = (xx.astype(np.float32) + f32(0.5) - c[0]) / r
+        py = (yy.astype(np.float32) + f32(0.5) - c[1]) / r
+        pz = (zz.astype(np.float32) + f32(0.5) - c[2]) / r
+        inside = (px * px + py * py) + pz * pz <= f32(1.0)
+        self.sigma[zz[inside], yy[inside], xx[inside]] = max(sig, f32(0.0))
+        self.eps[zz[inside], yy[inside], xx[inside]] = max(eps, f32(1.0))
+        return int(inside.sum())
+
+    def box(self, mn, mx, eps_r=1.0, sigma=None, conductivity=None) -> int:
+        """EnvironmentBuilder::box (src/Environment.cpp:99-101): cells with min <= centre < max."""
+        sig, eps = self.resolve(eps_r, sigma, conductivity)
+        mn = np.minimum(np.asarray(mn, np.float32), np.asarray(mx, np.float32))
+        mx2 = np.maximum(np.asarray(mn, np.float32), np.asarray(mx, np.float32))
+        idx = []
+        for a in range(3):
+            lo = min(max(int(math.floor(float(mn[a] - f32(0.5)))), 0), self.size[a])
+            hi = min(max(int(math.ceil(float(mx2[a] + f32(0.5)))), 0), self.size[a])
+            cc = np.arange(lo, hi).astype(np.float32) + f32(0.5)
+            idx.append(np.arange(lo, hi)[(cc >= mn[a]) & (cc < mx2[a])])
+        if any(len(i) == 0 for i in idx):
+            return 0
+        self.sigma[np.ix_(idx[2], idx[1], idx[0])] = max(sig, f32(0.0))
+        self.eps[np.ix_(idx[2], idx[1], idx[0])] = max(eps, f32(1.0))
+        return len(idx[0]) * len(idx[1]) * len(idx[2])
+
+    # -- sources --
+    def waveform(self, wavelength=None, waveform="cw", ramp=3.0, pulse_width=1.0, phase_deg=0.0, interval=0.0,
+                 delay=0.0):
+        """readWaveform (src/ScriptEngine.cpp:1082-1102); the scene wavelength is the default."""
+        if wavelength is None:
+            wavelength = self.wavelength
+        s = SourceDesc(wavelength=float(f32(wavelength)), phase=float(f32(math.radians(phase_deg))))
+        if waveform in ("cw", "continuous", "sine"):
+            s.waveform = WAVE_CW
+            s.ramp_periods = float(f32(ramp))
+        else:
+            s.waveform = WAVE_PULSE
+            s.ramp_periods = float(f32(pulse_width))
+        s.repeat_periods = float(f32(interval))
+        s.delay_periods = float(f32(delay))
+        return s
+
+    def hertzian(self, center, orientation=(0, 0, 1), amplitude=1.0, length=0.0, **wave):
+        """dipole{model="current"}: SourceDesc::dipole + copyTiming (src/ScriptEngine.cpp:1314-1321)."""
+        w = self.waveform(**wave)
+        soft = f32(0.75)
+        amp = f32(amplitude) / (f32(math.pi) * soft * soft)
+        ax = np.asarray(orientation, np.float32)
+        len2 = f32(np.dot(ax, ax))
+        n = ax / np.sqrt(len2) if len2 > 0 else np.array([0, 0, 1], np.float32)
+        half = np.abs(n) * (f32(0.5) * f32(max(length, 0.0)))
+        s = SourceDesc(center=tuple(float(v) for v in np.asarray(center, np.float32)),
+                       half_extent=tuple(float(v) for v in half), softness=(0.75, 0.75, 0.75),
+                       polarization=tuple(float(v) for v in ax), amplitude=float(amp))
+        for k in ("wavelength", "phase", "waveform", "ramp_periods", "repeat_periods", "delay_periods", "enabled"):
+            setattr(s, k, getattr(w, k))
+        self.sources.append(s)
+        return s
+
+    def add_thin_wire(self, node, axis, i0, i1, radius, gap_edge=None):
+        """addThinWire (src/ScriptEngine.cpp:1183-1200)."""
+        if i1 <= i0:
+            return
+        self.thin_wires.append(ThinWireSpec(tuple(int(v) for v in node), axis, i0, i1, float(f32(radius)), gap_edge))
+
+    def add_feed(self, wave: SourceDesc, texel, axis, sign, edges, impedance=50.0, drive=1.0, voltage=True,
+                 name="dipole"):
+        """addFeed (src/ScriptEngine.cpp:1205-1252)."""
+        u, v = (axis + 1) % 3, (axis + 2) % 3
+        s = SourceDesc(**{k: getattr(wave, k) for k in wave.__dataclass_fields__})
+        s.softness = (0.25, 0.25, 0.25)
+        pol = [0.0, 0.0, 0.0]
+        pol[axis] = float(sign)
+        s.polarization = tuple(pol)
+        if edges == 4:
+            s.center = tuple(float(f32(t) + f32(0.5)) for t in texel)
+            he = [0.5, 0.5, 0.5]
+            he[axis] = 0.0
+            s.half_extent = tuple(he)
+        else:
+            c = [float(t) for t in texel]
+            c[axis] = float(f32(c[axis]) + f32(0.5))
+            s.center = tuple(c)
+            s.half_extent = (0.0, 0.0, 0.0)
+        if voltage:
+            sigma = ETA0F / (f32(edges) * f32(impedance))
+            offs = [(0, 0, 0)]
+            eu = [0, 0, 0]
+            ev = [0, 0, 0]
+            eu[u] = 1
+            ev[v] = 1
+            offs = [(0, 0, 0), tuple(eu), tuple(ev), tuple(a + b for a, b in zip(eu, ev))]
+            for i in range(edges):
+                self.subcell.edges.append((tuple(t + o for t, o in zip(texel, offs[i])), axis, float(sigma)))
+            s.amplitude = float(-sigma * f32(drive))
+        else:
+            s.amplitude = float(f32(drive) / f32(edges))
+        s.port = len(self.ports)
+        if s.enabled:
+            self.sources.append(s)
+        self.ports.append(PortSetup(name=name, texel=tuple(int(t) for t in texel), axis=axis, sign=float(sign),
+                                    edges=edges, wavelength=wave.wavelength, voltage_feed=voltage,
+                                    impedance_ohm=float(f32(impedance)), drive=float(f32(drive))))
+
+    def dipole_thin(self, center, axis=2, length=11.0, radius=None, impedance=50.0, amplitude=1.0, sign=1.0,
+                    name="dipole", **wave):
+        """dipole{model="thin"} (src/ScriptEngine.cpp:1255-1313): arms, thin wire, resistive feed."""
+        w = self.waveform(**wave)
+        if length < 5.0:
+            raise ValueError("a thin-wire dipole needs length >= 5 cells")
+        arm = max(1, round_half_away((float(f32(length)) - 1.0) * 0.5))
+        if abs(2 * arm + 1 - length) >= 1.0:
+            self.warnings.append(f"dipole: length {length:.1f} realised as {2 * arm + 1} cells")
+        if radius is None:
+            radius = float(f32(w.wavelength) / f32(400.0))
+        c = np.asarray(center, np.float32)
+        node = [round_half_away(float(v)) for v in c]
+        node[axis] = int(math.floor(float(c[axis])))
+        g = node[axis]
+        self.add_thin_wire(node, axis, g - arm, g + 1 + arm, radius, g)
+        self.add_feed(w, tuple(node), axis, sign, 1, impedance, amplitude, True, name)
+
+    def wire(self, p0, p1, radius=None):
+        """wire{} (src/ScriptEngine.cpp:1380-1394)."""
+        a = [round_half_away(float(v)) for v in np.asarray(p0, np.float32)]
+        b = [round_half_away(float(v)) for v in np.asarray(p1, np.float32)]
+        d = [bb - aa for aa, bb in zip(a, b)]
+        if sum(1 for v in d if v != 0) != 1:
+            raise ValueError("from and to must differ along exactly one axis")
+        ax = next(i for i in range(3) if d[i] != 0)
+        if radius is None:
+            radius = float(f32(self.wavelength) / f32(400.0))
+        self.add_thin_wire(a, ax, min(a[ax], b[ax]), max(a[ax], b[ax]), radius)
+
+    # -- monitors --
+    def set_far_field(self, margin=4, wavelength=None, averaging=4.0, axis=(0, 0, 1)):
+        """far_field{} default box (src/ScriptEngine.cpp:2160-2189)."""
+        lo = tuple(self.pml + 1 + margin for _ in range(3))
+        hi = tuple(s - self.pml - 2 - margin for s in self.size)
+        self.far_field = FarFieldSetup(lo, hi, float(f32(wavelength or self.wavelength)), float(f32(averaging)),
+                                       tuple(float(v) for v in axis))
+        return self.far_field
+
+    def set_sweep(self, wavelengths, points=81):
+        """sweep{wavelengths=...} (src/ScriptEngine.cpp:1431-1454)."""
+        lo, hi = min(wavelengths), max(wavelengths)
+        self.sweep = SweepSetup(float(f32(lo)), float(f32(hi)), int(min(max(points, 2), 2001)))
+        return self.sweep
+
+    # -- emission (src/ScriptEngine.cpp:2346-2419) --
+    def emit_thin_wires(self) -> None:
+        def key(p):
+            return tuple(int(v) for v in p)
+
+        nodes = []
+        for t in self.thin_wires:
+            s = set()
+            for k in range(t.i0, t.i1 + 1):
+                p = list(t.node)
+                p[t.axis] = k
+                s.add(key(p))
+            nodes.append(s)
+        sheet_nodes = set()
+        for (tex, ax, sg) in self.subcell.edges:
+            if f32(sg) < f32(0.5) * PERFECT_EDGE_SIGMA:
+                continue
+            end = list(tex)
+            end[ax] += 1
+            sheet_nodes.add(key(tex))
+            sheet_nodes.add(key(end))
+        nx, ny, nz = self.size
+
+        def mat_sigma(p):
+            x, y, z = p
+            if 0 <= x < nx and 0 <= y < ny and 0 <= z < nz:
+                return self.sigma[z, y, x]
+            return f32(0.0)
+
+        def touches(self_i, p):
+            if key(p) in sheet_nodes:
+                return True
+            for w, s in enumerate(nodes):
+                if w != self_i and key(p) in s:
+                    return True
+            for dz in (-1, 0):
+                for dy in (-1, 0):
+                    for dx in (-1, 0):
+                        if mat_sigma((p[0] + dx, p[1] + dy, p[2] + dz)) >= f32(1.0):
+                            return True
+            return False
+
+        for w, t0 in enumerate(self.thin_wires):
+            t = ThinWireSpec(**t0.__dict__)
+            a = list(t.node)
+            b = list(t.node)
+            a[t.axis] = t.i0
+            b[t.axis] = t.i1
+            gap = t.gap_edge if t.gap_edge is not None else -(1 << 31)
+            freeA = (not touches(w, a)) and gap != t.i0
+            freeB = (not touches(w, b)) and gap != t.i1 - 1
+            keep = 2 if t.i0 <= gap < t.i1 else 1
+            if (t.i1 - t.i0) - int(freeA) - int(freeB) >= keep:
+                if freeA:
+                    t.i0 += 1
+                if freeB:
+                    t.i1 -= 1
+            elif freeA or freeB:
+                self.warnings.append(f"thin wire at {tuple(a)} is too short for the end correction")
+            fac = float(f32(2.0) / np.log(f32(1.0) / f32(t.radius)))
+            u, v = (t.axis + 1) % 3, (t.axis + 2) % 3
+            eu = [0, 0, 0]
+            ev = [0, 0, 0]
+            eu[u] = 1
+            ev[v] = 1
+
+            def term(comp):
+                return 0 if (comp + 2) % 3 == t.axis else 1
+
+            for k in range(t.i0, t.i1):
+                q = list(t.node)
+                q[t.axis] = k
+                if k != gap:
+                    self.subcell.edges.append((tuple(q), t.axis, float(PERFECT_EDGE_SIGMA)))
+                qmv = tuple(qq - e for qq, e in zip(q, ev))
+                qmu = tuple(qq - e for qq, e in zip(q, eu))
+                self.subcell.wire_terms.append((tuple(q), u, term(u), fac))
+                self.subcell.wire_terms.append((qmv, u, term(u), fac))
+                self.subcell.wire_terms.append((tuple(q), v, term(v), fac))
+                self.subcell.wire_terms.append((qmu, v, term(v), fac))
+        self.thin_wires = []
+
+    def flat_materials(self):
+        return np.ascontiguousarray(self.sigma.ravel()), np.ascontiguousarray(self.eps.ravel())
+
+
+# ---------------------------------------------------------------------------------------------
+# Benchmark / parity scenes
+# ---------------------------------------------------------------------------------------------
+DEFAULT_SEED = 20251203
+
+
+def hertzian_vacuum(n=64, pml=8, steps_wavelength=16.0, courant=MAX_COURANT) -> Scene:
+    """Plain FDTD: n^3 vacuum, PML, z Hertzian dipole at the centre, Gaussian pulse (lambda 16)."""
+    sc = Scene(size=(n, n, n), pml=pml, courant=courant, wavelength=steps_wavelength)
+    sc.hertzian((n / 2, n / 2, n / 2), (0, 0, 1), 1.0, wavelength=steps_wavelength, waveform="pulse", pulse_width=1.0)
+    sc.set_far_field(wavelength=steps_wavelength)
+    return sc
+
+
+def spheres_dipole(n=256, pml=10, n_spheres=32, wavelength=20.0, seed=DEFAULT_SEED, length=11,
+                   radius_range=(4.0, 20.0), sweep_points=32) -> Scene:
+    """n^3 grid, seeded random dielectric spheres and a z thin-wire half-wave dipole fed by a
+    50 ohm resistive voltage source (pulse); sweep, flux box and far-field monitors."""
+    sc = Scene(size=(n, n, n), pml=pml, courant=MAX_COURANT, cell_size=1.0e-3, wavelength=wavelength)
+    ff = sc.set_far_field(wavelength=wavelength)
+    rng = Xoshiro256ss(seed)
+    c0 = n / 2
+    dip_lo, dip_hi = c0 - length / 2 - 1, c0 + length / 2 + 1
+    placed = 0
+    attempts = 0
+    scale = n / 256.0
+    while placed < n_spheres and attempts < 100000:
+        attempts += 1
+        r = rng.uniform(radius_range[0] * scale, radius_range[1] * scale)
+        eps = rng.uniform(1.0, 12.0)
+        cond = rng.log_uniform(1e-4, 1.0)
+        lo = ff.box_min[0] + r + 2
+        hi = ff.box_max[0] - r - 2
+        if hi <= lo:
+            continue
+        c = (rng.uniform(lo, hi), rng.uniform(lo, hi), rng.uniform(lo, hi))
+        # distance from the dipole's axis segment minus the radius must be >= 3 cells
+        dz = max(dip_lo - c[2], 0.0, c[2] - dip_hi)
+        dist = math.sqrt((c[0] - c0) ** 2 + (c[1] - c0) ** 2 + dz * dz)
+        if dist - r < 3.0:
+            continue
+        sc.sphere(c, r, eps_r=eps, conductivity=cond)
+        placed += 1
+    radius = 0.005 * wavelength
+    sc.dipole_thin((c0, c0, c0), axis=2, length=length, radius=radius, impedance=50.0, amplitude=1.0,
+                   wavelength=wavelength, waveform="pulse", pulse_width=1.0)
+    sc.set_sweep((wavelength / 1.5, wavelength / 0.5), points=sweep_points)
+    m = ff.box_min[0] - 2
+    sc.fluxes.append(FluxSurface.box((m, m, m), (n - 1 - m, n - 1 - m, n - 1 - m), "flux box"))
+    sc.emit_thin_wires()
+    return sc

@@ -1,0 +1,285 @@
+/*
+ * emfdtd.h — C ABI of the MI355X FDTD engine (libemfdtd.so).
+ *
+ * This is the drop-in boundary for EM_Lab's solver path. EM_Lab's host classes bind GL texture
+ * names directly; here every class of that path becomes an opaque context plus plain-pointer
+ * entry points, so a C++ (or any FFI) caller can keep EM_Lab's call sequence:
+ *
+ *   SimulationGrid::configure / uploadMaterials / setSources / resetFields / step
+ *       -> emfd_grid_configure / emfd_grid_upload_materials / emfd_grid_set_sources /
+ *          emfd_grid_reset_fields / emfd_grid_step
+ *   PortMonitor, FarFieldMonitor, FieldAnalyzer -> emfd_ports_*, emfd_ntff_*, emfd_flux_*, emfd_probe_*
+ *
+ * Conventions
+ *   - Every function returns EMFD_OK (0) or a positive error code; emfd_last_error() gives a
+ *     thread-local message for the last failure on the calling thread. No exception crosses
+ *     the ABI.
+ *   - A context is bound to one HIP device and one HIP stream (default priority) that it
+ *     creates. It is not thread-safe, like the GL context it replaces.
+ *   - Host arrays use EM_Lab's lattice index (z*ny + y)*nx + x (x fastest,
+ *     src/SimulationGrid.hpp:55-57). Field downloads offer EM_Lab's texel layout (RGBA per
+ *     cell, alpha = cell sigma for E and cell eps_r for B) or structure-of-arrays.
+ *   - Silent clamping follows EM_Lab where it clamps (configure, sources, ports); sizes it
+ *     rejects with std::runtime_error are rejected here with EMFD_ERR_INVALID.
+ */
+#ifndef EMFDTD_H
+#define EMFDTD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMFD_OK 0
+#define EMFD_ERR_INVALID 1 /* bad argument or size mismatch */
+#define EMFD_ERR_HIP 2     /* HIP runtime failure (message has the HIP error string) */
+#define EMFD_ERR_STATE 3   /* call out of order (e.g. step before configure) */
+#define EMFD_ERR_NOMEM 4   /* device allocation failed */
+
+#define EMFD_LAYOUT_TEXEL 0 /* float[n][4]: (x, y, z, alpha) per cell, EM_Lab's RGBA32F texel */
+#define EMFD_LAYOUT_SOA 1   /* float[3][n]: component planes */
+
+#define EMFD_KERNEL_AUTO 0    /* fused B+E z-marching kernel (default) */
+#define EMFD_KERNEL_FUSED 1   /* same, explicit */
+#define EMFD_KERNEL_TWO_PASS 2 /* separate update_B / update_E passes, in place */
+
+typedef struct emfd_ctx emfd_ctx;
+
+/* GridConfig (src/SimulationGrid.hpp:25-31) */
+typedef struct {
+    int size[3];
+    float courant;
+    int pml_thickness;
+    float pml_sigma_max;
+    float pml_grading;
+} emfd_grid_config;
+
+/* EdgeOverride (src/SimulationGrid.hpp:76-81) */
+typedef struct {
+    int texel[3];
+    int axis;
+    float sigma;
+} emfd_edge_override;
+
+/* WireCoefficient (src/SimulationGrid.hpp:86-94) */
+typedef struct {
+    int texel[3];
+    int component;
+    int term;
+    float factor;
+} emfd_wire_coefficient;
+
+/* SourceDesc (src/SimulationGrid.hpp:113-150); waveform 0 = CW, 1 = Gaussian pulse */
+typedef struct {
+    float center[3];
+    float half_extent[3];
+    float softness[3];
+    float polarization[3];
+    float amplitude;
+    float wavelength;
+    float phase;
+    int waveform;
+    float ramp_periods;
+    float repeat_periods;
+    float delay_periods;
+    int enabled;
+} emfd_source_desc;
+
+/* PortSetup (src/Ports.hpp:30-40) */
+typedef struct {
+    int texel[3];
+    int axis;
+    float sign;
+    int edges; /* 1 = thin-wire gap edge, 4 = rod gap corner edges */
+    float wavelength;
+    int voltage_feed;
+    float impedance_ohm;
+    float drive;
+} emfd_port_setup;
+
+/* SweepSetup (src/Ports.hpp:42-47) */
+typedef struct {
+    float wavelength_min;
+    float wavelength_max;
+    int points;
+} emfd_sweep_setup;
+
+/* PortReading (src/Ports.hpp:49-58) */
+typedef struct {
+    int valid;
+    double v_re, v_im, i_re, i_im;
+    double z_re, z_im;
+    double gamma_re, gamma_im;
+    double vswr;
+    double return_loss_db;
+    double accepted_power;
+    double available_power;
+} emfd_port_reading;
+
+/* SweepPoint (src/Ports.hpp:60-65) */
+typedef struct {
+    double wavelength;
+    double z_re, z_im;
+    double s11_db;
+    double vswr;
+} emfd_sweep_point;
+
+/* PortSweep summary (src/Ports.hpp:67-74) */
+typedef struct {
+    int64_t best;
+    double best_wavelength;
+    int has_band;
+    double band_low_wavelength;
+    double band_high_wavelength;
+} emfd_port_sweep;
+
+/* FarFieldSetup (src/FarField.hpp:39-47) */
+typedef struct {
+    int box_min[3];
+    int box_max[3];
+    float wavelength;
+    float averaging_periods;
+    float map_axis[3];
+} emfd_far_field_setup;
+
+/* FluxSurface (src/FieldAnalyzer.hpp:32-51); kind 0 = plane, 1 = closed box */
+typedef struct {
+    int kind;
+    int axis;
+    int index;
+    int rect_min[2];
+    int rect_max[2];
+    int box_min[3];
+    int box_max[3];
+} emfd_flux_surface;
+
+/* ProbeSample (src/FieldAnalyzer.hpp:19-29) */
+typedef struct {
+    float position[3];
+    float E[3];
+    float B[3];
+    float S[3];
+    float sigma;
+    float epsilon_r;
+} emfd_probe_sample;
+
+/* ---------------------------------------------------------------------------------------
+ * Context
+ * --------------------------------------------------------------------------------------- */
+/* Replaces SimulationGrid::SimulationGrid(shaderDir) (src/SimulationGrid.cpp:135-147). */
+int emfd_create(int device, emfd_ctx **out);
+void emfd_destroy(emfd_ctx *ctx);
+const char *emfd_last_error(void);
+/* Library build string (kernel variants, target). */
+const char *emfd_version(void);
+
+/* ---------------------------------------------------------------------------------------
+ * SimulationGrid (src/SimulationGrid.hpp:155-216)
+ * --------------------------------------------------------------------------------------- */
+/* SimulationGrid::configure (src/SimulationGrid.cpp:149-199): same clamps; vacuum materials,
+ * zero fields, step count 0. `applied` (may be NULL) receives the clamped configuration. */
+int emfd_grid_configure(emfd_ctx *ctx, const emfd_grid_config *requested, emfd_grid_config *applied);
+/* SimulationGrid::uploadMaterials (src/SimulationGrid.cpp:209-295): sigma/eps_r per cell
+ * (n_cells must equal the grid's cell count), edge averaging, sub-cell overrides, binary16
+ * quantisation of edge sigma (clamped to 6e4), edge eps_r and thin-wire factors. Clears the
+ * fields and rewinds time, as the reference does. */
+int emfd_grid_upload_materials(emfd_ctx *ctx, const float *sigma, const float *eps_r, int64_t n_cells,
+                               const emfd_edge_override *edges, int64_t n_edges,
+                               const emfd_wire_coefficient *wires, int64_t n_wires);
+/* SimulationGrid::setSources (src/SimulationGrid.cpp:297-357): disabled, zero-polarisation and
+ * beyond-256 sources are skipped; *n_active (may be NULL) receives the kept count. */
+int emfd_grid_set_sources(emfd_ctx *ctx, const emfd_source_desc *sources, int n, int *n_active);
+/* SimulationGrid::resetFields (src/SimulationGrid.cpp:359-366). */
+int emfd_grid_reset_fields(emfd_ctx *ctx);
+/* SimulationGrid::step (src/SimulationGrid.cpp:368-433): count <= 0 is a no-op. Each step runs
+ * B, E, sources, then every configured monitor's per-step accumulation on the device; there is
+ * no host callback per step. Asynchronous on the context stream. */
+int emfd_grid_step(emfd_ctx *ctx, int count);
+/* Wait for all queued work of the context. */
+int emfd_grid_synchronize(emfd_ctx *ctx);
+/* time() = step_count * dt, stepCount(), dt() (src/SimulationGrid.hpp:185-187). Any pointer may be NULL. */
+int emfd_grid_state(emfd_ctx *ctx, double *time, uint64_t *step_count, float *dt);
+/* The configuration in force (after clamps). */
+int emfd_grid_get_config(emfd_ctx *ctx, emfd_grid_config *out);
+/* Field readback (replaces eFieldTexture()/bFieldTexture() reads). Either pointer may be NULL.
+ * Sizes: 4*n floats (TEXEL) or 3*n floats (SOA). Synchronises the stream. */
+int emfd_grid_download_fields(emfd_ctx *ctx, float *E, float *B, int layout);
+/* Overwrite the field state (restart or test set-up). The alpha channel of a TEXEL upload is
+ * ignored (materials are unchanged). */
+int emfd_grid_upload_fields(emfd_ctx *ctx, const float *E, const float *B, int layout);
+/* Device bytes held by the grid (fields, coefficients, source lists). */
+int64_t emfd_grid_memory_bytes(emfd_ctx *ctx);
+/* Choose the Yee kernel variant (EMFD_KERNEL_*). */
+int emfd_grid_set_kernel(emfd_ctx *ctx, int variant);
+/* Number of distinct coefficient classes after the last material upload (0 = uniform vacuum). */
+int emfd_grid_material_classes(emfd_ctx *ctx, int *classes);
+/* Live timing of the Yee kernel: with enable != 0, every Yee launch of later steps is bracketed
+ * by HIP events on the context stream; *total_ms / *launches (may be NULL) report the sum and
+ * count since the last enable. */
+int emfd_grid_kernel_timing(emfd_ctx *ctx, int enable, double *total_ms, int64_t *launches);
+
+/* ---------------------------------------------------------------------------------------
+ * PortMonitor (src/Ports.hpp:84-140)
+ * --------------------------------------------------------------------------------------- */
+/* PortMonitor::configure (src/Ports.cpp:183-215): at most 256 ports; sweep may be NULL;
+ * record_steps > 0 records the (V, I) series without a sweep. */
+int emfd_ports_configure(emfd_ctx *ctx, const emfd_port_setup *ports, int n, float averaging_periods,
+                         const emfd_sweep_setup *sweep, float cell_size, int64_t record_steps);
+int emfd_ports_disable(emfd_ctx *ctx);
+/* Raw EMA phasors: 8 floats per port (stage1 Vre Vim Ire Iim, stage2 ...). */
+int emfd_ports_read_phasors(emfd_ctx *ctx, float *out, int n);
+/* PortMonitor::evaluate + computeReadings (src/Ports.cpp:285-325): one reading per port. */
+int emfd_ports_evaluate(emfd_ctx *ctx, emfd_port_reading *out, int n);
+/* Recorded series steps so far (PortMonitor::recordedSteps). */
+int64_t emfd_ports_recorded_steps(emfd_ctx *ctx);
+/* PortMonitor::readSeries (src/Ports.cpp:278-283): 2 floats (V, I) per [step][port];
+ * cap_floats bounds the write. */
+int emfd_ports_read_series(emfd_ctx *ctx, float *out, int64_t cap_floats);
+/* PortMonitor::evaluateSweep (src/Ports.cpp:327-418). points: n_ports * sweep points entries,
+ * port-major; summary: n_ports entries. Any output pointer may be NULL. Returns EMFD_ERR_STATE
+ * when no sweep is configured or fewer than 16 steps are recorded (the reference returns). */
+int emfd_ports_evaluate_sweep(emfd_ctx *ctx, double dt, emfd_sweep_point *points, emfd_port_sweep *summary,
+                              double *residual, int64_t *samples, int *full);
+
+/* ---------------------------------------------------------------------------------------
+ * FarFieldMonitor (src/FarField.hpp:75-133)
+ * --------------------------------------------------------------------------------------- */
+/* FarFieldMonitor::configure (src/FarField.cpp:259-307): box clamped to [L+1, N-L-2];
+ * EMFD_ERR_INVALID when it becomes degenerate. `applied` and `n_points` may be NULL. */
+int emfd_ntff_configure(emfd_ctx *ctx, const emfd_far_field_setup *setup, emfd_far_field_setup *applied,
+                        int *n_points);
+int emfd_ntff_disable(emfd_ctx *ctx);
+/* FarFieldMonitor::resetAccumulation (src/FarField.cpp:321-325). */
+int emfd_ntff_reset(emfd_ctx *ctx);
+/* Phasors in EM_Lab's PhasorPair layout: 32 floats per point (stage1 eRe eIm hRe hIm, stage2
+ * ..., vec4 each with w = 0). */
+int emfd_ntff_read_phasors(emfd_ctx *ctx, float *out, int64_t n_points);
+/* EM_Lab's direction set (src/FarField.cpp:201-257): 10582 directions as 4 floats each.
+ * map_index (may be NULL): 2*81*81 ints. frame (may be NULL): normal, u, v (9 floats). */
+int emfd_ntff_directions(emfd_ctx *ctx, float *dirs, int cap, int *n_dirs, int *map_index, float *frame);
+/* Radiation integrals (ntff_pattern + ntff_reduce, shaders/ntff_pattern.comp:45-110,
+ * shaders/ntff_reduce.comp:19-39) for arbitrary directions (4 floats each, xyz used):
+ * f2[d] = |N_perp - r x L|^2 at the monitor's wavelength; *power = radiated power P. */
+int emfd_ntff_eval(emfd_ctx *ctx, const float *dirs, int n_dirs, float *f2, double *power);
+/* FarFieldMonitor::evaluate + finishEvaluation (src/FarField.cpp:411-468) on the reference
+ * direction set. directivity_dbi: n_dirs floats (may be NULL). cuts: 3 x 6 floats per cut
+ * (peak dBi, peak angle, beamwidth, front/back, unused, unused) (may be NULL). Returns
+ * *valid = 0 when P <= 1e-12 or non-finite. */
+int emfd_ntff_evaluate(emfd_ctx *ctx, int *valid, double *radiated_power, float *max_directivity_dbi,
+                       float *max_direction, float *directivity_dbi, float *cuts);
+
+/* ---------------------------------------------------------------------------------------
+ * FieldAnalyzer (src/FieldAnalyzer.hpp:73-130)
+ * --------------------------------------------------------------------------------------- */
+/* FieldAnalyzer::measureFlux (src/FieldAnalyzer.cpp:447-505) for n surfaces. */
+int emfd_flux_measure(emfd_ctx *ctx, const emfd_flux_surface *surfaces, int n, double *out);
+/* FieldAnalyzer::probe / submit+collect (src/FieldAnalyzer.cpp:274-303,367-388,404-445):
+ * positions: 3 floats each. */
+int emfd_probe_measure(emfd_ctx *ctx, const float *positions, int n, emfd_probe_sample *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

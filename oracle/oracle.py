@@ -1,0 +1,251 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so) — TEST INFRASTRUCTURE ONLY.
+
+Only tests, the smoke check and the benchmark's CPU-baseline leg may import this module. It
+drives the scalar C restatement of EM_Lab's shaders in EM_Lab's own texel layout (RGBA per
+cell, x fastest).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+_lib = None
+
+
+class OrGrid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dt", C.c_float), ("pml", C.c_int),
+                ("sigma_max", C.c_float), ("grading", C.c_float)]
+
+
+class OrSource(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("half_extent", C.c_float * 3), ("softness", C.c_float * 3),
+                ("polarization", C.c_float * 3), ("amplitude", C.c_float), ("wavelength", C.c_float),
+                ("phase", C.c_float), ("waveform", C.c_int), ("ramp_periods", C.c_float),
+                ("repeat_periods", C.c_float), ("delay_periods", C.c_float), ("enabled", C.c_int)]
+
+
+class OrPort(C.Structure):
+    _fields_ = [("texel", C.c_int * 3), ("axis", C.c_int), ("edges", C.c_int), ("sign", C.c_int)]
+
+
+EDGE_DTYPE = np.dtype([("texel", "<i4", 3), ("axis", "<i4"), ("sigma", "<f4")])
+WIRE_DTYPE = np.dtype([("texel", "<i4", 3), ("component", "<i4"), ("term", "<i4"), ("factor", "<f4")])
+
+_P = C.c_void_p
+_F = C.c_float
+_D = C.c_double
+_I = C.c_int
+_SIG = {
+    "or_f32_to_f16": (C.c_uint16, [_F]),
+    "or_f16_to_f32": (_F, [C.c_uint16]),
+    "or_pack_half2": (C.c_uint32, [_F, _F]),
+    "or_configure": (None, [_I, _I, _I, _F, _I, _F, _F, C.POINTER(OrGrid)]),
+    "or_pml_profile": (_F, [C.POINTER(OrGrid), _F]),
+    "or_pml_sigma": (_F, [C.POINTER(OrGrid), _F, _F, _F]),
+    "or_in_pml": (_I, [C.POINTER(OrGrid), _I, _I, _I]),
+    "or_phi": (_F, [_F]),
+    "or_upload_materials": (_I, [C.POINTER(OrGrid), _P, _P, _P, C.c_size_t, _P, C.c_size_t, _P, _P, _P, _P]),
+    "or_set_sources": (_I, [C.POINTER(OrGrid), _P, _I, _P, _P, _P, C.POINTER(_I), _P, C.POINTER(_I)]),
+    "or_signal": (_D, [C.POINTER(OrSource), _D]),
+    "or_update_B": (None, [C.POINTER(OrGrid), _P, _P, _P, _I]),
+    "or_update_E": (None, [C.POINTER(OrGrid), _P, _P, _P]),
+    "or_apply_sources": (None, [C.POINTER(OrGrid), _P, _P, _P, _P, _I, _P, _I, _P]),
+    "or_clear_fields": (None, [C.POINTER(OrGrid), _P, _P]),
+    "or_step": (None, [C.POINTER(OrGrid), _P, _P, _P, _P, _I, _P, _I, _I, C.POINTER(C.c_uint64)]),
+    "or_port_accumulate": (None, [C.POINTER(OrGrid), _P, _P, _P, _I, _P, _F, _I, _P, _P]),
+    "or_monitor_constants": (None, [_D, _F, _F, C.c_uint64, _P, C.POINTER(_F)]),
+    "or_ntff_point": (None, [_P, _P, _P, _I, _P, _P, C.POINTER(_F)]),
+    "or_ntff_face_offsets": (_I, [_P, _P, _P]),
+    "or_ntff_accumulate": (None, [C.POINTER(OrGrid), _P, _P, _P, _P, _P, _I, _P, _F, _P]),
+    "or_ntff_eval_f32": (None, [_P, _P, _P, _I, _P, _P, _I, _F, _P, _P]),
+    "or_ntff_eval_f64": (None, [_P, _P, _P, _I, _P, _P, _I, _D, _P, _P, C.POINTER(_D)]),
+    "or_build_directions": (_I, [_P, _P, _I, _P, _P]),
+    "or_e_at_node": (None, [C.POINTER(OrGrid), _P, _I, _I, _I, _P]),
+    "or_b_at_node": (None, [C.POINTER(OrGrid), _P, _I, _I, _I, _P]),
+    "or_plane_flux": (_D, [C.POINTER(OrGrid), _P, _P, _I, _I, _I, _I, _I, _I]),
+    "or_box_flux": (_D, [C.POINTER(OrGrid), _P, _P, _P, _P]),
+    "or_probe": (None, [C.POINTER(OrGrid), _P, _P, _P, _P]),
+    "or_sweep": (_D, [_P, C.c_size_t, _I, _I, _D, _D, _D, _I, _D, _P, _P]),
+}
+
+
+def build() -> Path:
+    r = subprocess.run(["make", "-s", "-C", str(HERE)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout)
+    return LIB
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        l = C.CDLL(str(LIB))
+        for name, (res, args) in _SIG.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def grid(nx, ny, nz, courant, pml, sigma_max=1.6, grading=3.0) -> OrGrid:
+    g = OrGrid()
+    lib().or_configure(nx, ny, nz, courant, pml, sigma_max, grading, C.byref(g))
+    return g
+
+
+def source_c(s) -> OrSource:
+    f3 = C.c_float * 3
+    return OrSource(f3(*s.center), f3(*s.half_extent), f3(*s.softness), f3(*s.polarization), s.amplitude,
+                    s.wavelength, s.phase, int(s.waveform), s.ramp_periods, s.repeat_periods, s.delay_periods,
+                    1 if s.enabled else 0)
+
+
+def edge_array(edges):
+    a = np.zeros(len(edges), EDGE_DTYPE)
+    for i, (t, ax, s) in enumerate(edges):
+        a[i] = (t, ax, s)
+    return a
+
+
+def wire_array(wires):
+    a = np.zeros(len(wires), WIRE_DTYPE)
+    for i, (t, comp, term, f) in enumerate(wires):
+        a[i] = (t, comp, term, f)
+    return a
+
+
+class OracleSim:
+    """A whole simulation on the CPU oracle: materials, sources, steps and per-step monitors,
+    in the order of SimulationGrid::step (src/SimulationGrid.cpp:368-433)."""
+
+    def __init__(self, size, courant, pml, sigma_max=1.6, grading=3.0):
+        L = lib()
+        self.g = grid(size[0], size[1], size[2], courant, pml, sigma_max, grading)
+        n = self.g.nx * self.g.ny * self.g.nz
+        self.n = n
+        self.E = np.zeros((n, 4), np.float32)
+        self.B = np.zeros((n, 4), np.float32)
+        self.B[:, 3] = 1.0
+        self.edge = np.zeros((n, 4), np.uint32)
+        self.edge[:, 0] = L.or_pack_half2(0.0, 0.0)
+        self.edge[:, 1] = L.or_pack_half2(0.0, 1.0)
+        self.edge[:, 2] = L.or_pack_half2(1.0, 1.0)
+        self.wire = np.zeros((1, 4), np.uint32)
+        self.thin = 0
+        self.step_count = 0
+        self.sources = []
+        self.src_c = None
+        self.ports = None
+        self.ntff = None
+
+    @property
+    def size(self):
+        return (self.g.nx, self.g.ny, self.g.nz)
+
+    def upload_materials(self, sigma, eps, edges=(), wires=()):
+        L = lib()
+        s = np.ascontiguousarray(sigma, np.float32).ravel()
+        e = np.ascontiguousarray(eps, np.float32).ravel()
+        ea = edge_array(list(edges))
+        wa = wire_array(list(wires))
+        self.wire = np.zeros((self.n if len(wa) else 1, 4), np.uint32)
+        self.thin = L.or_upload_materials(C.byref(self.g), ptr(s), ptr(e), ptr(ea) if len(ea) else None, len(ea),
+                                          ptr(wa) if len(wa) else None, len(wa), ptr(self.E), ptr(self.B),
+                                          ptr(self.edge), ptr(self.wire))
+        self.step_count = 0
+
+    def set_sources(self, sources):
+        self.sources = list(sources)
+        self.src_c = (OrSource * max(len(sources), 1))(*[source_c(s) for s in sources])
+
+    def configure_ports(self, ports, averaging, record=False, capacity=0):
+        arr = (OrPort * max(len(ports), 1))()
+        for i, p in enumerate(ports):
+            arr[i] = OrPort((C.c_int * 3)(*p.texel), p.axis, p.edges, 1 if p.sign >= 0 else -1)
+        wl = max(ports[0].wavelength, 2.0) if ports else 16.0
+        self.ports = dict(arr=arr, n=len(ports), averaging=max(averaging, 0.5), wavelength=np.float32(wl),
+                          phasors=np.zeros((max(len(ports), 1), 8), np.float32), record=record,
+                          series=np.zeros((capacity, max(len(ports), 1), 2), np.float32), recorded=0,
+                          capacity=capacity)
+
+    def configure_ntff(self, box_min, box_max, wavelength, averaging):
+        L = lib()
+        bmin = np.array(box_min, np.int32)
+        bmax = np.array(box_max, np.int32)
+        fo = np.zeros(7, np.int32)
+        npts = L.or_ntff_face_offsets(ptr(bmin), ptr(bmax), ptr(fo))
+        self.ntff = dict(bmin=bmin, bmax=bmax, fo=fo, n=npts, wavelength=np.float32(max(wavelength, 2.0)),
+                         averaging=max(averaging, 0.5), phasors=np.zeros((npts, 32), np.float32))
+
+    def step(self, count):
+        L = lib()
+        if count <= 0:
+            return
+        # one call per step keeps the per-step monitors in the reference's order
+        for _ in range(count):
+            sc = C.c_uint64(self.step_count)
+            L.or_step(C.byref(self.g), ptr(self.E), ptr(self.B), ptr(self.edge), ptr(self.wire), self.thin,
+                      C.cast(self.src_c, C.c_void_p) if self.src_c else None, len(self.sources), 1, C.byref(sc))
+            self.step_count = sc.value
+            self._monitors()
+
+    def _monitors(self):
+        L = lib()
+        if self.ports and self.ports["n"]:
+            P = self.ports
+            rot = np.zeros(4, np.float32)
+            blend = C.c_float()
+            L.or_monitor_constants(float(P["wavelength"]), P["averaging"], self.g.dt, self.step_count, ptr(rot),
+                                   C.byref(blend))
+            rec = -1
+            if P["record"] and P["recorded"] < P["capacity"]:
+                rec = P["recorded"]
+                P["recorded"] += 1
+            L.or_port_accumulate(C.byref(self.g), ptr(self.E), ptr(self.B), C.cast(P["arr"], C.c_void_p), P["n"],
+                                 ptr(rot), blend.value, rec, ptr(P["phasors"]), ptr(P["series"]))
+        if self.ntff:
+            N = self.ntff
+            rot = np.zeros(4, np.float32)
+            blend = C.c_float()
+            L.or_monitor_constants(float(N["wavelength"]), N["averaging"], self.g.dt, self.step_count, ptr(rot),
+                                   C.byref(blend))
+            L.or_ntff_accumulate(C.byref(self.g), ptr(self.E), ptr(self.B), ptr(N["bmin"]), ptr(N["bmax"]),
+                                 ptr(N["fo"]), N["n"], ptr(rot), blend.value, ptr(N["phasors"]))
+
+    def ntff_eval_f32(self, dirs, k=None):
+        L = lib()
+        N = self.ntff
+        d = np.ascontiguousarray(dirs, np.float32)
+        out = np.zeros(2 * d.shape[0], np.float32)
+        kk = np.float32(2.0 * np.pi / float(N["wavelength"])) if k is None else np.float32(k)
+        center = np.array([0.5 * (N["bmin"][a] + N["bmax"][a]) for a in range(3)], np.float32)
+        L.or_ntff_eval_f32(ptr(N["bmin"]), ptr(N["bmax"]), ptr(N["fo"]), N["n"], ptr(N["phasors"]), ptr(d), d.shape[0],
+                           float(kk), ptr(center), ptr(out))
+        return out[0::2], float(out[1])
+
+    def plane_flux(self, axis, index, rmin, rmax):
+        return lib().or_plane_flux(C.byref(self.g), ptr(self.E), ptr(self.B), axis, index, rmin[0], rmin[1], rmax[0],
+                                   rmax[1])
+
+    def box_flux(self, bmin, bmax):
+        a = np.array(bmin, np.int32)
+        b = np.array(bmax, np.int32)
+        return lib().or_box_flux(C.byref(self.g), ptr(self.E), ptr(self.B), ptr(a), ptr(b))
+
+    def probe(self, pos):
+        out = np.zeros(11, np.float32)
+        p = np.array(pos, np.float32)
+        lib().or_probe(C.byref(self.g), ptr(self.E), ptr(self.B), ptr(p), ptr(out))
+        return out
