@@ -62,6 +62,8 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--kernel", type=int, default=0, help="EMFD_KERNEL_* variant (| tile << 8 | zchunk << 16)")
     ap.add_argument("--no-monitors", action="store_true")
+    ap.add_argument("--scene", choices=["spheres", "vacuum"], default="spheres",
+                    help="spheres: the headline workload; vacuum: same grid, no materials (roofline check)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -70,9 +72,12 @@ def main() -> int:
         return 2
 
     from em_lab_amd import emfdtd as E
-    from em_lab_amd.scene import spheres_dipole
+    from em_lab_amd.scene import hertzian_vacuum, spheres_dipole
 
-    scene = spheres_dipole(n=args.size, pml=10)
+    if args.scene == "spheres":
+        scene = spheres_dipole(n=args.size, pml=10)
+    else:
+        scene = hertzian_vacuum(args.size, 10, steps_wavelength=20.0)
     g = E.SimulationGrid(0)
     g.configure(scene.grid_config())
     s, e = scene.flat_materials()
@@ -80,8 +85,9 @@ def main() -> int:
     g.set_sources(scene.sources)
     g.set_kernel(args.kernel)
     if not args.no_monitors:
-        pm = E.PortMonitor(g)
-        pm.configure(scene.ports, scene.far_field.averaging_periods, scene.sweep, scene.cell_size)
+        if scene.ports:
+            pm = E.PortMonitor(g)
+            pm.configure(scene.ports, scene.far_field.averaging_periods, scene.sweep, scene.cell_size)
         ff = E.FarFieldMonitor(g)
         ff.configure(scene.far_field)
     nx, ny, nz = g.size
@@ -122,9 +128,11 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{nx}^3 Yee leapfrog, 10-cell absorber, 32 seeded dielectric spheres, z thin-wire "
-                               f"half-wave dipole (lambda 20) on a 50 ohm resistive feed, port + far-field "
-                               f"accumulation every step" + (" (monitors off)" if args.no_monitors else ""),
+        "config": {"workload": (f"{nx}^3 Yee leapfrog, 10-cell absorber, 32 seeded dielectric spheres, z thin-wire "
+                                f"half-wave dipole (lambda 20) on a 50 ohm resistive feed, port + far-field "
+                                f"accumulation every step" if args.scene == "spheres" else
+                                f"{nx}^3 vacuum Yee leapfrog, 10-cell absorber, z Hertzian pulse, far-field "
+                                f"accumulation every step") + (" (monitors off)" if args.no_monitors else ""),
                    "grid": [nx, ny, nz], "courant": round(g.dt, 6), "material_classes": g.material_classes(),
                    "kernel": args.kernel},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -72,7 +72,7 @@ struct emfd_ctx {
     emfd::DevBuf field_buf[2];
     int cur = 0;
     emfd::DevBuf sigma_cell, eps_cell;  // EM_Lab's alpha channels (probes)
-    emfd::DevBuf mat_id, table, prof;
+    emfd::DevBuf mat_id, table, prof, pml_tab;
     int id_bytes = 0;
     int classes = 0;
     int has_wire = 0;
@@ -83,9 +83,13 @@ struct emfd_ctx {
     emfd::DevBuf d_packed, d_cells, d_lists;
     int n_cells = 0;
 
-    // per-call step constants
+    // per-call step constants: pinned host staging (two buffers, each released by an event once
+    // its copy has run) and the device copy the kernels read
     emfd::DevBuf d_consts;
-    std::vector<float> h_consts;
+    float *h_consts[2] = {nullptr, nullptr};
+    size_t h_consts_cap[2] = {0, 0};
+    hipEvent_t h_consts_ev[2] = {nullptr, nullptr};
+    int h_consts_next = 0;
 
     uint64_t step_count = 0;
     int kernel_variant = EMFD_KERNEL_AUTO;

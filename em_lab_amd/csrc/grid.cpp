@@ -4,6 +4,7 @@
 // match bit for bit (clamps, edge averaging, binary16 quantisation, source lists, FP64 signals);
 // the GPU state and launch sequence are this library's own design.
 #include "context.h"
+#include "expf.h"
 
 #include <algorithm>
 #include <array>
@@ -112,6 +113,8 @@ emfd::LatticeArgs emfd_ctx::lattice() const {
     a.id_bytes = id_bytes;
     a.table = table.as<float>();
     a.has_wire = has_wire;
+    a.pml_tab = pml_tab.as<float>();
+    a.pml_k = 2 * cfg.pml_thickness + 2;
     return a;
 }
 
@@ -148,12 +151,16 @@ void emfd_destroy(emfd_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &b : ctx->field_buf) b.release();
-    DevBuf *bufs[] = {&ctx->sigma_cell, &ctx->eps_cell, &ctx->mat_id, &ctx->table, &ctx->prof, &ctx->d_packed,
+    DevBuf *bufs[] = {&ctx->sigma_cell, &ctx->eps_cell, &ctx->mat_id, &ctx->table, &ctx->prof, &ctx->pml_tab, &ctx->d_packed,
                       &ctx->d_cells, &ctx->d_lists, &ctx->d_consts, &ctx->ports.d_ports, &ctx->ports.d_phasors,
                       &ctx->ports.d_series, &ctx->ntff.d_phasors, &ctx->ntff.d_dirs, &ctx->ntff.d_partials,
                       &ctx->ntff.d_out, &ctx->ntff.d_power, &ctx->scratch_partials, &ctx->scratch_misc};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t ev : ctx->ev_pool) (void)hipEventDestroy(ev);
+    for (int i = 0; i < 2; ++i) {
+        if (ctx->h_consts[i]) (void)hipHostFree(ctx->h_consts[i]);
+        if (ctx->h_consts_ev[i]) (void)hipEventDestroy(ctx->h_consts_ev[i]);
+    }
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -196,7 +203,7 @@ int emfd_grid_configure(emfd_ctx *ctx, const emfd_grid_config *requested, emfd_g
     auto profile = [&](float depth) -> float {
         if (depth <= 0.0f) return 0.0f;
         const float u = std::min(depth / static_cast<float>(c.pml_thickness), 1.0f);
-        return c.pml_sigma_max * (std::exp(c.pml_grading * u) - 1.0f) / (std::exp(c.pml_grading) - 1.0f);
+        return c.pml_sigma_max * (det_expf(c.pml_grading * u) - 1.0f) / (det_expf(c.pml_grading) - 1.0f);
     };
     for (int a = 0; a < 3; ++a) {
         const float hi = static_cast<float>(c.size[a] - 1) - L;
@@ -210,6 +217,9 @@ int emfd_grid_configure(emfd_ctx *ctx, const emfd_grid_config *requested, emfd_g
     if (int rc = ctx->prof.ensure(prof.size() * sizeof(float))) return rc;
     EMFD_HIP(hipMemcpyAsync(ctx->prof.ptr, prof.data(), prof.size() * sizeof(float), hipMemcpyHostToDevice,
                             ctx->stream));
+    const size_t kk = static_cast<size_t>(2 * c.pml_thickness + 2);
+    if (int rc = ctx->pml_tab.ensure(kk * kk * kk * kPmlStride * sizeof(float))) return rc;
+    EMFD_HIP(launch_build_pml_table(ctx->lattice(), ctx->pml_tab.as<float>(), ctx->stream));
 
     // vacuum everywhere: E = (0,0,0, sigma 0), B = (0,0,0, eps 1) (src/SimulationGrid.cpp:185-192)
     ctx->classes = 0;
@@ -459,7 +469,7 @@ int emfd_grid_reset_fields(emfd_ctx *ctx) {
 int emfd_grid_set_kernel(emfd_ctx *ctx, int variant) {
     if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
     const int v = variant & 0xff;
-    if (v < 0 || v > 2) return fail(EMFD_ERR_INVALID, "unknown kernel variant");
+    if (v < 0 || v > 4) return fail(EMFD_ERR_INVALID, "unknown kernel variant");
     ctx->kernel_variant = v;
     ctx->tile_cfg = variant >> 8;  // optional tile shape / z-chunk selector for the fused kernel
     return EMFD_OK;
@@ -501,17 +511,36 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
     const int nsrc = static_cast<int>(ctx->sources.size());
     const int mon = monitors_consts_floats(ctx);
     const int stride = std::max(nsrc + mon, 1);
-    ctx->h_consts.assign(static_cast<size_t>(stride) * count, 0.0f);
+    const size_t nfl = static_cast<size_t>(stride) * count;
+    const int hb = ctx->h_consts_next;
+    ctx->h_consts_next ^= 1;
+    if (ctx->h_consts_ev[hb]) {
+        EMFD_HIP(hipEventSynchronize(ctx->h_consts_ev[hb]));  // its previous copy has run
+    } else {
+        EMFD_HIP(hipEventCreateWithFlags(&ctx->h_consts_ev[hb], hipEventDisableTiming));
+    }
+    if (ctx->h_consts_cap[hb] < nfl) {
+        if (ctx->h_consts[hb]) EMFD_HIP(hipHostFree(ctx->h_consts[hb]));
+        ctx->h_consts[hb] = nullptr;
+        EMFD_HIP(hipHostMalloc(reinterpret_cast<void **>(&ctx->h_consts[hb]), nfl * sizeof(float),
+                               hipHostMallocDefault));
+        ctx->h_consts_cap[hb] = nfl;
+    }
+    float *hc = ctx->h_consts[hb];
+    std::fill(hc, hc + nfl, 0.0f);
     const double dt = static_cast<double>(ctx->cfg.courant);
     for (int k = 0; k < count; ++k) {
-        float *row = &ctx->h_consts[static_cast<size_t>(k) * stride];
+        float *row = hc + static_cast<size_t>(k) * stride;
         const double t = (static_cast<double>(ctx->step_count + static_cast<uint64_t>(k)) + 0.5) * dt;
         for (int s = 0; s < nsrc; ++s) row[s] = static_cast<float>(source_signal(ctx->sources[s], t));
         monitors_fill_consts(ctx, row + nsrc, ctx->step_count + static_cast<uint64_t>(k) + 1);
     }
-    if (int rc = ctx->d_consts.ensure(ctx->h_consts.size() * sizeof(float))) return rc;
-    EMFD_HIP(hipMemcpyAsync(ctx->d_consts.ptr, ctx->h_consts.data(), ctx->h_consts.size() * sizeof(float),
-                            hipMemcpyHostToDevice, ctx->stream));
+    if (ctx->d_consts.bytes < nfl * sizeof(float)) {
+        EMFD_HIP(hipStreamSynchronize(ctx->stream));  // earlier steps may still read the old buffer
+        if (int rc = ctx->d_consts.ensure(nfl * sizeof(float))) return rc;
+    }
+    EMFD_HIP(hipMemcpyAsync(ctx->d_consts.ptr, hc, nfl * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    EMFD_HIP(hipEventRecord(ctx->h_consts_ev[hb], ctx->stream));
 
     const LatticeArgs lat = ctx->lattice();
     for (int k = 0; k < count; ++k) {
@@ -530,8 +559,18 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
             ctx->ev_used += 2;
             EMFD_HIP(hipEventRecord(ev0, ctx->stream));
         }
-        if (ctx->kernel_variant == EMFD_KERNEL_TWO_PASS) {
+        int variant = ctx->kernel_variant;
+        if (variant == EMFD_KERNEL_AUTO) variant = yee_rows_supported(lat) ? EMFD_KERNEL_ROWS : EMFD_KERNEL_TWO_PASS;
+        if (variant == EMFD_KERNEL_ROWS && !yee_rows_supported(lat))
+            return fail(EMFD_ERR_INVALID, "the row-wavefront kernel supports nx <= 256");
+        if (variant == EMFD_KERNEL_TWO_PASS) {
             EMFD_HIP(launch_yee_two_pass(lat, ctx->fields(ctx->cur), ctx->stream));
+        } else if (variant == EMFD_KERNEL_POINT) {
+            EMFD_HIP(launch_yee_point(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->stream));
+            ctx->cur ^= 1;
+        } else if (variant == EMFD_KERNEL_ROWS) {
+            EMFD_HIP(launch_yee_rows(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->tile_cfg, ctx->stream));
+            ctx->cur ^= 1;
         } else {
             EMFD_HIP(launch_yee_fused(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->tile_cfg,
                                       ctx->stream));

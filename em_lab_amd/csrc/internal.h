@@ -33,7 +33,16 @@ struct LatticeArgs {
     int id_bytes;               // 2 or 4
     const float *table;         // classes x kTableStride floats
     int has_wire;               // scene has thin wires
+    const float *pml_tab;       // absorber coefficients per (kx, ky, kz) depth class, kPmlStride floats
+    int pml_k;                  // depth classes per axis: 2 * pml + 2
 };
+
+// Absorber coefficient table entry (vacuum edges): B: exp(-s dt), dt phi(s dt) per component;
+// E: Ca, Cb per component for sigma = sigma_pml, eps_r = 1. The depth class of lattice index i
+// on an axis of n cells is 0 inside, 1 + i below the layer's inner edge, 1 + L + (i - (n-1-L))
+// from the upper layer's first +1/2 sample on.
+constexpr int kPmlStride = 16;
+enum PmlField : int { kPmlBa = 0, kPmlBb = 3, kPmlEa = 6, kPmlEb = 9 };
 
 struct FieldPtrs {
     float *e[3];
@@ -54,10 +63,14 @@ struct StepConsts {
 hipError_t launch_build_table(const uint16_t *keys, int classes, float dt, float *table, hipStream_t s);
 hipError_t launch_yee_fused(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int tile_cfg,
                             hipStream_t s);
+hipError_t launch_yee_rows(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int zchunk, hipStream_t s);
+int yee_rows_supported(const LatticeArgs &a);
+hipError_t launch_yee_point(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, hipStream_t s);
 hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
 hipError_t launch_apply_sources(const LatticeArgs &a, const FieldPtrs &f, const int *cells, int n_cells,
                                 const int *lists, const float *packed, const float *signals, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
+hipError_t launch_build_pml_table(const LatticeArgs &a, float *tab, hipStream_t s);
 
 struct PortDev {
     int texel[3];

@@ -7,8 +7,11 @@
 //
 // Arithmetic follows the reference shaders operation by operation (shaders/common.glsl:21-54,
 // shaders/update_B.comp:31-70, shaders/update_E.comp:31-66, shaders/apply_sources.comp:37-79);
-// this file is compiled with -ffp-contract=off so no multiply-add is fused.
+// this file is compiled with -ffp-contract=off so no multiply-add is fused, and every exponential
+// goes through det_expf, which gives the same bits on the host and the device.
+#include "expf.h"
 #include "internal.h"
+#include "yee_device.h"
 
 #include <algorithm>
 
@@ -16,95 +19,6 @@ namespace emfd {
 namespace {
 
 constexpr int kXcds = 8;  // MI355X: 8 accelerator dies, each with its own L2
-
-__device__ __forceinline__ float phi_f(float x) {
-    // phi(x) = (1 - e^{-x}) / x, Taylor series below 1e-2 (shaders/common.glsl:50-54)
-    const float series = 1.0f - x * (0.5f - x * (1.0f / 6.0f - x * (1.0f / 24.0f)));
-    const float exact = (1.0f - expf(-x)) / fmaxf(x, 1e-12f);
-    return x >= 1e-2f ? exact : series;
-}
-
-__device__ __forceinline__ bool in_pml(const LatticeArgs &a, int x, int y, int z) {
-    // shaders/common.glsl:40-43
-    const int L = a.pml;
-    return x < L || y < L || z < L || x >= a.nx - (L + 1) || y >= a.ny - (L + 1) || z >= a.nz - (L + 1);
-}
-
-// Absorber sigma at a staggered point: (profile_x + profile_y) + profile_z as pmlSigma sums it
-// (shaders/common.glsl:31-36). hx/hy/hz select the +1/2 samples.
-__device__ __forceinline__ float pml_sigma(const LatticeArgs &a, int x, int hx, int y, int hy, int z, int hz) {
-    const int s = a.prof_stride;
-    return a.prof[(2 * 0 + hx) * s + x] + a.prof[(2 * 1 + hy) * s + y] + a.prof[(2 * 2 + hz) * s + z];
-}
-
-__device__ __forceinline__ uint32_t load_id(const LatticeArgs &a, size_t i) {
-    if (a.id_bytes == 2) return static_cast<const uint16_t *>(a.mat_id)[i];
-    return static_cast<const uint32_t *>(a.mat_id)[i];
-}
-
-// Faraday update of one texel (shaders/update_B.comp:35-69). e0: E at p; exn/eyn/ezn: E at
-// p+x, p+y, p+z (0 beyond the upper faces). id: class of p (0 when no table).
-__device__ __forceinline__ void faraday(const LatticeArgs &a, int x, int y, int z, const float e0[3],
-                                        const float exn[3], const float eyn[3], const float ezn[3], uint32_t id,
-                                        float b[3]) {
-    float t0x = eyn[2] - e0[2], t0y = ezn[0] - e0[0], t0z = exn[1] - e0[1];
-    float t1x = ezn[1] - e0[1], t1y = exn[2] - e0[2], t1z = eyn[0] - e0[0];
-    if (a.has_wire) {
-        const float *w = a.table + static_cast<size_t>(id) * kTableStride + kWire;
-        t0x *= w[0];
-        t1x *= w[1];
-        t0y *= w[2];
-        t1y *= w[3];
-        t0z *= w[4];
-        t1z *= w[5];
-    }
-    const float cx = t0x - t1x, cy = t0y - t1y, cz = t0z - t1z;
-    const float dt = a.dt;
-    if (in_pml(a, x, y, z)) {
-        const float sx = pml_sigma(a, x, 0, y, 1, z, 1) * dt;
-        const float sy = pml_sigma(a, x, 1, y, 0, z, 1) * dt;
-        const float sz = pml_sigma(a, x, 1, y, 1, z, 0) * dt;
-        b[0] = expf(-sx) * b[0] - (dt * phi_f(sx)) * cx;
-        b[1] = expf(-sy) * b[1] - (dt * phi_f(sy)) * cy;
-        b[2] = expf(-sz) * b[2] - (dt * phi_f(sz)) * cz;
-    } else {
-        b[0] = b[0] - dt * cx;
-        b[1] = b[1] - dt * cy;
-        b[2] = b[2] - dt * cz;
-    }
-}
-
-// Update coefficients Ca, Cb of the three E edges of texel p (shaders/update_E.comp:46-61).
-__device__ __forceinline__ void edge_coefficients(const LatticeArgs &a, int x, int y, int z, uint32_t id,
-                                                  float ca[3], float cb[3]) {
-    const float dt = a.dt;
-    if (in_pml(a, x, y, z)) {
-        float sig[3] = {0.0f, 0.0f, 0.0f}, eps[3] = {1.0f, 1.0f, 1.0f};
-        if (a.mat_id) {
-            const float *t = a.table + static_cast<size_t>(id) * kTableStride;
-            sig[0] = t[kSig + 0]; sig[1] = t[kSig + 1]; sig[2] = t[kSig + 2];
-            eps[0] = t[kEps + 0]; eps[1] = t[kEps + 1]; eps[2] = t[kEps + 2];
-        }
-        sig[0] += pml_sigma(a, x, 1, y, 0, z, 0);
-        sig[1] += pml_sigma(a, x, 0, y, 1, z, 0);
-        sig[2] += pml_sigma(a, x, 0, y, 0, z, 1);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float dtEps = dt / eps[c];
-            const float xx = sig[c] * dtEps;
-            ca[c] = expf(-xx);
-            cb[c] = dtEps * phi_f(xx);
-        }
-    } else if (a.mat_id) {
-        const float *t = a.table + static_cast<size_t>(id) * kTableStride;
-        ca[0] = t[kCa + 0]; ca[1] = t[kCa + 1]; ca[2] = t[kCa + 2];
-        cb[0] = t[kCb + 0]; cb[1] = t[kCb + 1]; cb[2] = t[kCb + 2];
-    } else {
-        // vacuum: x = 0 -> exp(-0) = 1, phi(0) = 1, dt / 1 = dt
-        ca[0] = ca[1] = ca[2] = 1.0f;
-        cb[0] = cb[1] = cb[2] = dt;
-    }
-}
 
 // ---------------------------------------------------------------------------------------------
 // Coefficient table from binary16 keys: 12 halves per class
@@ -130,13 +44,43 @@ __global__ void k_build_table(const uint16_t *keys, int classes, float dt, float
         const float xx = sig * dtEps;
         t[kSig + c] = sig;
         t[kEps + c] = eps;
-        t[kCa + c] = expf(-xx);
+        t[kCa + c] = det_expf(-xx);
         t[kCb + c] = dtEps * phi_f(xx);
     }
 #pragma unroll
     for (int j = 0; j < 6; ++j) t[kWire + j] = h[6 + j];
     t[18] = 0.0f;
     t[19] = 0.0f;
+}
+
+// Absorber coefficient table (one entry per (kx, ky, kz) depth class), built with the same
+// expressions faraday() / edge_coefficients() would evaluate per texel.
+__device__ __forceinline__ int pml_index_of(int k, int n, int L) {
+    return k == 0 ? L : (k <= L ? k - 1 : (n - 1 - L) + (k - 1 - L));
+}
+
+__global__ void k_build_pml_table(LatticeArgs a, float *tab) {
+    const int K = a.pml_k;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= K * K * K) return;
+    const int x = pml_index_of(e % K, a.nx, a.pml), y = pml_index_of((e / K) % K, a.ny, a.pml),
+              z = pml_index_of(e / (K * K), a.nz, a.pml);
+    const float dt = a.dt;
+    float *t = tab + static_cast<size_t>(e) * kPmlStride;
+    const float sB[3] = {pml_sigma(a, x, 0, y, 1, z, 1) * dt, pml_sigma(a, x, 1, y, 0, z, 1) * dt,
+                         pml_sigma(a, x, 1, y, 1, z, 0) * dt};
+    const float sE[3] = {0.0f + pml_sigma(a, x, 1, y, 0, z, 0), 0.0f + pml_sigma(a, x, 0, y, 1, z, 0),
+                         0.0f + pml_sigma(a, x, 0, y, 0, z, 1)};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        t[kPmlBa + c] = det_expf(-sB[c]);
+        t[kPmlBb + c] = dt * phi_f(sB[c]);
+        const float dtEps = dt / 1.0f;
+        const float xx = sE[c] * dtEps;
+        t[kPmlEa + c] = det_expf(-xx);
+        t[kPmlEb + c] = dtEps * phi_f(xx);
+    }
+    t[12] = t[13] = t[14] = t[15] = 0.0f;
 }
 
 __global__ void k_fill(float *p, size_t n, float v) {
@@ -439,7 +383,7 @@ __device__ __forceinline__ float footprint(const float *s, float x, float y, flo
     const float dx = fmaxf(fabsf(x - s[0]) - s[4], 0.0f) / s[12];
     const float dy = fmaxf(fabsf(y - s[1]) - s[5], 0.0f) / s[13];
     const float dz = fmaxf(fabsf(z - s[2]) - s[6], 0.0f) / s[14];
-    return expf(-(dx * dx + dy * dy + dz * dz));
+    return det_expf(-(dx * dx + dy * dy + dz * dz));
 }
 
 __global__ void __launch_bounds__(64) k_apply_sources(LatticeArgs a, FieldPtrs f, const int *cells, int n_cells,
@@ -478,6 +422,12 @@ __global__ void __launch_bounds__(64) k_apply_sources(LatticeArgs a, FieldPtrs f
 hipError_t launch_build_table(const uint16_t *keys, int classes, float dt, float *table, hipStream_t s) {
     if (classes <= 0) return hipSuccess;
     k_build_table<<<(classes + 127) / 128, 128, 0, s>>>(keys, classes, dt, table);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_pml_table(const LatticeArgs &a, float *tab, hipStream_t s) {
+    const int n = a.pml_k * a.pml_k * a.pml_k;
+    k_build_pml_table<<<(n + 127) / 128, 128, 0, s>>>(a, tab);
     return hipGetLastError();
 }
 

@@ -24,6 +24,8 @@ LAYOUT_SOA = 1
 KERNEL_AUTO = 0
 KERNEL_FUSED = 1
 KERNEL_TWO_PASS = 2
+KERNEL_ROWS = 3
+KERNEL_POINT = 4
 
 
 class EmfdError(RuntimeError):

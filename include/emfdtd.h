@@ -41,9 +41,11 @@ extern "C" {
 #define EMFD_LAYOUT_TEXEL 0 /* float[n][4]: (x, y, z, alpha) per cell, EM_Lab's RGBA32F texel */
 #define EMFD_LAYOUT_SOA 1   /* float[3][n]: component planes */
 
-#define EMFD_KERNEL_AUTO 0    /* fused B+E z-marching kernel (default) */
-#define EMFD_KERNEL_FUSED 1   /* same, explicit */
+#define EMFD_KERNEL_AUTO 0     /* fastest variant that supports the grid (default) */
+#define EMFD_KERNEL_FUSED 1    /* fused B+E, LDS-tiled z-march (ping-pong buffers) */
 #define EMFD_KERNEL_TWO_PASS 2 /* separate update_B / update_E passes, in place */
+#define EMFD_KERNEL_ROWS 3     /* fused B+E, one wavefront per x-row z-march (nx <= 256) */
+#define EMFD_KERNEL_POINT 4    /* fused B+E, one thread per texel, neighbour B rebuilt */
 
 typedef struct emfd_ctx emfd_ctx;
 
@@ -211,7 +213,8 @@ int emfd_grid_download_fields(emfd_ctx *ctx, float *E, float *B, int layout);
 int emfd_grid_upload_fields(emfd_ctx *ctx, const float *E, const float *B, int layout);
 /* Device bytes held by the grid (fields, coefficients, source lists). */
 int64_t emfd_grid_memory_bytes(emfd_ctx *ctx);
-/* Choose the Yee kernel variant (EMFD_KERNEL_*). */
+/* Choose the Yee kernel variant: EMFD_KERNEL_* in the low byte; bits 8.. carry a variant
+ * parameter (FUSED: tile shape | z-chunk << 8; ROWS: z-chunk), 0 = automatic. */
 int emfd_grid_set_kernel(emfd_ctx *ctx, int variant);
 /* Number of distinct coefficient classes after the last material upload (0 = uniform vacuum). */
 int emfd_grid_material_classes(emfd_ctx *ctx, int *classes);

@@ -33,6 +33,27 @@
 #define IDX(g, x, y, z) ((((size_t)(z)) * (size_t)(g)->ny + (size_t)(y)) * (size_t)(g)->nx + (size_t)(x))
 
 static inline float fmaxf_(float a, float b) { return a > b ? a : b; }
+
+/* The GLSL built-in exp is driver-defined (no version pin in the reference). Both this oracle
+ * and the GPU library evaluate it with the same published method so that their results can be
+ * compared bit for bit: Cody-Waite reduction x = n ln2 + r (ln2 as a 16-bit head plus a tail),
+ * Taylor series of e^r to degree 7 by Horner's rule, exact scaling by 2^n. Within 2 ulp. */
+float or_exp(float x) {
+    if (x != x) return x;
+    if (x > 88.72283935546875f) return HUGE_VALF;
+    if (x < -103.972084045410156f) return 0.0f;
+    const float n = rintf(x * 1.44269502162933349609f);
+    const float r = (x - n * 0.693145751953125f) - n * 1.42860682030941723212e-6f;
+    float p = 1.0f / 5040.0f;
+    p = 1.0f / 720.0f + r * p;
+    p = 1.0f / 120.0f + r * p;
+    p = 1.0f / 24.0f + r * p;
+    p = 1.0f / 6.0f + r * p;
+    p = 0.5f + r * p;
+    p = 1.0f + r * p;
+    p = 1.0f + r * p;
+    return ldexpf(p, (int)n);
+}
 static inline float fminf_(float a, float b) { return a < b ? a : b; }
 static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
@@ -122,7 +143,7 @@ void or_configure(int nx, int ny, int nz, float courant, int pml, float sigma_ma
 float or_pml_profile(const or_grid *g, float depth) {
     if (depth <= 0.0f) return 0.0f;
     float u = fminf_(depth / (float)g->pml, 1.0f); /* depth/0 = inf -> 1 */
-    return g->sigma_max * (expf(g->grading * u) - 1.0f) / (expf(g->grading) - 1.0f);
+    return g->sigma_max * (or_exp(g->grading * u) - 1.0f) / (or_exp(g->grading) - 1.0f);
 }
 
 float or_pml_sigma(const or_grid *g, float x, float y, float z) {
@@ -139,7 +160,7 @@ int or_in_pml(const or_grid *g, int x, int y, int z) {
 
 float or_phi(float x) {
     const float series = 1.0f - x * (0.5f - x * (1.0f / 6.0f - x * (1.0f / 24.0f)));
-    const float exact = (1.0f - expf(-x)) / fmaxf_(x, 1e-12f);
+    const float exact = (1.0f - or_exp(-x)) / fmaxf_(x, 1e-12f);
     return x >= 1e-2f ? exact : series; /* mix(series, exact, step(1e-2, x)) */
 }
 
@@ -376,7 +397,7 @@ void or_update_B(const or_grid *g, float *B, const float *E, const uint32_t *wir
                                          or_pml_sigma(g, px + 0.5f, py + 0.5f, pz + 0.0f)};
                     for (int c = 0; c < 3; ++c) {
                         const float xx = sm[c] * dt;
-                        b[c] = expf(-xx) * b[c] - (dt * or_phi(xx)) * curl[c];
+                        b[c] = or_exp(-xx) * b[c] - (dt * or_phi(xx)) * curl[c];
                     }
                 } else {
                     for (int c = 0; c < 3; ++c) b[c] = b[c] - dt * curl[c];
@@ -429,7 +450,7 @@ void or_update_E(const or_grid *g, float *E, const float *B, const uint32_t *edg
                 for (int c = 0; c < 3; ++c) {
                     const float dtEps = dt / epsr[c];
                     const float xx = sigma[c] * dtEps;
-                    const float ca = expf(-xx);
+                    const float ca = or_exp(-xx);
                     const float cb = dtEps * or_phi(xx);
                     e[c] = ca * e[c] + cb * curl[c];
                 }
@@ -443,7 +464,7 @@ static float footprint(const float *s, float x, float y, float z) {
     const float q[3] = {x, y, z};
     float d[3];
     for (int a = 0; a < 3; ++a) d[a] = fmaxf_(fabsf(q[a] - s[a]) - s[4 + a], 0.0f) / s[12 + a];
-    return expf(-(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+    return or_exp(-(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
 }
 
 void or_apply_sources(const or_grid *g, float *E, const uint32_t *edge_mat, const float *packed, const float *signals,

@@ -49,6 +49,9 @@ typedef struct {
 typedef struct { int texel[3]; int axis; float sigma; } or_edge_override;
 typedef struct { int texel[3]; int component; int term; float factor; } or_wire_coef;
 
+/* exp used for every float exponential of the path (see oracle.c) */
+float or_exp(float x);
+
 /* half floats (round to nearest even, as glm::packHalf2x16) */
 uint16_t or_f32_to_f16(float f);
 float or_f16_to_f32(uint16_t h);

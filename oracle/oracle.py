@@ -41,6 +41,7 @@ _F = C.c_float
 _D = C.c_double
 _I = C.c_int
 _SIG = {
+    "or_exp": (_F, [_F]),
     "or_f32_to_f16": (C.c_uint16, [_F]),
     "or_f16_to_f32": (_F, [C.c_uint16]),
     "or_pack_half2": (C.c_uint32, [_F, _F]),

@@ -27,7 +27,7 @@ def _fields_match(g, sim, tol=FIELD_TOL):
     return re, rb
 
 
-@pytest.mark.parametrize("kernel", [E.KERNEL_FUSED, E.KERNEL_TWO_PASS])
+@pytest.mark.parametrize("kernel", [E.KERNEL_FUSED, E.KERNEL_TWO_PASS, E.KERNEL_ROWS, E.KERNEL_POINT])
 def test_hertzian_vacuum_64(kernel):
     """Plain FDTD 64^3 vacuum, 8-cell absorber, Courant 0.99/sqrt(3), z Hertzian pulse at the
     centre (lambda 16), 400 steps: fields, far-field phasors and the directivity pattern."""
@@ -120,6 +120,29 @@ def test_fused_tile_shapes_odd_grid(shape):
     g.close()
 
 
+@pytest.mark.parametrize("size,zchunk", [((37, 29, 23), 5), ((100, 17, 30), 0), ((200, 14, 19), 7),
+                                         ((128, 20, 16), 3), ((256, 15, 14), 0)])
+def test_rows_kernel_odd_grids(size, zchunk):
+    """Row-wavefront kernel at every lanes-per-row width (1..4, full and partial rows), lossy
+    random materials, absorber on all faces, short z-chunks with ghost planes."""
+    print("case: rows", size, zchunk, flush=True)
+    sc = Scene(size=size, pml=3, courant=0.55)
+    rng = np.random.default_rng(sum(size))
+    sc.sigma[:] = (rng.random(sc.sigma.shape) < 0.3) * rng.uniform(0, 2.0, sc.sigma.shape).astype(np.float32)
+    sc.eps[:] = rng.choice(np.array([1.0, 2.5, 4.0], np.float32), sc.eps.shape)
+    c = [v / 2 for v in size]
+    sc.hertzian((c[0] + 0.2, c[1] - 0.4, c[2] + 0.3), (0.3, -0.5, 1.0), 1.0, wavelength=8.0, waveform="pulse",
+                pulse_width=0.8)
+    sc.wire((c[0] - 2, c[1], 2), (c[0] - 2, c[1], size[2] - 3), radius=0.05)
+    sc.emit_thin_wires()
+    g, _, _ = gpu_grid(sc, E.KERNEL_ROWS | (zchunk << 8), ports=False, ntff=False)
+    sim = oracle_sim(sc, ports=False, ntff=False)
+    g.step(50)
+    sim.step(50)
+    _fields_match(g, sim)
+    g.close()
+
+
 def test_wide_class_ids():
     """More than 65536 coefficient classes: the 32-bit class-id path."""
     print("case: 32-bit class ids", flush=True)
@@ -160,25 +183,27 @@ def test_pml_zero_cavity_energy():
 
 
 def test_full_size_256_properties():
-    """The 256^3 sphere + thin-dipole scene at full size: the first steps match the oracle, the
-    fused and two-pass kernels agree bit for bit over 200 steps, and the fields stay finite."""
+    """The 256^3 sphere + thin-dipole scene at full size: the first steps match the oracle bit for
+    bit, and the three kernel forms agree bit for bit over 200 steps with finite fields."""
     print("case: 256^3 full size", flush=True)
     sc = spheres_dipole(n=256, pml=10)
-    g1, _, _ = gpu_grid(sc, E.KERNEL_FUSED, ports=False, ntff=False)
-    g2, _, _ = gpu_grid(sc, E.KERNEL_TWO_PASS, ports=False, ntff=False)
+    grids = [gpu_grid(sc, k, ports=False, ntff=False)[0]
+             for k in (E.KERNEL_ROWS, E.KERNEL_FUSED, E.KERNEL_TWO_PASS, E.KERNEL_POINT)]
     sim = oracle_sim(sc, ports=False, ntff=False)
-    for g in (g1, g2):
+    for g in grids:
         g.step(3)
     sim.step(3)
-    _fields_match(g1, sim)
-    g1.step(197)
-    g2.step(197)
-    e1, b1 = g1.fields(E.LAYOUT_SOA)
-    e2, b2 = g2.fields(E.LAYOUT_SOA)
-    assert np.all(np.isfinite(e1)) and np.abs(e1).max() > 0
-    assert np.array_equal(e1, e2) and np.array_equal(b1, b2)
-    g1.close()
-    g2.close()
+    Eg, Bg = grids[0].fields(E.LAYOUT_TEXEL)
+    assert np.array_equal(Eg[:, :3], sim.E[:, :3]) and np.array_equal(Bg[:, :3], sim.B[:, :3])
+    for g in grids:
+        g.step(197)
+    ref = grids[0].fields(E.LAYOUT_SOA)
+    assert np.all(np.isfinite(ref[0])) and np.abs(ref[0]).max() > 0
+    for g in grids[1:]:
+        e, b = g.fields(E.LAYOUT_SOA)
+        assert np.array_equal(e, ref[0]) and np.array_equal(b, ref[1])
+    for g in grids:
+        g.close()
 
 
 def test_abi_errors():
