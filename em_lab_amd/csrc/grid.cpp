@@ -469,7 +469,7 @@ int emfd_grid_reset_fields(emfd_ctx *ctx) {
 int emfd_grid_set_kernel(emfd_ctx *ctx, int variant) {
     if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
     const int v = variant & 0xff;
-    if (v < 0 || v > 4) return fail(EMFD_ERR_INVALID, "unknown kernel variant");
+    if (v < 0 || v > 5) return fail(EMFD_ERR_INVALID, "unknown kernel variant");
     ctx->kernel_variant = v;
     ctx->tile_cfg = variant >> 8;  // optional tile shape / z-chunk selector for the fused kernel
     return EMFD_OK;
@@ -560,11 +560,16 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
             EMFD_HIP(hipEventRecord(ev0, ctx->stream));
         }
         int variant = ctx->kernel_variant;
-        if (variant == EMFD_KERNEL_AUTO) variant = yee_rows_supported(lat) ? EMFD_KERNEL_ROWS : EMFD_KERNEL_TWO_PASS;
+        if (variant == EMFD_KERNEL_AUTO)
+            variant = yee_two_pass4_supported(lat) ? EMFD_KERNEL_TWO_PASS4 : EMFD_KERNEL_TWO_PASS;
+        if (variant == EMFD_KERNEL_TWO_PASS4 && !yee_two_pass4_supported(lat))
+            return fail(EMFD_ERR_INVALID, "the vectorised two-pass kernel needs nx % 4 == 0");
         if (variant == EMFD_KERNEL_ROWS && !yee_rows_supported(lat))
             return fail(EMFD_ERR_INVALID, "the row-wavefront kernel supports nx <= 256");
         if (variant == EMFD_KERNEL_TWO_PASS) {
             EMFD_HIP(launch_yee_two_pass(lat, ctx->fields(ctx->cur), ctx->stream));
+        } else if (variant == EMFD_KERNEL_TWO_PASS4) {
+            EMFD_HIP(launch_yee_two_pass4(lat, ctx->fields(ctx->cur), ctx->stream));
         } else if (variant == EMFD_KERNEL_POINT) {
             EMFD_HIP(launch_yee_point(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->stream));
             ctx->cur ^= 1;

@@ -67,6 +67,8 @@ hipError_t launch_yee_rows(const LatticeArgs &a, const FieldPtrs &src, const Fie
 int yee_rows_supported(const LatticeArgs &a);
 hipError_t launch_yee_point(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, hipStream_t s);
 hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
+int yee_two_pass4_supported(const LatticeArgs &a);
 hipError_t launch_apply_sources(const LatticeArgs &a, const FieldPtrs &f, const int *cells, int n_cells,
                                 const int *lists, const float *packed, const float *signals, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);

@@ -62,17 +62,84 @@ __device__ __forceinline__ void store_row(float *__restrict__ p, int xb, int nx,
     }
 }
 
-template <int VPL>
+template <int VPL, bool FULL>
 __device__ __forceinline__ void load_ids(const LatticeArgs &a, size_t row, int xb, int nx, uint32_t (&o)[VPL]) {
+    if (!a.mat_id) {
 #pragma unroll
-    for (int j = 0; j < VPL; ++j) o[j] = (a.mat_id && xb + j < nx) ? load_id(a, row + xb + j) : 0u;
+        for (int j = 0; j < VPL; ++j) o[j] = 0u;
+        return;
+    }
+    if constexpr (FULL && VPL == 4) {
+        if (a.id_bytes == 2) {
+            const uint2 v = *reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(a.mat_id) + row + xb);
+            o[0] = v.x & 0xffffu; o[1] = v.x >> 16; o[2] = v.y & 0xffffu; o[3] = v.y >> 16;
+            return;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < VPL; ++j) o[j] = xb + j < nx ? load_id(a, row + xb + j) : 0u;
 }
 
+// Everything one plane iteration reads, loaded one iteration ahead (software pipelining: the
+// next plane's loads are in flight while the current plane is computed).
+template <int VPL>
+struct PlaneIn {
+    float eN[3][VPL];  // E(y, z+1)
+    float eU[2][VPL];  // E(y+1, z): x, z components
+    float eD[3][VPL];  // E(y-1, z)
+    float eDn[VPL];    // E(y-1, z+1): y component
+    float bD[2][VPL];  // B(y-1, z): x, z components
+    float bO[3][VPL];  // B(y, z)
+    uint32_t id[VPL], idD[VPL];
+};
+
 template <int VPL, bool FULL>
+__device__ __forceinline__ void load_plane(const LatticeArgs &a, const FieldPtrs &src, int y, int z, int xb,
+                                           size_t sy, size_t sz, bool has_up, bool has_dn, PlaneIn<VPL> &L) {
+    const int nx = a.nx, nz = a.nz;
+    const size_t row = z * sz + y * sy;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (z + 1 < nz) load_row<VPL, FULL>(src.e[c] + row + sz, xb, nx, L.eN[c]);
+        else
+#pragma unroll
+            for (int j = 0; j < VPL; ++j) L.eN[c][j] = 0.0f;
+        load_row<VPL, FULL>(src.b[c] + row, xb, nx, L.bO[c]);
+    }
+    if (has_up) {
+        load_row<VPL, FULL>(src.e[0] + row + sy, xb, nx, L.eU[0]);
+        load_row<VPL, FULL>(src.e[2] + row + sy, xb, nx, L.eU[1]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) L.eU[0][j] = L.eU[1][j] = 0.0f;
+    }
+    if (has_dn) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) load_row<VPL, FULL>(src.e[c] + row - sy, xb, nx, L.eD[c]);
+        if (z + 1 < nz) load_row<VPL, FULL>(src.e[1] + row - sy + sz, xb, nx, L.eDn);
+        else
+#pragma unroll
+            for (int j = 0; j < VPL; ++j) L.eDn[j] = 0.0f;
+        load_row<VPL, FULL>(src.b[0] + row - sy, xb, nx, L.bD[0]);
+        load_row<VPL, FULL>(src.b[2] + row - sy, xb, nx, L.bD[1]);
+        load_ids<VPL, FULL>(a, row - sy, xb, nx, L.idD);
+    } else {
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) {
+            L.eD[0][j] = L.eD[1][j] = L.eD[2][j] = L.eDn[j] = L.bD[0][j] = L.bD[1][j] = 0.0f;
+            L.idD[j] = 0u;
+        }
+    }
+    load_ids<VPL, FULL>(a, row, xb, nx, L.id);
+}
+
+template <int VPL, bool FULL, bool PIPE>
 __global__ void __launch_bounds__(64 * kWavesPerBlock) k_yee_rows(LatticeArgs a, FieldPtrs src, FieldPtrs dst,
                                                                     int zchunk, int n_tiles) {
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    // the wave index is uniform across the wave: say so, so that the tile, row and plane indices
+    // (and every branch and address base derived from them) live in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     const int per = static_cast<int>(gridDim.x) / kXcds;  // gridDim.x is a multiple of kXcds
     const int blk = (static_cast<int>(blockIdx.x) % kXcds) * per + static_cast<int>(blockIdx.x) / kXcds;
     const int tile = blk * kWavesPerBlock + wave;
@@ -93,48 +160,20 @@ __global__ void __launch_bounds__(64 * kWavesPerBlock) k_yee_rows(LatticeArgs a,
 #pragma unroll
     for (int j = 0; j < VPL; ++j) bprev[0][j] = bprev[1][j] = 0.0f;
 
+    PlaneIn<VPL> cur;
+    if constexpr (PIPE) load_plane<VPL, FULL>(a, src, y, zs, xb, sy, sz, has_up, has_dn, cur);
     for (int z = zs; z < z1; ++z) {
         const size_t row = z * sz + y * sy;
-        float eN[3][VPL], eU[2][VPL], eD[3][VPL], eDn[VPL], bD[2][VPL], bO[3][VPL];
-        uint32_t id[VPL], idD[VPL];
-        // ---- loads
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            if (z + 1 < nz) load_row<VPL, FULL>(src.e[c] + row + sz, xb, nx, eN[c]);
-            else
-#pragma unroll
-                for (int j = 0; j < VPL; ++j) eN[c][j] = 0.0f;
-            load_row<VPL, FULL>(src.b[c] + row, xb, nx, bO[c]);
-        }
-        if (has_up) {
-            load_row<VPL, FULL>(src.e[0] + row + sy, xb, nx, eU[0]);
-            load_row<VPL, FULL>(src.e[2] + row + sy, xb, nx, eU[1]);
+        PlaneIn<VPL> nxt;
+        if constexpr (PIPE) {
+            if (z + 1 < z1) load_plane<VPL, FULL>(a, src, y, z + 1, xb, sy, sz, has_up, has_dn, nxt);
         } else {
-#pragma unroll
-            for (int j = 0; j < VPL; ++j) eU[0][j] = eU[1][j] = 0.0f;
+            load_plane<VPL, FULL>(a, src, y, z, xb, sy, sz, has_up, has_dn, cur);
         }
-        if (has_dn) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) load_row<VPL, FULL>(src.e[c] + row - sy, xb, nx, eD[c]);
-            if (z + 1 < nz) load_row<VPL, FULL>(src.e[1] + row - sy + sz, xb, nx, eDn);
-            else
-#pragma unroll
-                for (int j = 0; j < VPL; ++j) eDn[j] = 0.0f;
-            load_row<VPL, FULL>(src.b[0] + row - sy, xb, nx, bD[0]);
-            load_row<VPL, FULL>(src.b[2] + row - sy, xb, nx, bD[1]);
-            load_ids<VPL>(a, row - sy, xb, nx, idD);
-        } else {
-#pragma unroll
-            for (int j = 0; j < VPL; ++j) {
-                eD[0][j] = eD[1][j] = eD[2][j] = eDn[j] = bD[0][j] = bD[1][j] = 0.0f;
-                idD[j] = 0u;
-            }
-        }
-        load_ids<VPL>(a, row, xb, nx, id);
 
         // ---- x+1 neighbours: inside the lane, or element 0 of the next lane
         const float nxt_ey = from_next_lane(eC[1][0]), nxt_ez = from_next_lane(eC[2][0]);
-        const float nxt_dy = from_next_lane(eD[1][0]);
+        const float nxt_dy = from_next_lane(cur.eD[1][0]);
 
         // ---- B^{n+1/2}(y, z)
         float bN[3][VPL];
@@ -142,10 +181,10 @@ __global__ void __launch_bounds__(64 * kWavesPerBlock) k_yee_rows(LatticeArgs a,
         for (int j = 0; j < VPL; ++j) {
             const float e0[3] = {eC[0][j], eC[1][j], eC[2][j]};
             const float exn[3] = {0.0f, j + 1 < VPL ? eC[1][j + 1] : nxt_ey, j + 1 < VPL ? eC[2][j + 1] : nxt_ez};
-            const float eyn[3] = {eU[0][j], 0.0f, eU[1][j]};
-            const float ezn[3] = {eN[0][j], eN[1][j], 0.0f};
-            float b[3] = {bO[0][j], bO[1][j], bO[2][j]};
-            if (xb + j < nx) faraday(a, xb + j, y, z, e0, exn, eyn, ezn, id[j], b);
+            const float eyn[3] = {cur.eU[0][j], 0.0f, cur.eU[1][j]};
+            const float ezn[3] = {cur.eN[0][j], cur.eN[1][j], 0.0f};
+            float b[3] = {cur.bO[0][j], cur.bO[1][j], cur.bO[2][j]};
+            if (xb + j < nx) faraday(a, xb + j, y, z, e0, exn, eyn, ezn, cur.id[j], b);
             else b[0] = b[1] = b[2] = 0.0f;
             bN[0][j] = b[0];
             bN[1][j] = b[1];
@@ -155,12 +194,12 @@ __global__ void __launch_bounds__(64 * kWavesPerBlock) k_yee_rows(LatticeArgs a,
         float bDN[2][VPL];
 #pragma unroll
         for (int j = 0; j < VPL; ++j) {
-            const float e0[3] = {eD[0][j], eD[1][j], eD[2][j]};
-            const float exn[3] = {0.0f, j + 1 < VPL ? eD[1][j + 1] : nxt_dy, 0.0f};
+            const float e0[3] = {cur.eD[0][j], cur.eD[1][j], cur.eD[2][j]};
+            const float exn[3] = {0.0f, j + 1 < VPL ? cur.eD[1][j + 1] : nxt_dy, 0.0f};
             const float eyn[3] = {eC[0][j], 0.0f, eC[2][j]};
-            const float ezn[3] = {0.0f, eDn[j], 0.0f};
-            float b[3] = {bD[0][j], 0.0f, bD[1][j]};
-            if (has_dn && xb + j < nx) faraday(a, xb + j, y - 1, z, e0, exn, eyn, ezn, idD[j], b);
+            const float ezn[3] = {0.0f, cur.eDn[j], 0.0f};
+            float b[3] = {cur.bD[0][j], 0.0f, cur.bD[1][j]};
+            if (has_dn && xb + j < nx) faraday(a, xb + j, y - 1, z, e0, exn, eyn, ezn, cur.idD[j], b);
             else b[0] = b[2] = 0.0f;
             bDN[0][j] = b[0];
             bDN[1][j] = b[2];
@@ -179,7 +218,7 @@ __global__ void __launch_bounds__(64 * kWavesPerBlock) k_yee_rows(LatticeArgs a,
                 const float curl[3] = {(b0z - bym_z) - (b0y - bzm_y), (b0x - bzm_x) - (b0z - bxm_z),
                                        (b0y - bxm_y) - (b0x - bym_x)};
                 float ca[3], cb[3];
-                edge_coefficients(a, min(xb + j, nx - 1), y, z, id[j], ca, cb);
+                edge_coefficients(a, min(xb + j, nx - 1), y, z, cur.id[j], ca, cb);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) eO[c][j] = ca[c] * eC[c][j] + cb[c] * curl[c];
             }
@@ -194,19 +233,23 @@ __global__ void __launch_bounds__(64 * kWavesPerBlock) k_yee_rows(LatticeArgs a,
             bprev[0][j] = bN[0][j];
             bprev[1][j] = bN[1][j];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) eC[c][j] = eN[c][j];
+            for (int c = 0; c < 3; ++c) eC[c][j] = cur.eN[c][j];
         }
+        if constexpr (PIPE) cur = nxt;
     }
 }
 
 template <int VPL, bool FULL>
-hipError_t launch_rows_cfg(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int zchunk,
+hipError_t launch_rows_cfg(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int zchunk, bool pipe,
                            hipStream_t s) {
     const int nchunks = (a.nz + zchunk - 1) / zchunk;
     const int n_tiles = a.ny * nchunks;
     int blocks = (n_tiles + kWavesPerBlock - 1) / kWavesPerBlock;
     blocks = (blocks + kXcds - 1) / kXcds * kXcds;
-    k_yee_rows<VPL, FULL><<<blocks, 64 * kWavesPerBlock, 0, s>>>(a, src, dst, zchunk, n_tiles);
+    if (pipe)
+        k_yee_rows<VPL, FULL, true><<<blocks, 64 * kWavesPerBlock, 0, s>>>(a, src, dst, zchunk, n_tiles);
+    else
+        k_yee_rows<VPL, FULL, false><<<blocks, 64 * kWavesPerBlock, 0, s>>>(a, src, dst, zchunk, n_tiles);
     return hipGetLastError();
 }
 
@@ -214,10 +257,13 @@ hipError_t launch_rows_cfg(const LatticeArgs &a, const FieldPtrs &src, const Fie
 
 int yee_rows_supported(const LatticeArgs &a) { return a.nx <= 256 ? 1 : 0; }
 
-// zchunk 0 = automatic: about 2048 row-waves in flight (8 per CU over 256 CUs), at least 8 planes
-hipError_t launch_yee_rows(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int zchunk,
+// param: bits 0..15 z-chunk (0 = automatic: about 2048 row-waves in flight, 8 per CU over 256 CUs,
+// at least 8 planes), bit 16 = software-pipelined plane loads
+hipError_t launch_yee_rows(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int param,
                            hipStream_t s) {
     if (!yee_rows_supported(a)) return hipErrorInvalidValue;
+    int zchunk = param & 0xffff;
+    const bool pipe = (param >> 16) & 1;
     if (zchunk <= 0) {
         int nch = (2048 + a.ny - 1) / a.ny;
         nch = std::max(1, std::min(nch, a.nz / 8));
@@ -226,13 +272,13 @@ hipError_t launch_yee_rows(const LatticeArgs &a, const FieldPtrs &src, const Fie
     const int vpl = (a.nx + 63) / 64;
     switch (vpl) {
     case 1:
-        return a.nx == 64 ? launch_rows_cfg<1, true>(a, src, dst, zchunk, s) : launch_rows_cfg<1, false>(a, src, dst, zchunk, s);
+        return a.nx == 64 ? launch_rows_cfg<1, true>(a, src, dst, zchunk, pipe, s) : launch_rows_cfg<1, false>(a, src, dst, zchunk, pipe, s);
     case 2:
-        return a.nx == 128 ? launch_rows_cfg<2, true>(a, src, dst, zchunk, s) : launch_rows_cfg<2, false>(a, src, dst, zchunk, s);
+        return a.nx == 128 ? launch_rows_cfg<2, true>(a, src, dst, zchunk, pipe, s) : launch_rows_cfg<2, false>(a, src, dst, zchunk, pipe, s);
     case 3:
-        return launch_rows_cfg<3, false>(a, src, dst, zchunk, s);
+        return launch_rows_cfg<3, false>(a, src, dst, zchunk, pipe, s);
     default:
-        return a.nx == 256 ? launch_rows_cfg<4, true>(a, src, dst, zchunk, s) : launch_rows_cfg<4, false>(a, src, dst, zchunk, s);
+        return a.nx == 256 ? launch_rows_cfg<4, true>(a, src, dst, zchunk, pipe, s) : launch_rows_cfg<4, false>(a, src, dst, zchunk, pipe, s);
     }
 }
 

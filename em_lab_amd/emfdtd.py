@@ -26,6 +26,7 @@ KERNEL_FUSED = 1
 KERNEL_TWO_PASS = 2
 KERNEL_ROWS = 3
 KERNEL_POINT = 4
+KERNEL_TWO_PASS4 = 5
 
 
 class EmfdError(RuntimeError):

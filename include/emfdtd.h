@@ -46,6 +46,7 @@ extern "C" {
 #define EMFD_KERNEL_TWO_PASS 2 /* separate update_B / update_E passes, in place */
 #define EMFD_KERNEL_ROWS 3     /* fused B+E, one wavefront per x-row z-march (nx <= 256) */
 #define EMFD_KERNEL_POINT 4    /* fused B+E, one thread per texel, neighbour B rebuilt */
+#define EMFD_KERNEL_TWO_PASS4 5 /* two passes, four cells per thread with float4 streams (nx % 4 == 0) */
 
 typedef struct emfd_ctx emfd_ctx;
 

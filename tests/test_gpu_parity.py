@@ -27,7 +27,8 @@ def _fields_match(g, sim, tol=FIELD_TOL):
     return re, rb
 
 
-@pytest.mark.parametrize("kernel", [E.KERNEL_FUSED, E.KERNEL_TWO_PASS, E.KERNEL_ROWS, E.KERNEL_POINT])
+@pytest.mark.parametrize("kernel", [E.KERNEL_FUSED, E.KERNEL_TWO_PASS, E.KERNEL_ROWS, E.KERNEL_POINT,
+                                    E.KERNEL_TWO_PASS4])
 def test_hertzian_vacuum_64(kernel):
     """Plain FDTD 64^3 vacuum, 8-cell absorber, Courant 0.99/sqrt(3), z Hertzian pulse at the
     centre (lambda 16), 400 steps: fields, far-field phasors and the directivity pattern."""
@@ -141,6 +142,11 @@ def test_rows_kernel_odd_grids(size, zchunk):
     sim.step(50)
     _fields_match(g, sim)
     g.close()
+    if size[0] % 4 == 0:
+        g4, _, _ = gpu_grid(sc, E.KERNEL_TWO_PASS4, ports=False, ntff=False)
+        g4.step(50)
+        _fields_match(g4, sim)
+        g4.close()
 
 
 def test_wide_class_ids():
@@ -188,7 +194,8 @@ def test_full_size_256_properties():
     print("case: 256^3 full size", flush=True)
     sc = spheres_dipole(n=256, pml=10)
     grids = [gpu_grid(sc, k, ports=False, ntff=False)[0]
-             for k in (E.KERNEL_ROWS, E.KERNEL_FUSED, E.KERNEL_TWO_PASS, E.KERNEL_POINT)]
+             for k in (E.KERNEL_TWO_PASS4, E.KERNEL_ROWS, E.KERNEL_FUSED, E.KERNEL_TWO_PASS,
+                       E.KERNEL_POINT)]
     sim = oracle_sim(sc, ports=False, ntff=False)
     for g in grids:
         g.step(3)
