@@ -6,8 +6,9 @@ source injection, port V/I accumulation with series recording, and the near-to-f
 phasor accumulation on the Huygens box. Prints one JSON line.
 
 value    = whole-job Mcell-updates/s (cells x timed steps / wall time between synchronisations)
-roofline = the fused Yee kernel: algorithmic 48 B per cell-update (E and B read once, written
-           once, FP32) / its average launch time from HIP events on the library's stream
+roofline = the Yee step (default: the two vectorised passes k_update_b4 + k_update_e4, timed
+           together): algorithmic 48 B per cell-update (E and B read once, written once, FP32) /
+           its average time from HIP events on the library's stream
 cpu_baseline = the CPU oracle (scalar C, OpenMP over z-planes) on a bounded number of steps
 """
 from __future__ import annotations
@@ -65,6 +66,8 @@ def main() -> int:
     ap.add_argument("--scene", choices=["spheres", "vacuum"], default="spheres",
                     help="spheres: the headline workload; vacuum: same grid, no materials (roofline check)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="only time a plain float4 device copy of MB megabytes (roofline calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -73,6 +76,15 @@ def main() -> int:
 
     from em_lab_amd import emfdtd as E
     from em_lab_amd.scene import hertzian_vacuum, spheres_dipole
+
+    if args.copy_probe:
+        g = E.SimulationGrid(0)
+        nbytes = args.copy_probe * 1000 * 1000
+        ms = g.copy_probe(nbytes, max(args.steps, 1))
+        print(json.dumps({"copy_probe_bytes": nbytes, "avg_ms": round(ms, 5),
+                          "read_plus_write_GBs": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}), flush=True)
+        g.close()
+        return 0
 
     if args.scene == "spheres":
         scene = spheres_dipole(n=args.size, pml=10)
@@ -111,10 +123,14 @@ def main() -> int:
     value = cells * args.steps / wall / 1e6
     avg_ms = yee_ms / max(launches, 1)
     achieved = BYTES_PER_CELL * cells / (avg_ms * 1e-3) / 1e9
+    # HBM bytes per Yee step from the calibrated rocprofv3 counter runs of this workload
+    # (profiles/yee_256_traffic.md); only quoted for the default kernel at 256^3
     traffic = None
-    tf = ROOT / "profiles" / "yee_fused_256_traffic.json"
-    if tf.exists():
+    tf = ROOT / "profiles" / "yee_256_traffic.json"
+    if tf.exists() and args.size == 256 and args.kernel in (0, 5) and args.scene == "spheres":
         traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+    kernel_names = {0: "k_update_b4+k_update_e4", 1: "k_yee_fused", 2: "k_update_b+k_update_e", 3: "k_yee_rows",
+                    4: "k_yee_point", 5: "k_update_b4+k_update_e4"}
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -137,7 +153,8 @@ def main() -> int:
                    "kernel": args.kernel},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_yee_fused", "avg_launch_ms": round(avg_ms, 5), "launches": launches,
+                     "kernel": kernel_names.get(args.kernel & 0xff, "?"), "avg_launch_ms": round(avg_ms, 5),
+                     "launches": launches,
                      "bytes_per_cell": BYTES_PER_CELL},
         "cpu_baseline": None,
     }

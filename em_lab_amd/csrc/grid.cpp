@@ -667,6 +667,29 @@ int emfd_grid_upload_fields(emfd_ctx *ctx, const float *E, const float *B, int l
     return EMFD_OK;
 }
 
+int emfd_copy_probe(emfd_ctx *ctx, int64_t bytes, int iterations, double *avg_ms) {
+    if (!ctx || bytes < 16 || iterations < 1) return fail(EMFD_ERR_INVALID, "emfd_copy_probe: bad arguments");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    DevBuf a, b;
+    if (int rc = a.ensure(static_cast<size_t>(bytes))) return rc;
+    if (int rc = b.ensure(static_cast<size_t>(bytes))) return rc;
+    EMFD_HIP(launch_fill(a.as<float>(), static_cast<size_t>(bytes) / 4, 1.0f, ctx->stream));
+    EMFD_HIP(launch_copy4(a.ptr, b.ptr, static_cast<size_t>(bytes), ctx->stream));  // warm-up
+    hipEvent_t e0, e1;
+    EMFD_HIP(hipEventCreate(&e0));
+    EMFD_HIP(hipEventCreate(&e1));
+    EMFD_HIP(hipEventRecord(e0, ctx->stream));
+    for (int i = 0; i < iterations; ++i) EMFD_HIP(launch_copy4(a.ptr, b.ptr, static_cast<size_t>(bytes), ctx->stream));
+    EMFD_HIP(hipEventRecord(e1, ctx->stream));
+    EMFD_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    EMFD_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (avg_ms) *avg_ms = ms / iterations;
+    return EMFD_OK;
+}
+
 int64_t emfd_grid_memory_bytes(emfd_ctx *ctx) {
     if (!ctx) return -1;
     int64_t total = 0;

@@ -72,6 +72,7 @@ int yee_two_pass4_supported(const LatticeArgs &a);
 hipError_t launch_apply_sources(const LatticeArgs &a, const FieldPtrs &f, const int *cells, int n_cells,
                                 const int *lists, const float *packed, const float *signals, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
+hipError_t launch_copy4(const void *src, void *dst, size_t bytes, hipStream_t s);
 hipError_t launch_build_pml_table(const LatticeArgs &a, float *tab, hipStream_t s);
 
 struct PortDev {

@@ -83,6 +83,13 @@ __global__ void k_build_pml_table(LatticeArgs a, float *tab) {
     t[12] = t[13] = t[14] = t[15] = 0.0f;
 }
 
+// Plain float4 copy: the bandwidth and counter calibration reference for the roofline figures.
+__global__ void __launch_bounds__(256) k_copy4(const float4 *__restrict__ a, float4 *__restrict__ b, size_t n) {
+    size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) b[i] = a[i];
+}
+
 __global__ void k_fill(float *p, size_t n, float v) {
     size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
@@ -428,6 +435,12 @@ hipError_t launch_build_table(const uint16_t *keys, int classes, float dt, float
 hipError_t launch_build_pml_table(const LatticeArgs &a, float *tab, hipStream_t s) {
     const int n = a.pml_k * a.pml_k * a.pml_k;
     k_build_pml_table<<<(n + 127) / 128, 128, 0, s>>>(a, tab);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy4(const void *src, void *dst, size_t bytes, hipStream_t s) {
+    const size_t n = bytes / 16;
+    k_copy4<<<16384, 256, 0, s>>>(static_cast<const float4 *>(src), static_cast<float4 *>(dst), n);
     return hipGetLastError();
 }
 

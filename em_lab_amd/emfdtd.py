@@ -134,6 +134,7 @@ _SIGNATURES = {
                                      _P]),
     "emfd_flux_measure": (C.c_int, [_P, _P, C.c_int, _P]),
     "emfd_probe_measure": (C.c_int, [_P, _P, C.c_int, _P]),
+    "emfd_copy_probe": (C.c_int, [_P, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
 }
 
 
@@ -408,6 +409,12 @@ class SimulationGrid:
     @property
     def step_count(self) -> int:
         return self._state()[1]
+
+    def copy_probe(self, nbytes: int, iterations: int = 20) -> float:
+        """Mean ms of a plain float4 device copy of nbytes (bandwidth / counter calibration)."""
+        ms = C.c_double()
+        _check(self._lib.emfd_copy_probe(self._h, int(nbytes), int(iterations), C.byref(ms)))
+        return ms.value
 
     def gpu_memory_bytes(self) -> int:
         return int(self._lib.emfd_grid_memory_bytes(self._h))

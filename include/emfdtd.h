@@ -283,6 +283,13 @@ int emfd_flux_measure(emfd_ctx *ctx, const emfd_flux_surface *surfaces, int n, d
  * positions: 3 floats each. */
 int emfd_probe_measure(emfd_ctx *ctx, const float *positions, int n, emfd_probe_sample *out);
 
+/* ---------------------------------------------------------------------------------------
+ * Measurement helper (no counterpart in the reference: it calibrates the roofline figures)
+ * --------------------------------------------------------------------------------------- */
+/* Copies `bytes` (rounded down to 16) between two scratch buffers `iterations` times on the
+ * context stream with a plain float4 kernel; *avg_ms = mean time per copy from HIP events. */
+int emfd_copy_probe(emfd_ctx *ctx, int64_t bytes, int iterations, double *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif
