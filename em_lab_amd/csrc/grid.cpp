@@ -5,6 +5,7 @@
 // the GPU state and launch sequence are this library's own design.
 #include "context.h"
 #include "expf.h"
+#include "host_common.h"
 
 #include <algorithm>
 #include <array>
@@ -86,6 +87,181 @@ double source_signal(const emfd_source_desc &s, double tAbs) {
     return std::exp(-u * u) * std::sin(w * (tl - t0) + s.phase);
 }
 
+
+emfd_grid_config clamp_config(const emfd_grid_config &requested) {
+    emfd_grid_config c = requested;
+    c.pml_thickness = std::clamp(c.pml_thickness, 0, 64);
+    const int min_size = 2 * c.pml_thickness + 8;
+    for (int a = 0; a < 3; ++a) c.size[a] = std::max(c.size[a], min_size);
+    c.courant = std::clamp(c.courant, 0.05f, kMaxCourant);
+    c.pml_sigma_max = std::max(c.pml_sigma_max, 0.0f);
+    c.pml_grading = std::clamp(c.pml_grading, 0.25f, 12.0f);
+    return c;
+}
+
+std::vector<float> absorber_profiles(const emfd_grid_config &c, int *stride_out) {
+    // evaluated in float exactly as the shader does
+    const int stride = std::max({c.size[0], c.size[1], c.size[2]});
+    std::vector<float> prof(static_cast<size_t>(6) * stride, 0.0f);
+    const float L = static_cast<float>(c.pml_thickness);
+    auto profile = [&](float depth) -> float {
+        if (depth <= 0.0f) return 0.0f;
+        const float u = std::min(depth / static_cast<float>(c.pml_thickness), 1.0f);
+        return c.pml_sigma_max * (det_expf(c.pml_grading * u) - 1.0f) / (det_expf(c.pml_grading) - 1.0f);
+    };
+    for (int a = 0; a < 3; ++a) {
+        const float hi = static_cast<float>(c.size[a] - 1) - L;
+        for (int h = 0; h < 2; ++h)
+            for (int i = 0; i < c.size[a]; ++i) {
+                const float x = static_cast<float>(i) + (h ? 0.5f : 0.0f);
+                prof[static_cast<size_t>(2 * a + h) * stride + i] = profile(std::max(L - x, x - hi));
+            }
+    }
+    *stride_out = stride;
+    return prof;
+}
+
+ClassKey ClassMap::vacuum() {
+    const uint16_t one = f32_to_f16(1.0f);
+    return {0, 0, 0, one, one, one, one, one, one, one, one, one};
+}
+
+ClassMap::ClassMap() {
+    map.emplace(vacuum(), 0u);
+    keys.push_back(vacuum());
+}
+
+void classify_cells(const emfd_grid_config &cfg, const float *sigma, const float *eps_r,
+                    const emfd_edge_override *edges, int64_t n_edges, const emfd_wire_coefficient *wires,
+                    int64_t n_wires, ClassMap &cm, uint32_t *ids) {
+    const int nx = cfg.size[0], ny = cfg.size[1], nz = cfg.size[2];
+    // sub-cell overrides sorted by texel, keeping their order (later entries win, as the
+    // reference's sequential loop at src/SimulationGrid.cpp:251-254)
+    struct EdgeOv { size_t idx; int axis; float sigma; };
+    std::vector<EdgeOv> ov;
+    for (int64_t k = 0; k < n_edges; ++k) {
+        const emfd_edge_override &o = edges[k];
+        if (o.texel[0] < 0 || o.texel[1] < 0 || o.texel[2] < 0 || o.texel[0] >= nx || o.texel[1] >= ny ||
+            o.texel[2] >= nz || o.axis < 0 || o.axis > 2)
+            continue;
+        ov.push_back({(static_cast<size_t>(o.texel[2]) * ny + o.texel[1]) * nx + o.texel[0], o.axis,
+                      std::max(o.sigma, 0.0f)});
+    }
+    std::stable_sort(ov.begin(), ov.end(), [](const EdgeOv &a, const EdgeOv &b) { return a.idx < b.idx; });
+    struct WireOv { size_t idx; int slot; float factor; };
+    std::vector<WireOv> wv;
+    for (int64_t k = 0; k < n_wires; ++k) {
+        const emfd_wire_coefficient &w = wires[k];
+        if (w.texel[0] < 0 || w.texel[1] < 0 || w.texel[2] < 0 || w.texel[0] >= nx || w.texel[1] >= ny ||
+            w.texel[2] >= nz || w.component < 0 || w.component > 2)
+            continue;
+        wv.push_back({(static_cast<size_t>(w.texel[2]) * ny + w.texel[1]) * nx + w.texel[0],
+                      2 * w.component + (w.term == 0 ? 0 : 1), w.factor});
+    }
+    std::stable_sort(wv.begin(), wv.end(), [](const WireOv &a, const WireOv &b) { return a.idx < b.idx; });
+    // edge averages (src/SimulationGrid.cpp:224-248), quantised to binary16, classified
+    auto idx = [&](int x, int y, int z) {
+        x = std::max(x, 0);
+        y = std::max(y, 0);
+        z = std::max(z, 0);
+        return (static_cast<size_t>(z) * ny + y) * nx + x;
+    };
+    size_t oi = 0, wi = 0;
+    ClassKey last = ClassMap::vacuum();
+    uint32_t last_id = cm.map.at(last);
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const size_t c00 = idx(x, y, z);
+                const size_t ex[4] = {c00, idx(x, y - 1, z), idx(x, y, z - 1), idx(x, y - 1, z - 1)};
+                const size_t ey[4] = {c00, idx(x - 1, y, z), idx(x, y, z - 1), idx(x - 1, y, z - 1)};
+                const size_t ez[4] = {c00, idx(x - 1, y, z), idx(x, y - 1, z), idx(x - 1, y - 1, z)};
+                auto avg = [](const float *v, const size_t (&c)[4]) {
+                    return 0.25f * (v[c[0]] + v[c[1]] + v[c[2]] + v[c[3]]);
+                };
+                float es[3] = {avg(sigma, ex), avg(sigma, ey), avg(sigma, ez)};
+                const float ee[3] = {avg(eps_r, ex), avg(eps_r, ey), avg(eps_r, ez)};
+                for (; oi < ov.size() && ov[oi].idx == c00; ++oi) es[ov[oi].axis] = ov[oi].sigma;
+                float wf[6] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f};
+                for (; wi < wv.size() && wv[wi].idx == c00; ++wi) wf[wv[wi].slot] = wv[wi].factor;
+                ClassKey k;
+                for (int c = 0; c < 3; ++c) {
+                    k[c] = f32_to_f16(std::min(es[c], kPerfectEdgeSigma));
+                    k[3 + c] = f32_to_f16(ee[c]);
+                }
+                for (int j = 0; j < 6; ++j) k[6 + j] = f32_to_f16(wf[j]);
+                uint32_t id;
+                if (k == last) {
+                    id = last_id;
+                } else {
+                    auto it = cm.map.find(k);
+                    if (it == cm.map.end()) {
+                        id = static_cast<uint32_t>(cm.keys.size());
+                        cm.map.emplace(k, id);
+                        cm.keys.push_back(k);
+                    } else {
+                        id = it->second;
+                    }
+                    last = k;
+                    last_id = id;
+                }
+                ids[c00] = id;
+            }
+}
+
+PackedSources pack_sources(const emfd_grid_config &cfg, const emfd_source_desc *sources, int n) {
+    PackedSources ps;
+    // packing (src/SimulationGrid.cpp:297-318)
+    for (int i = 0; i < n && static_cast<int>(ps.kept.size()) < kMaxSources; ++i) {
+        const emfd_source_desc &s = sources[i];
+        const float len = std::sqrt(s.polarization[0] * s.polarization[0] + s.polarization[1] * s.polarization[1] +
+                                    s.polarization[2] * s.polarization[2]);
+        if (!s.enabled || !(len > 1e-6f) || !std::isfinite(len)) continue;
+        float g[16];
+        for (int a = 0; a < 3; ++a) {
+            g[a] = s.center[a];
+            g[4 + a] = std::clamp(s.half_extent[a], 0.0f, 1.0e5f);
+            g[8 + a] = s.polarization[a] / len;
+            g[12 + a] = std::max(s.softness[a], 0.25f);
+        }
+        g[3] = g[7] = g[15] = 0.0f;
+        g[11] = s.amplitude;
+        ps.packed.insert(ps.packed.end(), g, g + 16);
+        ps.kept.push_back(s);
+    }
+    // per-texel source lists (src/SimulationGrid.cpp:320-349)
+    const int nx = cfg.size[0], ny = cfg.size[1], nz = cfg.size[2];
+    const int dims[3] = {nx, ny, nz};
+    std::vector<std::pair<uint32_t, uint32_t>> pairs;
+    for (size_t i = 0; i < ps.kept.size(); ++i) {
+        const float *g = &ps.packed[16 * i];
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::clamp(static_cast<int>(std::floor(g[a] - g[4 + a] - 3.5f * g[12 + a])) - 1, 0, dims[a] - 1);
+            hi[a] = std::clamp(static_cast<int>(std::ceil(g[a] + g[4 + a] + 3.5f * g[12 + a])), 0, dims[a] - 1);
+        }
+        for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int x = lo[0]; x <= hi[0]; ++x)
+                    pairs.emplace_back(static_cast<uint32_t>((z * ny + y) * nx + x), static_cast<uint32_t>(i));
+    }
+    std::sort(pairs.begin(), pairs.end());
+    for (size_t k = 0; k < pairs.size(); ++k) {
+        if (k == 0 || pairs[k].first != pairs[k - 1].first) {
+            const uint32_t id = pairs[k].first;
+            ps.cells.push_back(static_cast<int>(id % static_cast<uint32_t>(nx)));
+            ps.cells.push_back(static_cast<int>((id / static_cast<uint32_t>(nx)) % static_cast<uint32_t>(ny)));
+            ps.cells.push_back(static_cast<int>(id / static_cast<uint32_t>(nx * ny)));
+            ps.cells.push_back(static_cast<int>(ps.lists.size()));
+        }
+        ps.lists.push_back(static_cast<int>(pairs[k].second));
+    }
+    ps.n_cells = static_cast<int>(ps.cells.size() / 4);
+    ps.cells.insert(ps.cells.end(), {0, 0, 0, static_cast<int>(ps.lists.size())});
+    if (ps.lists.empty()) ps.lists.push_back(0);
+    if (ps.packed.empty()) ps.packed.assign(16, 0.0f);
+    return ps;
+}
 }  // namespace emfd
 
 using namespace emfd;
@@ -169,14 +345,7 @@ int emfd_grid_configure(emfd_ctx *ctx, const emfd_grid_config *requested, emfd_g
     if (!ctx || !requested) return fail(EMFD_ERR_INVALID, "emfd_grid_configure: null argument");
     EMFD_HIP(hipSetDevice(ctx->device));
     EMFD_HIP(hipStreamSynchronize(ctx->stream));  // buffers may be reallocated below
-    // SimulationGrid::configure clamps (src/SimulationGrid.cpp:150-156)
-    emfd_grid_config c = *requested;
-    c.pml_thickness = std::clamp(c.pml_thickness, 0, 64);
-    const int min_size = 2 * c.pml_thickness + 8;
-    for (int a = 0; a < 3; ++a) c.size[a] = std::max(c.size[a], min_size);
-    c.courant = std::clamp(c.courant, 0.05f, kMaxCourant);
-    c.pml_sigma_max = std::max(c.pml_sigma_max, 0.0f);
-    c.pml_grading = std::clamp(c.pml_grading, 0.25f, 12.0f);
+    const emfd_grid_config c = clamp_config(*requested);
     // The reference rejects grids beyond GL_MAX_3D_TEXTURE_SIZE; here the limit is the 32-bit
     // plane index used by the kernels for one component plane.
     const size_t n = static_cast<size_t>(c.size[0]) * c.size[1] * c.size[2];
@@ -195,24 +364,8 @@ int emfd_grid_configure(emfd_ctx *ctx, const emfd_grid_config *requested, emfd_g
     ctx->configured = true;
     ctx->cur = 0;
 
-    // absorber profiles pmlProfile(depth) per axis at i and i + 1/2 (shaders/common.glsl:23-36),
-    // evaluated in float exactly as the shader does
-    const int stride = std::max({c.size[0], c.size[1], c.size[2]});
-    std::vector<float> prof(static_cast<size_t>(6) * stride, 0.0f);
-    const float L = static_cast<float>(c.pml_thickness);
-    auto profile = [&](float depth) -> float {
-        if (depth <= 0.0f) return 0.0f;
-        const float u = std::min(depth / static_cast<float>(c.pml_thickness), 1.0f);
-        return c.pml_sigma_max * (det_expf(c.pml_grading * u) - 1.0f) / (det_expf(c.pml_grading) - 1.0f);
-    };
-    for (int a = 0; a < 3; ++a) {
-        const float hi = static_cast<float>(c.size[a] - 1) - L;
-        for (int h = 0; h < 2; ++h)
-            for (int i = 0; i < c.size[a]; ++i) {
-                const float x = static_cast<float>(i) + (h ? 0.5f : 0.0f);
-                prof[static_cast<size_t>(2 * a + h) * stride + i] = profile(std::max(L - x, x - hi));
-            }
-    }
+    int stride = 0;
+    const std::vector<float> prof = absorber_profiles(c, &stride);
     ctx->prof_stride = stride;
     if (int rc = ctx->prof.ensure(prof.size() * sizeof(float))) return rc;
     EMFD_HIP(hipMemcpyAsync(ctx->prof.ptr, prof.data(), prof.size() * sizeof(float), hipMemcpyHostToDevice,
@@ -251,103 +404,17 @@ int emfd_grid_upload_materials(emfd_ctx *ctx, const float *sigma, const float *e
         return fail(EMFD_ERR_INVALID, "emfd_grid_upload_materials: null override array");
     EMFD_HIP(hipSetDevice(ctx->device));
     EMFD_HIP(hipStreamSynchronize(ctx->stream));  // buffers may be reallocated below
-    const int nx = ctx->cfg.size[0], ny = ctx->cfg.size[1], nz = ctx->cfg.size[2];
     const size_t n = ctx->n;
 
     // cell materials (alpha channels of the reference's field textures)
     EMFD_HIP(hipMemcpyAsync(ctx->sigma_cell.ptr, sigma, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     EMFD_HIP(hipMemcpyAsync(ctx->eps_cell.ptr, eps_r, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
 
-    // sub-cell overrides sorted by texel, keeping their order (later entries win, as the
-    // reference's sequential loop at src/SimulationGrid.cpp:251-254)
-    struct EdgeOv { size_t idx; int axis; float sigma; };
-    std::vector<EdgeOv> ov;
-    ov.reserve(static_cast<size_t>(std::max<int64_t>(n_edges, 0)));
-    for (int64_t k = 0; k < n_edges; ++k) {
-        const emfd_edge_override &o = edges[k];
-        if (o.texel[0] < 0 || o.texel[1] < 0 || o.texel[2] < 0 || o.texel[0] >= nx || o.texel[1] >= ny ||
-            o.texel[2] >= nz || o.axis < 0 || o.axis > 2)
-            continue;
-        ov.push_back({(static_cast<size_t>(o.texel[2]) * ny + o.texel[1]) * nx + o.texel[0], o.axis,
-                      std::max(o.sigma, 0.0f)});
-    }
-    std::stable_sort(ov.begin(), ov.end(), [](const EdgeOv &a, const EdgeOv &b) { return a.idx < b.idx; });
-    struct WireOv { size_t idx; int slot; float factor; };
-    std::vector<WireOv> wv;
-    for (int64_t k = 0; k < n_wires; ++k) {
-        const emfd_wire_coefficient &w = wires[k];
-        if (w.texel[0] < 0 || w.texel[1] < 0 || w.texel[2] < 0 || w.texel[0] >= nx || w.texel[1] >= ny ||
-            w.texel[2] >= nz || w.component < 0 || w.component > 2)
-            continue;
-        wv.push_back({(static_cast<size_t>(w.texel[2]) * ny + w.texel[1]) * nx + w.texel[0],
-                      2 * w.component + (w.term == 0 ? 0 : 1), w.factor});
-    }
-    std::stable_sort(wv.begin(), wv.end(), [](const WireOv &a, const WireOv &b) { return a.idx < b.idx; });
     const bool thin = n_wires > 0;  // the reference keys thin-wire mode on a non-empty list
-
-    // edge averages (src/SimulationGrid.cpp:224-248), quantised to binary16, classified
-    using Key = std::array<uint16_t, 12>;
-    struct KeyHash {
-        size_t operator()(const Key &k) const noexcept {
-            uint64_t h = 1469598103934665603ull;
-            for (uint16_t v : k) h = (h ^ v) * 1099511628211ull;
-            return static_cast<size_t>(h);
-        }
-    };
-    const uint16_t h_one = f32_to_f16(1.0f);
-    const Key vacuum = {0, 0, 0, h_one, h_one, h_one, h_one, h_one, h_one, h_one, h_one, h_one};
-    std::unordered_map<Key, uint32_t, KeyHash> classes;
-    std::vector<Key> keys;
-    classes.emplace(vacuum, 0u);
-    keys.push_back(vacuum);
+    ClassMap cm;
     std::vector<uint32_t> ids(n);
-    auto idx = [&](int x, int y, int z) {
-        x = std::max(x, 0);
-        y = std::max(y, 0);
-        z = std::max(z, 0);
-        return (static_cast<size_t>(z) * ny + y) * nx + x;
-    };
-    size_t oi = 0, wi = 0;
-    Key last = vacuum;
-    uint32_t last_id = 0;
-    for (int z = 0; z < nz; ++z)
-        for (int y = 0; y < ny; ++y)
-            for (int x = 0; x < nx; ++x) {
-                const size_t c00 = idx(x, y, z);
-                const size_t ex[4] = {c00, idx(x, y - 1, z), idx(x, y, z - 1), idx(x, y - 1, z - 1)};
-                const size_t ey[4] = {c00, idx(x - 1, y, z), idx(x, y, z - 1), idx(x - 1, y, z - 1)};
-                const size_t ez[4] = {c00, idx(x - 1, y, z), idx(x, y - 1, z), idx(x - 1, y - 1, z)};
-                auto avg = [](const float *v, const size_t (&c)[4]) {
-                    return 0.25f * (v[c[0]] + v[c[1]] + v[c[2]] + v[c[3]]);
-                };
-                float es[3] = {avg(sigma, ex), avg(sigma, ey), avg(sigma, ez)};
-                const float ee[3] = {avg(eps_r, ex), avg(eps_r, ey), avg(eps_r, ez)};
-                for (; oi < ov.size() && ov[oi].idx == c00; ++oi) es[ov[oi].axis] = ov[oi].sigma;
-                float wf[6] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f};
-                for (; wi < wv.size() && wv[wi].idx == c00; ++wi) wf[wv[wi].slot] = wv[wi].factor;
-                Key k;
-                for (int c = 0; c < 3; ++c) {
-                    k[c] = f32_to_f16(std::min(es[c], kPerfectEdgeSigma));
-                    k[3 + c] = f32_to_f16(ee[c]);
-                }
-                for (int j = 0; j < 6; ++j) k[6 + j] = f32_to_f16(wf[j]);
-                uint32_t id;
-                if (k == last) {
-                    id = last_id;
-                } else {
-                    auto it = classes.find(k);
-                    if (it == classes.end()) {
-                        id = static_cast<uint32_t>(keys.size());
-                        classes.emplace(k, id);
-                        keys.push_back(k);
-                    } else {
-                        id = it->second;
-                    }
-                    last = k;
-                    last_id = id;
-                }
-                ids[c00] = id;
-            }
+    classify_cells(ctx->cfg, sigma, eps_r, edges, n_edges, wires, n_wires, cm, ids.data());
+    const std::vector<ClassKey> &keys = cm.keys;
 
     ctx->has_wire = thin ? 1 : 0;
     if (keys.size() == 1 && !thin) {
@@ -365,9 +432,9 @@ int emfd_grid_upload_materials(emfd_ctx *ctx, const float *sigma, const float *e
             EMFD_HIP(hipMemcpy(ctx->mat_id.ptr, ids.data(), n * 4, hipMemcpyHostToDevice));
         }
         DevBuf d_keys;
-        if (int rc = d_keys.ensure(keys.size() * sizeof(Key))) return rc;
+        if (int rc = d_keys.ensure(keys.size() * sizeof(ClassKey))) return rc;
         if (int rc = ctx->table.ensure(keys.size() * kTableStride * sizeof(float))) return rc;
-        EMFD_HIP(hipMemcpy(d_keys.ptr, keys.data(), keys.size() * sizeof(Key), hipMemcpyHostToDevice));
+        EMFD_HIP(hipMemcpy(d_keys.ptr, keys.data(), keys.size() * sizeof(ClassKey), hipMemcpyHostToDevice));
         EMFD_HIP(launch_build_table(d_keys.as<uint16_t>(), ctx->classes, ctx->cfg.courant, ctx->table.as<float>(),
                                     ctx->stream));
         EMFD_HIP(hipStreamSynchronize(ctx->stream));
@@ -392,58 +459,11 @@ int emfd_grid_set_sources(emfd_ctx *ctx, const emfd_source_desc *sources, int n,
     if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
     if (n > 0 && !sources) return fail(EMFD_ERR_INVALID, "null sources");
     EMFD_HIP(hipSetDevice(ctx->device));
-    // packing (src/SimulationGrid.cpp:297-318)
-    std::vector<float> packed;
-    std::vector<emfd_source_desc> kept;
-    for (int i = 0; i < n && static_cast<int>(kept.size()) < kMaxSources; ++i) {
-        const emfd_source_desc &s = sources[i];
-        const float len = std::sqrt(s.polarization[0] * s.polarization[0] + s.polarization[1] * s.polarization[1] +
-                                    s.polarization[2] * s.polarization[2]);
-        if (!s.enabled || !(len > 1e-6f) || !std::isfinite(len)) continue;
-        float g[16];
-        for (int a = 0; a < 3; ++a) {
-            g[a] = s.center[a];
-            g[4 + a] = std::clamp(s.half_extent[a], 0.0f, 1.0e5f);
-            g[8 + a] = s.polarization[a] / len;
-            g[12 + a] = std::max(s.softness[a], 0.25f);
-        }
-        g[3] = g[7] = g[15] = 0.0f;
-        g[11] = s.amplitude;
-        packed.insert(packed.end(), g, g + 16);
-        kept.push_back(s);
-    }
-    // per-texel source lists (src/SimulationGrid.cpp:320-349)
-    const int nx = ctx->cfg.size[0], ny = ctx->cfg.size[1], nz = ctx->cfg.size[2];
-    const int dims[3] = {nx, ny, nz};
-    std::vector<std::pair<uint32_t, uint32_t>> pairs;
-    for (size_t i = 0; i < kept.size(); ++i) {
-        const float *g = &packed[16 * i];
-        int lo[3], hi[3];
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = std::clamp(static_cast<int>(std::floor(g[a] - g[4 + a] - 3.5f * g[12 + a])) - 1, 0, dims[a] - 1);
-            hi[a] = std::clamp(static_cast<int>(std::ceil(g[a] + g[4 + a] + 3.5f * g[12 + a])), 0, dims[a] - 1);
-        }
-        for (int z = lo[2]; z <= hi[2]; ++z)
-            for (int y = lo[1]; y <= hi[1]; ++y)
-                for (int x = lo[0]; x <= hi[0]; ++x)
-                    pairs.emplace_back(static_cast<uint32_t>((z * ny + y) * nx + x), static_cast<uint32_t>(i));
-    }
-    std::sort(pairs.begin(), pairs.end());
-    std::vector<int> cells, lists;
-    for (size_t k = 0; k < pairs.size(); ++k) {
-        if (k == 0 || pairs[k].first != pairs[k - 1].first) {
-            const uint32_t id = pairs[k].first;
-            cells.push_back(static_cast<int>(id % static_cast<uint32_t>(nx)));
-            cells.push_back(static_cast<int>((id / static_cast<uint32_t>(nx)) % static_cast<uint32_t>(ny)));
-            cells.push_back(static_cast<int>(id / static_cast<uint32_t>(nx * ny)));
-            cells.push_back(static_cast<int>(lists.size()));
-        }
-        lists.push_back(static_cast<int>(pairs[k].second));
-    }
-    ctx->n_cells = static_cast<int>(cells.size() / 4);
-    cells.insert(cells.end(), {0, 0, 0, static_cast<int>(lists.size())});
-    if (lists.empty()) lists.push_back(0);
-    if (packed.empty()) packed.assign(16, 0.0f);
+    PackedSources ps = pack_sources(ctx->cfg, sources, n);
+    ctx->n_cells = ps.n_cells;
+    std::vector<float> &packed = ps.packed;
+    std::vector<int> &cells = ps.cells, &lists = ps.lists;
+    const std::vector<emfd_source_desc> &kept = ps.kept;
     if (int rc = ctx->d_packed.ensure(packed.size() * sizeof(float))) return rc;
     if (int rc = ctx->d_cells.ensure(cells.size() * sizeof(int))) return rc;
     if (int rc = ctx->d_lists.ensure(lists.size() * sizeof(int))) return rc;
