@@ -6,6 +6,7 @@
 #include "../../include/emfdtd.h"
 
 #include <array>
+#include <complex>
 #include <cstdint>
 #include <unordered_map>
 #include <vector>
@@ -53,5 +54,21 @@ struct PackedSources {
     int n_cells = 0;
 };
 PackedSources pack_sources(const emfd_grid_config &c, const emfd_source_desc *src, int n);
+
+// PortMonitor sweep (src/Ports.cpp:336-414): nu evenly spaced over [1/lambda_max, 1/lambda_min]
+struct SweepBand {
+    int points = 2;
+    double nu_lo = 0.0, nu_hi = 0.0;
+    double nu(int f) const { return nu_lo + (nu_hi - nu_lo) * f / (points - 1); }
+};
+SweepBand sweep_band(const emfd_sweep_setup &sweep);
+// Z, S11, VSWR per point and the best/band summary of one port from its DFT sums (sumI already
+// carries the e^{+i w dt / 2} half-step factor). points: band.points entries, or null.
+void sweep_finish(const SweepBand &band, const std::complex<double> *sumV, const std::complex<double> *sumI,
+                  double impedance, emfd_sweep_point *points, emfd_port_sweep *summary);
+// Rotation factors (cos w tE, sin w tE, cos w tH, sin w tH) at step_after, FP64 cast to float,
+// and the per-stage EMA weight of the port and far-field accumulators.
+void port_rotation(float wavelength, float dt, uint64_t step_after, float rot[4]);
+float port_blend(float averaging, float wavelength, float dt);
 
 }  // namespace emfd

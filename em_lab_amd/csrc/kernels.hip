@@ -11,6 +11,7 @@
 // goes through det_expf, which gives the same bits on the host and the device.
 #include "expf.h"
 #include "internal.h"
+#include "sources_device.h"
 #include "yee_device.h"
 
 #include <algorithm>
@@ -382,43 +383,11 @@ __global__ void __launch_bounds__(TX *TY) k_yee_fused(LatticeArgs a, FieldPtrs s
     }
 }
 
-// ---------------------------------------------------------------------------------------------
-// apply_sources.comp (shaders/apply_sources.comp:37-79): one thread per affected texel.
-// packed: 16 floats per source (center, halfExtent, polAmp, softness; vec4 each).
-// ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ float footprint(const float *s, float x, float y, float z) {
-    const float dx = fmaxf(fabsf(x - s[0]) - s[4], 0.0f) / s[12];
-    const float dy = fmaxf(fabsf(y - s[1]) - s[5], 0.0f) / s[13];
-    const float dz = fmaxf(fabsf(z - s[2]) - s[6], 0.0f) / s[14];
-    return det_expf(-(dx * dx + dy * dy + dz * dz));
-}
-
 __global__ void __launch_bounds__(64) k_apply_sources(LatticeArgs a, FieldPtrs f, const int *cells, int n_cells,
                                                       const int *lists, const float *packed, const float *signals) {
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= n_cells) return;
-    const int x = cells[4 * id + 0], y = cells[4 * id + 1], z = cells[4 * id + 2];
-    const int first = cells[4 * id + 3], end = cells[4 * (id + 1) + 3];
-    const float cx = static_cast<float>(x), cy = static_cast<float>(y), cz = static_cast<float>(z);
-    float J[3] = {0.0f, 0.0f, 0.0f};
-    for (int k = first; k < end; ++k) {
-        const int i = lists[k];
-        const float signal = signals[i];
-        if (signal == 0.0f) continue;
-        const float *s = packed + 16 * i;
-        const float w[3] = {footprint(s, cx + 0.5f, cy + 0.0f, cz + 0.0f), footprint(s, cx + 0.0f, cy + 0.5f, cz + 0.0f),
-                            footprint(s, cx + 0.0f, cy + 0.0f, cz + 0.5f)};
-        const float as = s[11] * signal;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) J[c] += s[8 + c] * as * w[c];
-    }
-    if (J[0] == 0.0f && J[1] == 0.0f && J[2] == 0.0f) return;
-    const size_t p = (static_cast<size_t>(z) * a.ny + y) * a.nx + x;
-    const uint32_t cid = a.mat_id ? load_id(a, p) : 0u;
-    float ca[3], cb[3];
-    edge_coefficients(a, x, y, z, cid, ca, cb);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) f.e[c][p] = f.e[c][p] - cb[c] * J[c];
+    source_cell(a, f, cells, lists, packed, signals, id);
 }
 
 }  // namespace

@@ -3,6 +3,7 @@
 // on the device inside emfd_grid_step; evaluation math that the reference does on the CPU in
 // double precision stays on the host here, with the same formulas.
 #include "context.h"
+#include "host_common.h"
 
 #include <algorithm>
 #include <cmath>
@@ -39,6 +40,75 @@ float blend_of(float averaging, float wavelength, float dt) {
     const double tau = averaging * wavelength;  // float product, as the reference
     return static_cast<float>(1.0 - std::exp(-static_cast<double>(dt) / (0.5 * tau)));
 }
+
+}  // namespace
+
+SweepBand sweep_band(const emfd_sweep_setup &sw) {
+    SweepBand b;
+    b.points = std::max(sw.points, 2);
+    b.nu_lo = 1.0 / std::max(sw.wavelength_max, sw.wavelength_min);
+    b.nu_hi = 1.0 / std::min(sw.wavelength_max, sw.wavelength_min);
+    return b;
+}
+
+void sweep_finish(const SweepBand &band, const std::complex<double> *sumV, const std::complex<double> *sumI,
+                  double impedance, emfd_sweep_point *points, emfd_port_sweep *summary) {
+    // src/Ports.cpp:370-414
+    const double z0 = std::max(impedance, 1e-6);
+    const int npts = band.points;
+    std::vector<emfd_sweep_point> pts;
+    for (int f = 0; f < npts; ++f) {
+        emfd_sweep_point pt{};
+        pt.wavelength = 1.0 / band.nu(f);
+        const std::complex<double> &sv = sumV[f], &si = sumI[f];
+        if (std::abs(si) > 1e-30) {
+            const std::complex<double> z = -sv / si * kEta0;
+            const double g = std::abs(reflection(z, z0));
+            pt.z_re = z.real();
+            pt.z_im = z.imag();
+            pt.s11_db = g > 1e-12 ? 20.0 * std::log10(g) : -200.0;
+            pt.vswr = vswr_of(g);
+        }
+        pts.push_back(pt);
+    }
+    emfd_port_sweep ps{};
+    size_t best = 0;
+    for (size_t i = 1; i < pts.size(); ++i)
+        if (pts[i].s11_db < pts[best].s11_db) best = i;
+    ps.best = static_cast<int64_t>(best);
+    double nuBest = 1.0 / pts[best].wavelength;
+    if (best > 0 && best + 1 < pts.size()) {
+        const double a = pts[best - 1].s11_db, b = pts[best].s11_db, c = pts[best + 1].s11_db;
+        const double denom = a - 2.0 * b + c;
+        const double dnu = (band.nu_hi - band.nu_lo) / (npts - 1);
+        if (std::abs(denom) > 1e-12) nuBest += std::clamp(0.5 * (a - c) / denom, -1.0, 1.0) * dnu;
+    }
+    ps.best_wavelength = 1.0 / nuBest;
+    if (pts[best].s11_db < -10.0) {
+        auto edge = [&](int dir) {
+            int i = static_cast<int>(best);
+            while (i + dir >= 0 && i + dir < static_cast<int>(pts.size()) &&
+                   pts[static_cast<size_t>(i + dir)].s11_db < -10.0)
+                i += dir;
+            const int j = i + dir;
+            const double nuI = 1.0 / pts[static_cast<size_t>(i)].wavelength;
+            if (j < 0 || j >= static_cast<int>(pts.size())) return nuI;
+            const double nuJ = 1.0 / pts[static_cast<size_t>(j)].wavelength;
+            const double sI = pts[static_cast<size_t>(i)].s11_db, sJ = pts[static_cast<size_t>(j)].s11_db;
+            return nuI + (nuJ - nuI) * (-10.0 - sI) / (sJ - sI);
+        };
+        ps.has_band = 1;
+        ps.band_low_wavelength = 1.0 / edge(-1);
+        ps.band_high_wavelength = 1.0 / edge(+1);
+    }
+    if (points) std::copy(pts.begin(), pts.end(), points);
+    if (summary) *summary = ps;
+}
+
+void port_rotation(float wavelength, float dt, uint64_t step_after, float rot[4]) { rotation(wavelength, dt, step_after, rot); }
+float port_blend(float averaging, float wavelength, float dt) { return blend_of(averaging, wavelength, dt); }
+
+namespace {
 
 void build_directions(NtffState &st, const float axis[3]) {
     // FarFieldMonitor::buildDirections (src/FarField.cpp:201-257)
@@ -283,9 +353,7 @@ int emfd_ports_evaluate_sweep(emfd_ctx *ctx, double dt, emfd_sweep_point *points
     std::vector<float> series(n * nPorts * 2);
     if (int rc = emfd_ports_read_series(ctx, series.data(), static_cast<int64_t>(series.size()))) return rc;
     // PortMonitor::evaluateSweep (src/Ports.cpp:336-414)
-    const int npts = std::max(P.sweep.points, 2);
-    const double nuLo = 1.0 / std::max(P.sweep.wavelength_max, P.sweep.wavelength_min);
-    const double nuHi = 1.0 / std::min(P.sweep.wavelength_max, P.sweep.wavelength_min);
+    const SweepBand band = sweep_band(P.sweep);
     double peak = 0.0, late = 0.0;
     for (size_t k = 0; k < n; ++k) {
         const double v = std::abs(static_cast<double>(series[2 * (k * nPorts)]));
@@ -295,65 +363,23 @@ int emfd_ports_evaluate_sweep(emfd_ctx *ctx, double dt, emfd_sweep_point *points
     if (residual) *residual = peak > 0.0 ? late / peak : 1.0;
     if (samples) *samples = static_cast<int64_t>(n);
     if (full) *full = P.recorded >= P.capacity_steps;
+    std::vector<std::complex<double>> sumV(static_cast<size_t>(band.points)), sumI(static_cast<size_t>(band.points));
     for (size_t p = 0; p < nPorts; ++p) {
-        const double z0 = std::max(static_cast<double>(P.ports[p].impedance_ohm), 1e-6);
-        std::vector<emfd_sweep_point> pts;
-        for (int f = 0; f < npts; ++f) {
-            const double nu = nuLo + (nuHi - nuLo) * f / (npts - 1);
-            const double w = kTwoPi * nu;
+        for (int f = 0; f < band.points; ++f) {
+            const double w = kTwoPi * band.nu(f);
             const std::complex<double> step = std::polar(1.0, -w * dt);
-            std::complex<double> rot(1.0, 0.0), sumV(0.0), sumI(0.0);
+            std::complex<double> rot(1.0, 0.0), sv(0.0), si(0.0);
             for (size_t k = 0; k < n; ++k) {
-                sumV += static_cast<double>(series[2 * (k * nPorts + p)]) * rot;
-                sumI += static_cast<double>(series[2 * (k * nPorts + p) + 1]) * rot;
+                sv += static_cast<double>(series[2 * (k * nPorts + p)]) * rot;
+                si += static_cast<double>(series[2 * (k * nPorts + p) + 1]) * rot;
                 rot *= step;
                 if ((k & 1023) == 1023) rot /= std::abs(rot);
             }
-            sumI *= std::polar(1.0, w * 0.5 * dt);
-            emfd_sweep_point pt{};
-            pt.wavelength = 1.0 / nu;
-            if (std::abs(sumI) > 1e-30) {
-                const std::complex<double> z = -sumV / sumI * kEta0;
-                const double g = std::abs(reflection(z, z0));
-                pt.z_re = z.real();
-                pt.z_im = z.imag();
-                pt.s11_db = g > 1e-12 ? 20.0 * std::log10(g) : -200.0;
-                pt.vswr = vswr_of(g);
-            }
-            pts.push_back(pt);
+            sumV[static_cast<size_t>(f)] = sv;
+            sumI[static_cast<size_t>(f)] = si * std::polar(1.0, w * 0.5 * dt);
         }
-        emfd_port_sweep ps{};
-        size_t best = 0;
-        for (size_t i = 1; i < pts.size(); ++i)
-            if (pts[i].s11_db < pts[best].s11_db) best = i;
-        ps.best = static_cast<int64_t>(best);
-        double nuBest = 1.0 / pts[best].wavelength;
-        if (best > 0 && best + 1 < pts.size()) {
-            const double a = pts[best - 1].s11_db, b = pts[best].s11_db, c = pts[best + 1].s11_db;
-            const double denom = a - 2.0 * b + c;
-            const double dnu = (nuHi - nuLo) / (npts - 1);
-            if (std::abs(denom) > 1e-12) nuBest += std::clamp(0.5 * (a - c) / denom, -1.0, 1.0) * dnu;
-        }
-        ps.best_wavelength = 1.0 / nuBest;
-        if (pts[best].s11_db < -10.0) {
-            auto edge = [&](int dir) {
-                int i = static_cast<int>(best);
-                while (i + dir >= 0 && i + dir < static_cast<int>(pts.size()) &&
-                       pts[static_cast<size_t>(i + dir)].s11_db < -10.0)
-                    i += dir;
-                const int j = i + dir;
-                const double nuI = 1.0 / pts[static_cast<size_t>(i)].wavelength;
-                if (j < 0 || j >= static_cast<int>(pts.size())) return nuI;
-                const double nuJ = 1.0 / pts[static_cast<size_t>(j)].wavelength;
-                const double sI = pts[static_cast<size_t>(i)].s11_db, sJ = pts[static_cast<size_t>(j)].s11_db;
-                return nuI + (nuJ - nuI) * (-10.0 - sI) / (sJ - sI);
-            };
-            ps.has_band = 1;
-            ps.band_low_wavelength = 1.0 / edge(-1);
-            ps.band_high_wavelength = 1.0 / edge(+1);
-        }
-        if (points) std::copy(pts.begin(), pts.end(), points + p * static_cast<size_t>(npts));
-        if (summary) summary[p] = ps;
+        sweep_finish(band, sumV.data(), sumI.data(), static_cast<double>(P.ports[p].impedance_ohm),
+                     points ? points + p * static_cast<size_t>(band.points) : nullptr, summary ? summary + p : nullptr);
     }
     return EMFD_OK;
 }

@@ -2,85 +2,18 @@
 // near-to-far-field Huygens-box phasors, radiation integrals, Poynting flux, probe gather.
 // Compiled with -ffp-contract=off, like the Yee kernels, so each expression rounds as written.
 #include "internal.h"
+#include "port_device.h"
 
 namespace emfd {
 namespace {
-
-__device__ __forceinline__ float fetch_c(const LatticeArgs &a, const float *plane, int x, int y, int z) {
-    // texelFetch with indices clamped to the grid (shaders/field_sampling.glsl:14-15,
-    // shaders/port_accumulate.comp:40-41)
-    x = min(max(x, 0), a.nx - 1);
-    y = min(max(y, 0), a.ny - 1);
-    z = min(max(z, 0), a.nz - 1);
-    return plane[(static_cast<size_t>(z) * a.ny + y) * a.nx + x];
-}
-
-// eAtNode / bAtNode (shaders/field_sampling.glsl:17-32)
-__device__ __forceinline__ void e_at_node(const LatticeArgs &a, const FieldPtrs &f, int x, int y, int z, float o[3]) {
-    o[0] = 0.5f * (fetch_c(a, f.e[0], x, y, z) + fetch_c(a, f.e[0], x - 1, y, z));
-    o[1] = 0.5f * (fetch_c(a, f.e[1], x, y, z) + fetch_c(a, f.e[1], x, y - 1, z));
-    o[2] = 0.5f * (fetch_c(a, f.e[2], x, y, z) + fetch_c(a, f.e[2], x, y, z - 1));
-}
-__device__ __forceinline__ void b_at_node(const LatticeArgs &a, const FieldPtrs &f, int x, int y, int z, float o[3]) {
-    const float bx = fetch_c(a, f.b[0], x, y, z) + fetch_c(a, f.b[0], x, y - 1, z) + fetch_c(a, f.b[0], x, y, z - 1) +
-                     fetch_c(a, f.b[0], x, y - 1, z - 1);
-    const float by = fetch_c(a, f.b[1], x, y, z) + fetch_c(a, f.b[1], x - 1, y, z) + fetch_c(a, f.b[1], x, y, z - 1) +
-                     fetch_c(a, f.b[1], x - 1, y, z - 1);
-    const float bz = fetch_c(a, f.b[2], x, y, z) + fetch_c(a, f.b[2], x - 1, y, z) + fetch_c(a, f.b[2], x, y - 1, z) +
-                     fetch_c(a, f.b[2], x - 1, y - 1, z);
-    o[0] = 0.25f * bx;
-    o[1] = 0.25f * by;
-    o[2] = 0.25f * bz;
-}
-
-// ---------------------------------------------------------------------------------------------
-// port_accumulate.comp (shaders/port_accumulate.comp:45-83)
-// ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ float edge_current(const LatticeArgs &a, const FieldPtrs &f, int p[3], int ax) {
-    const int u = (ax + 1) % 3, v = (ax + 2) % 3;
-    int pu[3] = {p[0], p[1], p[2]}, pv[3] = {p[0], p[1], p[2]};
-    pu[u] -= 1;
-    pv[v] -= 1;
-    const float b0v = fetch_c(a, f.b[v], p[0], p[1], p[2]), buv = fetch_c(a, f.b[v], pu[0], pu[1], pu[2]);
-    const float b0u = fetch_c(a, f.b[u], p[0], p[1], p[2]), bvu = fetch_c(a, f.b[u], pv[0], pv[1], pv[2]);
-    return (b0v - buv) - (b0u - bvu);
-}
 
 __global__ void __launch_bounds__(64) k_port_accumulate(LatticeArgs a, FieldPtrs f, const PortDev *ports, int n_ports,
                                                         const float *rot, float blend, float *ph, float *series,
                                                         int64_t rec) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_ports) return;
-    const PortDev pt = ports[i];
-    int p[3] = {pt.texel[0], pt.texel[1], pt.texel[2]};
-    const int ax = pt.axis;
-    float V = 0.0f, I = 0.0f;
-    if (pt.edges == 4) {
-        const int u = (ax + 1) % 3, v = (ax + 2) % 3;
-        for (int k = 0; k < 4; ++k) {
-            int q[3] = {p[0], p[1], p[2]};
-            if (k == 1 || k == 3) q[u] += 1;
-            if (k == 2 || k == 3) q[v] += 1;
-            V += 0.25f * fetch_c(a, f.e[ax], q[0], q[1], q[2]);
-            I += edge_current(a, f, q, ax);
-        }
-    } else {
-        V = fetch_c(a, f.e[ax], p[0], p[1], p[2]);
-        I = edge_current(a, f, p, ax);
-    }
-    const float s = static_cast<float>(pt.sign);
-    V *= s;
-    I *= s;
-    if (rec >= 0 && series) {
-        series[2 * (rec * n_ports + i) + 0] = V;
-        series[2 * (rec * n_ports + i) + 1] = I;
-    }
-    const float obs[4] = {2.0f * rot[0] * V, -2.0f * rot[1] * V, 2.0f * rot[2] * I, -2.0f * rot[3] * I};
-    float *s1 = ph + 8 * i, *s2 = s1 + 4;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) s1[c] += blend * (obs[c] - s1[c]);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) s2[c] += blend * (s1[c] - s2[c]);
+    port_sample(a, f, ports[i], rot, blend, ph + 8 * i,
+                (rec >= 0 && series) ? series + 2 * (rec * n_ports + i) : nullptr);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -1,0 +1,96 @@
+// Per-quad bodies of the vectorised Yee passes: one thread owns four consecutive x cells so
+// every field stream moves as float4 (16 B per lane). Requires nx % 4 == 0 and x0 % 4 == 0.
+// The x+1 (B pass) and x-1 (E pass) neighbours of the edge cells are extra dword loads that hit
+// the same cache lines. Same per-texel arithmetic as faraday() and edge_coefficients(), so every
+// kernel built on these bodies (single grid or batched) gives bit-identical fields.
+#pragma once
+
+#include "yee_device.h"
+
+namespace emfd {
+
+__device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+__device__ __forceinline__ void st4(float *p, float a, float b, float c, float d) {
+    *reinterpret_cast<float4 *>(p) = make_float4(a, b, c, d);
+}
+__device__ __forceinline__ void unpack(const float4 v, float (&o)[4]) {
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+
+__device__ __forceinline__ void load_ids4(const LatticeArgs &a, size_t p, uint32_t (&id)[4]) {
+    if (!a.mat_id) {
+        id[0] = id[1] = id[2] = id[3] = 0u;
+    } else if (a.id_bytes == 2) {
+        const uint2 v = *reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(a.mat_id) + p);
+        id[0] = v.x & 0xffffu; id[1] = v.x >> 16; id[2] = v.y & 0xffffu; id[3] = v.y >> 16;
+    } else {
+        const uint4 v = *reinterpret_cast<const uint4 *>(static_cast<const uint32_t *>(a.mat_id) + p);
+        id[0] = v.x; id[1] = v.y; id[2] = v.z; id[3] = v.w;
+    }
+}
+
+// update_B for the four cells x0..x0+3 of row (y, z) (shaders/update_B.comp:31-70)
+__device__ __forceinline__ void b_quad(const LatticeArgs &a, const FieldPtrs &f, int x0, int y, int z) {
+    const size_t sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
+    const size_t p = z * sz + y * sy + x0;
+    const bool up_x = x0 + 4 < a.nx, up_y = y + 1 < a.ny, up_z = z + 1 < a.nz;
+    float e0[3][4], eX[3], eY[3][4], eZ[3][4], b[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float *e = f.e[c];
+        unpack(ld4(e + p), e0[c]);
+        eX[c] = up_x ? e[p + 4] : 0.0f;
+        if (up_y) unpack(ld4(e + p + sy), eY[c]); else eY[c][0] = eY[c][1] = eY[c][2] = eY[c][3] = 0.0f;
+        if (up_z) unpack(ld4(e + p + sz), eZ[c]); else eZ[c][0] = eZ[c][1] = eZ[c][2] = eZ[c][3] = 0.0f;
+        unpack(ld4(f.b[c] + p), b[c]);
+    }
+    uint32_t id[4];
+    load_ids4(a, p, id);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float c0[3] = {e0[0][j], e0[1][j], e0[2][j]};
+        const float cx[3] = {j < 3 ? e0[0][j + 1] : eX[0], j < 3 ? e0[1][j + 1] : eX[1], j < 3 ? e0[2][j + 1] : eX[2]};
+        const float cy[3] = {eY[0][j], eY[1][j], eY[2][j]};
+        const float cz[3] = {eZ[0][j], eZ[1][j], eZ[2][j]};
+        float bb[3] = {b[0][j], b[1][j], b[2][j]};
+        faraday(a, x0 + j, y, z, c0, cx, cy, cz, id[j], bb);
+        b[0][j] = bb[0];
+        b[1][j] = bb[1];
+        b[2][j] = bb[2];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) st4(f.b[c] + p, b[c][0], b[c][1], b[c][2], b[c][3]);
+}
+
+// update_E for the four cells x0..x0+3 of row (y, z) (shaders/update_E.comp:31-66)
+__device__ __forceinline__ void e_quad(const LatticeArgs &a, const FieldPtrs &f, int x0, int y, int z) {
+    const size_t sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
+    const size_t p = z * sz + y * sy + x0;
+    const bool dn_x = x0 > 0, dn_y = y > 0, dn_z = z > 0;
+    float b0[3][4], bX[3], bY[3][4], bZ[3][4], e[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float *bp = f.b[c];
+        unpack(ld4(bp + p), b0[c]);
+        bX[c] = dn_x ? bp[p - 1] : 0.0f;
+        if (dn_y) unpack(ld4(bp + p - sy), bY[c]); else bY[c][0] = bY[c][1] = bY[c][2] = bY[c][3] = 0.0f;
+        if (dn_z) unpack(ld4(bp + p - sz), bZ[c]); else bZ[c][0] = bZ[c][1] = bZ[c][2] = bZ[c][3] = 0.0f;
+        unpack(ld4(f.e[c] + p), e[c]);
+    }
+    uint32_t id[4];
+    load_ids4(a, p, id);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float bxm[3] = {j > 0 ? b0[0][j - 1] : bX[0], j > 0 ? b0[1][j - 1] : bX[1], j > 0 ? b0[2][j - 1] : bX[2]};
+        const float curl[3] = {(b0[2][j] - bY[2][j]) - (b0[1][j] - bZ[1][j]), (b0[0][j] - bZ[0][j]) - (b0[2][j] - bxm[2]),
+                               (b0[1][j] - bxm[1]) - (b0[0][j] - bY[0][j])};
+        float ca[3], cb[3];
+        edge_coefficients(a, x0 + j, y, z, id[j], ca, cb);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) e[c][j] = ca[c] * e[c][j] + cb[c] * curl[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) st4(f.e[c] + p, e[c][0], e[c][1], e[c][2], e[c][3]);
+}
+
+}  // namespace emfd
