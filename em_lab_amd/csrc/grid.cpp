@@ -8,6 +8,7 @@
 #include "host_common.h"
 
 #include <algorithm>
+#include <cstdint>
 #include <array>
 #include <cmath>
 #include <cstring>
@@ -169,6 +170,45 @@ void classify_cells(const emfd_grid_config &cfg, const float *sigma, const float
     size_t oi = 0, wi = 0;
     ClassKey last = ClassMap::vacuum();
     uint32_t last_id = cm.map.at(last);
+    auto classify = [&](size_t c00, float (&es)[3], const float (&ee)[3]) {
+        for (; oi < ov.size() && ov[oi].idx == c00; ++oi) es[ov[oi].axis] = ov[oi].sigma;
+        float wf[6] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f};
+        for (; wi < wv.size() && wv[wi].idx == c00; ++wi) wf[wv[wi].slot] = wv[wi].factor;
+        ClassKey k;
+        for (int c = 0; c < 3; ++c) {
+            k[c] = f32_to_f16(std::min(es[c], kPerfectEdgeSigma));
+            k[3 + c] = f32_to_f16(ee[c]);
+        }
+        for (int j = 0; j < 6; ++j) k[6 + j] = f32_to_f16(wf[j]);
+        uint32_t id;
+        if (k == last) {
+            id = last_id;
+        } else {
+            auto it = cm.map.find(k);
+            if (it == cm.map.end()) {
+                id = static_cast<uint32_t>(cm.keys.size());
+                cm.map.emplace(k, id);
+                cm.keys.push_back(k);
+            } else {
+                id = it->second;
+            }
+            last = k;
+            last_id = id;
+        }
+        ids[c00] = id;
+    };
+    if (!sigma && !eps_r) {
+        // vacuum volume: every edge average is (0, 1), so only the overridden texels can leave
+        // class 0; visiting them in lattice order numbers the classes as the full scan does
+        std::fill(ids, ids + static_cast<size_t>(nx) * ny * nz, 0u);
+        const float ee[3] = {1.0f, 1.0f, 1.0f};
+        while (oi < ov.size() || wi < wv.size()) {
+            const size_t a = oi < ov.size() ? ov[oi].idx : SIZE_MAX, b = wi < wv.size() ? wv[wi].idx : SIZE_MAX;
+            float es[3] = {0.0f, 0.0f, 0.0f};
+            classify(std::min(a, b), es, ee);
+        }
+        return;
+    }
     for (int z = 0; z < nz; ++z)
         for (int y = 0; y < ny; ++y)
             for (int x = 0; x < nx; ++x) {
@@ -181,31 +221,7 @@ void classify_cells(const emfd_grid_config &cfg, const float *sigma, const float
                 };
                 float es[3] = {avg(sigma, ex), avg(sigma, ey), avg(sigma, ez)};
                 const float ee[3] = {avg(eps_r, ex), avg(eps_r, ey), avg(eps_r, ez)};
-                for (; oi < ov.size() && ov[oi].idx == c00; ++oi) es[ov[oi].axis] = ov[oi].sigma;
-                float wf[6] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f};
-                for (; wi < wv.size() && wv[wi].idx == c00; ++wi) wf[wv[wi].slot] = wv[wi].factor;
-                ClassKey k;
-                for (int c = 0; c < 3; ++c) {
-                    k[c] = f32_to_f16(std::min(es[c], kPerfectEdgeSigma));
-                    k[3 + c] = f32_to_f16(ee[c]);
-                }
-                for (int j = 0; j < 6; ++j) k[6 + j] = f32_to_f16(wf[j]);
-                uint32_t id;
-                if (k == last) {
-                    id = last_id;
-                } else {
-                    auto it = cm.map.find(k);
-                    if (it == cm.map.end()) {
-                        id = static_cast<uint32_t>(cm.keys.size());
-                        cm.map.emplace(k, id);
-                        cm.keys.push_back(k);
-                    } else {
-                        id = it->second;
-                    }
-                    last = k;
-                    last_id = id;
-                }
-                ids[c00] = id;
+                classify(c00, es, ee);
             }
 }
 

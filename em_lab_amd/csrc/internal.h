@@ -107,4 +107,53 @@ hipError_t launch_flux(const LatticeArgs &a, const FieldPtrs &f, const FluxPlane
 hipError_t launch_probe_gather(const LatticeArgs &a, const FieldPtrs &f, const int *origins, int n_probes,
                                const float *sigma_cell, const float *eps_cell, float *blocks, hipStream_t s);
 
+
+// ---- batched stepper (batch.hip / batch.cpp) ----
+// One scene of a batch as the device sees it. Quads are 4-cell x runs; a quad index q maps to
+// (x0, y, z) with r = q / qx, x0 = 4 (q - r qx), z = r / ny, y = r - z ny, the divisions done as
+// __umulhi by ceil(2^32 / d) (exact while q * d < 2^32, which the host checks).
+struct BatchScene {
+    LatticeArgs lat;
+    FieldPtrs f;
+    uint32_t nq, qx, mq, my;
+    // sources: per-texel lists as launch_apply_sources; signals at sig_offset of the step row
+    const int *src_cells;
+    const int *src_lists;
+    const float *src_packed;
+    int sig_offset;
+    // ports: rotation at rot_offset of the step row; series [step][port][2] or null
+    int n_ports;
+    const PortDev *ports;
+    float *phasors;
+    float *series;
+    int rot_offset;
+    float blend;
+};
+
+struct BatchLaunchArgs {
+    const BatchScene *scenes;
+    const int2 *tiles;  // (scene, first quad)
+    int n_tiles;
+    int slices;         // 1 or 8
+    int quads;          // quads per thread: 1, 2 or 4
+};
+hipError_t launch_batch_yee(const BatchLaunchArgs &a, int persistent_blocks, hipStream_t s);
+hipError_t launch_batch_sources(const BatchScene *scenes, const int2 *list, int n, const float *consts,
+                                int64_t stride, const int64_t *d_step, int k, hipStream_t s);
+hipError_t launch_batch_ports(const BatchScene *scenes, const int2 *list, int n, const float *consts,
+                              int64_t stride, const int64_t *d_step, int k, hipStream_t s);
+hipError_t launch_batch_advance(int64_t *d_step, int count, hipStream_t s);
+
+// FP64 DFT of one scene's recorded port series (PortMonitor::evaluateSweep's sums)
+struct BatchSweepJob {
+    const float *series;    // [n][ports][2]
+    int n_ports;
+    int points;
+    int64_t n;              // recorded samples
+    const double2 *step;    // per point: e^{-i w dt}
+    double2 *sums;          // [port][point][2]: sum V rot, sum I rot
+    double *residual;       // late / peak of |V| of port 0
+};
+hipError_t launch_batch_sweep(const BatchSweepJob *jobs, int n_jobs, hipStream_t s);
+
 }  // namespace emfd

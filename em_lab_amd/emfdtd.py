@@ -93,6 +93,17 @@ class ProbeSampleC(C.Structure):
                 ("sigma", C.c_float), ("epsilon_r", C.c_float)]
 
 
+class BatchSceneC(C.Structure):
+    _fields_ = [("config", GridConfigC), ("sigma", C.c_void_p), ("eps_r", C.c_void_p), ("n_cells", C.c_int64),
+                ("edges", C.c_void_p), ("n_edges", C.c_int64), ("wires", C.c_void_p), ("n_wires", C.c_int64),
+                ("sources", C.c_void_p), ("n_sources", C.c_int), ("ports", C.c_void_p), ("n_ports", C.c_int),
+                ("averaging_periods", C.c_float), ("sweep", C.c_void_p)]
+
+
+class BatchLaunchC(C.Structure):
+    _fields_ = [("quads_per_thread", C.c_int), ("persistent_blocks", C.c_int), ("xcd_slices", C.c_int)]
+
+
 EDGE_DTYPE = np.dtype([("texel", "<i4", 3), ("axis", "<i4"), ("sigma", "<f4")])
 WIRE_DTYPE = np.dtype([("texel", "<i4", 3), ("component", "<i4"), ("term", "<i4"), ("factor", "<f4")])
 
@@ -135,6 +146,27 @@ _SIGNATURES = {
     "emfd_flux_measure": (C.c_int, [_P, _P, C.c_int, _P]),
     "emfd_probe_measure": (C.c_int, [_P, _P, C.c_int, _P]),
     "emfd_copy_probe": (C.c_int, [_P, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
+    "emfd_batch_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "emfd_batch_destroy": (C.c_int, [_P]),
+    "emfd_batch_add_scene": (C.c_int, [_P, C.POINTER(BatchSceneC), C.POINTER(C.c_int), C.POINTER(GridConfigC)]),
+    "emfd_batch_finalize": (C.c_int, [_P, C.c_int64]),
+    "emfd_batch_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_size_t)]),
+    "emfd_batch_set_launch": (C.c_int, [_P, C.POINTER(BatchLaunchC)]),
+    "emfd_batch_get_launch": (C.c_int, [_P, C.POINTER(BatchLaunchC)]),
+    "emfd_batch_autotune": (C.c_int, [_P, C.c_int, C.POINTER(BatchLaunchC), _P, C.c_int, C.POINTER(C.c_int)]),
+    "emfd_batch_set_graph": (C.c_int, [_P, C.c_int]),
+    "emfd_batch_step": (C.c_int, [_P, C.c_int]),
+    "emfd_batch_synchronize": (C.c_int, [_P]),
+    "emfd_batch_state": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "emfd_batch_reset": (C.c_int, [_P]),
+    "emfd_batch_download_fields": (C.c_int, [_P, C.c_int, _P, _P]),
+    "emfd_batch_upload_fields": (C.c_int, [_P, C.c_int, _P, _P]),
+    "emfd_batch_read_phasors": (C.c_int, [_P, C.c_int, _P, C.c_int]),
+    "emfd_batch_read_series": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
+    "emfd_batch_evaluate_sweeps": (C.c_int, [_P]),
+    "emfd_batch_sweep_result": (C.c_int, [_P, C.c_int, _P, C.c_int, _P, C.POINTER(C.c_double)]),
+    "emfd_batch_kernel_timing": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
 }
 
 
@@ -605,6 +637,172 @@ class FieldAnalyzer:
         _check(self._lib.emfd_probe_measure(self.grid.handle, _ptr(p), p.shape[0], C.cast(arr, C.c_void_p)))
         return [dict(position=np.array(s.position), E=np.array(s.E), B=np.array(s.B), S=np.array(s.S),
                      sigma=s.sigma, epsilon_r=s.epsilon_r) for s in arr[: p.shape[0]]]
+
+
+class Batch:
+    """Batched stepper: many independent scenes advance together (emfd_batch_* in emfdtd.h).
+
+    Each scene is what SimulationGrid.configure + upload_materials + set_sources and
+    PortMonitor.configure (with an optional sweep) would set up alone, and gives bit-identical
+    fields, port phasors and series."""
+
+    def __init__(self, device: int = 0):
+        lib = load_library()
+        h = C.c_void_p()
+        _check(lib.emfd_batch_create(device, C.byref(h)))
+        self._h = h
+        self._lib = lib
+        self.configs: list = []
+        self.ports: list = []
+        self.sweeps: list = []
+        self._keep: list = []
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.emfd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_scene(self, config: GridConfig, sigma: Optional[np.ndarray] = None, eps_r: Optional[np.ndarray] = None,
+                  subcell: Optional[SubcellModel] = None, sources: Sequence[SourceDesc] = (),
+                  ports: Sequence[PortSetup] = (), averaging_periods: float = 4.0,
+                  sweep: Optional[SweepSetup] = None) -> int:
+        sc = BatchSceneC()
+        sc.config = config.to_c()
+        keep = []
+        if sigma is not None:
+            sg = np.ascontiguousarray(sigma, np.float32).ravel()
+            ep = np.ascontiguousarray(eps_r, np.float32).ravel()
+            keep += [sg, ep]
+            sc.sigma, sc.eps_r, sc.n_cells = _ptr(sg), _ptr(ep), sg.size
+        edges = subcell.edge_array() if subcell else np.zeros(0, EDGE_DTYPE)
+        wires = subcell.wire_array() if subcell else np.zeros(0, WIRE_DTYPE)
+        keep += [edges, wires]
+        sc.edges, sc.n_edges = (_ptr(edges) if edges.size else None), edges.size
+        sc.wires, sc.n_wires = (_ptr(wires) if wires.size else None), wires.size
+        src = (SourceDescC * max(len(sources), 1))(*[x.to_c() for x in sources])
+        prt = (PortSetupC * max(len(ports), 1))(*[x.to_c() for x in ports])
+        keep += [src, prt]
+        sc.sources, sc.n_sources = C.cast(src, C.c_void_p), len(sources)
+        sc.ports, sc.n_ports = C.cast(prt, C.c_void_p), len(ports)
+        sc.averaging_periods = averaging_periods
+        if sweep is not None:
+            sw = SweepSetupC(sweep.wavelength_min, sweep.wavelength_max, sweep.points)
+            keep.append(sw)
+            sc.sweep = C.cast(C.pointer(sw), C.c_void_p)
+        idx = C.c_int()
+        applied = GridConfigC()
+        _check(self._lib.emfd_batch_add_scene(self._h, C.byref(sc), C.byref(idx), C.byref(applied)))
+        self.configs.append(GridConfig.from_c(applied))
+        self.ports.append(list(ports))
+        self.sweeps.append(sweep)
+        return idx.value
+
+    def add(self, scene) -> int:
+        """Adds an em_lab_amd.scene.Scene (its thin wires must already be emitted)."""
+        s, e = scene.flat_materials()
+        vacuum = not np.any(s) and np.all(e == 1.0)
+        return self.add_scene(scene.grid_config(), None if vacuum else s, None if vacuum else e, scene.subcell,
+                              scene.sources, scene.ports,
+                              scene.far_field.averaging_periods if scene.far_field else 4.0, scene.sweep)
+
+    def finalize(self, max_steps: int) -> None:
+        _check(self._lib.emfd_batch_finalize(self._h, int(max_steps)))
+
+    def info(self) -> dict:
+        n, cells, tiles, nbytes = C.c_int(), C.c_int64(), C.c_int64(), C.c_size_t()
+        _check(self._lib.emfd_batch_info(self._h, C.byref(n), C.byref(cells), C.byref(tiles), C.byref(nbytes)))
+        return dict(scenes=n.value, cells=cells.value, tiles=tiles.value, device_bytes=nbytes.value)
+
+    def set_launch(self, quads_per_thread: int = 1, persistent_blocks: int = 0, xcd_slices: int = 8) -> None:
+        _check(self._lib.emfd_batch_set_launch(self._h, C.byref(BatchLaunchC(quads_per_thread, persistent_blocks,
+                                                                               xcd_slices))))
+
+    def launch(self) -> tuple:
+        l = BatchLaunchC()
+        _check(self._lib.emfd_batch_get_launch(self._h, C.byref(l)))
+        return (l.quads_per_thread, l.persistent_blocks, l.xcd_slices)
+
+    def autotune(self, iterations: int = 5):
+        chosen = BatchLaunchC()
+        cap = 64
+        table = np.zeros(3 * cap, np.float64)
+        n = C.c_int()
+        _check(self._lib.emfd_batch_autotune(self._h, iterations, C.byref(chosen), _ptr(table), cap, C.byref(n)))
+        return (chosen.quads_per_thread, chosen.persistent_blocks, chosen.xcd_slices), table[:3 * n.value].reshape(-1, 3)
+
+    def set_graph(self, steps_per_graph: int) -> None:
+        _check(self._lib.emfd_batch_set_graph(self._h, int(steps_per_graph)))
+
+    def step(self, count: int) -> None:
+        _check(self._lib.emfd_batch_step(self._h, int(count)))
+
+    def synchronize(self) -> None:
+        _check(self._lib.emfd_batch_synchronize(self._h))
+
+    @property
+    def step_count(self) -> int:
+        n = C.c_uint64()
+        _check(self._lib.emfd_batch_state(self._h, C.byref(n)))
+        return n.value
+
+    def reset(self) -> None:
+        _check(self._lib.emfd_batch_reset(self._h))
+
+    def fields(self, scene: int):
+        nx, ny, nz = self.configs[scene].size
+        n = nx * ny * nz
+        E = np.empty(3 * n, np.float32)
+        B = np.empty(3 * n, np.float32)
+        _check(self._lib.emfd_batch_download_fields(self._h, scene, _ptr(E), _ptr(B)))
+        return E, B
+
+    def set_fields(self, scene: int, E: Optional[np.ndarray], B: Optional[np.ndarray]) -> None:
+        e = None if E is None else np.ascontiguousarray(E, np.float32).ravel()
+        b = None if B is None else np.ascontiguousarray(B, np.float32).ravel()
+        _check(self._lib.emfd_batch_upload_fields(self._h, scene, _ptr(e), _ptr(b)))
+
+    def phasors(self, scene: int) -> np.ndarray:
+        n = len(self.ports[scene])
+        out = np.zeros(8 * max(n, 1), np.float32)
+        _check(self._lib.emfd_batch_read_phasors(self._h, scene, _ptr(out), out.size))
+        return out[:8 * n].reshape(n, 8)
+
+    def series(self, scene: int) -> np.ndarray:
+        n = len(self.ports[scene])
+        out = np.zeros(2 * n * max(self.step_count, 1), np.float32)
+        _check(self._lib.emfd_batch_read_series(self._h, scene, _ptr(out), out.size))
+        return out[:2 * n * self.step_count].reshape(self.step_count, n, 2)
+
+    def evaluate_sweeps(self) -> None:
+        _check(self._lib.emfd_batch_evaluate_sweeps(self._h))
+
+    def sweep_result(self, scene: int) -> dict:
+        n = len(self.ports[scene])
+        pts = max(self.sweeps[scene].points, 2)
+        parr = (SweepPointC * (n * pts))()
+        sarr = (PortSweepC * max(n, 1))()
+        res = C.c_double()
+        _check(self._lib.emfd_batch_sweep_result(self._h, scene, C.cast(parr, C.c_void_p), n * pts,
+                                                  C.cast(sarr, C.c_void_p), C.byref(res)))
+        ports = []
+        for p in range(n):
+            points = np.array([(q.wavelength, q.z_re, q.z_im, q.s11_db, q.vswr) for q in parr[p * pts:(p + 1) * pts]])
+            s = sarr[p]
+            ports.append(dict(points=points, best=s.best, best_wavelength=s.best_wavelength, has_band=bool(s.has_band),
+                              band_low_wavelength=s.band_low_wavelength, band_high_wavelength=s.band_high_wavelength))
+        return dict(residual=res.value, ports=ports)
+
+    def kernel_timing(self, enable: int = -1):
+        ms = C.c_double()
+        n = C.c_int64()
+        _check(self._lib.emfd_batch_kernel_timing(self._h, enable, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
 
 def version() -> str:

@@ -431,3 +431,85 @@ def spheres_dipole(n=256, pml=10, n_spheres=32, wavelength=20.0, seed=DEFAULT_SE
     sc.fluxes.append(FluxSurface.box((m, m, m), (n - 1 - m, n - 1 - m, n - 1 - m), "flux box"))
     sc.emit_thin_wires()
     return sc
+
+
+# ---------------------------------------------------------------------------------------------
+# Batched antenna scenes: parallel z thin-wire elements in vacuum, element 0 fed
+# ---------------------------------------------------------------------------------------------
+@dataclass
+class ArrayGeometry:
+    """Realised geometry of a parallel-wire antenna, in cells: lengths[i] of element i,
+    spacings[i] = x distance from element i-1 (spacings[0] = 0), radii[i]."""
+    lengths: list
+    spacings: list
+    radii: list
+
+    @property
+    def elements(self) -> int:
+        return len(self.lengths)
+
+    def span(self) -> int:
+        return int(sum(self.spacings))
+
+
+ARRAY_LAMBDA0 = 16.0
+ARRAY_PML = 8
+ARRAY_MAX_ELEMENTS = 6
+
+
+def draw_array_geometry(rng: "Xoshiro256ss", lambda0: float = ARRAY_LAMBDA0) -> ArrayGeometry:
+    """1-6 parallel z wires: length U(0.3, 0.55) lambda0 (element 0 realised as an odd dipole of
+    at least 5 cells, the others as whole cells), spacing U(0.1, 0.4) lambda0 (whole cells, at
+    least 1), radius log-U(0.001, 0.01) lambda0."""
+    n = 1 + min(int(rng.uniform(0.0, 6.0)), 5)
+    lengths, spacings, radii = [], [], []
+    for i in range(n):
+        L = rng.uniform(0.3, 0.55) * lambda0
+        if i == 0:
+            arm = max(2, round_half_away((max(L, 5.0) - 1.0) * 0.5))
+            lengths.append(2 * arm + 1)
+            spacings.append(0)
+        else:
+            lengths.append(max(1, round_half_away(L)))
+            spacings.append(max(1, round_half_away(rng.uniform(0.1, 0.4) * lambda0)))
+        radii.append(float(f32(rng.log_uniform(0.001, 0.01) * lambda0)))
+    return ArrayGeometry(lengths, spacings, radii)
+
+
+def array_scene(geom: ArrayGeometry, size, lambda0: float = ARRAY_LAMBDA0, pml: int = ARRAY_PML,
+                sweep_points: int = 64, pulse_width: float = 0.75) -> Scene:
+    """Vacuum grid of `size` with the wires of `geom` centred in x/y/z; element 0 is a thin-wire
+    dipole behind a 50 ohm resistive feed (Gaussian pulse at lambda0), the others are passive
+    wire{} elements; a port sweep over 0.5-1.5 f0."""
+    nx, ny, nz = (int(v) for v in size)
+    sc = Scene(size=(nx, ny, nz), pml=pml, courant=MAX_COURANT, cell_size=1.0e-3, wavelength=lambda0)
+    xs = [0]
+    for s in geom.spacings[1:]:
+        xs.append(xs[-1] + int(s))
+    x0 = nx // 2 - geom.span() // 2
+    cy, cz = ny // 2, nz // 2
+    sc.dipole_thin((x0 + xs[0], cy, cz + 0.5), axis=2, length=float(geom.lengths[0]), radius=geom.radii[0],
+                   impedance=50.0, amplitude=1.0, wavelength=lambda0, waveform="pulse", pulse_width=pulse_width)
+    for i in range(1, geom.elements):
+        n = int(geom.lengths[i])
+        z0 = cz - n // 2
+        sc.wire((x0 + xs[i], cy, z0), (x0 + xs[i], cy, z0 + n), radius=geom.radii[i])
+    sc.set_sweep((lambda0 / 1.5, lambda0 / 0.5), points=sweep_points)
+    sc.emit_thin_wires()
+    return sc
+
+
+def draw_array_scene(index: int, seed: int = DEFAULT_SEED, lambda0: float = ARRAY_LAMBDA0):
+    """Scene `index` of the batched generator: its own xoshiro256** stream (seed + index), the
+    geometry first, then the grid size over multiples of 8 in [48,128] x [48,128] x [96,160]
+    (x at least the array span plus both absorbers and 8 cells)."""
+    rng = Xoshiro256ss(seed * 1000003 + index)
+    geom = draw_array_geometry(rng, lambda0)
+    lo_x = max(48, -(-(geom.span() + 2 * ARRAY_PML + 8) // 8) * 8)
+
+    def pick(lo, hi):
+        k = (hi - lo) // 8 + 1
+        return lo + 8 * min(int(rng.uniform(0.0, float(k))), k - 1)
+
+    size = (pick(lo_x, 128), pick(48, 128), pick(96, 160))
+    return geom, array_scene(geom, size, lambda0)

@@ -284,6 +284,87 @@ int emfd_flux_measure(emfd_ctx *ctx, const emfd_flux_surface *surfaces, int n, d
 int emfd_probe_measure(emfd_ctx *ctx, const float *positions, int n, emfd_probe_sample *out);
 
 /* ---------------------------------------------------------------------------------------
+ * Batched stepper (no counterpart in the reference, which runs one grid per process,
+ * src/Main.cpp:239-246): many independent scenes of uneven sizes advance together. Each scene
+ * is what emfd_grid_configure + emfd_grid_upload_materials + emfd_grid_set_sources +
+ * emfd_ports_configure would set up on its own context, and its fields, port phasors and
+ * series are bit-identical to that single-grid run (same per-texel arithmetic). The Yee
+ * passes run as one launch over a flat list of (scene, 1024..4096-cell) tiles; a step
+ * sequence can be captured in a HIP graph; a launch-shape autotuner picks the tiling.
+ * --------------------------------------------------------------------------------------- */
+typedef struct emfd_batch emfd_batch;
+
+typedef struct {
+    emfd_grid_config config;                /* clamped as emfd_grid_configure; nx % 4 == 0 after clamping */
+    const float *sigma;                     /* cell sigma / eps_r (n cells each), or both NULL for vacuum */
+    const float *eps_r;
+    int64_t n_cells;
+    const emfd_edge_override *edges;
+    int64_t n_edges;
+    const emfd_wire_coefficient *wires;
+    int64_t n_wires;
+    const emfd_source_desc *sources;
+    int n_sources;
+    const emfd_port_setup *ports;           /* at most EMFD_BATCH_MAX_PORTS */
+    int n_ports;
+    float averaging_periods;
+    const emfd_sweep_setup *sweep;          /* NULL: no sweep for this scene */
+} emfd_batch_scene;
+
+#define EMFD_BATCH_MAX_PORTS 8
+
+/* Launch shape of the batched Yee passes. */
+typedef struct {
+    int quads_per_thread;  /* 1, 2 or 4: a tile is 256 threads x this many 4-cell x-quads */
+    int persistent_blocks; /* 0: one workgroup per tile; > 0: this many workgroups loop over the tiles */
+    int xcd_slices;        /* 1: plain order; 8: the tile list is cut in 8 contiguous slices, one per XCD */
+} emfd_batch_launch;
+
+int emfd_batch_create(int device, emfd_batch **out);
+int emfd_batch_destroy(emfd_batch *b);
+/* Adds a scene (allocates its device buffers and uploads its materials); *index = its number.
+ * `applied` (may be NULL) receives the clamped configuration. */
+int emfd_batch_add_scene(emfd_batch *b, const emfd_batch_scene *scene, int *index, emfd_grid_config *applied);
+/* Builds the tile list and the per-step constant table for max_steps steps (source signals and
+ * port rotations, FP64 on the host cast to float, as emfd_grid_step) and the port series
+ * buffers. Scenes cannot be added afterwards. */
+int emfd_batch_finalize(emfd_batch *b, int64_t max_steps);
+int emfd_batch_info(emfd_batch *b, int *n_scenes, int64_t *total_cells, int64_t *n_tiles, size_t *device_bytes);
+int emfd_batch_set_launch(emfd_batch *b, const emfd_batch_launch *launch);
+int emfd_batch_get_launch(emfd_batch *b, emfd_batch_launch *launch);
+/* Times each candidate launch shape over `iterations` Yee steps and keeps the fastest. Only
+ * before the first step: the fields are zero then, and the Yee passes map zero to zero, so the
+ * state is unchanged. ms_per_step: 3 doubles per candidate tried (may be NULL, cap entries). */
+int emfd_batch_autotune(emfd_batch *b, int iterations, emfd_batch_launch *chosen, double *ms_per_step, int cap,
+                        int *n_tried);
+/* steps_per_graph > 0: emfd_batch_step replays a HIP graph of that many steps (captured on
+ * first use) for whole chunks and launches the remainder directly; 0: direct launches only. */
+int emfd_batch_set_graph(emfd_batch *b, int steps_per_graph);
+/* Advances every scene by `count` steps (B, E, sources, ++step, ports: emfd_grid_step's order). */
+int emfd_batch_step(emfd_batch *b, int count);
+int emfd_batch_synchronize(emfd_batch *b);
+int emfd_batch_state(emfd_batch *b, uint64_t *step_count);
+/* Zero all fields, port phasors and series; rewind to step 0. */
+int emfd_batch_reset(emfd_batch *b);
+/* Scene fields as EMFD_LAYOUT_SOA planes (3 x n floats each; either pointer may be NULL). */
+int emfd_batch_download_fields(emfd_batch *b, int scene, float *E, float *B);
+int emfd_batch_upload_fields(emfd_batch *b, int scene, const float *E, const float *B);
+int emfd_batch_read_phasors(emfd_batch *b, int scene, float *out, int n);
+/* (V, I) per recorded step and port: out[2 * (k * n_ports + p) + {0, 1}]. */
+int emfd_batch_read_series(emfd_batch *b, int scene, float *out, int64_t n);
+/* PortMonitor::evaluateSweep for every scene with a sweep: the FP64 DFT of the recorded series
+ * runs on the GPU (one workgroup per scene, one lane per frequency, the same rotation recurrence
+ * renormalised every 1024 samples); Z, S11, VSWR and the band summary follow on the host as in
+ * emfd_ports_evaluate_sweep. Results are kept for emfd_batch_sweep_result. */
+int emfd_batch_evaluate_sweeps(emfd_batch *b);
+/* points: n_ports x sweep points of the scene (may be NULL); summary: n_ports entries (may be
+ * NULL); residual: ring-down ratio of port 0 (may be NULL). */
+int emfd_batch_sweep_result(emfd_batch *b, int scene, emfd_sweep_point *points, int cap_points,
+                            emfd_port_sweep *summary, double *residual);
+/* Yee launch timing as emfd_grid_kernel_timing (both passes of a step form one launch). */
+int emfd_batch_kernel_timing(emfd_batch *b, int enable, double *total_ms, int64_t *launches);
+
+/* ---------------------------------------------------------------------------------------
  * Measurement helper (no counterpart in the reference: it calibrates the roofline figures)
  * --------------------------------------------------------------------------------------- */
 /* Copies `bytes` (rounded down to 16) between two scratch buffers `iterations` times on the
