@@ -16,8 +16,8 @@ LIB_DIR = PKG / "_lib"
 LIB = LIB_DIR / "libemfdtd.so"
 OBJ_DIR = PKG / "_build"
 
-SOURCES = ["kernels.hip", "yee_rows.hip", "yee_point.hip", "yee_two_pass4.hip", "monitors.hip", "batch.hip", "grid.cpp",
-           "monitors.cpp", "batch.cpp"]
+SOURCES = ["kernels.hip", "yee_rows.hip", "yee_point.hip", "yee_two_pass4.hip", "yee_half.hip", "monitors.hip", "batch.hip", "mlp.hip",
+           "grid.cpp", "monitors.cpp", "batch.cpp", "mlp.cpp"]
 
 # -ffp-contract=off: every expression rounds as the reference shader writes it (no fused
 # multiply-add), so results can be compared against the CPU restatement to the last ulp

@@ -284,11 +284,22 @@ using namespace emfd;
 
 emfd::FieldPtrs emfd_ctx::fields(int which) const {
     FieldPtrs f;
-    float *base = field_buf[which].as<float>();
-    for (int c = 0; c < 3; ++c) {
-        f.e[c] = base + static_cast<size_t>(c) * n;
-        f.b[c] = base + static_cast<size_t>(3 + c) * n;
+    if (storage == kStorageF32) {
+        float *base = field_buf[which].as<float>();
+        for (int c = 0; c < 3; ++c) {
+            f.e[c] = base + static_cast<size_t>(c) * n;
+            f.b[c] = base + static_cast<size_t>(3 + c) * n;
+        }
+        return f;
     }
+    // 16-bit planes: the pointers carry 16-bit element addresses
+    uint16_t *base = field_buf[which].as<uint16_t>();
+    for (int c = 0; c < 3; ++c) {
+        f.e[c] = reinterpret_cast<float *>(base + static_cast<size_t>(c) * n);
+        f.b[c] = reinterpret_cast<float *>(base + static_cast<size_t>(3 + c) * n);
+    }
+    f.storage = storage;
+    for (int c = 0; c < 6; ++c) f.scale[c] = std::ldexp(1.0f, scale_log2[c]);
     return f;
 }
 
@@ -368,8 +379,10 @@ int emfd_grid_configure(emfd_ctx *ctx, const emfd_grid_config *requested, emfd_g
     if (n > (size_t(1) << 34)) return fail(EMFD_ERR_INVALID, "SimulationGrid: grid too large");
 
     if (n != ctx->n || !ctx->field_buf[0].ptr) {
-        for (int k = 0; k < 2; ++k) {
-            int rc = ctx->field_buf[k].ensure(n * 6 * sizeof(float));
+        ctx->n = n;
+        // the ping-pong buffer serves the fused FP32 kernels only
+        for (int k = 0; k < (ctx->storage == kStorageF32 ? 2 : 1); ++k) {
+            int rc = ctx->field_buf[k].ensure(ctx->field_bytes());
             if (rc) return rc;
         }
         if (int rc = ctx->sigma_cell.ensure(n * sizeof(float))) return rc;
@@ -394,7 +407,7 @@ int emfd_grid_configure(emfd_ctx *ctx, const emfd_grid_config *requested, emfd_g
     ctx->classes = 0;
     ctx->has_wire = 0;
     ctx->id_bytes = 0;
-    EMFD_HIP(hipMemsetAsync(ctx->field_buf[0].ptr, 0, n * 6 * sizeof(float), ctx->stream));
+    EMFD_HIP(hipMemsetAsync(ctx->field_buf[0].ptr, 0, ctx->field_bytes(), ctx->stream));
     EMFD_HIP(hipMemsetAsync(ctx->sigma_cell.ptr, 0, n * sizeof(float), ctx->stream));
     EMFD_HIP(launch_fill(ctx->eps_cell.as<float>(), n, 1.0f, ctx->stream));
     ctx->step_count = 0;
@@ -458,7 +471,7 @@ int emfd_grid_upload_materials(emfd_ctx *ctx, const float *sigma, const float *e
     }
     // uploadMaterials clears E and B and rewinds time (src/SimulationGrid.cpp:216-222,294)
     ctx->cur = 0;
-    EMFD_HIP(hipMemsetAsync(ctx->field_buf[0].ptr, 0, n * 6 * sizeof(float), ctx->stream));
+    EMFD_HIP(hipMemsetAsync(ctx->field_buf[0].ptr, 0, ctx->field_bytes(), ctx->stream));
     ctx->step_count = 0;
     EMFD_HIP(hipStreamSynchronize(ctx->stream));
     return EMFD_OK;
@@ -497,7 +510,36 @@ int emfd_grid_reset_fields(emfd_ctx *ctx) {
     if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
     EMFD_HIP(hipSetDevice(ctx->device));
     ctx->cur = 0;
-    EMFD_HIP(launch_fill(ctx->field_buf[0].as<float>(), ctx->n * 6, 0.0f, ctx->stream));
+    EMFD_HIP(launch_fill(ctx->field_buf[0].as<float>(), ctx->field_bytes() / 4, 0.0f, ctx->stream));
+    ctx->step_count = 0;
+    return EMFD_OK;
+}
+
+int emfd_grid_set_storage(emfd_ctx *ctx, int storage, const int *scale_log2) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
+    if (storage != EMFD_STORAGE_F32 && storage != EMFD_STORAGE_F16 && storage != EMFD_STORAGE_BF16)
+        return fail(EMFD_ERR_INVALID, "unknown field storage");
+    if (storage != EMFD_STORAGE_F32 && ctx->cfg.size[0] % 8 != 0)
+        return fail(EMFD_ERR_INVALID, "16-bit field storage needs nx % 8 == 0");
+    int k[6] = {0, 0, 0, 0, 0, 0};
+    if (scale_log2)
+        for (int c = 0; c < 6; ++c) {
+            if (scale_log2[c] < -100 || scale_log2[c] > 100) return fail(EMFD_ERR_INVALID, "scale exponent out of range");
+            k[c] = scale_log2[c];
+        }
+    EMFD_HIP(hipSetDevice(ctx->device));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->storage = storage;
+    std::copy(k, k + 6, ctx->scale_log2);
+    ctx->field_buf[0].release();
+    ctx->field_buf[1].release();
+    for (int b = 0; b < (storage == EMFD_STORAGE_F32 ? 2 : 1); ++b)
+        if (int rc = ctx->field_buf[b].ensure(ctx->field_bytes())) return rc;
+    // as uploadMaterials: fields cleared, time rewound
+    ctx->cur = 0;
+    EMFD_HIP(hipMemsetAsync(ctx->field_buf[0].ptr, 0, ctx->field_bytes(), ctx->stream));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
     ctx->step_count = 0;
     return EMFD_OK;
 }
@@ -596,13 +638,20 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
             EMFD_HIP(hipEventRecord(ev0, ctx->stream));
         }
         int variant = ctx->kernel_variant;
-        if (variant == EMFD_KERNEL_AUTO)
+        if (ctx->storage != kStorageF32) {
+            if (variant != EMFD_KERNEL_AUTO)
+                return fail(EMFD_ERR_INVALID, "16-bit field storage runs the automatic (two-pass) kernel only");
+            EMFD_HIP(launch_yee_half(lat, ctx->fields(ctx->cur), ctx->stream));
+            variant = -1;
+        } else if (variant == EMFD_KERNEL_AUTO)
             variant = yee_two_pass4_supported(lat) ? EMFD_KERNEL_TWO_PASS4 : EMFD_KERNEL_TWO_PASS;
         if (variant == EMFD_KERNEL_TWO_PASS4 && !yee_two_pass4_supported(lat))
             return fail(EMFD_ERR_INVALID, "the vectorised two-pass kernel needs nx % 4 == 0");
         if (variant == EMFD_KERNEL_ROWS && !yee_rows_supported(lat))
             return fail(EMFD_ERR_INVALID, "the row-wavefront kernel supports nx <= 256");
-        if (variant == EMFD_KERNEL_TWO_PASS) {
+        if (variant == -1) {
+            // launched above
+        } else if (variant == EMFD_KERNEL_TWO_PASS) {
             EMFD_HIP(launch_yee_two_pass(lat, ctx->fields(ctx->cur), ctx->stream));
         } else if (variant == EMFD_KERNEL_TWO_PASS4) {
             EMFD_HIP(launch_yee_two_pass4(lat, ctx->fields(ctx->cur), ctx->stream));
@@ -647,6 +696,67 @@ int emfd_grid_state(emfd_ctx *ctx, double *time, uint64_t *step_count, float *dt
     return EMFD_OK;
 }
 
+namespace {
+// Component planes of the current fields as FP32 values on the host (3 planes of n for E or B).
+int read_planes(emfd_ctx *ctx, int which, std::vector<float> &out) {
+    const size_t n = ctx->n;
+    out.resize(3 * n);
+    if (ctx->storage == kStorageF32) {
+        EMFD_HIP(hipMemcpy(out.data(), ctx->field_buf[ctx->cur].as<float>() + static_cast<size_t>(3 * which) * n,
+                           3 * n * sizeof(float), hipMemcpyDeviceToHost));
+        return EMFD_OK;
+    }
+    std::vector<uint16_t> raw(3 * n);
+    EMFD_HIP(hipMemcpy(raw.data(), ctx->field_buf[ctx->cur].as<uint16_t>() + static_cast<size_t>(3 * which) * n,
+                       3 * n * 2, hipMemcpyDeviceToHost));
+    for (int c = 0; c < 3; ++c) {
+        const float s = std::ldexp(1.0f, ctx->scale_log2[3 * which + c]);
+        for (size_t i = 0; i < n; ++i) {
+            const uint16_t h = raw[static_cast<size_t>(c) * n + i];
+            float v;
+            if (ctx->storage == kStorageF16) {
+                _Float16 x;
+                std::memcpy(&x, &h, 2);
+                v = static_cast<float>(x);
+            } else {
+                const uint32_t u = static_cast<uint32_t>(h) << 16;
+                std::memcpy(&v, &u, 4);
+            }
+            out[static_cast<size_t>(c) * n + i] = v * s;
+        }
+    }
+    return EMFD_OK;
+}
+
+uint16_t f32_to_bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);  // quiet NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+
+int write_planes(emfd_ctx *ctx, int which, const std::vector<float> &in) {
+    const size_t n = ctx->n;
+    if (ctx->storage == kStorageF32) {
+        EMFD_HIP(hipMemcpy(ctx->field_buf[ctx->cur].as<float>() + static_cast<size_t>(3 * which) * n, in.data(),
+                           3 * n * sizeof(float), hipMemcpyHostToDevice));
+        return EMFD_OK;
+    }
+    std::vector<uint16_t> raw(3 * n);
+    for (int c = 0; c < 3; ++c) {
+        const float inv = std::ldexp(1.0f, -ctx->scale_log2[3 * which + c]);
+        for (size_t i = 0; i < n; ++i) {
+            const float v = in[static_cast<size_t>(c) * n + i] * inv;
+            raw[static_cast<size_t>(c) * n + i] = ctx->storage == kStorageF16 ? f32_to_f16(v) : f32_to_bf16(v);
+        }
+    }
+    EMFD_HIP(hipMemcpy(ctx->field_buf[ctx->cur].as<uint16_t>() + static_cast<size_t>(3 * which) * n, raw.data(),
+                       3 * n * 2, hipMemcpyHostToDevice));
+    return EMFD_OK;
+}
+}  // namespace
+
 int emfd_grid_download_fields(emfd_ctx *ctx, float *E, float *B, int layout) {
     if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
     if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
@@ -654,18 +764,21 @@ int emfd_grid_download_fields(emfd_ctx *ctx, float *E, float *B, int layout) {
     EMFD_HIP(hipSetDevice(ctx->device));
     EMFD_HIP(hipStreamSynchronize(ctx->stream));
     const size_t n = ctx->n;
-    const float *base = ctx->field_buf[ctx->cur].as<float>();
-    if (layout == EMFD_LAYOUT_SOA) {
+    if (layout == EMFD_LAYOUT_SOA && ctx->storage == kStorageF32) {
+        const float *base = ctx->field_buf[ctx->cur].as<float>();
         if (E) EMFD_HIP(hipMemcpy(E, base, 3 * n * sizeof(float), hipMemcpyDeviceToHost));
         if (B) EMFD_HIP(hipMemcpy(B, base + 3 * n, 3 * n * sizeof(float), hipMemcpyDeviceToHost));
         return EMFD_OK;
     }
-    std::vector<float> plane(3 * n), alpha(n);
+    std::vector<float> plane, alpha(n);
     for (int f = 0; f < 2; ++f) {
         float *dst = f == 0 ? E : B;
         if (!dst) continue;
-        EMFD_HIP(hipMemcpy(plane.data(), base + static_cast<size_t>(3 * f) * n, 3 * n * sizeof(float),
-                           hipMemcpyDeviceToHost));
+        if (int rc = read_planes(ctx, f, plane)) return rc;
+        if (layout == EMFD_LAYOUT_SOA) {
+            std::copy(plane.begin(), plane.end(), dst);
+            continue;
+        }
         EMFD_HIP(hipMemcpy(alpha.data(), f == 0 ? ctx->sigma_cell.ptr : ctx->eps_cell.ptr, n * sizeof(float),
                            hipMemcpyDeviceToHost));
         for (size_t i = 0; i < n; ++i) {
@@ -685,20 +798,17 @@ int emfd_grid_upload_fields(emfd_ctx *ctx, const float *E, const float *B, int l
     EMFD_HIP(hipSetDevice(ctx->device));
     EMFD_HIP(hipStreamSynchronize(ctx->stream));
     const size_t n = ctx->n;
-    float *base = ctx->field_buf[ctx->cur].as<float>();
     for (int f = 0; f < 2; ++f) {
         const float *srcp = f == 0 ? E : B;
         if (!srcp) continue;
+        std::vector<float> plane(3 * n);
         if (layout == EMFD_LAYOUT_SOA) {
-            EMFD_HIP(hipMemcpy(base + static_cast<size_t>(3 * f) * n, srcp, 3 * n * sizeof(float),
-                               hipMemcpyHostToDevice));
+            std::copy(srcp, srcp + 3 * n, plane.begin());
         } else {
-            std::vector<float> plane(3 * n);
             for (size_t i = 0; i < n; ++i)
                 for (int c = 0; c < 3; ++c) plane[static_cast<size_t>(c) * n + i] = srcp[4 * i + c];
-            EMFD_HIP(hipMemcpy(base + static_cast<size_t>(3 * f) * n, plane.data(), 3 * n * sizeof(float),
-                               hipMemcpyHostToDevice));
         }
+        if (int rc = write_planes(ctx, f, plane)) return rc;
     }
     return EMFD_OK;
 }

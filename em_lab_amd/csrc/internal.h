@@ -44,9 +44,15 @@ struct LatticeArgs {
 constexpr int kPmlStride = 16;
 enum PmlField : int { kPmlBa = 0, kPmlBb = 3, kPmlEa = 6, kPmlEb = 9 };
 
+// Field storage of a lattice: FP32 planes, or 16-bit planes (binary16 or bfloat16) holding
+// value / scale with a power-of-two scale per component (updates still run in FP32).
+enum FieldStorage : int { kStorageF32 = 0, kStorageF16 = 1, kStorageBF16 = 2 };
+
 struct FieldPtrs {
-    float *e[3];
+    float *e[3];  // component planes; for 16-bit storage they point at 16-bit elements
     float *b[3];
+    int storage = kStorageF32;
+    float scale[6] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f};  // E x y z, B x y z: value = stored * scale
 };
 
 // Device-side step constants: per step k, nsrc signals, then port rotation (4), then the
@@ -69,6 +75,8 @@ hipError_t launch_yee_point(const LatticeArgs &a, const FieldPtrs &src, const Fi
 hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
 hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
 int yee_two_pass4_supported(const LatticeArgs &a);
+hipError_t launch_yee_half(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
+int yee_half_supported(const LatticeArgs &a);
 hipError_t launch_apply_sources(const LatticeArgs &a, const FieldPtrs &f, const int *cells, int n_cells,
                                 const int *lists, const float *packed, const float *signals, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
@@ -155,5 +163,30 @@ struct BatchSweepJob {
     double *residual;       // late / peak of |V| of port 0
 };
 hipError_t launch_batch_sweep(const BatchSweepJob *jobs, int n_jobs, hipStream_t s);
+
+// ---- MLP surrogate (mlp.hip / mlp.cpp) ----
+// BF16 GEMM with an FP32-accumulating MFMA core and a fused epilogue; see mlp.hip.
+enum MlpEpilogue : int { MLP_EPI_STORE = 0, MLP_EPI_BIAS = 1, MLP_EPI_BIAS_GELU = 2, MLP_EPI_GELU_GRAD = 3 };
+struct MlpGemm {
+    const void *A;  // bf16
+    const void *B;  // bf16
+    bool a_t, b_t;
+    int M, N, K, lda, ldb;
+    int epi;
+    const float *bias;
+    const float *aux;
+    float *out_f32;
+    void *out_bf16;
+    int ldc;
+};
+hipError_t launch_mlp_gemm(const MlpGemm &m, hipStream_t s);
+hipError_t launch_mlp_mse(const float *y, const float *t, size_t n, float scale, float *g, void *gb,
+                          double *partials, int blocks, hipStream_t s);
+hipError_t launch_mlp_colsum(const float *g, int M, int N, float *db, hipStream_t s);
+hipError_t launch_mlp_gather(const float *x, int k_in, int ld, const float *t, int n_t, const int *idx, int rows,
+                             void *xb, float *tb, hipStream_t s);
+hipError_t launch_mlp_adam(float *p, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2,
+                           float eps, float wd, float c1, float c2, const int *bf16_index, void *wb, hipStream_t s);
+hipError_t launch_mlp_to_bf16(const float *p, const int *bf16_index, size_t n, void *wb, hipStream_t s);
 
 }  // namespace emfd

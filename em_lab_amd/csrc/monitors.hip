@@ -256,9 +256,9 @@ __global__ void __launch_bounds__(64) k_probe_gather(LatticeArgs a, FieldPtrs f,
     const size_t p = (static_cast<size_t>(z) * a.ny + y) * a.nx + x;
     float *o = blocks + 4 * static_cast<size_t>(id);
     if (k < 27) {
-        o[0] = f.e[0][p]; o[1] = f.e[1][p]; o[2] = f.e[2][p]; o[3] = sigma_cell[p];
+        o[0] = field_load(f, 0, p); o[1] = field_load(f, 1, p); o[2] = field_load(f, 2, p); o[3] = sigma_cell[p];
     } else {
-        o[0] = f.b[0][p]; o[1] = f.b[1][p]; o[2] = f.b[2][p]; o[3] = eps_cell[p];
+        o[0] = field_load(f, 3, p); o[1] = field_load(f, 4, p); o[2] = field_load(f, 5, p); o[3] = eps_cell[p];
     }
 }
 

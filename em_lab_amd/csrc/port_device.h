@@ -6,28 +6,47 @@
 
 namespace emfd {
 
-__device__ __forceinline__ float fetch_c(const LatticeArgs &a, const float *plane, int x, int y, int z) {
+// One field value of component comp (0..2: E, 3..5: B) at lattice index i, any storage.
+__device__ __forceinline__ float field_load(const FieldPtrs &f, int comp, size_t i) {
+    const void *plane = comp < 3 ? static_cast<const void *>(f.e[comp]) : static_cast<const void *>(f.b[comp - 3]);
+    if (f.storage == kStorageF32) return static_cast<const float *>(plane)[i];
+    if (f.storage == kStorageF16) return static_cast<float>(static_cast<const _Float16 *>(plane)[i]) * f.scale[comp];
+    return static_cast<float>(static_cast<const __bf16 *>(plane)[i]) * f.scale[comp];
+}
+
+__device__ __forceinline__ void field_store(const FieldPtrs &f, int comp, size_t i, float v) {
+    void *plane = comp < 3 ? static_cast<void *>(f.e[comp]) : static_cast<void *>(f.b[comp - 3]);
+    if (f.storage == kStorageF32) {
+        static_cast<float *>(plane)[i] = v;
+    } else if (f.storage == kStorageF16) {
+        static_cast<_Float16 *>(plane)[i] = static_cast<_Float16>(v / f.scale[comp]);
+    } else {
+        static_cast<__bf16 *>(plane)[i] = static_cast<__bf16>(v / f.scale[comp]);
+    }
+}
+
+__device__ __forceinline__ float fetch_c(const LatticeArgs &a, const FieldPtrs &f, int comp, int x, int y, int z) {
     // texelFetch with indices clamped to the grid (shaders/field_sampling.glsl:14-15,
     // shaders/port_accumulate.comp:40-41)
     x = min(max(x, 0), a.nx - 1);
     y = min(max(y, 0), a.ny - 1);
     z = min(max(z, 0), a.nz - 1);
-    return plane[(static_cast<size_t>(z) * a.ny + y) * a.nx + x];
+    return field_load(f, comp, (static_cast<size_t>(z) * a.ny + y) * a.nx + x);
 }
 
 // eAtNode / bAtNode (shaders/field_sampling.glsl:17-32)
 __device__ __forceinline__ void e_at_node(const LatticeArgs &a, const FieldPtrs &f, int x, int y, int z, float o[3]) {
-    o[0] = 0.5f * (fetch_c(a, f.e[0], x, y, z) + fetch_c(a, f.e[0], x - 1, y, z));
-    o[1] = 0.5f * (fetch_c(a, f.e[1], x, y, z) + fetch_c(a, f.e[1], x, y - 1, z));
-    o[2] = 0.5f * (fetch_c(a, f.e[2], x, y, z) + fetch_c(a, f.e[2], x, y, z - 1));
+    o[0] = 0.5f * (fetch_c(a, f, 0, x, y, z) + fetch_c(a, f, 0, x - 1, y, z));
+    o[1] = 0.5f * (fetch_c(a, f, 1, x, y, z) + fetch_c(a, f, 1, x, y - 1, z));
+    o[2] = 0.5f * (fetch_c(a, f, 2, x, y, z) + fetch_c(a, f, 2, x, y, z - 1));
 }
 __device__ __forceinline__ void b_at_node(const LatticeArgs &a, const FieldPtrs &f, int x, int y, int z, float o[3]) {
-    const float bx = fetch_c(a, f.b[0], x, y, z) + fetch_c(a, f.b[0], x, y - 1, z) + fetch_c(a, f.b[0], x, y, z - 1) +
-                     fetch_c(a, f.b[0], x, y - 1, z - 1);
-    const float by = fetch_c(a, f.b[1], x, y, z) + fetch_c(a, f.b[1], x - 1, y, z) + fetch_c(a, f.b[1], x, y, z - 1) +
-                     fetch_c(a, f.b[1], x - 1, y, z - 1);
-    const float bz = fetch_c(a, f.b[2], x, y, z) + fetch_c(a, f.b[2], x - 1, y, z) + fetch_c(a, f.b[2], x, y - 1, z) +
-                     fetch_c(a, f.b[2], x - 1, y - 1, z);
+    const float bx = fetch_c(a, f, 3 + 0, x, y, z) + fetch_c(a, f, 3 + 0, x, y - 1, z) + fetch_c(a, f, 3 + 0, x, y, z - 1) +
+                     fetch_c(a, f, 3 + 0, x, y - 1, z - 1);
+    const float by = fetch_c(a, f, 3 + 1, x, y, z) + fetch_c(a, f, 3 + 1, x - 1, y, z) + fetch_c(a, f, 3 + 1, x, y, z - 1) +
+                     fetch_c(a, f, 3 + 1, x - 1, y, z - 1);
+    const float bz = fetch_c(a, f, 3 + 2, x, y, z) + fetch_c(a, f, 3 + 2, x - 1, y, z) + fetch_c(a, f, 3 + 2, x, y - 1, z) +
+                     fetch_c(a, f, 3 + 2, x - 1, y - 1, z);
     o[0] = 0.25f * bx;
     o[1] = 0.25f * by;
     o[2] = 0.25f * bz;
@@ -41,8 +60,8 @@ __device__ __forceinline__ float edge_current(const LatticeArgs &a, const FieldP
     int pu[3] = {p[0], p[1], p[2]}, pv[3] = {p[0], p[1], p[2]};
     pu[u] -= 1;
     pv[v] -= 1;
-    const float b0v = fetch_c(a, f.b[v], p[0], p[1], p[2]), buv = fetch_c(a, f.b[v], pu[0], pu[1], pu[2]);
-    const float b0u = fetch_c(a, f.b[u], p[0], p[1], p[2]), bvu = fetch_c(a, f.b[u], pv[0], pv[1], pv[2]);
+    const float b0v = fetch_c(a, f, 3 + v, p[0], p[1], p[2]), buv = fetch_c(a, f, 3 + v, pu[0], pu[1], pu[2]);
+    const float b0u = fetch_c(a, f, 3 + u, p[0], p[1], p[2]), bvu = fetch_c(a, f, 3 + u, pv[0], pv[1], pv[2]);
     return (b0v - buv) - (b0u - bvu);
 }
 
@@ -59,11 +78,11 @@ __device__ __forceinline__ void port_sample(const LatticeArgs &a, const FieldPtr
             int q[3] = {p[0], p[1], p[2]};
             if (k == 1 || k == 3) q[u] += 1;
             if (k == 2 || k == 3) q[v] += 1;
-            V += 0.25f * fetch_c(a, f.e[ax], q[0], q[1], q[2]);
+            V += 0.25f * fetch_c(a, f, ax, q[0], q[1], q[2]);
             I += edge_current(a, f, q, ax);
         }
     } else {
-        V = fetch_c(a, f.e[ax], p[0], p[1], p[2]);
+        V = fetch_c(a, f, ax, p[0], p[1], p[2]);
         I = edge_current(a, f, p, ax);
     }
     const float s = static_cast<float>(pt.sign);

@@ -1,6 +1,7 @@
 // Device body of the source pass, shared by the single-grid and batched launches.
 #pragma once
 
+#include "port_device.h"
 #include "yee_device.h"
 
 namespace emfd {
@@ -41,7 +42,7 @@ __device__ __forceinline__ void source_cell(const LatticeArgs &a, const FieldPtr
     float ca[3], cb[3];
     edge_coefficients(a, x, y, z, cid, ca, cb);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) f.e[c][p] = f.e[c][p] - cb[c] * J[c];
+    for (int c = 0; c < 3; ++c) field_store(f, c, p, field_load(f, c, p) - cb[c] * J[c]);
 }
 
 }  // namespace emfd

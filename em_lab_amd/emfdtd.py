@@ -27,6 +27,9 @@ KERNEL_TWO_PASS = 2
 KERNEL_ROWS = 3
 KERNEL_POINT = 4
 KERNEL_TWO_PASS4 = 5
+STORAGE_F32 = 0
+STORAGE_F16 = 1
+STORAGE_BF16 = 2
 
 
 class EmfdError(RuntimeError):
@@ -126,6 +129,7 @@ _SIGNATURES = {
     "emfd_grid_memory_bytes": (C.c_int64, [_P]),
     "emfd_grid_set_kernel": (C.c_int, [_P, C.c_int]),
     "emfd_grid_material_classes": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "emfd_grid_set_storage": (C.c_int, [_P, C.c_int, _P]),
     "emfd_grid_kernel_timing": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "emfd_ports_configure": (C.c_int, [_P, _P, C.c_int, C.c_float, _P, C.c_float, C.c_int64]),
     "emfd_ports_disable": (C.c_int, [_P]),
@@ -167,6 +171,18 @@ _SIGNATURES = {
     "emfd_batch_evaluate_sweeps": (C.c_int, [_P]),
     "emfd_batch_sweep_result": (C.c_int, [_P, C.c_int, _P, C.c_int, _P, C.POINTER(C.c_double)]),
     "emfd_batch_kernel_timing": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "emfd_mlp_create": (C.c_int, [C.c_int, _P, C.c_int, C.c_int, C.POINTER(_P)]),
+    "emfd_mlp_destroy": (C.c_int, [_P]),
+    "emfd_mlp_param_count": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "emfd_mlp_set_params": (C.c_int, [_P, _P, C.c_int64]),
+    "emfd_mlp_get_params": (C.c_int, [_P, _P, C.c_int64]),
+    "emfd_mlp_forward": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "emfd_mlp_gradients": (C.c_int, [_P, _P, _P, C.c_int64, _P, C.POINTER(C.c_double)]),
+    "emfd_mlp_set_adam": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "emfd_mlp_train_step": (C.c_int, [_P, _P, _P, C.c_int64, C.c_float, C.POINTER(C.c_double)]),
+    "emfd_mlp_set_dataset": (C.c_int, [_P, _P, _P, C.c_int64]),
+    "emfd_mlp_train": (C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, _P, C.c_int]),
+    "emfd_mlp_time_forward": (C.c_int, [_P, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
 }
 
 
@@ -401,6 +417,12 @@ class SimulationGrid:
 
     def set_kernel(self, variant: int) -> None:
         _check(self._lib.emfd_grid_set_kernel(self._h, int(variant)))
+
+    def set_storage(self, storage: int, scale_log2=None) -> None:
+        """Field storage STORAGE_F32 / STORAGE_F16 / STORAGE_BF16 with a power-of-two scale per
+        component (E x y z, B x y z). Clears the fields and rewinds time."""
+        k = None if scale_log2 is None else (C.c_int * 6)(*[int(v) for v in scale_log2])
+        _check(self._lib.emfd_grid_set_storage(self._h, int(storage), None if k is None else C.cast(k, C.c_void_p)))
 
     def material_classes(self) -> int:
         c = C.c_int(0)
@@ -803,6 +825,95 @@ class Batch:
         n = C.c_int64()
         _check(self._lib.emfd_batch_kernel_timing(self._h, enable, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+class Mlp:
+    """The surrogate network on one HIP device (emfd_mlp_* in emfdtd.h): BF16 MFMA forward and
+    backward, FP32 master parameters, Adam. Parameters: per layer W [n_out][n_in], then b [n_out]."""
+
+    def __init__(self, dims: Sequence[int], max_batch: int = 65536, device: int = 0):
+        lib = load_library()
+        h = C.c_void_p()
+        d = (C.c_int * len(dims))(*[int(v) for v in dims])
+        _check(lib.emfd_mlp_create(device, C.cast(d, C.c_void_p), len(dims), int(max_batch), C.byref(h)))
+        self._h = h
+        self._lib = lib
+        self.dims = tuple(int(v) for v in dims)
+        self.max_batch = int(max_batch)
+        n = C.c_int64()
+        _check(lib.emfd_mlp_param_count(h, C.byref(n)))
+        self.n_params = n.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.emfd_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_params(self, seed: int = 0) -> np.ndarray:
+        """He-style uniform initialisation (seeded), biases zero; sets and returns the vector."""
+        rng = np.random.default_rng(seed)
+        parts = []
+        for i, o in zip(self.dims[:-1], self.dims[1:]):
+            lim = math.sqrt(6.0 / i)
+            parts.append(rng.uniform(-lim, lim, size=(o, i)).astype(np.float32).ravel())
+            parts.append(np.zeros(o, np.float32))
+        p = np.concatenate(parts)
+        self.set_params(p)
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        _check(self._lib.emfd_mlp_set_params(self._h, _ptr(p), p.size))
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        _check(self._lib.emfd_mlp_get_params(self._h, _ptr(p), p.size))
+        return p
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((x.shape[0], self.dims[-1]), np.float32)
+        _check(self._lib.emfd_mlp_forward(self._h, _ptr(x), x.shape[0], _ptr(y)))
+        return y
+
+    def gradients(self, x: np.ndarray, t: np.ndarray):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_double()
+        _check(self._lib.emfd_mlp_gradients(self._h, _ptr(x), _ptr(t), x.shape[0], _ptr(g), C.byref(loss)))
+        return loss.value, g
+
+    def set_adam(self, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0) -> None:
+        _check(self._lib.emfd_mlp_set_adam(self._h, beta1, beta2, eps, weight_decay))
+
+    def train_step(self, x: np.ndarray, t: np.ndarray, lr: float) -> float:
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = C.c_double()
+        _check(self._lib.emfd_mlp_train_step(self._h, _ptr(x), _ptr(t), x.shape[0], lr, C.byref(loss)))
+        return loss.value
+
+    def set_dataset(self, x: np.ndarray, t: np.ndarray) -> None:
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        _check(self._lib.emfd_mlp_set_dataset(self._h, _ptr(x), _ptr(t), x.shape[0]))
+
+    def train(self, batch: int, epochs: int, lr_start: float, lr_end: float, seed: int = 0) -> np.ndarray:
+        losses = np.zeros(max(epochs, 1), np.float64)
+        _check(self._lib.emfd_mlp_train(self._h, batch, epochs, lr_start, lr_end, seed, _ptr(losses), losses.size))
+        return losses[:epochs]
+
+    def time_forward(self, batch: int, iterations: int = 20) -> float:
+        ms = C.c_double()
+        _check(self._lib.emfd_mlp_time_forward(self._h, batch, iterations, C.byref(ms)))
+        return ms.value
 
 
 def version() -> str:

@@ -499,10 +499,10 @@ def array_scene(geom: ArrayGeometry, size, lambda0: float = ARRAY_LAMBDA0, pml: 
     return sc
 
 
-def draw_array_scene(index: int, seed: int = DEFAULT_SEED, lambda0: float = ARRAY_LAMBDA0):
-    """Scene `index` of the batched generator: its own xoshiro256** stream (seed + index), the
-    geometry first, then the grid size over multiples of 8 in [48,128] x [48,128] x [96,160]
-    (x at least the array span plus both absorbers and 8 cells)."""
+def draw_array_design(index: int, seed: int = DEFAULT_SEED, lambda0: float = ARRAY_LAMBDA0):
+    """Design `index` of the batched generator: its own xoshiro256** stream (seed * 1000003 +
+    index), the geometry first, then the grid size over multiples of 8 in [48,128] x [48,128] x
+    [96,160] (x at least the array span plus both absorbers and 8 cells)."""
     rng = Xoshiro256ss(seed * 1000003 + index)
     geom = draw_array_geometry(rng, lambda0)
     lo_x = max(48, -(-(geom.span() + 2 * ARRAY_PML + 8) // 8) * 8)
@@ -511,5 +511,10 @@ def draw_array_scene(index: int, seed: int = DEFAULT_SEED, lambda0: float = ARRA
         k = (hi - lo) // 8 + 1
         return lo + 8 * min(int(rng.uniform(0.0, float(k))), k - 1)
 
-    size = (pick(lo_x, 128), pick(48, 128), pick(96, 160))
+    return geom, (pick(lo_x, 128), pick(48, 128), pick(96, 160))
+
+
+def draw_array_scene(index: int, seed: int = DEFAULT_SEED, lambda0: float = ARRAY_LAMBDA0):
+    """(geometry, Scene) of design `index` (draw_array_design)."""
+    geom, size = draw_array_design(index, seed, lambda0)
     return geom, array_scene(geom, size, lambda0)

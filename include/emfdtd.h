@@ -48,6 +48,10 @@ extern "C" {
 #define EMFD_KERNEL_POINT 4    /* fused B+E, one thread per texel, neighbour B rebuilt */
 #define EMFD_KERNEL_TWO_PASS4 5 /* two passes, four cells per thread with float4 streams (nx % 4 == 0) */
 
+#define EMFD_STORAGE_F32 0  /* FP32 field planes (default; EM_Lab's RGBA32F) */
+#define EMFD_STORAGE_F16 1  /* binary16 planes of value / 2^k per component, FP32 updates */
+#define EMFD_STORAGE_BF16 2 /* bfloat16 planes of value / 2^k per component, FP32 updates */
+
 typedef struct emfd_ctx emfd_ctx;
 
 /* GridConfig (src/SimulationGrid.hpp:25-31) */
@@ -217,6 +221,13 @@ int64_t emfd_grid_memory_bytes(emfd_ctx *ctx);
 /* Choose the Yee kernel variant: EMFD_KERNEL_* in the low byte; bits 8.. carry a variant
  * parameter (FUSED: tile shape | z-chunk << 8; ROWS: z-chunk), 0 = automatic. */
 int emfd_grid_set_kernel(emfd_ctx *ctx, int variant);
+/* Field storage (no counterpart in the reference, which keeps RGBA32F textures): EMFD_STORAGE_*
+ * with scale_log2[6] = k per component (E x y z, B x y z; NULL = all 0), the stored value being
+ * field / 2^k rounded to nearest even. Every update still computes in FP32; only the planes are
+ * 16-bit, which halves the field traffic. 16-bit storage needs nx % 8 == 0 and runs the
+ * two-pass kernel (EMFD_KERNEL_AUTO). Clears the fields and rewinds time, like
+ * emfd_grid_upload_materials. Downloads and uploads convert to and from FP32 values. */
+int emfd_grid_set_storage(emfd_ctx *ctx, int storage, const int *scale_log2);
 /* Number of distinct coefficient classes after the last material upload (0 = uniform vacuum). */
 int emfd_grid_material_classes(emfd_ctx *ctx, int *classes);
 /* Live timing of the Yee kernel: with enable != 0, every Yee launch of later steps is bracketed
@@ -363,6 +374,41 @@ int emfd_batch_sweep_result(emfd_batch *b, int scene, emfd_sweep_point *points, 
                             emfd_port_sweep *summary, double *residual);
 /* Yee launch timing as emfd_grid_kernel_timing (both passes of a step form one launch). */
 int emfd_batch_kernel_timing(emfd_batch *b, int enable, double *total_ms, int64_t *launches);
+
+/* ---------------------------------------------------------------------------------------
+ * MLP surrogate (no counterpart in the reference): a fully connected network with GELU hidden
+ * layers, BF16 weights and activations, FP32 accumulation on the matrix cores, FP32 master
+ * parameters and Adam. Forward, backward and the optimizer are this library's own kernels.
+ * Parameter layout (host side): for each layer, W [n_out][n_in] row-major, then b [n_out].
+ * The loss is the mean squared error over batch x n_out outputs.
+ * --------------------------------------------------------------------------------------- */
+typedef struct emfd_mlp emfd_mlp;
+
+/* dims: n_layers + 1 widths, input first (e.g. 18, 256, 256, 256, 128); max_batch: rows per
+ * device pass (forward calls with more rows are processed in chunks). */
+int emfd_mlp_create(int device, const int *dims, int n_dims, int max_batch, emfd_mlp **out);
+int emfd_mlp_destroy(emfd_mlp *m);
+int emfd_mlp_param_count(emfd_mlp *m, int64_t *n);
+/* Sets the FP32 master parameters (and their BF16 copies); resets the Adam state. */
+int emfd_mlp_set_params(emfd_mlp *m, const float *params, int64_t n);
+int emfd_mlp_get_params(emfd_mlp *m, float *params, int64_t n);
+/* y [batch][n_out] = network(x [batch][n_in]). */
+int emfd_mlp_forward(emfd_mlp *m, const float *x, int64_t batch, float *y);
+/* Loss and parameter gradients (host layout) of one batch (batch <= max_batch); grads may be NULL. */
+int emfd_mlp_gradients(emfd_mlp *m, const float *x, const float *t, int64_t batch, float *grads, double *loss);
+int emfd_mlp_set_adam(emfd_mlp *m, float beta1, float beta2, float eps, float weight_decay);
+/* One Adam step on a batch (batch <= max_batch); *loss (may be NULL) = loss before the step. */
+int emfd_mlp_train_step(emfd_mlp *m, const float *x, const float *t, int64_t batch, float lr, double *loss);
+/* Keeps a training set on the device (x [n][n_in], t [n][n_out]). */
+int emfd_mlp_set_dataset(emfd_mlp *m, const float *x, const float *t, int64_t n);
+/* `epochs` passes over the device training set in seeded random minibatches of `batch` rows
+ * (the last partial minibatch included) with learning rate lr_start decaying to lr_end
+ * (cosine); epoch_loss (may be NULL, cap entries): mean minibatch loss per epoch. */
+int emfd_mlp_train(emfd_mlp *m, int batch, int epochs, float lr_start, float lr_end, uint64_t seed,
+                   double *epoch_loss, int cap);
+/* Mean time of one forward pass over `batch` device-resident rows (batch <= max_batch), from
+ * HIP events over `iterations` passes. */
+int emfd_mlp_time_forward(emfd_mlp *m, int64_t batch, int iterations, double *avg_ms);
 
 /* ---------------------------------------------------------------------------------------
  * Measurement helper (no counterpart in the reference: it calibrates the roofline figures)

@@ -507,9 +507,69 @@ void or_clear_fields(const or_grid *g, float *E, float *B) {
     }
 }
 
-/* SimulationGrid::step (src/SimulationGrid.cpp:368-433) */
-void or_step(const or_grid *g, float *E, float *B, const uint32_t *edge_mat, const uint32_t *wire_tex, int thin_wires,
-             const or_source_desc *src, int n_src, int count, uint64_t *step_count) {
+/* ------------------------------------------------------------------------------------------
+ * 16-bit field storage (no reference counterpart): values are kept as binary16 or bfloat16 of
+ * value / 2^k, rounded to nearest even, and every update reads them back as float.
+ * ------------------------------------------------------------------------------------------ */
+float or_round_f16(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    const uint32_t sign = u & 0x80000000u, a = u & 0x7fffffffu;
+    if (a >= 0x7f800000u) return f;                 /* inf, nan */
+    uint32_t r;
+    if (a >= 0x477ff000u) {                         /* >= 65520: rounds to infinity */
+        r = 0x7f800000u;
+    } else if (a < 0x38800000u) {                   /* below 2^-14: binary16 subnormal, quantum 2^-24 */
+        float v;
+        memcpy(&v, &a, 4);
+        v = rintf(v * 16777216.0f) * 5.9604644775390625e-8f;
+        memcpy(&r, &v, 4);
+    } else {                                        /* normal: keep 10 fraction bits */
+        r = (a + 0xfffu + ((a >> 13) & 1u)) & ~0x1fffu;
+    }
+    r |= sign;
+    float o;
+    memcpy(&o, &r, 4);
+    return o;
+}
+
+float or_round_bf16(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return f;  /* nan */
+    u = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
+    float o;
+    memcpy(&o, &u, 4);
+    return o;
+}
+
+/* Rounds the three components of every texel of F (4 floats per texel) to the storage. */
+static void quantize(const or_grid *g, float *F, int storage, const int *k3) {
+    if (storage == 0) return;
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    float s[3], inv[3];
+    for (int c = 0; c < 3; ++c) {
+        s[c] = ldexpf(1.0f, k3[c]);
+        inv[c] = ldexpf(1.0f, -k3[c]);
+    }
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) {
+            const float v = F[4 * i + c] * inv[c];
+            F[4 * i + c] = (storage == 1 ? or_round_f16(v) : or_round_bf16(v)) * s[c];
+        }
+}
+
+void or_quantize_fields(const or_grid *g, float *E, float *B, int storage, const int scale_log2[6]) {
+    quantize(g, E, storage, scale_log2);
+    quantize(g, B, storage, scale_log2 + 3);
+}
+
+/* SimulationGrid::step (src/SimulationGrid.cpp:368-433); storage 1/2 rounds every pass's
+ * output to binary16 / bfloat16 as the 16-bit storage of the library does. */
+void or_step_storage(const or_grid *g, float *E, float *B, const uint32_t *edge_mat, const uint32_t *wire_tex,
+                     int thin_wires, const or_source_desc *src, int n_src, int count, uint64_t *step_count,
+                     int storage, const int *scale_log2) {
     if (count <= 0) return;
     float packed[256 * 16];
     int kept_index[256];
@@ -523,12 +583,22 @@ void or_step(const or_grid *g, float *E, float *B, const uint32_t *edge_mat, con
         const double t = ((double)(*step_count) + 0.5) * (double)g->dt;
         for (int s = 0; s < kept; ++s) signals[s] = (float)or_signal(&src[kept_index[s]], t);
         or_update_B(g, B, E, wire_tex, thin_wires);
+        if (storage) quantize(g, B, storage, scale_log2 + 3);
         or_update_E(g, E, B, edge_mat);
-        if (n_cells > 0) or_apply_sources(g, E, edge_mat, packed, signals, 0, cells, n_cells, lists);
+        if (storage) quantize(g, E, storage, scale_log2);
+        if (n_cells > 0) {
+            or_apply_sources(g, E, edge_mat, packed, signals, 0, cells, n_cells, lists);
+            if (storage) quantize(g, E, storage, scale_log2);
+        }
         ++(*step_count);
     }
     free(cells);
     free(lists);
+}
+
+void or_step(const or_grid *g, float *E, float *B, const uint32_t *edge_mat, const uint32_t *wire_tex, int thin_wires,
+             const or_source_desc *src, int n_src, int count, uint64_t *step_count) {
+    or_step_storage(g, E, B, edge_mat, wire_tex, thin_wires, src, n_src, count, step_count, 0, NULL);
 }
 
 /* ------------------------------------------------------------------------------------------

@@ -93,6 +93,15 @@ void or_clear_fields(const or_grid *g, float *E, float *B);
  * step_count is advanced. */
 void or_step(const or_grid *g, float *E, float *B, const uint32_t *edge_mat, const uint32_t *wire_tex, int thin_wires,
              const or_source_desc *src, int n_src, int count, uint64_t *step_count);
+/* 16-bit field storage emulation (no reference counterpart): or_step with every pass's output
+ * rounded to binary16 (storage 1) or bfloat16 (storage 2) of value / 2^k, k per component
+ * (E x y z, B x y z), round to nearest even. */
+void or_step_storage(const or_grid *g, float *E, float *B, const uint32_t *edge_mat, const uint32_t *wire_tex,
+                     int thin_wires, const or_source_desc *src, int n_src, int count, uint64_t *step_count,
+                     int storage, const int *scale_log2);
+void or_quantize_fields(const or_grid *g, float *E, float *B, int storage, const int scale_log2[6]);
+float or_round_f16(float f);
+float or_round_bf16(float f);
 
 /* ---- monitors ---- */
 typedef struct { int texel[3]; int axis; int edges; int sign; } or_port;

@@ -58,6 +58,10 @@ _SIG = {
     "or_apply_sources": (None, [C.POINTER(OrGrid), _P, _P, _P, _P, _I, _P, _I, _P]),
     "or_clear_fields": (None, [C.POINTER(OrGrid), _P, _P]),
     "or_step": (None, [C.POINTER(OrGrid), _P, _P, _P, _P, _I, _P, _I, _I, C.POINTER(C.c_uint64)]),
+    "or_step_storage": (None, [C.POINTER(OrGrid), _P, _P, _P, _P, _I, _P, _I, _I, C.POINTER(C.c_uint64), _I, _P]),
+    "or_quantize_fields": (None, [C.POINTER(OrGrid), _P, _P, _I, _P]),
+    "or_round_f16": (C.c_float, [C.c_float]),
+    "or_round_bf16": (C.c_float, [C.c_float]),
     "or_port_accumulate": (None, [C.POINTER(OrGrid), _P, _P, _P, _I, _P, _F, _I, _P, _P]),
     "or_monitor_constants": (None, [_D, _F, _F, C.c_uint64, _P, C.POINTER(_F)]),
     "or_ntff_point": (None, [_P, _P, _P, _I, _P, _P, C.POINTER(_F)]),
@@ -150,6 +154,17 @@ class OracleSim:
         self.src_c = None
         self.ports = None
         self.ntff = None
+        self.storage = 0
+        self.scale_log2 = np.zeros(6, np.int32)
+
+    def set_storage(self, storage, scale_log2=None):
+        """Emulates the library's 16-bit field storage (1: binary16, 2: bfloat16): clears the
+        fields and rewinds time, as emfd_grid_set_storage."""
+        self.storage = int(storage)
+        self.scale_log2 = np.zeros(6, np.int32) if scale_log2 is None else np.asarray(scale_log2, np.int32).copy()
+        self.E[:, :3] = 0.0
+        self.B[:, :3] = 0.0
+        self.step_count = 0
 
     @property
     def size(self):
@@ -197,8 +212,9 @@ class OracleSim:
         # one call per step keeps the per-step monitors in the reference's order
         for _ in range(count):
             sc = C.c_uint64(self.step_count)
-            L.or_step(C.byref(self.g), ptr(self.E), ptr(self.B), ptr(self.edge), ptr(self.wire), self.thin,
-                      C.cast(self.src_c, C.c_void_p) if self.src_c else None, len(self.sources), 1, C.byref(sc))
+            L.or_step_storage(C.byref(self.g), ptr(self.E), ptr(self.B), ptr(self.edge), ptr(self.wire), self.thin,
+                              C.cast(self.src_c, C.c_void_p) if self.src_c else None, len(self.sources), 1,
+                              C.byref(sc), self.storage, ptr(self.scale_log2))
             self.step_count = sc.value
             self._monitors()
 

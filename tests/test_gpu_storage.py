@@ -1,0 +1,71 @@
+"""16-bit field storage on the GPU against the oracle's storage emulation: every pass's output
+rounded to binary16 / bfloat16 of value / 2^k; all arithmetic FP32. Bit-exact, like the FP32
+path."""
+import numpy as np
+import pytest
+
+from em_lab_amd import emfdtd as E
+from em_lab_amd.scene import hertzian_vacuum, spheres_dipole
+
+from gpu_helpers import gpu_grid, oracle_sim, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("storage", [E.STORAGE_F16, E.STORAGE_BF16])
+def test_hertzian_storage_bit_exact(storage):
+    print("case: hertzian 64^3 storage", storage, flush=True)
+    sc = hertzian_vacuum(64, 8)
+    k = [2, 2, 2, -3, -3, -3]
+    g, _, ff = gpu_grid(sc)
+    g.set_storage(storage, k)
+    sim = oracle_sim(sc, gpu_ntff_setup=ff.setup)
+    sim.set_storage(storage, k)
+    g.step(200)
+    sim.step(200)
+    Eg, Bg = g.fields(E.LAYOUT_TEXEL)
+    assert np.array_equal(Eg[:, :3], sim.E[:, :3]) and np.array_equal(Bg[:, :3], sim.B[:, :3])
+    assert np.any(Eg[:, :3])
+    assert np.array_equal(ff.phasors(), sim.ntff["phasors"])
+    g.close()
+
+
+def test_spheres_dipole_f16_ports_bit_exact():
+    print("case: spheres+dipole 48^3 f16 storage with ports", flush=True)
+    sc = spheres_dipole(n=48, pml=8, n_spheres=4, radius_range=(3.0, 6.0), sweep_points=16)
+    k = [0, 0, 0, -4, -4, -4]
+    g, pm, _ = gpu_grid(sc, ntff=False)
+    g.set_storage(E.STORAGE_F16, k)
+    sim = oracle_sim(sc, ntff=False)
+    sim.set_storage(1, k)
+    g.step(150)
+    sim.step(150)
+    Eg, Bg = g.fields(E.LAYOUT_TEXEL)
+    assert np.array_equal(Eg[:, :3], sim.E[:, :3]) and np.array_equal(Bg[:, :3], sim.B[:, :3])
+    assert np.array_equal(pm.phasors().ravel(), sim.ports["phasors"].ravel()[:8 * len(sc.ports)])
+    # the 16-bit run stays close to the FP32 run of the same scene
+    g32, _, _ = gpu_grid(sc, ntff=False)
+    g32.step(150)
+    E32, _ = g32.fields(E.LAYOUT_TEXEL)
+    assert rel_l2(Eg[:, :3], E32[:, :3]) < 2e-2
+    g.close()
+    g32.close()
+
+
+def test_storage_roundtrip_and_errors():
+    sc = hertzian_vacuum(32, 4)
+    g, _, _ = gpu_grid(sc, ntff=False)
+    g.set_storage(E.STORAGE_F16, [1, 0, -1, 0, 0, 0])
+    n = 32 ** 3
+    rng = np.random.default_rng(0)
+    e0 = rng.standard_normal((3, n)).astype(np.float32)
+    g.set_fields(e0, None, E.LAYOUT_SOA)
+    e1, _ = g.fields(E.LAYOUT_SOA)
+    scale = np.array([2.0, 1.0, 0.5], np.float32)[:, None]
+    assert np.array_equal(e1, (e0 / scale).astype(np.float16).astype(np.float32) * scale)
+    with pytest.raises(E.EmfdError):
+        g.set_kernel(E.KERNEL_ROWS)
+        g.step(1)
+    g.set_kernel(E.KERNEL_AUTO)
+    g.step(2)
+    g.close()
