@@ -63,8 +63,11 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--kernel", type=int, default=0, help="EMFD_KERNEL_* variant (| tile << 8 | zchunk << 16)")
     ap.add_argument("--no-monitors", action="store_true")
-    ap.add_argument("--scene", choices=["spheres", "vacuum"], default="spheres",
-                    help="spheres: the headline workload; vacuum: same grid, no materials (roofline check)")
+    ap.add_argument("--scene", choices=["spheres", "vacuum", "array"], default="spheres",
+                    help="spheres: the headline workload; vacuum: same grid, no materials (roofline check); "
+                         "array: seeded spheres around a 4x4 thin-wire dipole array (the precision-form scene)")
+    ap.add_argument("--storage", choices=["f32", "f16", "bf16"], default="f32",
+                    help="field storage: FP32 planes, or 16-bit planes with FP32 updates")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only time a plain float4 device copy of MB megabytes (roofline calibration)")
@@ -75,7 +78,7 @@ def main() -> int:
         return 2
 
     from em_lab_amd import emfdtd as E
-    from em_lab_amd.scene import hertzian_vacuum, spheres_dipole
+    from em_lab_amd.scene import hertzian_vacuum, spheres_array, spheres_dipole
 
     if args.copy_probe:
         g = E.SimulationGrid(0)
@@ -88,6 +91,8 @@ def main() -> int:
 
     if args.scene == "spheres":
         scene = spheres_dipole(n=args.size, pml=10)
+    elif args.scene == "array":
+        scene = spheres_array(n=args.size, pml=10)
     else:
         scene = hertzian_vacuum(args.size, 10, steps_wavelength=20.0)
     g = E.SimulationGrid(0)
@@ -96,6 +101,11 @@ def main() -> int:
     g.upload_materials(s, e, scene.subcell)
     g.set_sources(scene.sources)
     g.set_kernel(args.kernel)
+    storage = {"f32": E.STORAGE_F32, "f16": E.STORAGE_F16, "bf16": E.STORAGE_BF16}[args.storage]
+    if storage != E.STORAGE_F32:
+        # E near the feeds peaks at O(1); B at O(1/8): stored values stay far below 65504
+        g.set_storage(storage, [0, 0, 0, -3, -3, -3])
+    bytes_per_cell = BYTES_PER_CELL if storage == E.STORAGE_F32 else BYTES_PER_CELL // 2
     if not args.no_monitors:
         if scene.ports:
             pm = E.PortMonitor(g)
@@ -122,15 +132,17 @@ def main() -> int:
 
     value = cells * args.steps / wall / 1e6
     avg_ms = yee_ms / max(launches, 1)
-    achieved = BYTES_PER_CELL * cells / (avg_ms * 1e-3) / 1e9
+    achieved = bytes_per_cell * cells / (avg_ms * 1e-3) / 1e9
     # HBM bytes per Yee step from the calibrated rocprofv3 counter runs of this workload
     # (profiles/yee_256_traffic.md); only quoted for the default kernel at 256^3
     traffic = None
     tf = ROOT / "profiles" / "yee_256_traffic.json"
-    if tf.exists() and args.size == 256 and args.kernel in (0, 5) and args.scene == "spheres":
+    if (tf.exists() and args.size == 256 and args.kernel in (0, 5) and args.scene == "spheres"
+            and storage == E.STORAGE_F32):
         traffic = json.loads(tf.read_text()).get("bytes_per_launch")
     kernel_names = {0: "k_update_b4+k_update_e4", 1: "k_yee_fused", 2: "k_update_b+k_update_e", 3: "k_yee_rows",
                     4: "k_yee_point", 5: "k_update_b4+k_update_e4"}
+    kname = kernel_names.get(args.kernel & 0xff, "?") if storage == E.STORAGE_F32 else "k_update_b8h+k_update_e8h"
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -147,15 +159,19 @@ def main() -> int:
         "config": {"workload": (f"{nx}^3 Yee leapfrog, 10-cell absorber, 32 seeded dielectric spheres, z thin-wire "
                                 f"half-wave dipole (lambda 20) on a 50 ohm resistive feed, port + far-field "
                                 f"accumulation every step" if args.scene == "spheres" else
+                                f"{nx}^3 Yee leapfrog, 10-cell absorber, 32 seeded dielectric spheres around a 4x4 "
+                                f"thin-wire dipole array on 50 ohm feeds, port + far-field accumulation every step"
+                                if args.scene == "array" else
                                 f"{nx}^3 vacuum Yee leapfrog, 10-cell absorber, z Hertzian pulse, far-field "
-                                f"accumulation every step") + (" (monitors off)" if args.no_monitors else ""),
+                                f"accumulation every step") + (" (monitors off)" if args.no_monitors else "")
+                               + ("" if args.storage == "f32" else f", {args.storage} field storage"),
                    "grid": [nx, ny, nz], "courant": round(g.dt, 6), "material_classes": g.material_classes(),
-                   "kernel": args.kernel},
+                   "kernel": args.kernel, "field_storage": args.storage},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": kernel_names.get(args.kernel & 0xff, "?"), "avg_launch_ms": round(avg_ms, 5),
+                     "kernel": kname, "avg_launch_ms": round(avg_ms, 5),
                      "launches": launches,
-                     "bytes_per_cell": BYTES_PER_CELL},
+                     "bytes_per_cell": bytes_per_cell},
         "cpu_baseline": None,
     }
     g.close()

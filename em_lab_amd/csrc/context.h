@@ -71,9 +71,12 @@ struct emfd_ctx {
     // fields: two sets (ping-pong for the fused kernel); cur = the one holding the state
     emfd::DevBuf field_buf[2];
     int cur = 0;
+    emfd::DevBuf d_src_descs;  // kept sources (FP64 stepper evaluates their signals on the device)
     int storage = emfd::kStorageF32;  // field storage (EMFD_STORAGE_*)
     int scale_log2[6] = {0, 0, 0, 0, 0, 0};
-    size_t field_bytes() const { return n * 6 * (storage == emfd::kStorageF32 ? 4 : 2); }
+    size_t field_bytes() const {
+        return n * 6 * (storage == emfd::kStorageF32 ? 4 : storage == emfd::kStorageF64 ? 8 : 2);
+    }
     emfd::DevBuf sigma_cell, eps_cell;  // EM_Lab's alpha channels (probes)
     emfd::DevBuf mat_id, table, prof, pml_tab;
     int id_bytes = 0;

@@ -6,6 +6,7 @@
 #include "context.h"
 #include "expf.h"
 #include "host_common.h"
+#include "signal.h"
 
 #include <algorithm>
 #include <cstdint>
@@ -61,33 +62,7 @@ uint16_t f32_to_f16(float f) {
 }
 
 // SourceDesc::signal (src/SimulationGrid.cpp:99-130), double precision
-double source_signal(const emfd_source_desc &s, double tAbs) {
-    const double wl = std::max(static_cast<double>(s.wavelength), 1e-3);
-    const double t = tAbs - std::max(static_cast<double>(s.delay_periods), 0.0) * wl;
-    if (!s.enabled || t < 0.0) return 0.0;
-    const double period = wl;
-    const double w = 6.283185307179586476925286766559 / wl;
-    if (s.waveform == 0) {
-        const double rampT = std::max(static_cast<double>(s.ramp_periods), 0.0) * period;
-        double ramp = 1.0;
-        if (rampT > 0.0 && t < rampT) {
-            const double r = t / rampT;
-            ramp = r * r * r * (10.0 + r * (-15.0 + 6.0 * r));
-        }
-        return ramp * std::sin(w * t + s.phase);
-    }
-    const double tau = std::max(static_cast<double>(s.ramp_periods), 0.1) * period;
-    const double t0 = 3.0 * tau;
-    double tl = t;
-    if (s.repeat_periods > 0.0f) {
-        const double rep = std::max(static_cast<double>(s.repeat_periods) * period, 2.0 * t0);
-        tl = std::fmod(t, rep);
-    }
-    const double u = (tl - t0) / tau;
-    if (std::abs(u) > 6.0) return 0.0;
-    return std::exp(-u * u) * std::sin(w * (tl - t0) + s.phase);
-}
-
+double source_signal(const emfd_source_desc &s, double tAbs) { return source_signal_d(s, tAbs); }
 
 emfd_grid_config clamp_config(const emfd_grid_config &requested) {
     emfd_grid_config c = requested;
@@ -292,6 +267,15 @@ emfd::FieldPtrs emfd_ctx::fields(int which) const {
         }
         return f;
     }
+    f.storage = storage;
+    if (storage == kStorageF64) {
+        double *base = field_buf[which].as<double>();
+        for (int c = 0; c < 3; ++c) {
+            f.e[c] = reinterpret_cast<float *>(base + static_cast<size_t>(c) * n);
+            f.b[c] = reinterpret_cast<float *>(base + static_cast<size_t>(3 + c) * n);
+        }
+        return f;
+    }
     // 16-bit planes: the pointers carry 16-bit element addresses
     uint16_t *base = field_buf[which].as<uint16_t>();
     for (int c = 0; c < 3; ++c) {
@@ -354,7 +338,7 @@ void emfd_destroy(emfd_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &b : ctx->field_buf) b.release();
-    DevBuf *bufs[] = {&ctx->sigma_cell, &ctx->eps_cell, &ctx->mat_id, &ctx->table, &ctx->prof, &ctx->pml_tab, &ctx->d_packed,
+    DevBuf *bufs[] = {&ctx->d_src_descs, &ctx->sigma_cell, &ctx->eps_cell, &ctx->mat_id, &ctx->table, &ctx->prof, &ctx->pml_tab, &ctx->d_packed,
                       &ctx->d_cells, &ctx->d_lists, &ctx->d_consts, &ctx->ports.d_ports, &ctx->ports.d_phasors,
                       &ctx->ports.d_series, &ctx->ntff.d_phasors, &ctx->ntff.d_dirs, &ctx->ntff.d_partials,
                       &ctx->ntff.d_out, &ctx->ntff.d_power, &ctx->scratch_partials, &ctx->scratch_misc};
@@ -494,10 +478,14 @@ int emfd_grid_set_sources(emfd_ctx *ctx, const emfd_source_desc *sources, int n,
     std::vector<int> &cells = ps.cells, &lists = ps.lists;
     const std::vector<emfd_source_desc> &kept = ps.kept;
     if (int rc = ctx->d_packed.ensure(packed.size() * sizeof(float))) return rc;
+    if (int rc = ctx->d_src_descs.ensure(std::max<size_t>(kept.size(), 1) * sizeof(emfd_source_desc))) return rc;
     if (int rc = ctx->d_cells.ensure(cells.size() * sizeof(int))) return rc;
     if (int rc = ctx->d_lists.ensure(lists.size() * sizeof(int))) return rc;
     EMFD_HIP(hipStreamSynchronize(ctx->stream));
     EMFD_HIP(hipMemcpy(ctx->d_packed.ptr, packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (!kept.empty())
+        EMFD_HIP(hipMemcpy(ctx->d_src_descs.ptr, kept.data(), kept.size() * sizeof(emfd_source_desc),
+                           hipMemcpyHostToDevice));
     EMFD_HIP(hipMemcpy(ctx->d_cells.ptr, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice));
     EMFD_HIP(hipMemcpy(ctx->d_lists.ptr, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice));
     ctx->sources = kept;
@@ -518,9 +506,10 @@ int emfd_grid_reset_fields(emfd_ctx *ctx) {
 int emfd_grid_set_storage(emfd_ctx *ctx, int storage, const int *scale_log2) {
     if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
     if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
-    if (storage != EMFD_STORAGE_F32 && storage != EMFD_STORAGE_F16 && storage != EMFD_STORAGE_BF16)
+    if (storage != EMFD_STORAGE_F32 && storage != EMFD_STORAGE_F16 && storage != EMFD_STORAGE_BF16 &&
+        storage != EMFD_STORAGE_F64)
         return fail(EMFD_ERR_INVALID, "unknown field storage");
-    if (storage != EMFD_STORAGE_F32 && ctx->cfg.size[0] % 8 != 0)
+    if ((storage == EMFD_STORAGE_F16 || storage == EMFD_STORAGE_BF16) && ctx->cfg.size[0] % 8 != 0)
         return fail(EMFD_ERR_INVALID, "16-bit field storage needs nx % 8 == 0");
     int k[6] = {0, 0, 0, 0, 0, 0};
     if (scale_log2)
@@ -640,8 +629,11 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
         int variant = ctx->kernel_variant;
         if (ctx->storage != kStorageF32) {
             if (variant != EMFD_KERNEL_AUTO)
-                return fail(EMFD_ERR_INVALID, "16-bit field storage runs the automatic (two-pass) kernel only");
-            EMFD_HIP(launch_yee_half(lat, ctx->fields(ctx->cur), ctx->stream));
+                return fail(EMFD_ERR_INVALID, "16-bit and FP64 field storage run the automatic kernel only");
+            if (ctx->storage == kStorageF64)
+                EMFD_HIP(launch_yee_f64(lat, ctx->fields(ctx->cur), ctx->stream));
+            else
+                EMFD_HIP(launch_yee_half(lat, ctx->fields(ctx->cur), ctx->stream));
             variant = -1;
         } else if (variant == EMFD_KERNEL_AUTO)
             variant = yee_two_pass4_supported(lat) ? EMFD_KERNEL_TWO_PASS4 : EMFD_KERNEL_TWO_PASS;
@@ -667,9 +659,17 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
             ctx->cur ^= 1;
         }
         if (ev1) EMFD_HIP(hipEventRecord(ev1, ctx->stream));
-        if (ctx->n_cells > 0 && nsrc > 0)
-            EMFD_HIP(launch_apply_sources(lat, ctx->fields(ctx->cur), ctx->d_cells.as<int>(), ctx->n_cells,
-                                          ctx->d_lists.as<int>(), ctx->d_packed.as<float>(), consts, ctx->stream));
+        if (ctx->n_cells > 0 && nsrc > 0) {
+            if (ctx->storage == kStorageF64) {
+                const double t = (static_cast<double>(ctx->step_count) + 0.5) * static_cast<double>(ctx->cfg.courant);
+                EMFD_HIP(launch_sources_f64(lat, ctx->fields(ctx->cur), ctx->d_cells.as<int>(), ctx->n_cells,
+                                            ctx->d_lists.as<int>(), ctx->d_packed.as<float>(),
+                                            ctx->d_src_descs.as<emfd_source_desc>(), t, ctx->stream));
+            } else {
+                EMFD_HIP(launch_apply_sources(lat, ctx->fields(ctx->cur), ctx->d_cells.as<int>(), ctx->n_cells,
+                                              ctx->d_lists.as<int>(), ctx->d_packed.as<float>(), consts, ctx->stream));
+            }
+        }
         ++ctx->step_count;  // observers see the time of the step that just completed
         if (int rc = monitors_after_step(ctx, consts + nsrc, k)) return rc;
         if (ctx->timing && ctx->ev_used >= 4096) {
@@ -704,6 +704,13 @@ int read_planes(emfd_ctx *ctx, int which, std::vector<float> &out) {
     if (ctx->storage == kStorageF32) {
         EMFD_HIP(hipMemcpy(out.data(), ctx->field_buf[ctx->cur].as<float>() + static_cast<size_t>(3 * which) * n,
                            3 * n * sizeof(float), hipMemcpyDeviceToHost));
+        return EMFD_OK;
+    }
+    if (ctx->storage == kStorageF64) {
+        std::vector<double> d(3 * n);
+        EMFD_HIP(hipMemcpy(d.data(), ctx->field_buf[ctx->cur].as<double>() + static_cast<size_t>(3 * which) * n,
+                           3 * n * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < 3 * n; ++i) out[i] = static_cast<float>(d[i]);
         return EMFD_OK;
     }
     std::vector<uint16_t> raw(3 * n);
@@ -741,6 +748,12 @@ int write_planes(emfd_ctx *ctx, int which, const std::vector<float> &in) {
     if (ctx->storage == kStorageF32) {
         EMFD_HIP(hipMemcpy(ctx->field_buf[ctx->cur].as<float>() + static_cast<size_t>(3 * which) * n, in.data(),
                            3 * n * sizeof(float), hipMemcpyHostToDevice));
+        return EMFD_OK;
+    }
+    if (ctx->storage == kStorageF64) {
+        std::vector<double> d(in.begin(), in.end());
+        EMFD_HIP(hipMemcpy(ctx->field_buf[ctx->cur].as<double>() + static_cast<size_t>(3 * which) * n, d.data(),
+                           3 * n * sizeof(double), hipMemcpyHostToDevice));
         return EMFD_OK;
     }
     std::vector<uint16_t> raw(3 * n);
@@ -786,6 +799,27 @@ int emfd_grid_download_fields(emfd_ctx *ctx, float *E, float *B, int layout) {
             dst[4 * i + 1] = plane[n + i];
             dst[4 * i + 2] = plane[2 * n + i];
             dst[4 * i + 3] = alpha[i];
+        }
+    }
+    return EMFD_OK;
+}
+
+int emfd_grid_download_fields_f64(emfd_ctx *ctx, double *E, double *B) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    const size_t n = ctx->n;
+    for (int f = 0; f < 2; ++f) {
+        double *dst = f == 0 ? E : B;
+        if (!dst) continue;
+        if (ctx->storage == kStorageF64) {
+            EMFD_HIP(hipMemcpy(dst, ctx->field_buf[ctx->cur].as<double>() + static_cast<size_t>(3 * f) * n,
+                               3 * n * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> plane;
+            if (int rc = read_planes(ctx, f, plane)) return rc;
+            for (size_t i = 0; i < 3 * n; ++i) dst[i] = plane[i];
         }
     }
     return EMFD_OK;

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/emfdtd.h"
+
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -46,10 +48,10 @@ enum PmlField : int { kPmlBa = 0, kPmlBb = 3, kPmlEa = 6, kPmlEb = 9 };
 
 // Field storage of a lattice: FP32 planes, or 16-bit planes (binary16 or bfloat16) holding
 // value / scale with a power-of-two scale per component (updates still run in FP32).
-enum FieldStorage : int { kStorageF32 = 0, kStorageF16 = 1, kStorageBF16 = 2 };
+enum FieldStorage : int { kStorageF32 = 0, kStorageF16 = 1, kStorageBF16 = 2, kStorageF64 = 3 };
 
 struct FieldPtrs {
-    float *e[3];  // component planes; for 16-bit storage they point at 16-bit elements
+    float *e[3];  // component planes; for 16-bit / FP64 storage they point at those elements
     float *b[3];
     int storage = kStorageF32;
     float scale[6] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f};  // E x y z, B x y z: value = stored * scale
@@ -76,6 +78,10 @@ hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStre
 hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
 int yee_two_pass4_supported(const LatticeArgs &a);
 hipError_t launch_yee_half(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_yee_f64(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_sources_f64(const LatticeArgs &a, const FieldPtrs &f, const int *cells, int n_cells,
+                              const int *lists, const float *packed, const emfd_source_desc *descs, double t,
+                              hipStream_t s);
 int yee_half_supported(const LatticeArgs &a);
 hipError_t launch_apply_sources(const LatticeArgs &a, const FieldPtrs &f, const int *cells, int n_cells,
                                 const int *lists, const float *packed, const float *signals, hipStream_t s);

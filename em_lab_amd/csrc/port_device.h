@@ -11,6 +11,7 @@ __device__ __forceinline__ float field_load(const FieldPtrs &f, int comp, size_t
     const void *plane = comp < 3 ? static_cast<const void *>(f.e[comp]) : static_cast<const void *>(f.b[comp - 3]);
     if (f.storage == kStorageF32) return static_cast<const float *>(plane)[i];
     if (f.storage == kStorageF16) return static_cast<float>(static_cast<const _Float16 *>(plane)[i]) * f.scale[comp];
+    if (f.storage == kStorageF64) return static_cast<float>(static_cast<const double *>(plane)[i]);
     return static_cast<float>(static_cast<const __bf16 *>(plane)[i]) * f.scale[comp];
 }
 
@@ -20,6 +21,8 @@ __device__ __forceinline__ void field_store(const FieldPtrs &f, int comp, size_t
         static_cast<float *>(plane)[i] = v;
     } else if (f.storage == kStorageF16) {
         static_cast<_Float16 *>(plane)[i] = static_cast<_Float16>(v / f.scale[comp]);
+    } else if (f.storage == kStorageF64) {
+        static_cast<double *>(plane)[i] = static_cast<double>(v);
     } else {
         static_cast<__bf16 *>(plane)[i] = static_cast<__bf16>(v / f.scale[comp]);
     }

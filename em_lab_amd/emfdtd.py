@@ -30,6 +30,7 @@ KERNEL_TWO_PASS4 = 5
 STORAGE_F32 = 0
 STORAGE_F16 = 1
 STORAGE_BF16 = 2
+STORAGE_F64 = 3
 
 
 class EmfdError(RuntimeError):
@@ -130,6 +131,7 @@ _SIGNATURES = {
     "emfd_grid_set_kernel": (C.c_int, [_P, C.c_int]),
     "emfd_grid_material_classes": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "emfd_grid_set_storage": (C.c_int, [_P, C.c_int, _P]),
+    "emfd_grid_download_fields_f64": (C.c_int, [_P, _P, _P]),
     "emfd_grid_kernel_timing": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "emfd_ports_configure": (C.c_int, [_P, _P, C.c_int, C.c_float, _P, C.c_float, C.c_int64]),
     "emfd_ports_disable": (C.c_int, [_P]),
@@ -423,6 +425,15 @@ class SimulationGrid:
         component (E x y z, B x y z). Clears the fields and rewinds time."""
         k = None if scale_log2 is None else (C.c_int * 6)(*[int(v) for v in scale_log2])
         _check(self._lib.emfd_grid_set_storage(self._h, int(storage), None if k is None else C.cast(k, C.c_void_p)))
+
+    def fields_f64(self):
+        """(E, B) as (3, n) float64 planes (exact for STORAGE_F64)."""
+        nx, ny, nz = self.size
+        n = nx * ny * nz
+        E = np.empty((3, n), np.float64)
+        B = np.empty((3, n), np.float64)
+        _check(self._lib.emfd_grid_download_fields_f64(self._h, _ptr(E), _ptr(B)))
+        return E, B
 
     def material_classes(self) -> int:
         c = C.c_int(0)

@@ -433,6 +433,51 @@ def spheres_dipole(n=256, pml=10, n_spheres=32, wavelength=20.0, seed=DEFAULT_SE
     return sc
 
 
+def spheres_array(n=192, pml=10, n_spheres=32, wavelength=20.0, seed=DEFAULT_SEED, length=11, rows=4,
+                  cols=4, sweep_points=32) -> Scene:
+    """Precision-study scene: the seeded sphere medium of spheres_dipole (radii and placement
+    scaled by n / 256) around a rows x cols array of z thin-wire dipoles at half-wave spacing in
+    x and y, each on its own 50 ohm resistive feed (uniform amplitude, Gaussian pulse); port
+    sweep and far-field box as spheres_dipole."""
+    sc = Scene(size=(n, n, n), pml=pml, courant=MAX_COURANT, cell_size=1.0e-3, wavelength=wavelength)
+    ff = sc.set_far_field(wavelength=wavelength)
+    rng = Xoshiro256ss(seed)
+    c0 = n / 2
+    spacing = round_half_away(wavelength / 2)
+    xs = [round_half_away(c0 + (i - (cols - 1) / 2) * spacing) for i in range(cols)]
+    ys = [round_half_away(c0 + (j - (rows - 1) / 2) * spacing) for j in range(rows)]
+    keep_lo = (min(xs) - 3, min(ys) - 3, c0 - length / 2 - 1 - 3)
+    keep_hi = (max(xs) + 3, max(ys) + 3, c0 + length / 2 + 1 + 3)
+    scale = n / 256.0
+    placed = attempts = 0
+    while placed < n_spheres and attempts < 100000:
+        attempts += 1
+        r = rng.uniform(4.0 * scale, 20.0 * scale)
+        eps = rng.uniform(1.0, 12.0)
+        cond = rng.log_uniform(1e-4, 1.0)
+        lo = ff.box_min[0] + r + 2
+        hi = ff.box_max[0] - r - 2
+        if hi <= lo:
+            continue
+        c = (rng.uniform(lo, hi), rng.uniform(lo, hi), rng.uniform(lo, hi))
+        # clear of the array's bounding box (plus 3 cells)
+        d2 = sum(max(keep_lo[a] - c[a], 0.0, c[a] - keep_hi[a]) ** 2 for a in range(3))
+        if math.sqrt(d2) - r < 0.0:
+            continue
+        sc.sphere(c, r, eps_r=eps, conductivity=cond)
+        placed += 1
+    for j, y in enumerate(ys):
+        for i, x in enumerate(xs):
+            sc.dipole_thin((x, y, c0), axis=2, length=length, radius=0.005 * wavelength, impedance=50.0,
+                           amplitude=1.0, name=f"element {j * cols + i}", wavelength=wavelength, waveform="pulse",
+                           pulse_width=1.0)
+    sc.set_sweep((wavelength / 1.5, wavelength / 0.5), points=sweep_points)
+    m = ff.box_min[0] - 2
+    sc.fluxes.append(FluxSurface.box((m, m, m), (n - 1 - m, n - 1 - m, n - 1 - m), "flux box"))
+    sc.emit_thin_wires()
+    return sc
+
+
 # ---------------------------------------------------------------------------------------------
 # Batched antenna scenes: parallel z thin-wire elements in vacuum, element 0 fed
 # ---------------------------------------------------------------------------------------------

@@ -51,6 +51,7 @@ extern "C" {
 #define EMFD_STORAGE_F32 0  /* FP32 field planes (default; EM_Lab's RGBA32F) */
 #define EMFD_STORAGE_F16 1  /* binary16 planes of value / 2^k per component, FP32 updates */
 #define EMFD_STORAGE_BF16 2 /* bfloat16 planes of value / 2^k per component, FP32 updates */
+#define EMFD_STORAGE_F64 3  /* FP64 planes and FP64 updates: the accuracy reference */
 
 typedef struct emfd_ctx emfd_ctx;
 
@@ -225,9 +226,13 @@ int emfd_grid_set_kernel(emfd_ctx *ctx, int variant);
  * with scale_log2[6] = k per component (E x y z, B x y z; NULL = all 0), the stored value being
  * field / 2^k rounded to nearest even. Every update still computes in FP32; only the planes are
  * 16-bit, which halves the field traffic. 16-bit storage needs nx % 8 == 0 and runs the
- * two-pass kernel (EMFD_KERNEL_AUTO). Clears the fields and rewinds time, like
+ * two-pass kernel (EMFD_KERNEL_AUTO). EMFD_STORAGE_F64 runs every update and source in double
+ * precision (the reference for accuracy studies); monitors read its fields as float. Clears the fields and rewinds time, like
  * emfd_grid_upload_materials. Downloads and uploads convert to and from FP32 values. */
 int emfd_grid_set_storage(emfd_ctx *ctx, int storage, const int *scale_log2);
+/* E and B as FP64 structure-of-arrays planes (3 x n doubles each; either may be NULL); exact
+ * for EMFD_STORAGE_F64, converted otherwise. */
+int emfd_grid_download_fields_f64(emfd_ctx *ctx, double *E, double *B);
 /* Number of distinct coefficient classes after the last material upload (0 = uniform vacuum). */
 int emfd_grid_material_classes(emfd_ctx *ctx, int *classes);
 /* Live timing of the Yee kernel: with enable != 0, every Yee launch of later steps is bracketed
