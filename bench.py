@@ -10,6 +10,10 @@ roofline = the Yee step (default: the two vectorised passes k_update_b4 + k_upda
            together): algorithmic 48 B per cell-update (E and B read once, written once, FP32) /
            its average time from HIP events on the library's stream
 cpu_baseline = the CPU oracle (scalar C, OpenMP over z-planes) on a bounded number of steps
+surrogate = the learned screening path: 65,536 seeded thin-wire antenna candidates through the
+           trained MLP (BF16 MFMA), the best 256 by predicted -10 dB bandwidth verified by the
+           batched FDTD stepper (4000 steps each); designs screened per second against
+           one-at-a-time FDTD of the same designs, and how many verified within 0.5 dB of S11
 """
 from __future__ import annotations
 
@@ -23,7 +27,7 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-METRIC = "Mcell-updates/s & %HBM peak"
+METRIC = "Mcell-updates/s & %HBM peak; designs/s vs plain FDTD at |\u0394S11|\u22640.5 dB"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = 48  # FP32 E and B (6 components), each read once and written once
 
@@ -69,6 +73,8 @@ def main() -> int:
     ap.add_argument("--storage", choices=["f32", "f16", "bf16"], default="f32",
                     help="field storage: FP32 planes, or 16-bit planes with FP32 updates")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the learned screening measurement (designs/s against one-at-a-time FDTD)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only time a plain float4 device copy of MB megabytes (roofline calibration)")
     args = ap.parse_args()
@@ -141,7 +147,7 @@ def main() -> int:
             and storage == E.STORAGE_F32):
         traffic = json.loads(tf.read_text()).get("bytes_per_launch")
     kernel_names = {0: "k_update_b4+k_update_e4", 1: "k_yee_fused", 2: "k_update_b+k_update_e", 3: "k_yee_rows",
-                    4: "k_yee_point", 5: "k_update_b4+k_update_e4"}
+                    4: "k_yee_point", 5: "k_update_b4+k_update_e4", 6: "k_update_b4+k_update_e4 (z-slabs)"}
     kname = kernel_names.get(args.kernel & 0xff, "?") if storage == E.STORAGE_F32 else "k_update_b8h+k_update_e8h"
     out = {
         "metric": METRIC,
@@ -175,6 +181,10 @@ def main() -> int:
         "cpu_baseline": None,
     }
     g.close()
+    if (not args.no_surrogate and args.size == 256 and args.scene == "spheres" and args.storage == "f32"
+            and args.kernel == 0):
+        from em_lab_amd import surrogate as S
+        out["surrogate"] = S.screening_run(65536, 256, plain_designs=4)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene)
     print(json.dumps(out), flush=True)

@@ -536,7 +536,7 @@ int emfd_grid_set_storage(emfd_ctx *ctx, int storage, const int *scale_log2) {
 int emfd_grid_set_kernel(emfd_ctx *ctx, int variant) {
     if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
     const int v = variant & 0xff;
-    if (v < 0 || v > 5) return fail(EMFD_ERR_INVALID, "unknown kernel variant");
+    if (v < 0 || v > 6) return fail(EMFD_ERR_INVALID, "unknown kernel variant");
     ctx->kernel_variant = v;
     ctx->tile_cfg = variant >> 8;  // optional tile shape / z-chunk selector for the fused kernel
     return EMFD_OK;
@@ -647,6 +647,11 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
             EMFD_HIP(launch_yee_two_pass(lat, ctx->fields(ctx->cur), ctx->stream));
         } else if (variant == EMFD_KERNEL_TWO_PASS4) {
             EMFD_HIP(launch_yee_two_pass4(lat, ctx->fields(ctx->cur), ctx->stream));
+        } else if (variant == EMFD_KERNEL_SLABS) {
+            if (!yee_two_pass4_supported(lat))
+                return fail(EMFD_ERR_INVALID, "the slab-interleaved kernel needs nx % 4 == 0");
+            EMFD_HIP(launch_yee_two_pass4_slabs(lat, ctx->fields(ctx->cur), ctx->tile_cfg > 0 ? ctx->tile_cfg : 32,
+                                                ctx->stream));
         } else if (variant == EMFD_KERNEL_POINT) {
             EMFD_HIP(launch_yee_point(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->stream));
             ctx->cur ^= 1;

@@ -76,6 +76,7 @@ int yee_rows_supported(const LatticeArgs &a);
 hipError_t launch_yee_point(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, hipStream_t s);
 hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
 hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_yee_two_pass4_slabs(const LatticeArgs &a, const FieldPtrs &f, int slab, hipStream_t s);
 int yee_two_pass4_supported(const LatticeArgs &a);
 hipError_t launch_yee_half(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
 hipError_t launch_yee_f64(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);

@@ -6,14 +6,14 @@
 namespace emfd {
 namespace {
 
-__global__ void __launch_bounds__(256) k_update_b4(LatticeArgs a, FieldPtrs f) {
-    const int x0 = (blockIdx.x * 64 + threadIdx.x) * 4, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+__global__ void __launch_bounds__(256) k_update_b4(LatticeArgs a, FieldPtrs f, int z_base) {
+    const int x0 = (blockIdx.x * 64 + threadIdx.x) * 4, y = blockIdx.y * 4 + threadIdx.y, z = z_base + blockIdx.z;
     if (x0 >= a.nx || y >= a.ny) return;
     b_quad(a, f, x0, y, z);
 }
 
-__global__ void __launch_bounds__(256) k_update_e4(LatticeArgs a, FieldPtrs f) {
-    const int x0 = (blockIdx.x * 64 + threadIdx.x) * 4, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+__global__ void __launch_bounds__(256) k_update_e4(LatticeArgs a, FieldPtrs f, int z_base) {
+    const int x0 = (blockIdx.x * 64 + threadIdx.x) * 4, y = blockIdx.y * 4 + threadIdx.y, z = z_base + blockIdx.z;
     if (x0 >= a.nx || y >= a.ny) return;
     e_quad(a, f, x0, y, z);
 }
@@ -25,11 +25,32 @@ int yee_two_pass4_supported(const LatticeArgs &a) { return a.nx % 4 == 0 ? 1 : 0
 hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s) {
     if (!yee_two_pass4_supported(a)) return hipErrorInvalidValue;
     dim3 block(64, 4, 1), grid((a.nx / 4 + 63) / 64, (a.ny + 3) / 4, a.nz);
-    k_update_b4<<<grid, block, 0, s>>>(a, f);
+    k_update_b4<<<grid, block, 0, s>>>(a, f, 0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_update_e4<<<grid, block, 0, s>>>(a, f);
+    k_update_e4<<<grid, block, 0, s>>>(a, f, 0);
     return hipGetLastError();
+}
+
+// Slab-interleaved order: B then E on z-slab 0, then on slab 1, ... A valid leapfrog schedule:
+// E on slab k needs B^{n+1/2} on slab k and on the plane below it (slab k-1, already done), and
+// B on slab k+1 needs E^n on slab k+1 and the plane above it, which no earlier E pass has
+// touched. The E pass of a slab re-reads what the B pass just read and wrote, so with a slab of
+// a few tens of MB those reads are served by the memory-side cache instead of HBM.
+hipError_t launch_yee_two_pass4_slabs(const LatticeArgs &a, const FieldPtrs &f, int slab, hipStream_t s) {
+    if (!yee_two_pass4_supported(a)) return hipErrorInvalidValue;
+    if (slab < 1) slab = 1;
+    for (int z0 = 0; z0 < a.nz; z0 += slab) {
+        const int nzs = a.nz - z0 < slab ? a.nz - z0 : slab;
+        dim3 block(64, 4, 1), grid((a.nx / 4 + 63) / 64, (a.ny + 3) / 4, nzs);
+        k_update_b4<<<grid, block, 0, s>>>(a, f, z0);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        k_update_e4<<<grid, block, 0, s>>>(a, f, z0);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace emfd

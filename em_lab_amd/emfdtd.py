@@ -27,6 +27,7 @@ KERNEL_TWO_PASS = 2
 KERNEL_ROWS = 3
 KERNEL_POINT = 4
 KERNEL_TWO_PASS4 = 5
+KERNEL_SLABS = 6
 STORAGE_F32 = 0
 STORAGE_F16 = 1
 STORAGE_BF16 = 2

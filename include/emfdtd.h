@@ -47,6 +47,8 @@ extern "C" {
 #define EMFD_KERNEL_ROWS 3     /* fused B+E, one wavefront per x-row z-march (nx <= 256) */
 #define EMFD_KERNEL_POINT 4    /* fused B+E, one thread per texel, neighbour B rebuilt */
 #define EMFD_KERNEL_TWO_PASS4 5 /* two passes, four cells per thread with float4 streams (nx % 4 == 0) */
+#define EMFD_KERNEL_SLABS 6     /* TWO_PASS4 interleaved per z-slab (B, E on slab 0, then slab 1, ...); bits 8..
+                                   of the variant = slab thickness in planes (default 32) */
 
 #define EMFD_STORAGE_F32 0  /* FP32 field planes (default; EM_Lab's RGBA32F) */
 #define EMFD_STORAGE_F16 1  /* binary16 planes of value / 2^k per component, FP32 updates */
