@@ -36,9 +36,9 @@ __global__ void __launch_bounds__(256) k_batch_yee(BatchLaunchArgs args) {
                 const uint32_t z = __umulhi(r, sc.my);
                 const uint32_t y = r - z * static_cast<uint32_t>(sc.lat.ny);
                 if (kB)
-                    b_quad(sc.lat, sc.f, static_cast<int>(x0), static_cast<int>(y), static_cast<int>(z));
+                    b_quad<2>(sc.lat, sc.f, static_cast<int>(x0), static_cast<int>(y), static_cast<int>(z));
                 else
-                    e_quad(sc.lat, sc.f, static_cast<int>(x0), static_cast<int>(y), static_cast<int>(z));
+                    e_quad<2>(sc.lat, sc.f, static_cast<int>(x0), static_cast<int>(y), static_cast<int>(z));
             }
         }
     }

@@ -646,7 +646,9 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
         } else if (variant == EMFD_KERNEL_TWO_PASS) {
             EMFD_HIP(launch_yee_two_pass(lat, ctx->fields(ctx->cur), ctx->stream));
         } else if (variant == EMFD_KERNEL_TWO_PASS4) {
-            EMFD_HIP(launch_yee_two_pass4(lat, ctx->fields(ctx->cur), ctx->stream));
+            // streaming level: explicit bits 8..9 of the variant, else 2 (measured fastest)
+            const int st = ctx->kernel_variant == EMFD_KERNEL_AUTO ? 2 : (ctx->tile_cfg & 3);
+            EMFD_HIP(launch_yee_two_pass4(lat, ctx->fields(ctx->cur), st, ctx->stream));
         } else if (variant == EMFD_KERNEL_SLABS) {
             if (!yee_two_pass4_supported(lat))
                 return fail(EMFD_ERR_INVALID, "the slab-interleaved kernel needs nx % 4 == 0");

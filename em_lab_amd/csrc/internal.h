@@ -75,7 +75,7 @@ hipError_t launch_yee_rows(const LatticeArgs &a, const FieldPtrs &src, const Fie
 int yee_rows_supported(const LatticeArgs &a);
 hipError_t launch_yee_point(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, hipStream_t s);
 hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
-hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, int streaming, hipStream_t s);
 hipError_t launch_yee_two_pass4_slabs(const LatticeArgs &a, const FieldPtrs &f, int slab, hipStream_t s);
 int yee_two_pass4_supported(const LatticeArgs &a);
 hipError_t launch_yee_half(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
@@ -173,7 +173,10 @@ hipError_t launch_batch_sweep(const BatchSweepJob *jobs, int n_jobs, hipStream_t
 
 // ---- MLP surrogate (mlp.hip / mlp.cpp) ----
 // BF16 GEMM with an FP32-accumulating MFMA core and a fused epilogue; see mlp.hip.
-enum MlpEpilogue : int { MLP_EPI_STORE = 0, MLP_EPI_BIAS = 1, MLP_EPI_BIAS_GELU = 2, MLP_EPI_GELU_GRAD = 3 };
+enum MlpEpilogue : int {
+    MLP_EPI_STORE = 0, MLP_EPI_BIAS = 1, MLP_EPI_BIAS_GELU = 2, MLP_EPI_GELU_GRAD = 3,
+    MLP_EPI_BIAS_GELU_INFER = 4  // as BIAS_GELU without keeping the pre-activation
+};
 struct MlpGemm {
     const void *A;  // bf16
     const void *B;  // bf16

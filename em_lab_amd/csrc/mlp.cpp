@@ -45,7 +45,7 @@ struct emfd_mlp {
         for (auto &b : z) b.release();
         for (auto &b : h) b.release();
     }
-    int forward(int rows);
+    int forward(int rows, bool keep_preactivation = true);
     int backward(int rows, double *loss_partials_dst, int blocks);
 };
 
@@ -62,9 +62,9 @@ uint64_t splitmix(uint64_t &s) {
 }
 }  // namespace
 
-// Forward over `rows` rows already in xb: hidden layers write z (FP32) and h (BF16), the last
-// layer writes y (FP32).
-int emfd_mlp::forward(int rows) {
+// Forward over `rows` rows already in xb: hidden layers write h (BF16) and, for training, z
+// (FP32 pre-activation); the last layer writes y (FP32).
+int emfd_mlp::forward(int rows, bool keep_preactivation) {
     const void *in = xb.ptr;
     for (int l = 0; l < layers; ++l) {
         const bool last = l == layers - 1;
@@ -78,7 +78,7 @@ int emfd_mlp::forward(int rows) {
         gm.K = kp[static_cast<size_t>(l)];
         gm.lda = kp[static_cast<size_t>(l)];
         gm.ldb = kp[static_cast<size_t>(l)];
-        gm.epi = last ? MLP_EPI_BIAS : MLP_EPI_BIAS_GELU;
+        gm.epi = last ? MLP_EPI_BIAS : (keep_preactivation ? MLP_EPI_BIAS_GELU : MLP_EPI_BIAS_GELU_INFER);
         gm.bias = p.as<float>() + b_off[static_cast<size_t>(l)];
         gm.out_f32 = last ? y.as<float>() : z[static_cast<size_t>(l)].as<float>();
         gm.out_bf16 = last ? nullptr : h[static_cast<size_t>(l)].ptr;
@@ -317,7 +317,7 @@ int emfd_mlp_forward(emfd_mlp *m, const float *x, int64_t batch, float *y) {
     for (int64_t r0 = 0; r0 < batch; r0 += m->max_batch) {
         const int rows = static_cast<int>(std::min<int64_t>(m->max_batch, batch - r0));
         if (int rc = stage_rows(m, x + r0 * m->n_in(), nullptr, rows, s.x, s.t)) return rc;
-        if (int rc = m->forward(rows)) return rc;
+        if (int rc = m->forward(rows, false)) return rc;
         EMFD_HIP(hipMemcpyAsync(y + r0 * m->n_out(), m->y.ptr, static_cast<size_t>(rows) * m->n_out() * sizeof(float),
                                 hipMemcpyDeviceToHost, m->stream));
         EMFD_HIP(hipStreamSynchronize(m->stream));
@@ -469,10 +469,10 @@ int emfd_mlp_time_forward(emfd_mlp *m, int64_t batch, int iterations, double *av
     hipEvent_t a, b;
     EMFD_HIP(hipEventCreate(&a));
     EMFD_HIP(hipEventCreate(&b));
-    if (int rc = m->forward(rows)) return rc;  // warm-up
+    if (int rc = m->forward(rows, false)) return rc;  // warm-up
     EMFD_HIP(hipEventRecord(a, m->stream));
     for (int i = 0; i < iterations; ++i)
-        if (int rc = m->forward(rows)) return rc;
+        if (int rc = m->forward(rows, false)) return rc;
     EMFD_HIP(hipEventRecord(b, m->stream));
     EMFD_HIP(hipEventSynchronize(b));
     float ms = 0.0f;

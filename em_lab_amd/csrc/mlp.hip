@@ -3,8 +3,9 @@
 //
 // GEMM: C[i][j] = sum_k A(i, k) B(k, j), A(i, k) = A[i * lda + k] (A_T = false) or A[k * lda + i]
 // (A_T = true), B(k, j) = B[k * ldb + j] (B_T = false) or B[j * ldb + k] (B_T = true).
-// Workgroup tile 64 x 64, four wavefronts of 32 x 32, K in steps of 32 staged through LDS with the
-// k index contiguous per row (so every MFMA operand is one 16-byte LDS read per lane). Global
+// Workgroup tile 64 x 64 (one MFMA tile per wavefront) or 128 x 128 (2 x 2 per wavefront), four
+// wavefronts, K in steps of 32 staged through LDS with the k index contiguous per row (so every
+// MFMA operand is one 16-byte LDS read per lane). Global
 // loads are 16 B along whichever index is contiguous in memory. Rows and columns past M / N and
 // k past K read as zero.
 //
@@ -19,7 +20,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8;  // LDS row stride (bf16): 80 B, 16 B aligned
+constexpr int BK = 32, LDK = BK + 8;  // LDS row stride (bf16): 80 B, 16 B aligned
 
 __device__ __forceinline__ float gelu_f(float x) {
     // tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))
@@ -47,108 +48,137 @@ struct GemmArgs {
     int ldc;
 };
 
-enum : int { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2, EPI_GELU_GRAD = 3 };
+enum : int { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2, EPI_GELU_GRAD = 3, EPI_BIAS_GELU_INFER = 4 };
 
-template <bool A_T, bool B_T>
+template <bool A_T, bool B_T, int W>
 __device__ __forceinline__ void stage(const GemmArgs &g, int i0, int j0, int k0, __bf16 (*sA)[LDK],
                                       __bf16 (*sB)[LDK]) {
     const int t = threadIdx.x;
     const __bf16 zero = static_cast<__bf16>(0.0f);
-    // A tile: rows i0..i0+63, k0..k0+31 -> sA[i][k]
-    if (!A_T) {
-        const int i = t >> 2, kk = (t & 3) * 8;
-        const int gi = i0 + i, gk = k0 + kk;
-        __bf16 v[8];
-        if (gi < g.M && gk + 8 <= g.K && (g.lda % 8) == 0) {
-            *reinterpret_cast<uint4 *>(v) = *reinterpret_cast<const uint4 *>(g.A + static_cast<size_t>(gi) * g.lda + gk);
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+        // A tile: rows i0 .. i0 + 64 W - 1, k0 .. k0 + 31 -> sA[i][k]
+        if (!A_T) {
+            const int i = (t >> 2) + 64 * e, kk = (t & 3) * 8;
+            const int gi = i0 + i, gk = k0 + kk;
+            __bf16 v[8];
+            if (gi < g.M && gk + 8 <= g.K && (g.lda % 8) == 0) {
+                *reinterpret_cast<uint4 *>(v) =
+                    *reinterpret_cast<const uint4 *>(g.A + static_cast<size_t>(gi) * g.lda + gk);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    v[q] = (gi < g.M && gk + q < g.K) ? g.A[static_cast<size_t>(gi) * g.lda + gk + q] : zero;
+            }
+            *reinterpret_cast<uint4 *>(&sA[i][kk]) = *reinterpret_cast<const uint4 *>(v);
         } else {
+            const int kk = t >> 3, i = (t & 7) * 8 + 64 * e;
+            const int gk = k0 + kk, gi = i0 + i;
+            __bf16 v[8];
+            if (gk < g.K && gi + 8 <= g.M && (g.lda % 8) == 0) {
+                *reinterpret_cast<uint4 *>(v) =
+                    *reinterpret_cast<const uint4 *>(g.A + static_cast<size_t>(gk) * g.lda + gi);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e)
-                v[e] = (gi < g.M && gk + e < g.K) ? g.A[static_cast<size_t>(gi) * g.lda + gk + e] : zero;
+                for (int q = 0; q < 8; ++q)
+                    v[q] = (gk < g.K && gi + q < g.M) ? g.A[static_cast<size_t>(gk) * g.lda + gi + q] : zero;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) sA[i + q][kk] = v[q];
         }
-        *reinterpret_cast<uint4 *>(&sA[i][kk]) = *reinterpret_cast<const uint4 *>(v);
-    } else {
-        const int kk = t >> 3, i = (t & 7) * 8;
-        const int gk = k0 + kk, gi = i0 + i;
-        __bf16 v[8];
-        if (gk < g.K && gi + 8 <= g.M && (g.lda % 8) == 0) {
-            *reinterpret_cast<uint4 *>(v) = *reinterpret_cast<const uint4 *>(g.A + static_cast<size_t>(gk) * g.lda + gi);
+        // B tile: k0 .. k0 + 31, cols j0 .. j0 + 64 W - 1 -> sB[j][k]
+        if (B_T) {
+            const int j = (t >> 2) + 64 * e, kk = (t & 3) * 8;
+            const int gj = j0 + j, gk = k0 + kk;
+            __bf16 v[8];
+            if (gj < g.N && gk + 8 <= g.K && (g.ldb % 8) == 0) {
+                *reinterpret_cast<uint4 *>(v) =
+                    *reinterpret_cast<const uint4 *>(g.B + static_cast<size_t>(gj) * g.ldb + gk);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    v[q] = (gj < g.N && gk + q < g.K) ? g.B[static_cast<size_t>(gj) * g.ldb + gk + q] : zero;
+            }
+            *reinterpret_cast<uint4 *>(&sB[j][kk]) = *reinterpret_cast<const uint4 *>(v);
         } else {
+            const int kk = t >> 3, j = (t & 7) * 8 + 64 * e;
+            const int gk = k0 + kk, gj = j0 + j;
+            __bf16 v[8];
+            if (gk < g.K && gj + 8 <= g.N && (g.ldb % 8) == 0) {
+                *reinterpret_cast<uint4 *>(v) =
+                    *reinterpret_cast<const uint4 *>(g.B + static_cast<size_t>(gk) * g.ldb + gj);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e)
-                v[e] = (gk < g.K && gi + e < g.M) ? g.A[static_cast<size_t>(gk) * g.lda + gi + e] : zero;
+                for (int q = 0; q < 8; ++q)
+                    v[q] = (gk < g.K && gj + q < g.N) ? g.B[static_cast<size_t>(gk) * g.ldb + gj + q] : zero;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) sB[j + q][kk] = v[q];
         }
-#pragma unroll
-        for (int e = 0; e < 8; ++e) sA[i + e][kk] = v[e];
-    }
-    // B tile: k0..k0+31, cols j0..j0+63 -> sB[j][k]
-    if (B_T) {
-        const int j = t >> 2, kk = (t & 3) * 8;
-        const int gj = j0 + j, gk = k0 + kk;
-        __bf16 v[8];
-        if (gj < g.N && gk + 8 <= g.K && (g.ldb % 8) == 0) {
-            *reinterpret_cast<uint4 *>(v) = *reinterpret_cast<const uint4 *>(g.B + static_cast<size_t>(gj) * g.ldb + gk);
-        } else {
-#pragma unroll
-            for (int e = 0; e < 8; ++e)
-                v[e] = (gj < g.N && gk + e < g.K) ? g.B[static_cast<size_t>(gj) * g.ldb + gk + e] : zero;
-        }
-        *reinterpret_cast<uint4 *>(&sB[j][kk]) = *reinterpret_cast<const uint4 *>(v);
-    } else {
-        const int kk = t >> 3, j = (t & 7) * 8;
-        const int gk = k0 + kk, gj = j0 + j;
-        __bf16 v[8];
-        if (gk < g.K && gj + 8 <= g.N && (g.ldb % 8) == 0) {
-            *reinterpret_cast<uint4 *>(v) = *reinterpret_cast<const uint4 *>(g.B + static_cast<size_t>(gk) * g.ldb + gj);
-        } else {
-#pragma unroll
-            for (int e = 0; e < 8; ++e)
-                v[e] = (gk < g.K && gj + e < g.N) ? g.B[static_cast<size_t>(gk) * g.ldb + gj + e] : zero;
-        }
-#pragma unroll
-        for (int e = 0; e < 8; ++e) sB[j + e][kk] = v[e];
     }
 }
 
-template <bool A_T, bool B_T>
+// Workgroup tile (64 W) x (64 W); each wavefront owns (32 W) x (32 W) = W x W MFMA tiles.
+template <bool A_T, bool B_T, int W>
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
-    __shared__ __attribute__((aligned(16))) __bf16 sA[BM][LDK];
-    __shared__ __attribute__((aligned(16))) __bf16 sB[BN][LDK];
-    const int i0 = blockIdx.y * BM, j0 = blockIdx.x * BN;
+    __shared__ __attribute__((aligned(16))) __bf16 sA[64 * W][LDK];
+    __shared__ __attribute__((aligned(16))) __bf16 sB[64 * W][LDK];
+    const int i0 = blockIdx.y * 64 * W, j0 = blockIdx.x * 64 * W;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    floatx16 acc = {};
+    const int wr = (wave >> 1) * 32 * W, wc = (wave & 1) * 32 * W;
+    floatx16 acc[W][W];
+#pragma unroll
+    for (int tm = 0; tm < W; ++tm)
+#pragma unroll
+        for (int tn = 0; tn < W; ++tn) acc[tm][tn] = floatx16{};
     for (int k0 = 0; k0 < g.K; k0 += BK) {
-        stage<A_T, B_T>(g, i0, j0, k0, sA, sB);
+        stage<A_T, B_T, W>(g, i0, j0, k0, sA, sB);
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 16) {
-            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&sA[wr + (lane & 31)][ks + 8 * (lane >> 5)]);
-            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&sB[wc + (lane & 31)][ks + 8 * (lane >> 5)]);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            bf16x8 a[W], b[W];
+#pragma unroll
+            for (int tm = 0; tm < W; ++tm)
+                a[tm] = *reinterpret_cast<const bf16x8 *>(&sA[wr + 32 * tm + (lane & 31)][ks + 8 * (lane >> 5)]);
+#pragma unroll
+            for (int tn = 0; tn < W; ++tn)
+                b[tn] = *reinterpret_cast<const bf16x8 *>(&sB[wc + 32 * tn + (lane & 31)][ks + 8 * (lane >> 5)]);
+#pragma unroll
+            for (int tm = 0; tm < W; ++tm)
+#pragma unroll
+                for (int tn = 0; tn < W; ++tn)
+                    acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[tm], b[tn], acc[tm][tn], 0, 0, 0);
         }
         __syncthreads();
     }
-    const int j = j0 + wc + (lane & 31);
-    if (j >= g.N) return;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int i = i0 + wr + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        if (i >= g.M) continue;
-        const size_t o = static_cast<size_t>(i) * g.ldc + j;
-        float v = acc[r];
-        if (g.epi == EPI_BIAS || g.epi == EPI_BIAS_GELU) v += g.bias[j];
-        if (g.epi == EPI_BIAS_GELU) {
-            g.out_f32[o] = v;  // pre-activation, kept for the backward pass
-            g.out_bf16[o] = static_cast<__bf16>(gelu_f(v));
-        } else if (g.epi == EPI_GELU_GRAD) {
-            const float d = v * gelu_grad_f(g.aux[o]);
-            g.out_f32[o] = d;
-            g.out_bf16[o] = static_cast<__bf16>(d);
-        } else {
-            if (g.out_f32) g.out_f32[o] = v;
-            if (g.out_bf16) g.out_bf16[o] = static_cast<__bf16>(v);
-        }
+    for (int tn = 0; tn < W; ++tn) {
+        const int j = j0 + wc + 32 * tn + (lane & 31);
+        if (j >= g.N) continue;
+        const float bj = (g.epi == EPI_BIAS || g.epi == EPI_BIAS_GELU || g.epi == EPI_BIAS_GELU_INFER) ? g.bias[j] : 0.0f;
+#pragma unroll
+        for (int tm = 0; tm < W; ++tm)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = i0 + wr + 32 * tm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (i >= g.M) continue;
+                const size_t o = static_cast<size_t>(i) * g.ldc + j;
+                float v = acc[tm][tn][r];
+                if (g.epi == EPI_BIAS || g.epi == EPI_BIAS_GELU || g.epi == EPI_BIAS_GELU_INFER) v += bj;
+                if (g.epi == EPI_BIAS_GELU_INFER) {
+                    g.out_bf16[o] = static_cast<__bf16>(gelu_f(v));
+                } else if (g.epi == EPI_BIAS_GELU) {
+                    g.out_f32[o] = v;  // pre-activation, kept for the backward pass
+                    g.out_bf16[o] = static_cast<__bf16>(gelu_f(v));
+                } else if (g.epi == EPI_GELU_GRAD) {
+                    const float d = v * gelu_grad_f(g.aux[o]);
+                    g.out_f32[o] = d;
+                    g.out_bf16[o] = static_cast<__bf16>(d);
+                } else {
+                    if (g.out_f32) g.out_f32[o] = v;
+                    if (g.out_bf16) g.out_bf16[o] = static_cast<__bf16>(v);
+                }
+            }
     }
 }
 
@@ -238,11 +268,22 @@ hipError_t launch_mlp_gemm(const MlpGemm &m, hipStream_t s) {
     g.out_bf16 = static_cast<__bf16 *>(m.out_bf16);
     g.ldc = m.ldc;
     if (m.M <= 0 || m.N <= 0) return hipSuccess;
-    dim3 grid((m.N + BN - 1) / BN, (m.M + BM - 1) / BM);
-    if (!m.a_t && !m.b_t) k_gemm<false, false><<<grid, 256, 0, s>>>(g);
-    else if (!m.a_t && m.b_t) k_gemm<false, true><<<grid, 256, 0, s>>>(g);
-    else if (m.a_t && !m.b_t) k_gemm<true, false><<<grid, 256, 0, s>>>(g);
-    else k_gemm<true, true><<<grid, 256, 0, s>>>(g);
+    // 128 x 128 workgroup tiles (2 x 2 MFMA tiles per wavefront) when they still give every CU
+    // several workgroups; otherwise 64 x 64 tiles balance better
+    const long tiles128 = static_cast<long>((m.M + 127) / 128) * ((m.N + 127) / 128);
+    if (m.M >= 128 && m.N >= 128 && tiles128 >= 2048) {
+        dim3 grid((m.N + 127) / 128, (m.M + 127) / 128);
+        if (!m.a_t && !m.b_t) k_gemm<false, false, 2><<<grid, 256, 0, s>>>(g);
+        else if (!m.a_t && m.b_t) k_gemm<false, true, 2><<<grid, 256, 0, s>>>(g);
+        else if (m.a_t && !m.b_t) k_gemm<true, false, 2><<<grid, 256, 0, s>>>(g);
+        else k_gemm<true, true, 2><<<grid, 256, 0, s>>>(g);
+        return hipGetLastError();
+    }
+    dim3 grid((m.N + 63) / 64, (m.M + 63) / 64);
+    if (!m.a_t && !m.b_t) k_gemm<false, false, 1><<<grid, 256, 0, s>>>(g);
+    else if (!m.a_t && m.b_t) k_gemm<false, true, 1><<<grid, 256, 0, s>>>(g);
+    else if (m.a_t && !m.b_t) k_gemm<true, false, 1><<<grid, 256, 0, s>>>(g);
+    else k_gemm<true, true, 1><<<grid, 256, 0, s>>>(g);
     return hipGetLastError();
 }
 

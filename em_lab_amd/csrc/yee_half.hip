@@ -17,13 +17,25 @@ __device__ __forceinline__ void load8(const void *plane, size_t p, float s, floa
     for (int j = 0; j < 8; ++j) o[j] = static_cast<float>(h[j]) * s;
 }
 
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// the pass's output plane: streaming (non-temporal) 16-byte store
 template <typename T>
 __device__ __forceinline__ void store8(void *plane, size_t p, float inv, const float (&v)[8]) {
-    uint4 raw;
+    u32x4 raw;
     T *h = reinterpret_cast<T *>(&raw);
 #pragma unroll
     for (int j = 0; j < 8; ++j) h[j] = static_cast<T>(v[j] * inv);
-    *reinterpret_cast<uint4 *>(static_cast<T *>(plane) + p) = raw;
+    __builtin_nontemporal_store(raw, reinterpret_cast<u32x4 *>(static_cast<T *>(plane) + p));
+}
+
+// the plane a pass reads exactly once: streaming load
+template <typename T>
+__device__ __forceinline__ void load8s(const void *plane, size_t p, float s, float (&o)[8]) {
+    const u32x4 raw = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(static_cast<const T *>(plane) + p));
+    const T *h = reinterpret_cast<const T *>(&raw);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o[j] = static_cast<float>(h[j]) * s;
 }
 
 template <typename T>
@@ -72,7 +84,7 @@ __global__ void __launch_bounds__(256) k_update_b8h(LatticeArgs a, FieldPtrs f) 
         else
 #pragma unroll
             for (int j = 0; j < 8; ++j) eZ[c][j] = 0.0f;
-        load8<T>(f.b[c], p, f.scale[3 + c], b[c]);
+        load8s<T>(f.b[c], p, f.scale[3 + c], b[c]);
     }
     uint32_t id[8];
     load_ids8(a, p, id);
@@ -113,7 +125,7 @@ __global__ void __launch_bounds__(256) k_update_e8h(LatticeArgs a, FieldPtrs f) 
         else
 #pragma unroll
             for (int j = 0; j < 8; ++j) bZ[c][j] = 0.0f;
-        load8<T>(f.e[c], p, f.scale[c], e[c]);
+        load8s<T>(f.e[c], p, f.scale[c], e[c]);
     }
     uint32_t id[8];
     load_ids8(a, p, id);

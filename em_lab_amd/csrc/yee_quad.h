@@ -9,19 +9,40 @@
 
 namespace emfd {
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
 __device__ __forceinline__ void st4(float *p, float a, float b, float c, float d) {
     *reinterpret_cast<float4 *>(p) = make_float4(a, b, c, d);
+}
+// Streaming forms (non-temporal hint): for the plane a pass reads or writes exactly once.
+__device__ __forceinline__ float4 ld4s(const float *p) {
+    const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st4s(float *p, float a, float b, float c, float d) {
+    const f32x4 v = {a, b, c, d};
+    __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(p));
 }
 __device__ __forceinline__ void unpack(const float4 v, float (&o)[4]) {
     o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
 }
 
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+template <bool STREAM = false>
 __device__ __forceinline__ void load_ids4(const LatticeArgs &a, size_t p, uint32_t (&id)[4]) {
     if (!a.mat_id) {
         id[0] = id[1] = id[2] = id[3] = 0u;
     } else if (a.id_bytes == 2) {
-        const uint2 v = *reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(a.mat_id) + p);
+        uint2 v;
+        if (STREAM) {
+            const u32x2 w = __builtin_nontemporal_load(
+                reinterpret_cast<const u32x2 *>(static_cast<const uint16_t *>(a.mat_id) + p));
+            v = make_uint2(w.x, w.y);
+        } else {
+            v = *reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(a.mat_id) + p);
+        }
         id[0] = v.x & 0xffffu; id[1] = v.x >> 16; id[2] = v.y & 0xffffu; id[3] = v.y >> 16;
     } else {
         const uint4 v = *reinterpret_cast<const uint4 *>(static_cast<const uint32_t *>(a.mat_id) + p);
@@ -29,7 +50,11 @@ __device__ __forceinline__ void load_ids4(const LatticeArgs &a, size_t p, uint32
     }
 }
 
-// update_B for the four cells x0..x0+3 of row (y, z) (shaders/update_B.comp:31-70)
+// update_B for the four cells x0..x0+3 of row (y, z) (shaders/update_B.comp:31-70). ST: 0 plain
+// memory operations, 1 streaming stores, 2 streaming stores and streaming loads of the B plane
+// (read once per pass), 3 as 2 plus streaming loads of the class ids; the arithmetic is the same
+// for all.
+template <int ST = 0>
 __device__ __forceinline__ void b_quad(const LatticeArgs &a, const FieldPtrs &f, int x0, int y, int z) {
     const size_t sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
     const size_t p = z * sz + y * sy + x0;
@@ -42,10 +67,10 @@ __device__ __forceinline__ void b_quad(const LatticeArgs &a, const FieldPtrs &f,
         eX[c] = up_x ? e[p + 4] : 0.0f;
         if (up_y) unpack(ld4(e + p + sy), eY[c]); else eY[c][0] = eY[c][1] = eY[c][2] = eY[c][3] = 0.0f;
         if (up_z) unpack(ld4(e + p + sz), eZ[c]); else eZ[c][0] = eZ[c][1] = eZ[c][2] = eZ[c][3] = 0.0f;
-        unpack(ld4(f.b[c] + p), b[c]);
+        unpack(ST >= 2 ? ld4s(f.b[c] + p) : ld4(f.b[c] + p), b[c]);
     }
     uint32_t id[4];
-    load_ids4(a, p, id);
+    load_ids4<(ST >= 3)>(a, p, id);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const float c0[3] = {e0[0][j], e0[1][j], e0[2][j]};
@@ -59,10 +84,15 @@ __device__ __forceinline__ void b_quad(const LatticeArgs &a, const FieldPtrs &f,
         b[2][j] = bb[2];
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) st4(f.b[c] + p, b[c][0], b[c][1], b[c][2], b[c][3]);
+    for (int c = 0; c < 3; ++c) {
+        if (ST >= 1) st4s(f.b[c] + p, b[c][0], b[c][1], b[c][2], b[c][3]);
+        else st4(f.b[c] + p, b[c][0], b[c][1], b[c][2], b[c][3]);
+    }
 }
 
-// update_E for the four cells x0..x0+3 of row (y, z) (shaders/update_E.comp:31-66)
+// update_E for the four cells x0..x0+3 of row (y, z) (shaders/update_E.comp:31-66); ST as b_quad
+// with the E plane as the once-read plane.
+template <int ST = 0>
 __device__ __forceinline__ void e_quad(const LatticeArgs &a, const FieldPtrs &f, int x0, int y, int z) {
     const size_t sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
     const size_t p = z * sz + y * sy + x0;
@@ -75,10 +105,10 @@ __device__ __forceinline__ void e_quad(const LatticeArgs &a, const FieldPtrs &f,
         bX[c] = dn_x ? bp[p - 1] : 0.0f;
         if (dn_y) unpack(ld4(bp + p - sy), bY[c]); else bY[c][0] = bY[c][1] = bY[c][2] = bY[c][3] = 0.0f;
         if (dn_z) unpack(ld4(bp + p - sz), bZ[c]); else bZ[c][0] = bZ[c][1] = bZ[c][2] = bZ[c][3] = 0.0f;
-        unpack(ld4(f.e[c] + p), e[c]);
+        unpack(ST >= 2 ? ld4s(f.e[c] + p) : ld4(f.e[c] + p), e[c]);
     }
     uint32_t id[4];
-    load_ids4(a, p, id);
+    load_ids4<(ST >= 3)>(a, p, id);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const float bxm[3] = {j > 0 ? b0[0][j - 1] : bX[0], j > 0 ? b0[1][j - 1] : bX[1], j > 0 ? b0[2][j - 1] : bX[2]};
@@ -90,7 +120,10 @@ __device__ __forceinline__ void e_quad(const LatticeArgs &a, const FieldPtrs &f,
         for (int c = 0; c < 3; ++c) e[c][j] = ca[c] * e[c][j] + cb[c] * curl[c];
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) st4(f.e[c] + p, e[c][0], e[c][1], e[c][2], e[c][3]);
+    for (int c = 0; c < 3; ++c) {
+        if (ST >= 1) st4s(f.e[c] + p, e[c][0], e[c][1], e[c][2], e[c][3]);
+        else st4(f.e[c] + p, e[c][0], e[c][1], e[c][2], e[c][3]);
+    }
 }
 
 }  // namespace emfd

@@ -6,30 +6,44 @@
 namespace emfd {
 namespace {
 
+template <int ST>
 __global__ void __launch_bounds__(256) k_update_b4(LatticeArgs a, FieldPtrs f, int z_base) {
     const int x0 = (blockIdx.x * 64 + threadIdx.x) * 4, y = blockIdx.y * 4 + threadIdx.y, z = z_base + blockIdx.z;
     if (x0 >= a.nx || y >= a.ny) return;
-    b_quad(a, f, x0, y, z);
+    b_quad<ST>(a, f, x0, y, z);
 }
 
+template <int ST>
 __global__ void __launch_bounds__(256) k_update_e4(LatticeArgs a, FieldPtrs f, int z_base) {
     const int x0 = (blockIdx.x * 64 + threadIdx.x) * 4, y = blockIdx.y * 4 + threadIdx.y, z = z_base + blockIdx.z;
     if (x0 >= a.nx || y >= a.ny) return;
-    e_quad(a, f, x0, y, z);
+    e_quad<ST>(a, f, x0, y, z);
+}
+
+template <int ST>
+hipError_t launch_st(const LatticeArgs &a, const FieldPtrs &f, int z0, int nzs, hipStream_t s) {
+    dim3 block(64, 4, 1), grid((a.nx / 4 + 63) / 64, (a.ny + 3) / 4, nzs);
+    k_update_b4<ST><<<grid, block, 0, s>>>(a, f, z0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_update_e4<ST><<<grid, block, 0, s>>>(a, f, z0);
+    return hipGetLastError();
+}
+
+hipError_t launch_range(const LatticeArgs &a, const FieldPtrs &f, int z0, int nzs, int streaming, hipStream_t s) {
+    if (streaming == 1) return launch_st<1>(a, f, z0, nzs, s);
+    if (streaming == 2) return launch_st<2>(a, f, z0, nzs, s);
+    if (streaming == 3) return launch_st<3>(a, f, z0, nzs, s);
+    return launch_st<0>(a, f, z0, nzs, s);
 }
 
 }  // namespace
 
 int yee_two_pass4_supported(const LatticeArgs &a) { return a.nx % 4 == 0 ? 1 : 0; }
 
-hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s) {
+hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, int streaming, hipStream_t s) {
     if (!yee_two_pass4_supported(a)) return hipErrorInvalidValue;
-    dim3 block(64, 4, 1), grid((a.nx / 4 + 63) / 64, (a.ny + 3) / 4, a.nz);
-    k_update_b4<<<grid, block, 0, s>>>(a, f, 0);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    k_update_e4<<<grid, block, 0, s>>>(a, f, 0);
-    return hipGetLastError();
+    return launch_range(a, f, 0, a.nz, streaming, s);
 }
 
 // Slab-interleaved order: B then E on z-slab 0, then on slab 1, ... A valid leapfrog schedule:
@@ -42,12 +56,7 @@ hipError_t launch_yee_two_pass4_slabs(const LatticeArgs &a, const FieldPtrs &f, 
     if (slab < 1) slab = 1;
     for (int z0 = 0; z0 < a.nz; z0 += slab) {
         const int nzs = a.nz - z0 < slab ? a.nz - z0 : slab;
-        dim3 block(64, 4, 1), grid((a.nx / 4 + 63) / 64, (a.ny + 3) / 4, nzs);
-        k_update_b4<<<grid, block, 0, s>>>(a, f, z0);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        k_update_e4<<<grid, block, 0, s>>>(a, f, z0);
-        e = hipGetLastError();
+        const hipError_t e = launch_range(a, f, z0, nzs, 0, s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
