@@ -56,6 +56,8 @@ struct NtffState {
     float frame[9] = {0, 0, 1, 1, 0, 0, 0, 1, 0};
     double start_time = 0.0;
     DevBuf d_phasors;  // SoA: 24 planes of n_points floats
+    DevBuf d_side;     // x-face side columns (SideCols layout)
+    SideCols side{};   // geometry of d_side (buf set when the Yee kernel maintains it)
     DevBuf d_dirs, d_partials, d_out, d_power;
 };
 
@@ -73,6 +75,7 @@ struct emfd_ctx {
     int cur = 0;
     emfd::DevBuf d_src_descs;  // kept sources (FP64 stepper evaluates their signals on the device)
     int storage = emfd::kStorageF32;  // field storage (EMFD_STORAGE_*)
+    bool side_active = false;         // the current step's Yee kernel writes the far-field side columns
     int scale_log2[6] = {0, 0, 0, 0, 0, 0};
     size_t field_bytes() const {
         return n * 6 * (storage == emfd::kStorageF32 ? 4 : storage == emfd::kStorageF64 ? 8 : 2);

@@ -302,6 +302,10 @@ emfd::LatticeArgs emfd_ctx::lattice() const {
     a.has_wire = has_wire;
     a.pml_tab = pml_tab.as<float>();
     a.pml_k = 2 * cfg.pml_thickness + 2;
+    if (side_active && ntff.enabled && ntff.d_side.ptr) {
+        a.side = ntff.side;
+        a.side.buf = ntff.d_side.as<float>();
+    }
     return a;
 }
 
@@ -340,7 +344,7 @@ void emfd_destroy(emfd_ctx *ctx) {
     for (auto &b : ctx->field_buf) b.release();
     DevBuf *bufs[] = {&ctx->d_src_descs, &ctx->sigma_cell, &ctx->eps_cell, &ctx->mat_id, &ctx->table, &ctx->prof, &ctx->pml_tab, &ctx->d_packed,
                       &ctx->d_cells, &ctx->d_lists, &ctx->d_consts, &ctx->ports.d_ports, &ctx->ports.d_phasors,
-                      &ctx->ports.d_series, &ctx->ntff.d_phasors, &ctx->ntff.d_dirs, &ctx->ntff.d_partials,
+                      &ctx->ports.d_series, &ctx->ntff.d_phasors, &ctx->ntff.d_side, &ctx->ntff.d_dirs, &ctx->ntff.d_partials,
                       &ctx->ntff.d_out, &ctx->ntff.d_power, &ctx->scratch_partials, &ctx->scratch_misc};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t ev : ctx->ev_pool) (void)hipEventDestroy(ev);
@@ -609,6 +613,13 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
     EMFD_HIP(hipMemcpyAsync(ctx->d_consts.ptr, hc, nfl * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     EMFD_HIP(hipEventRecord(ctx->h_consts_ev[hb], ctx->stream));
 
+    {
+        // the vectorised FP32 two-pass kernels copy the far-field x-face columns as they go
+        const int v = ctx->kernel_variant;
+        const bool tp4 = yee_two_pass4_supported(ctx->lattice()) &&
+                         (v == EMFD_KERNEL_AUTO || v == EMFD_KERNEL_TWO_PASS4 || v == EMFD_KERNEL_SLABS);
+        ctx->side_active = ctx->ntff.enabled && ctx->storage == kStorageF32 && tp4;
+    }
     const LatticeArgs lat = ctx->lattice();
     for (int k = 0; k < count; ++k) {
         const float *consts = ctx->d_consts.as<float>() + static_cast<size_t>(k) * stride;

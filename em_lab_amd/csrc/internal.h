@@ -23,6 +23,17 @@ enum TableField : int {
     kWire = 12,  // 6: thin-wire factors (Bx t0, Bx t1, By t0, By t1, Bz t0, Bz t1)
 };
 
+// Side copies of the lattice columns x = X - 1 and x = X for the two far-field box faces normal to
+// x (X = box min / max x): the Yee passes (and the source pass) write each value there as they
+// store it, so the far-field accumulation reads those faces coalesced instead of gathering one or
+// two floats per cache line. Layout [face][component 0..5 (E, B)][x - (X - 1)][z - z0][y - y0].
+struct SideCols {
+    float *buf;   // null: not maintained
+    int x[2];     // X of face 0 and face 1
+    int y0, ny;   // y range (box min y - 1 .. box max y)
+    int z0, nz;   // z range
+};
+
 // Per-grid constants every lattice kernel needs.
 struct LatticeArgs {
     int nx, ny, nz;
@@ -37,7 +48,15 @@ struct LatticeArgs {
     int has_wire;               // scene has thin wires
     const float *pml_tab;       // absorber coefficients per (kx, ky, kz) depth class, kPmlStride floats
     int pml_k;                  // depth classes per axis: 2 * pml + 2
+    SideCols side;              // far-field x-face columns (buf null when not maintained)
 };
+
+// Index of (face, comp, x, y, z) in the side columns, or -1 when (x, y, z) is not a side column.
+__host__ __device__ inline long side_index(const SideCols &s, int face, int comp, int x, int y, int z) {
+    const int xo = x - (s.x[face] - 1);
+    if (xo < 0 || xo > 1 || y < s.y0 || y >= s.y0 + s.ny || z < s.z0 || z >= s.z0 + s.nz) return -1;
+    return ((static_cast<long>(face * 6 + comp) * 2 + xo) * s.nz + (z - s.z0)) * s.ny + (y - s.y0);
+}
 
 // Absorber coefficient table entry (vacuum edges): B: exp(-s dt), dt phi(s dt) per component;
 // E: Ca, Cb per component for sigma = sigma_pml, eps_r = 1. The depth class of lattice index i

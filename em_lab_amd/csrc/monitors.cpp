@@ -426,6 +426,15 @@ int emfd_ntff_configure(emfd_ctx *ctx, const emfd_far_field_setup *setup, emfd_f
     }
     b.n_points = b.face_offset[6];
     if (int rc = N.d_phasors.ensure(static_cast<size_t>(b.n_points) * 24 * sizeof(float))) return rc;
+    // side columns x = X-1, X of the two x faces over y, z in [box min - 1, box max]
+    N.side = SideCols{};
+    N.side.x[0] = b.bmin[0];
+    N.side.x[1] = b.bmax[0];
+    N.side.y0 = b.bmin[1] - 1;
+    N.side.ny = b.bmax[1] - b.bmin[1] + 2;
+    N.side.z0 = b.bmin[2] - 1;
+    N.side.nz = b.bmax[2] - b.bmin[2] + 2;
+    if (int rc = N.d_side.ensure(static_cast<size_t>(2 * 6 * 2) * N.side.ny * N.side.nz * sizeof(float))) return rc;
     EMFD_HIP(hipMemset(N.d_phasors.ptr, 0, static_cast<size_t>(b.n_points) * 24 * sizeof(float)));
     N.box = b;
     N.setup = s;

@@ -50,8 +50,16 @@ __global__ void __launch_bounds__(256) k_ntff_accumulate(LatticeArgs a, FieldPtr
     float n[3], w;
     ntff_point(box, i, node, n, &w);
     float E[3], H[3];
-    e_at_node(a, f, node[0], node[1], node[2], E);
-    b_at_node(a, f, node[0], node[1], node[2], H);
+    if (a.side.buf && n[0] != 0.0f) {
+        // faces normal to x: the same values from the coalesced side columns
+        const int face = n[0] > 0.0f ? 1 : 0;
+        const auto side = [&](int c, int x, int y, int z) { return a.side.buf[side_index(a.side, face, c, x, y, z)]; };
+        e_at_node_with(side, node[0], node[1], node[2], E);
+        b_at_node_with(side, node[0], node[1], node[2], H);
+    } else {
+        e_at_node(a, f, node[0], node[1], node[2], E);
+        b_at_node(a, f, node[0], node[1], node[2], H);
+    }
     const size_t np = static_cast<size_t>(box.n_points);
     const float co[4] = {2.0f * rot[0], -2.0f * rot[1], 2.0f * rot[2], -2.0f * rot[3]};
 #pragma unroll

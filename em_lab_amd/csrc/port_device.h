@@ -37,22 +37,27 @@ __device__ __forceinline__ float fetch_c(const LatticeArgs &a, const FieldPtrs &
     return field_load(f, comp, (static_cast<size_t>(z) * a.ny + y) * a.nx + x);
 }
 
-// eAtNode / bAtNode (shaders/field_sampling.glsl:17-32)
-__device__ __forceinline__ void e_at_node(const LatticeArgs &a, const FieldPtrs &f, int x, int y, int z, float o[3]) {
-    o[0] = 0.5f * (fetch_c(a, f, 0, x, y, z) + fetch_c(a, f, 0, x - 1, y, z));
-    o[1] = 0.5f * (fetch_c(a, f, 1, x, y, z) + fetch_c(a, f, 1, x, y - 1, z));
-    o[2] = 0.5f * (fetch_c(a, f, 2, x, y, z) + fetch_c(a, f, 2, x, y, z - 1));
+// eAtNode / bAtNode (shaders/field_sampling.glsl:17-32) over any fetch(comp, x, y, z)
+template <class Fetch>
+__device__ __forceinline__ void e_at_node_with(const Fetch &fetch, int x, int y, int z, float o[3]) {
+    o[0] = 0.5f * (fetch(0, x, y, z) + fetch(0, x - 1, y, z));
+    o[1] = 0.5f * (fetch(1, x, y, z) + fetch(1, x, y - 1, z));
+    o[2] = 0.5f * (fetch(2, x, y, z) + fetch(2, x, y, z - 1));
 }
-__device__ __forceinline__ void b_at_node(const LatticeArgs &a, const FieldPtrs &f, int x, int y, int z, float o[3]) {
-    const float bx = fetch_c(a, f, 3 + 0, x, y, z) + fetch_c(a, f, 3 + 0, x, y - 1, z) + fetch_c(a, f, 3 + 0, x, y, z - 1) +
-                     fetch_c(a, f, 3 + 0, x, y - 1, z - 1);
-    const float by = fetch_c(a, f, 3 + 1, x, y, z) + fetch_c(a, f, 3 + 1, x - 1, y, z) + fetch_c(a, f, 3 + 1, x, y, z - 1) +
-                     fetch_c(a, f, 3 + 1, x - 1, y, z - 1);
-    const float bz = fetch_c(a, f, 3 + 2, x, y, z) + fetch_c(a, f, 3 + 2, x - 1, y, z) + fetch_c(a, f, 3 + 2, x, y - 1, z) +
-                     fetch_c(a, f, 3 + 2, x - 1, y - 1, z);
+template <class Fetch>
+__device__ __forceinline__ void b_at_node_with(const Fetch &fetch, int x, int y, int z, float o[3]) {
+    const float bx = fetch(3, x, y, z) + fetch(3, x, y - 1, z) + fetch(3, x, y, z - 1) + fetch(3, x, y - 1, z - 1);
+    const float by = fetch(4, x, y, z) + fetch(4, x - 1, y, z) + fetch(4, x, y, z - 1) + fetch(4, x - 1, y, z - 1);
+    const float bz = fetch(5, x, y, z) + fetch(5, x - 1, y, z) + fetch(5, x, y - 1, z) + fetch(5, x - 1, y - 1, z);
     o[0] = 0.25f * bx;
     o[1] = 0.25f * by;
     o[2] = 0.25f * bz;
+}
+__device__ __forceinline__ void e_at_node(const LatticeArgs &a, const FieldPtrs &f, int x, int y, int z, float o[3]) {
+    e_at_node_with([&](int c, int xx, int yy, int zz) { return fetch_c(a, f, c, xx, yy, zz); }, x, y, z, o);
+}
+__device__ __forceinline__ void b_at_node(const LatticeArgs &a, const FieldPtrs &f, int x, int y, int z, float o[3]) {
+    b_at_node_with([&](int c, int xx, int yy, int zz) { return fetch_c(a, f, c, xx, yy, zz); }, x, y, z, o);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -42,7 +42,15 @@ __device__ __forceinline__ void source_cell(const LatticeArgs &a, const FieldPtr
     float ca[3], cb[3];
     edge_coefficients(a, x, y, z, cid, ca, cb);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) field_store(f, c, p, field_load(f, c, p) - cb[c] * J[c]);
+    for (int c = 0; c < 3; ++c) {
+        const float v = field_load(f, c, p) - cb[c] * J[c];
+        field_store(f, c, p, v);
+        if (a.side.buf)
+            for (int face = 0; face < 2; ++face) {
+                const long k = side_index(a.side, face, c, x, y, z);
+                if (k >= 0) a.side.buf[k] = v;
+            }
+    }
 }
 
 }  // namespace emfd

@@ -30,6 +30,25 @@ __device__ __forceinline__ void unpack(const float4 v, float (&o)[4]) {
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
+// Copies the four cells' values of components comp0..comp0+2 into the far-field side columns when
+// the quad holds one of them.
+__device__ __forceinline__ void side_write4(const LatticeArgs &a, int comp0, int x0, int y, int z,
+                                            const float (&v)[3][4]) {
+    const SideCols &s = a.side;
+    if (!s.buf || y < s.y0 || y >= s.y0 + s.ny || z < s.z0 || z >= s.z0 + s.nz) return;
+    if (x0 + 3 < s.x[0] - 1 || (x0 > s.x[0] && x0 + 3 < s.x[1] - 1) || x0 > s.x[1]) return;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int face = 0; face < 2; ++face) {
+            const long k = side_index(s, face, comp0, x0 + j, y, z);
+            if (k < 0) continue;
+            const long plane = 2L * s.nz * s.ny;  // component stride
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s.buf[k + c * plane] = v[c][j];
+        }
+}
+
 template <bool STREAM = false>
 __device__ __forceinline__ void load_ids4(const LatticeArgs &a, size_t p, uint32_t (&id)[4]) {
     if (!a.mat_id) {
@@ -88,6 +107,7 @@ __device__ __forceinline__ void b_quad(const LatticeArgs &a, const FieldPtrs &f,
         if (ST >= 1) st4s(f.b[c] + p, b[c][0], b[c][1], b[c][2], b[c][3]);
         else st4(f.b[c] + p, b[c][0], b[c][1], b[c][2], b[c][3]);
     }
+    side_write4(a, 3, x0, y, z, b);
 }
 
 // update_E for the four cells x0..x0+3 of row (y, z) (shaders/update_E.comp:31-66); ST as b_quad
@@ -124,6 +144,7 @@ __device__ __forceinline__ void e_quad(const LatticeArgs &a, const FieldPtrs &f,
         if (ST >= 1) st4s(f.e[c] + p, e[c][0], e[c][1], e[c][2], e[c][3]);
         else st4(f.e[c] + p, e[c][0], e[c][1], e[c][2], e[c][3]);
     }
+    side_write4(a, 0, x0, y, z, e);
 }
 
 }  // namespace emfd
