@@ -540,7 +540,7 @@ int emfd_grid_set_storage(emfd_ctx *ctx, int storage, const int *scale_log2) {
 int emfd_grid_set_kernel(emfd_ctx *ctx, int variant) {
     if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
     const int v = variant & 0xff;
-    if (v < 0 || v > 6) return fail(EMFD_ERR_INVALID, "unknown kernel variant");
+    if (v < 0 || v > 7) return fail(EMFD_ERR_INVALID, "unknown kernel variant");
     ctx->kernel_variant = v;
     ctx->tile_cfg = variant >> 8;  // optional tile shape / z-chunk selector for the fused kernel
     return EMFD_OK;
@@ -650,6 +650,8 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
             variant = yee_two_pass4_supported(lat) ? EMFD_KERNEL_TWO_PASS4 : EMFD_KERNEL_TWO_PASS;
         if (variant == EMFD_KERNEL_TWO_PASS4 && !yee_two_pass4_supported(lat))
             return fail(EMFD_ERR_INVALID, "the vectorised two-pass kernel needs nx % 4 == 0");
+        if (variant == EMFD_KERNEL_ZMARCH && !yee_zmarch_supported(lat))
+            return fail(EMFD_ERR_INVALID, "the z-march kernel supports nx <= 256 with nx % 4 == 0");
         if (variant == EMFD_KERNEL_ROWS && !yee_rows_supported(lat))
             return fail(EMFD_ERR_INVALID, "the row-wavefront kernel supports nx <= 256");
         if (variant == -1) {
@@ -667,6 +669,10 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
                                                 ctx->stream));
         } else if (variant == EMFD_KERNEL_POINT) {
             EMFD_HIP(launch_yee_point(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->stream));
+            ctx->cur ^= 1;
+        } else if (variant == EMFD_KERNEL_ZMARCH) {
+            EMFD_HIP(launch_yee_zmarch(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->tile_cfg,
+                                       ctx->stream));
             ctx->cur ^= 1;
         } else if (variant == EMFD_KERNEL_ROWS) {
             EMFD_HIP(launch_yee_rows(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->tile_cfg, ctx->stream));

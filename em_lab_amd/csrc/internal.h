@@ -92,6 +92,9 @@ hipError_t launch_yee_fused(const LatticeArgs &a, const FieldPtrs &src, const Fi
                             hipStream_t s);
 hipError_t launch_yee_rows(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int zchunk, hipStream_t s);
 int yee_rows_supported(const LatticeArgs &a);
+hipError_t launch_yee_zmarch(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int zchunk,
+                             hipStream_t s);
+int yee_zmarch_supported(const LatticeArgs &a);
 hipError_t launch_yee_point(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, hipStream_t s);
 hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
 hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, int streaming, hipStream_t s);

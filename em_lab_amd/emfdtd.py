@@ -28,6 +28,7 @@ KERNEL_ROWS = 3
 KERNEL_POINT = 4
 KERNEL_TWO_PASS4 = 5
 KERNEL_SLABS = 6
+KERNEL_ZMARCH = 7
 STORAGE_F32 = 0
 STORAGE_F16 = 1
 STORAGE_BF16 = 2

@@ -51,6 +51,8 @@ extern "C" {
                                    plane each pass reads once (AUTO uses 2) */
 #define EMFD_KERNEL_SLABS 6     /* TWO_PASS4 interleaved per z-slab (B, E on slab 0, then slab 1, ...); bits 8..
                                    of the variant = slab thickness in planes (default 32) */
+#define EMFD_KERNEL_ZMARCH 7    /* fused B+E, z-march of 4-row groups with LDS row exchange and a helper
+                                   wave (nx <= 256, nx % 4 == 0); bits 8.. = z-chunk (0 = automatic) */
 
 #define EMFD_STORAGE_F32 0  /* FP32 field planes (default; EM_Lab's RGBA32F) */
 #define EMFD_STORAGE_F16 1  /* binary16 planes of value / 2^k per component, FP32 updates */
