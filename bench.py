@@ -118,6 +118,9 @@ def main() -> int:
             pm.configure(scene.ports, scene.far_field.averaging_periods, scene.sweep, scene.cell_size)
         ff = E.FarFieldMonitor(g)
         ff.configure(scene.far_field)
+        # the workload's far field is evaluated at three wavelengths (lambda / 1.333, lambda, 1.5 lambda)
+        wl = scene.far_field.wavelength
+        ff.set_wavelengths([wl, wl * 0.75, wl * 1.5])
     nx, ny, nz = g.size
     cells = nx * ny * nz
 
@@ -137,6 +140,21 @@ def main() -> int:
         return 2
 
     value = cells * args.steps / wall / 1e6
+    pattern = None
+    if not args.no_monitors:
+        # far-field pattern of the workload: a 181 x 361 (theta, phi) grid at the three wavelengths
+        dirs = E.FarFieldMonitor.theta_phi_grid(181, 361)
+        ff.radiation(dirs[:256], 0)  # warm-up
+        t_p = time.perf_counter()
+        peaks = []
+        for f in range(3):
+            f2, P = ff.radiation(dirs, f)
+            peaks.append(float(f2.max()))
+        dt_p = time.perf_counter() - t_p
+        pairs = 3 * dirs.shape[0] * ff.n_points
+        pattern = {"directions": int(dirs.shape[0]), "points": int(ff.n_points), "wavelengths": 3,
+                   "seconds": round(dt_p, 4), "pairs_per_s": round(pairs / dt_p, 1),
+                   "flop_per_pair": "sincos + 12 FMA"}
     avg_ms = yee_ms / max(launches, 1)
     achieved = bytes_per_cell * cells / (avg_ms * 1e-3) / 1e9
     # HBM bytes per Yee step from the calibrated rocprofv3 counter runs of this workload
@@ -163,8 +181,8 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": (f"{nx}^3 Yee leapfrog, 10-cell absorber, 32 seeded dielectric spheres, z thin-wire "
-                                f"half-wave dipole (lambda 20) on a 50 ohm resistive feed, port + far-field "
-                                f"accumulation every step" if args.scene == "spheres" else
+                                f"half-wave dipole (lambda 20) on a 50 ohm resistive feed, port (32-point sweep) "
+                                f"+ far-field (3 wavelengths) accumulation every step" if args.scene == "spheres" else
                                 f"{nx}^3 Yee leapfrog, 10-cell absorber, 32 seeded dielectric spheres around a 4x4 "
                                 f"thin-wire dipole array on 50 ohm feeds, port + far-field accumulation every step"
                                 if args.scene == "array" else
@@ -180,6 +198,8 @@ def main() -> int:
                      "bytes_per_cell": bytes_per_cell},
         "cpu_baseline": None,
     }
+    if pattern is not None:
+        out["ntff_pattern"] = pattern
     g.close()
     if (not args.no_surrogate and args.size == 256 and args.scene == "spheres" and args.storage == "f32"
             and args.kernel == 0):

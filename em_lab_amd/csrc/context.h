@@ -57,6 +57,9 @@ struct NtffState {
     double start_time = 0.0;
     DevBuf d_phasors;  // SoA: 24 planes of n_points floats
     DevBuf d_side;     // x-face side columns (SideCols layout)
+    std::vector<float> wavelengths;  // accumulated wavelengths; [0] is setup.wavelength
+    int n_freq() const { return wavelengths.empty() ? 1 : static_cast<int>(wavelengths.size()); }
+    float wavelength(int f) const { return wavelengths.empty() ? setup.wavelength : wavelengths[static_cast<size_t>(f)]; }
     SideCols side{};   // geometry of d_side (buf set when the Yee kernel maintains it)
     DevBuf d_dirs, d_partials, d_out, d_power;
 };

@@ -314,8 +314,8 @@ extern "C" {
 const char *emfd_last_error(void) { return g_error.c_str(); }
 
 const char *emfd_version(void) {
-    return "emfdtd 0.1 gfx950: fused z-march Yee (LDS tiles, ping-pong), two-pass Yee, sources, ports, ntff, "
-           "flux, probes";
+    return "emfdtd 0.2 gfx950: streaming two-pass Yee (FP32 / binary16 / bfloat16 / FP64 storage), fused z-march "
+           "options, sources, ports, multi-wavelength far field, flux, probes, batched stepper, BF16 MFMA MLP";
 }
 
 int emfd_create(int device, emfd_ctx **out) {

@@ -127,8 +127,15 @@ struct NtffBox {
     int face_offset[7];
     int n_points;
 };
+// Far-field accumulation at up to kMaxNtffFreq wavelengths: phasor block f (24 planes of
+// n_points floats) at phasors + 24 f n_points, rotation factors at rot + 4 f, EMA weight blend[f].
+constexpr int kMaxNtffFreq = 8;
+struct NtffFreqs {
+    int n;
+    float blend[kMaxNtffFreq];
+};
 hipError_t launch_ntff_accumulate(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, const float *rot,
-                                  float blend, float *phasors, hipStream_t s);
+                                  const NtffFreqs &freqs, float *phasors, hipStream_t s);
 hipError_t launch_ntff_pattern(const NtffBox &box, const float *phasors, const float *dirs, int n_dirs, float k,
                                const float center[3], float *partials, int slices, float *out_f2,
                                float *power_partials, hipStream_t s);

@@ -191,13 +191,14 @@ float to_dbi(double d) { return d > 1e-12 ? static_cast<float>(10.0 * std::log10
 
 // ---- hooks used by emfd_grid_step ----
 int monitors_consts_floats(emfd_ctx *ctx) {
-    (void)ctx;
-    return 8;  // port rotation (4) + far-field rotation (4)
+    return 4 + 4 * ctx->ntff.n_freq();  // port rotation (4) + far-field rotation (4 per wavelength)
 }
 
 int monitors_fill_consts(emfd_ctx *ctx, float *row, uint64_t step_after) {
     if (ctx->ports.enabled()) rotation(ctx->ports.wavelength, ctx->cfg.courant, step_after, row);
-    if (ctx->ntff.enabled) rotation(ctx->ntff.setup.wavelength, ctx->cfg.courant, step_after, row + 4);
+    if (ctx->ntff.enabled)
+        for (int f = 0; f < ctx->ntff.n_freq(); ++f)
+            rotation(ctx->ntff.wavelength(f), ctx->cfg.courant, step_after, row + 4 + 4 * f);
     return EMFD_OK;
 }
 
@@ -215,9 +216,11 @@ int monitors_after_step(emfd_ctx *ctx, const float *consts, int64_t k) {
     }
     NtffState &N = ctx->ntff;
     if (N.enabled) {
-        EMFD_HIP(launch_ntff_accumulate(lat, f, N.box, consts + 4,
-                                        blend_of(N.setup.averaging_periods, N.setup.wavelength, ctx->cfg.courant),
-                                        N.d_phasors.as<float>(), ctx->stream));
+        NtffFreqs fr{};
+        fr.n = N.n_freq();
+        for (int q = 0; q < fr.n; ++q)
+            fr.blend[q] = blend_of(N.setup.averaging_periods, N.wavelength(q), ctx->cfg.courant);
+        EMFD_HIP(launch_ntff_accumulate(lat, f, N.box, consts + 4, fr, N.d_phasors.as<float>(), ctx->stream));
     }
     return EMFD_OK;
 }
@@ -425,6 +428,7 @@ int emfd_ntff_configure(emfd_ctx *ctx, const emfd_far_field_setup *setup, emfd_f
         b.bmax[a] = s.box_max[a];
     }
     b.n_points = b.face_offset[6];
+    N.wavelengths.assign(1, s.wavelength);  // configure resets the list to the setup's wavelength
     if (int rc = N.d_phasors.ensure(static_cast<size_t>(b.n_points) * 24 * sizeof(float))) return rc;
     // side columns x = X-1, X of the two x faces over y, z in [box min - 1, box max]
     N.side = SideCols{};
@@ -455,19 +459,43 @@ int emfd_ntff_disable(emfd_ctx *ctx) {
 int emfd_ntff_reset(emfd_ctx *ctx) {
     if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
     if (!ctx->ntff.enabled) return EMFD_OK;
-    EMFD_HIP(launch_fill(ctx->ntff.d_phasors.as<float>(), static_cast<size_t>(ctx->ntff.box.n_points) * 24, 0.0f,
-                         ctx->stream));
+    EMFD_HIP(launch_fill(ctx->ntff.d_phasors.as<float>(),
+                         static_cast<size_t>(ctx->ntff.box.n_points) * 24 * ctx->ntff.n_freq(), 0.0f, ctx->stream));
+    return EMFD_OK;
+}
+
+int emfd_ntff_set_wavelengths(emfd_ctx *ctx, const float *wavelengths, int n) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    NtffState &N = ctx->ntff;
+    if (!N.enabled) return fail(EMFD_ERR_STATE, "far-field monitor not configured");
+    if (n < 1 || n > kMaxNtffFreq || !wavelengths) return fail(EMFD_ERR_INVALID, "1 to 8 wavelengths");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    N.wavelengths.clear();
+    for (int f = 0; f < n; ++f) N.wavelengths.push_back(std::max(wavelengths[f], 2.0f));  // as configure
+    N.setup.wavelength = N.wavelengths[0];
+    const size_t floats = static_cast<size_t>(N.box.n_points) * 24 * n;
+    if (int rc = N.d_phasors.ensure(floats * sizeof(float))) return rc;
+    EMFD_HIP(launch_fill(N.d_phasors.as<float>(), floats, 0.0f, ctx->stream));
+    N.start_time = ctx->time();
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
     return EMFD_OK;
 }
 
 int emfd_ntff_read_phasors(emfd_ctx *ctx, float *out, int64_t n_points) {
+    return emfd_ntff_read_phasors_at(ctx, 0, out, n_points);
+}
+
+int emfd_ntff_read_phasors_at(emfd_ctx *ctx, int freq, float *out, int64_t n_points) {
     if (!ctx || !out) return fail(EMFD_ERR_INVALID, "null argument");
     if (!ctx->ntff.enabled) return fail(EMFD_ERR_STATE, "far-field monitor not configured");
+    if (freq < 0 || freq >= ctx->ntff.n_freq()) return fail(EMFD_ERR_INVALID, "no such far-field wavelength");
     const size_t np = static_cast<size_t>(ctx->ntff.box.n_points);
     if (n_points < static_cast<int64_t>(np)) return fail(EMFD_ERR_INVALID, "output too small");
     EMFD_HIP(hipStreamSynchronize(ctx->stream));
     std::vector<float> soa(np * 24);
-    EMFD_HIP(hipMemcpy(soa.data(), ctx->ntff.d_phasors.ptr, soa.size() * sizeof(float), hipMemcpyDeviceToHost));
+    EMFD_HIP(hipMemcpy(soa.data(), ctx->ntff.d_phasors.as<float>() + np * 24 * static_cast<size_t>(freq),
+                       soa.size() * sizeof(float), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < np; ++i) {
         float *o = out + 32 * i;
         for (int s = 0; s < 2; ++s)
@@ -498,9 +526,14 @@ int emfd_ntff_directions(emfd_ctx *ctx, float *dirs, int cap, int *n_dirs, int *
 }
 
 int emfd_ntff_eval(emfd_ctx *ctx, const float *dirs, int n_dirs, float *f2, double *power) {
+    return emfd_ntff_eval_at(ctx, 0, dirs, n_dirs, f2, power);
+}
+
+int emfd_ntff_eval_at(emfd_ctx *ctx, int freq, const float *dirs, int n_dirs, float *f2, double *power) {
     if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
     NtffState &N = ctx->ntff;
     if (!N.enabled) return fail(EMFD_ERR_STATE, "far-field monitor not configured");
+    if (freq < 0 || freq >= N.n_freq()) return fail(EMFD_ERR_INVALID, "no such far-field wavelength");
     if (n_dirs < 0 || (n_dirs > 0 && (!dirs || !f2))) return fail(EMFD_ERR_INVALID, "bad direction arguments");
     EMFD_HIP(hipSetDevice(ctx->device));
     const int slices = ntff_pattern_slices(N.box.n_points);
@@ -513,11 +546,12 @@ int emfd_ntff_eval(emfd_ctx *ctx, const float *dirs, int n_dirs, float *f2, doub
     if (n_dirs > 0)
         EMFD_HIP(hipMemcpyAsync(N.d_dirs.ptr, dirs, static_cast<size_t>(n_dirs) * 4 * sizeof(float),
                                 hipMemcpyHostToDevice, ctx->stream));
-    const float k = static_cast<float>(kTwoPi / N.setup.wavelength);  // src/FarField.cpp:293
+    const float k = static_cast<float>(kTwoPi / N.wavelength(freq));  // src/FarField.cpp:293
     const float center[3] = {0.5f * static_cast<float>(N.box.bmin[0] + N.box.bmax[0]),
                              0.5f * static_cast<float>(N.box.bmin[1] + N.box.bmax[1]),
                              0.5f * static_cast<float>(N.box.bmin[2] + N.box.bmax[2])};
-    EMFD_HIP(launch_ntff_pattern(N.box, N.d_phasors.as<float>(), N.d_dirs.as<float>(), n_dirs, k, center,
+    const float *ph = N.d_phasors.as<float>() + static_cast<size_t>(N.box.n_points) * 24 * static_cast<size_t>(freq);
+    EMFD_HIP(launch_ntff_pattern(N.box, ph, N.d_dirs.as<float>(), n_dirs, k, center,
                                  N.d_partials.as<float>(), slices, N.d_out.as<float>(),
                                  power ? N.d_power.as<float>() : nullptr, ctx->stream));
     EMFD_HIP(hipStreamSynchronize(ctx->stream));

@@ -43,7 +43,7 @@ __device__ __forceinline__ void ntff_point(const NtffBox &b, int index, int node
 // ntff_accumulate.comp (shaders/ntff_accumulate.comp:30-52). Phasors are stored as 24 planes of
 // n_points floats: stage s (0,1) x quantity q (eRe, eIm, hRe, hIm) x component c -> plane 12 s + 3 q + c.
 __global__ void __launch_bounds__(256) k_ntff_accumulate(LatticeArgs a, FieldPtrs f, NtffBox box, const float *rot,
-                                                         float blend, float *ph) {
+                                                         NtffFreqs freqs, float *ph) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= box.n_points) return;
     int node[3];
@@ -61,19 +61,24 @@ __global__ void __launch_bounds__(256) k_ntff_accumulate(LatticeArgs a, FieldPtr
         b_at_node(a, f, node[0], node[1], node[2], H);
     }
     const size_t np = static_cast<size_t>(box.n_points);
-    const float co[4] = {2.0f * rot[0], -2.0f * rot[1], 2.0f * rot[2], -2.0f * rot[3]};
+    for (int fq = 0; fq < freqs.n; ++fq) {
+        const float *r = rot + 4 * fq;
+        const float blend = freqs.blend[fq];
+        float *phf = ph + static_cast<size_t>(24 * fq) * np;
+        const float co[4] = {2.0f * r[0], -2.0f * r[1], 2.0f * r[2], -2.0f * r[3]};
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float obs = co[q] * (q < 2 ? E[c] : H[c]);
-            float *p1 = ph + (3 * q + c) * np + i, *p2 = ph + (12 + 3 * q + c) * np + i;
-            float s1 = *p1, s2 = *p2;
-            s1 += blend * (obs - s1);
-            s2 += blend * (s1 - s2);
-            *p1 = s1;
-            *p2 = s2;
-        }
+            for (int c = 0; c < 3; ++c) {
+                const float obs = co[q] * (q < 2 ? E[c] : H[c]);
+                float *p1 = phf + (3 * q + c) * np + i, *p2 = phf + (12 + 3 * q + c) * np + i;
+                float s1 = *p1, s2 = *p2;
+                s1 += blend * (obs - s1);
+                s2 += blend * (s1 - s2);
+                *p1 = s1;
+                *p2 = s2;
+            }
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -282,9 +287,9 @@ hipError_t launch_port_accumulate(const LatticeArgs &a, const FieldPtrs &f, cons
 }
 
 hipError_t launch_ntff_accumulate(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, const float *rot,
-                                  float blend, float *phasors, hipStream_t s) {
+                                  const NtffFreqs &freqs, float *phasors, hipStream_t s) {
     if (box.n_points <= 0) return hipSuccess;
-    k_ntff_accumulate<<<(box.n_points + 255) / 256, 256, 0, s>>>(a, f, box, rot, blend, phasors);
+    k_ntff_accumulate<<<(box.n_points + 255) / 256, 256, 0, s>>>(a, f, box, rot, freqs, phasors);
     return hipGetLastError();
 }
 

@@ -149,6 +149,9 @@ _SIGNATURES = {
     "emfd_ntff_read_phasors": (C.c_int, [_P, _P, C.c_int64]),
     "emfd_ntff_directions": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_int), _P, _P]),
     "emfd_ntff_eval": (C.c_int, [_P, _P, C.c_int, _P, C.POINTER(C.c_double)]),
+    "emfd_ntff_set_wavelengths": (C.c_int, [_P, _P, C.c_int]),
+    "emfd_ntff_eval_at": (C.c_int, [_P, C.c_int, _P, C.c_int, _P, C.POINTER(C.c_double)]),
+    "emfd_ntff_read_phasors_at": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
     "emfd_ntff_evaluate": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_float), _P, _P,
                                      _P]),
     "emfd_flux_measure": (C.c_int, [_P, _P, C.c_int, _P]),
@@ -608,10 +611,18 @@ class FarFieldMonitor:
     def reset_accumulation(self) -> None:
         _check(self._lib.emfd_ntff_reset(self.grid.handle))
 
-    def phasors(self) -> np.ndarray:
+    def phasors(self, freq: int = 0) -> np.ndarray:
         out = np.zeros((self.n_points, 32), np.float32)
-        _check(self._lib.emfd_ntff_read_phasors(self.grid.handle, _ptr(out), self.n_points))
+        _check(self._lib.emfd_ntff_read_phasors_at(self.grid.handle, int(freq), _ptr(out), self.n_points))
         return out
+
+    def set_wavelengths(self, wavelengths) -> None:
+        """Accumulate at several wavelengths (the first becomes the monitor's wavelength)."""
+        w = np.ascontiguousarray(wavelengths, np.float32)
+        _check(self._lib.emfd_ntff_set_wavelengths(self.grid.handle, _ptr(w), w.size))
+        self.wavelengths = [float(max(v, 2.0)) for v in w]
+        if self.setup is not None:
+            self.setup.wavelength = self.wavelengths[0]
 
     def directions(self):
         n = C.c_int()
@@ -623,15 +634,25 @@ class FarFieldMonitor:
                                               _ptr(frame)))
         return dirs, mi, frame
 
-    def radiation(self, dirs: np.ndarray):
-        """|N_perp - r x L|^2 per direction and the radiated power P."""
+    def radiation(self, dirs: np.ndarray, freq: int = 0):
+        """|N_perp - r x L|^2 per direction and the radiated power P (wavelength `freq` of the list)."""
         d = np.ascontiguousarray(dirs, np.float32)
         if d.ndim == 2 and d.shape[1] == 3:
             d = np.ascontiguousarray(np.concatenate([d, np.zeros((d.shape[0], 1), np.float32)], 1))
         out = np.zeros(d.shape[0], np.float32)
         P = C.c_double()
-        _check(self._lib.emfd_ntff_eval(self.grid.handle, _ptr(d), d.shape[0], _ptr(out), C.byref(P)))
+        _check(self._lib.emfd_ntff_eval_at(self.grid.handle, int(freq), _ptr(d), d.shape[0], _ptr(out), C.byref(P)))
         return out, P.value
+
+    @staticmethod
+    def theta_phi_grid(n_theta: int, n_phi: int) -> np.ndarray:
+        """Directions of a (theta, phi) grid, theta in [0, 180], phi in [0, 360] degrees inclusive,
+        theta-major: (n_theta * n_phi, 4) float32 (x, y, z, 0)."""
+        th = np.radians(np.linspace(0.0, 180.0, n_theta))
+        ph = np.radians(np.linspace(0.0, 360.0, n_phi))
+        t, p = np.meshgrid(th, ph, indexing="ij")
+        d = np.stack([np.sin(t) * np.cos(p), np.sin(t) * np.sin(p), np.cos(t), np.zeros_like(t)], -1)
+        return np.ascontiguousarray(d.reshape(-1, 4), np.float32)
 
     def evaluate(self) -> dict:
         dirs, _, _ = self.directions()

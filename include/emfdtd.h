@@ -289,6 +289,15 @@ int emfd_ntff_directions(emfd_ctx *ctx, float *dirs, int cap, int *n_dirs, int *
  * shaders/ntff_reduce.comp:19-39) for arbitrary directions (4 floats each, xyz used):
  * f2[d] = |N_perp - r x L|^2 at the monitor's wavelength; *power = radiated power P. */
 int emfd_ntff_eval(emfd_ctx *ctx, const float *dirs, int n_dirs, float *f2, double *power);
+/* Several far-field wavelengths at once (no counterpart in the reference, whose far_field{} has one
+ * wavelength): the monitor accumulates the two-stage phasors of every wavelength in the list (1 to
+ * 8, each clamped to >= 2 as configure does) from the same field samples; wavelengths[0] becomes
+ * the monitor's wavelength, used by emfd_ntff_evaluate. Clears the accumulation. Every wavelength's
+ * phasors are those a monitor configured with that wavelength alone would hold. */
+int emfd_ntff_set_wavelengths(emfd_ctx *ctx, const float *wavelengths, int n);
+/* emfd_ntff_eval / emfd_ntff_read_phasors for wavelength `freq` of the list. */
+int emfd_ntff_eval_at(emfd_ctx *ctx, int freq, const float *dirs, int n_dirs, float *f2, double *power);
+int emfd_ntff_read_phasors_at(emfd_ctx *ctx, int freq, float *out, int64_t n_points);
 /* FarFieldMonitor::evaluate + finishEvaluation (src/FarField.cpp:411-468) on the reference
  * direction set. directivity_dbi: n_dirs floats (may be NULL). cuts: 3 x 6 floats per cut
  * (peak dBi, peak angle, beamwidth, front/back, unused, unused) (may be NULL). Returns

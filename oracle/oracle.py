@@ -204,6 +204,15 @@ class OracleSim:
         npts = L.or_ntff_face_offsets(ptr(bmin), ptr(bmax), ptr(fo))
         self.ntff = dict(bmin=bmin, bmax=bmax, fo=fo, n=npts, wavelength=np.float32(max(wavelength, 2.0)),
                          averaging=max(averaging, 0.5), phasors=np.zeros((npts, 32), np.float32))
+        self.ntff_extra = []
+
+    def add_ntff_wavelength(self, wavelength):
+        """A further far-field accumulator on the same box at another wavelength (the library's
+        multi-wavelength monitor keeps one per wavelength); returns its phasor array."""
+        N = self.ntff
+        extra = dict(wavelength=np.float32(max(wavelength, 2.0)), phasors=np.zeros((N["n"], 32), np.float32))
+        self.ntff_extra.append(extra)
+        return extra["phasors"]
 
     def step(self, count):
         L = lib()
@@ -240,15 +249,23 @@ class OracleSim:
                                    C.byref(blend))
             L.or_ntff_accumulate(C.byref(self.g), ptr(self.E), ptr(self.B), ptr(N["bmin"]), ptr(N["bmax"]),
                                  ptr(N["fo"]), N["n"], ptr(rot), blend.value, ptr(N["phasors"]))
+            for X in getattr(self, "ntff_extra", []):
+                L.or_monitor_constants(float(X["wavelength"]), N["averaging"], self.g.dt, self.step_count, ptr(rot),
+                                       C.byref(blend))
+                L.or_ntff_accumulate(C.byref(self.g), ptr(self.E), ptr(self.B), ptr(N["bmin"]), ptr(N["bmax"]),
+                                     ptr(N["fo"]), N["n"], ptr(rot), blend.value, ptr(X["phasors"]))
 
-    def ntff_eval_f32(self, dirs, k=None):
+    def ntff_eval_f32(self, dirs, k=None, extra=None):
+        """Pattern |N_perp - r x L|^2 and power of the primary accumulator, or of the extra
+        wavelength `extra` (index into add_ntff_wavelength's list)."""
         L = lib()
         N = self.ntff
+        src = N if extra is None else self.ntff_extra[extra]
         d = np.ascontiguousarray(dirs, np.float32)
         out = np.zeros(2 * d.shape[0], np.float32)
-        kk = np.float32(2.0 * np.pi / float(N["wavelength"])) if k is None else np.float32(k)
+        kk = np.float32(2.0 * np.pi / float(src["wavelength"])) if k is None else np.float32(k)
         center = np.array([0.5 * (N["bmin"][a] + N["bmax"][a]) for a in range(3)], np.float32)
-        L.or_ntff_eval_f32(ptr(N["bmin"]), ptr(N["bmax"]), ptr(N["fo"]), N["n"], ptr(N["phasors"]), ptr(d), d.shape[0],
+        L.or_ntff_eval_f32(ptr(N["bmin"]), ptr(N["bmax"]), ptr(N["fo"]), N["n"], ptr(src["phasors"]), ptr(d), d.shape[0],
                            float(kk), ptr(center), ptr(out))
         return out[0::2], float(out[1])
 
