@@ -228,4 +228,18 @@ hipError_t launch_mlp_adam(float *p, const float *g, float *m, float *v, size_t 
                            float eps, float wd, float c1, float c2, const int *bf16_index, void *wb, hipStream_t s);
 hipError_t launch_mlp_to_bf16(const float *p, const int *bf16_index, size_t n, void *wb, hipStream_t s);
 
+// Fused inference pass over every layer (mlp.hip): rows of 64 stay in LDS between layers.
+constexpr int kMlpMaxLayers = 8;
+struct MlpFusedArgs {
+    const float *x;                   // [M][n_in] FP32
+    float *y;                         // [M][n_out of the last layer] FP32
+    int M, n_in, layers;
+    int kp[kMlpMaxLayers];            // padded input width of each layer (multiple of 16, <= 256)
+    int n_out[kMlpMaxLayers];         // output width of each layer (128 or 256)
+    const __bf16 *w[kMlpMaxLayers];   // BF16 weights [n_out][kp]
+    const float *bias[kMlpMaxLayers]; // FP32 biases
+};
+int mlp_fused_supported(int layers, const int *kp, const int *n_out);
+hipError_t launch_mlp_forward_fused(const MlpFusedArgs &a, hipStream_t s);
+
 }  // namespace emfd

@@ -46,6 +46,8 @@ struct emfd_mlp {
         for (auto &b : h) b.release();
     }
     int forward(int rows, bool keep_preactivation = true);
+    bool fused_ok = false;  // the fused all-layer inference kernel fits these widths
+    int forward_infer(const float *dx, int rows);
     int backward(int rows, double *loss_partials_dst, int blocks);
 };
 
@@ -87,6 +89,29 @@ int emfd_mlp::forward(int rows, bool keep_preactivation) {
         if (!last) in = h[static_cast<size_t>(l)].ptr;
     }
     return EMFD_OK;
+}
+
+// Inference over `rows` FP32 device rows dx into y: the fused all-layer kernel when the widths
+// fit it, else the layered kernels.
+int emfd_mlp::forward_infer(const float *dx, int rows) {
+    if (fused_ok) {
+        MlpFusedArgs fa{};
+        fa.x = dx;
+        fa.y = y.as<float>();
+        fa.M = rows;
+        fa.n_in = n_in();
+        fa.layers = layers;
+        for (int l = 0; l < layers; ++l) {
+            fa.kp[l] = kp[static_cast<size_t>(l)];
+            fa.n_out[l] = dims[static_cast<size_t>(l) + 1];
+            fa.w[l] = static_cast<const __bf16 *>(wb.ptr) + wb_off[static_cast<size_t>(l)];
+            fa.bias[l] = p.as<float>() + b_off[static_cast<size_t>(l)];
+        }
+        EMFD_HIP(launch_mlp_forward_fused(fa, stream));
+        return EMFD_OK;
+    }
+    EMFD_HIP(launch_mlp_gather(dx, n_in(), kp[0], nullptr, n_out(), nullptr, rows, xb.ptr, tb.as<float>(), stream));
+    return forward(rows, false);
 }
 
 // Loss gradient from y and tb, then the backward pass into g; the loss partial sums go to
@@ -191,6 +216,10 @@ int emfd_mlp_create(int device, const int *dims, int n_dims, int max_batch, emfd
     }
     m->n_params = off;
     m->n_wb = woff;
+    {
+        std::vector<int> nout(m->dims.begin() + 1, m->dims.end());
+        m->fused_ok = mlp_fused_supported(m->layers, m->kp.data(), nout.data()) != 0;
+    }
     std::vector<int> bfi(m->n_params, -1);
     for (int l = 0; l < m->layers; ++l)
         for (size_t i = 0; i < static_cast<size_t>(dims[l + 1]) * m->kp[static_cast<size_t>(l)]; ++i)
@@ -314,10 +343,12 @@ int emfd_mlp_forward(emfd_mlp *m, const float *x, int64_t batch, float *y) {
     if (!x || !y || batch < 0) return fail(EMFD_ERR_INVALID, "bad forward arguments");
     EMFD_HIP(hipSetDevice(m->device));
     Staging s;
+    if (int rc = s.x.ensure(static_cast<size_t>(m->max_batch) * m->n_in() * sizeof(float))) return rc;
     for (int64_t r0 = 0; r0 < batch; r0 += m->max_batch) {
         const int rows = static_cast<int>(std::min<int64_t>(m->max_batch, batch - r0));
-        if (int rc = stage_rows(m, x + r0 * m->n_in(), nullptr, rows, s.x, s.t)) return rc;
-        if (int rc = m->forward(rows, false)) return rc;
+        EMFD_HIP(hipMemcpyAsync(s.x.ptr, x + r0 * m->n_in(), static_cast<size_t>(rows) * m->n_in() * sizeof(float),
+                                hipMemcpyHostToDevice, m->stream));
+        if (int rc = m->forward_infer(s.x.as<float>(), rows)) return rc;
         EMFD_HIP(hipMemcpyAsync(y + r0 * m->n_out(), m->y.ptr, static_cast<size_t>(rows) * m->n_out() * sizeof(float),
                                 hipMemcpyDeviceToHost, m->stream));
         EMFD_HIP(hipStreamSynchronize(m->stream));
@@ -469,10 +500,13 @@ int emfd_mlp_time_forward(emfd_mlp *m, int64_t batch, int iterations, double *av
     hipEvent_t a, b;
     EMFD_HIP(hipEventCreate(&a));
     EMFD_HIP(hipEventCreate(&b));
-    if (int rc = m->forward(rows, false)) return rc;  // warm-up
+    Staging st;  // device input rows (zeros): timing does not depend on the values
+    if (int rc = st.x.ensure(static_cast<size_t>(rows) * m->n_in() * sizeof(float))) return rc;
+    EMFD_HIP(hipMemsetAsync(st.x.ptr, 0, static_cast<size_t>(rows) * m->n_in() * sizeof(float), m->stream));
+    if (int rc = m->forward_infer(st.x.as<float>(), rows)) return rc;  // warm-up
     EMFD_HIP(hipEventRecord(a, m->stream));
     for (int i = 0; i < iterations; ++i)
-        if (int rc = m->forward(rows, false)) return rc;
+        if (int rc = m->forward_infer(st.x.as<float>(), rows)) return rc;
     EMFD_HIP(hipEventRecord(b, m->stream));
     EMFD_HIP(hipEventSynchronize(b));
     float ms = 0.0f;

@@ -250,6 +250,86 @@ __global__ void __launch_bounds__(256) k_to_bf16(const float *p, const int *bf16
     }
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// Fused inference: one workgroup carries 64 rows through every layer. The activations stay in
+// LDS (64 x 256 BF16, rows padded to 264 to spread the banks); each of the 4 wavefronts computes
+// a quarter of the next layer's columns for all 64 rows (up to 2 x 2 MFMA tiles of 32 x 32,
+// K in steps of 16 in increasing order, exactly as k_gemm accumulates), weight fragments come
+// straight from global memory one k-step ahead (the whole network is 338 KB: L2-resident), and
+// the epilogue (bias,
+// GELU, BF16) writes the next layer's input back into LDS. Same arithmetic as the layered forward,
+// so the outputs are bit-identical to it.
+// ---------------------------------------------------------------------------------------------
+constexpr int kFuseRows = 64, kFuseTm = kFuseRows / 32, kFuseLd = 264;
+
+__global__ void __launch_bounds__(256) k_mlp_forward_fused(MlpFusedArgs g) {
+    __shared__ __attribute__((aligned(16))) __bf16 h[kFuseRows][kFuseLd];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r0 = blockIdx.x * kFuseRows;
+    // input rows -> BF16, zero padded to the first layer's width
+    const int k0w = g.kp[0];
+    for (int e = threadIdx.x; e < kFuseRows * k0w; e += 256) {
+        const int r = e / k0w, c = e - r * k0w;
+        const int gr = r0 + r;
+        const float v = (gr < g.M && c < g.n_in) ? g.x[static_cast<size_t>(gr) * g.n_in + c] : 0.0f;
+        h[r][c] = static_cast<__bf16>(v);
+    }
+    __syncthreads();
+    for (int l = 0; l < g.layers; ++l) {
+        const int K = g.kp[l], N = g.n_out[l];
+        const int ncw = N / 4;           // columns per wavefront: 64 or 32
+        const int tn_n = ncw / 32;       // column tiles per wavefront: 2 or 1
+        const int c0 = wave * ncw;
+        const __bf16 *W = g.w[l];
+        floatx16 acc[kFuseTm][2];
+#pragma unroll
+        for (int tm = 0; tm < kFuseTm; ++tm)
+#pragma unroll
+            for (int tn = 0; tn < 2; ++tn) acc[tm][tn] = floatx16{};
+        // weight fragments of column tile tn at k, one k-step ahead of their use
+        const __bf16 *wrow[2] = {W + static_cast<size_t>(c0 + (lane & 31)) * K + 8 * (lane >> 5),
+                                 W + static_cast<size_t>(c0 + 32 + (lane & 31)) * K + 8 * (lane >> 5)};
+        bf16x8 bnext[2];
+        bnext[0] = *reinterpret_cast<const bf16x8 *>(wrow[0]);
+        bnext[1] = tn_n > 1 ? *reinterpret_cast<const bf16x8 *>(wrow[1]) : bnext[0];
+        for (int k = 0; k < K; k += 16) {
+            const bf16x8 b0 = bnext[0], b1 = bnext[1];
+            if (k + 16 < K) {
+                bnext[0] = *reinterpret_cast<const bf16x8 *>(wrow[0] + k + 16);
+                if (tn_n > 1) bnext[1] = *reinterpret_cast<const bf16x8 *>(wrow[1] + k + 16);
+            }
+#pragma unroll
+            for (int tm = 0; tm < kFuseTm; ++tm) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&h[32 * tm + (lane & 31)][k + 8 * (lane >> 5)]);
+                acc[tm][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, acc[tm][0], 0, 0, 0);
+                if (tn_n > 1) acc[tm][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1, acc[tm][1], 0, 0, 0);
+            }
+        }
+        __syncthreads();  // every wavefront is done reading this layer's input
+        const bool last = l == g.layers - 1;
+#pragma unroll
+        for (int tn = 0; tn < 2; ++tn) {
+            if (tn >= tn_n) continue;
+            const int col = c0 + 32 * tn + (lane & 31);
+            const float bj = g.bias[l][col];
+#pragma unroll
+            for (int tm = 0; tm < kFuseTm; ++tm)
+#pragma unroll 4
+                for (int r = 0; r < 16; ++r) {
+                    const int row = 32 * tm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    const float v = acc[tm][tn][r] + bj;
+                    if (last) {
+                        if (r0 + row < g.M) g.y[static_cast<size_t>(r0 + row) * N + col] = v;
+                    } else {
+                        h[row][col] = static_cast<__bf16>(gelu_f(v));
+                    }
+                }
+        }
+        __syncthreads();  // the next layer's input is complete
+    }
+}
+
 }  // namespace
 
 hipError_t launch_mlp_gemm(const MlpGemm &m, hipStream_t s) {
@@ -313,6 +393,22 @@ hipError_t launch_mlp_adam(float *p, const float *g, float *m, float *v, size_t 
 
 hipError_t launch_mlp_to_bf16(const float *p, const int *bf16_index, size_t n, void *wb, hipStream_t s) {
     k_to_bf16<<<512, 256, 0, s>>>(p, bf16_index, n, static_cast<__bf16 *>(wb));
+    return hipGetLastError();
+}
+
+int mlp_fused_supported(int layers, const int *kp, const int *n_out) {
+    if (layers < 1 || layers > kMlpMaxLayers) return 0;
+    for (int l = 0; l < layers; ++l) {
+        if (kp[l] % 16 != 0 || kp[l] > 256) return 0;
+        if (n_out[l] != 128 && n_out[l] != 256) return 0;
+        if (l + 1 < layers && kp[l + 1] != n_out[l]) return 0;
+    }
+    return 1;
+}
+
+hipError_t launch_mlp_forward_fused(const MlpFusedArgs &a, hipStream_t s) {
+    if (a.M <= 0) return hipSuccess;
+    k_mlp_forward_fused<<<(a.M + kFuseRows - 1) / kFuseRows, 256, 0, s>>>(a);
     return hipGetLastError();
 }
 
