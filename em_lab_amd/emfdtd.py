@@ -672,6 +672,54 @@ class FarFieldMonitor:
                           for c in range(3)})
 
 
+# ---------------------------------------------------------------------------------------------
+# Report files (PortMonitor::writeSweepCsv, src/Ports.cpp:280-297; FarFieldMonitor::writeCsv,
+# src/FarField.cpp:337-350): the same columns, numbers printed as C++ streams print them (%g).
+# ---------------------------------------------------------------------------------------------
+C0 = 299792458.0  # src/Ports.cpp kC0
+
+
+def _g(v) -> str:
+    return format(float(v), "g")
+
+
+def write_sweep_csv(path, port_names, sweep: dict, cell_size: float) -> bool:
+    """Sweep CSV: frequency_Hz, wavelength_cells, then R, X, S11 dB, VSWR per port; `sweep` is
+    PortMonitor.evaluate_sweep()'s (or Batch.sweep_result()'s) dictionary."""
+    ports = sweep.get("ports") or []
+    if not ports:
+        return False
+    lines = ["frequency_Hz,wavelength_cells" +
+             "".join(f",{n}_R_ohm,{n}_X_ohm,{n}_S11_dB,{n}_VSWR" for n in port_names)]
+    pts0 = ports[0]["points"]
+    for i in range(len(pts0)):
+        wl = float(pts0[i][0])
+        row = [_g(C0 / (wl * cell_size)), _g(wl)]
+        for p in ports:
+            q = p["points"][i]
+            row += [_g(q[1]), _g(q[2]), _g(q[3]), _g(q[4])]
+        lines.append(",".join(row))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return True
+
+
+def write_pattern_csv(path, pattern: dict) -> bool:
+    """Far-field cut CSV: angle_deg, then xy_dBi, xz_dBi, yz_dBi; `pattern` is
+    FarFieldMonitor.evaluate()'s dictionary (cut c holds directions 180 c .. 180 c + 179)."""
+    if not pattern.get("valid"):
+        return False
+    d = np.asarray(pattern["directivity_dbi"], np.float32)
+    names = ("xy", "xz", "yz")
+    lines = ["angle_deg" + "".join(f",{n}_dBi" for n in names)]
+    for i in range(180):
+        angle = np.float32(360.0) * np.float32(i) / np.float32(180)
+        lines.append(",".join([_g(angle)] + [_g(d[180 * c + i]) for c in range(3)]))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return True
+
+
 class FieldAnalyzer:
     """FieldAnalyzer probes and flux meters on a grid's context (src/FieldAnalyzer.hpp:73-130)."""
 

@@ -81,3 +81,23 @@ def test_theta_phi_grid():
     assert d.shape == (91 * 181, 4)
     assert np.allclose(np.linalg.norm(d[:, :3], axis=1), 1.0, atol=1e-6)
     assert np.allclose(d[0, :3], [0, 0, 1], atol=1e-6) and np.allclose(d[-1, :3], [0, 0, -1], atol=1e-6)
+
+
+def test_report_csv_formats(tmp_path):
+    """The sweep and pattern CSVs keep EM_Lab's columns and C++ stream number formatting."""
+    from em_lab_amd import emfdtd as E
+    pts = np.array([[20.0, 50.0, -3.25, -20.5, 1.2], [10.0, 73.125, 42.5, -3.1, 5.75]])
+    sweep = dict(ports=[dict(points=pts)])
+    p = tmp_path / "sweep.csv"
+    assert E.write_sweep_csv(p, ["dipole"], sweep, 1.0e-3)
+    lines = p.read_text().splitlines()
+    assert lines[0] == "frequency_Hz,wavelength_cells,dipole_R_ohm,dipole_X_ohm,dipole_S11_dB,dipole_VSWR"
+    assert lines[1] == "1.49896e+10,20,50,-3.25,-20.5,1.2"
+    assert lines[2] == "2.99792e+10,10,73.125,42.5,-3.1,5.75"
+    pat = dict(valid=True, directivity_dbi=np.arange(540, dtype=np.float32) / 100.0)
+    q = tmp_path / "pattern.csv"
+    assert E.write_pattern_csv(q, pat)
+    rows = q.read_text().splitlines()
+    assert rows[0] == "angle_deg,xy_dBi,xz_dBi,yz_dBi" and len(rows) == 181
+    assert rows[2] == "2,0.01,1.81,3.61"
+    assert not E.write_pattern_csv(q, dict(valid=False))
