@@ -177,7 +177,54 @@ def train_surrogate(X, Z, epochs=400, batch=256, lr=(2e-3, 2e-5), seed=1, holdou
     return m, rep
 
 
+class Ensemble:
+    """Several surrogate networks (same widths, different seeds) whose outputs are averaged."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        self.dims = self.members[0].dims
+
+    def forward(self, X):
+        y = self.members[0].forward(X).astype(np.float64)
+        for m in self.members[1:]:
+            y += m.forward(X)
+        return (y / len(self.members)).astype(np.float32)
+
+    def time_forward(self, batch, iterations=20):
+        return sum(m.time_forward(batch, iterations) for m in self.members)
+
+    def params(self):
+        return np.stack([m.params() for m in self.members])
+
+    def close(self):
+        for m in self.members:
+            m.close()
+
+
+def train_ensemble(X, Z, members=4, epochs=600, batch=256, lr=(2e-3, 2e-5), holdout=1024, log=None):
+    """`members` networks trained from different seeds on the same data; the report scores their
+    averaged prediction on the held-out designs."""
+    nets, reps = [], []
+    for k in range(members):
+        m, rep = train_surrogate(X, Z, epochs=epochs, batch=batch, lr=lr, seed=1 + k, holdout=holdout, log=log)
+        nets.append(m)
+        reps.append(rep)
+    ens = Ensemble(nets)
+    rep = dict(members=members, member_holdout_s11_err_median_db=[r.get("holdout_s11_err_median_db") for r in reps],
+               train_s=round(sum(r["train_s"] for r in reps), 2))
+    if holdout:
+        ntr = X.shape[0] - holdout
+        y = ens.forward(X[ntr:])
+        err = s11_error_db(s11_db(gamma_from_output(y)), s11_db(z_to_gamma(Z[ntr:])))
+        rep.update(holdout_s11_err_median_db=float(np.median(err)), holdout_s11_err_p90_db=float(np.percentile(err, 90)),
+                   holdout_within_0p5db=float(np.mean(err <= 0.5)))
+    if log:
+        log(f"ensemble: {rep}")
+    return ens, rep
+
+
 def save_weights(m, path: Path = WEIGHTS, report=None) -> None:
+    """Parameters of one network (params: n) or of an ensemble (params: members x n)."""
     path.parent.mkdir(parents=True, exist_ok=True)
     np.savez_compressed(path, params=m.params(), dims=np.array(m.dims),
                         report=np.array(repr(report or {})))
@@ -186,9 +233,14 @@ def save_weights(m, path: Path = WEIGHTS, report=None) -> None:
 def load_surrogate(path: Path = WEIGHTS, max_batch: int = 65536):
     from em_lab_amd import emfdtd as E
     d = np.load(path)
-    m = E.Mlp(tuple(int(v) for v in d["dims"]), max_batch=max_batch)
-    m.set_params(d["params"])
-    return m
+    dims = tuple(int(v) for v in d["dims"])
+    params = np.atleast_2d(d["params"])
+    nets = []
+    for p in params:
+        m = E.Mlp(dims, max_batch=max_batch)
+        m.set_params(p)
+        nets.append(m)
+    return nets[0] if len(nets) == 1 else Ensemble(nets)
 
 
 def candidates(n: int, seed: int = CANDIDATE_SEED):
@@ -253,6 +305,7 @@ def screening_run(n_candidates: int = 65536, top: int = 256, plain_designs: int 
     nu = sweep_nu()
     bw_fdtd = bandwidth_10db(s_fdtd, nu)
     ms_fwd = m.time_forward(min(n_candidates, 65536), 10)
+    members = len(m.members) if isinstance(m, Ensemble) else 1
     m.close()
     plain = plain_fdtd_rate(plain_designs)
     screen_s = (t1 - t0) + (t3 - t2) + (t4 - t3)
@@ -263,8 +316,9 @@ def screening_run(n_candidates: int = 65536, top: int = 256, plain_designs: int 
                accepted=int(np.sum(err <= 0.5)), s11_err_median_db=round(float(np.median(err)), 3),
                s11_err_max_db=round(float(np.max(err)), 3),
                verified_bw_mean=round(float(np.mean(bw_fdtd)), 4), predicted_bw_mean=round(float(np.mean(bw_pred[idx])), 4),
-               forward_ms_65536=round(ms_fwd, 4),
-               forward_tflops=round(2 * 65536 * sum(i * o for i, o in zip(DIMS[:-1], DIMS[1:])) / (ms_fwd * 1e-3) / 1e12, 2))
+               surrogate_members=members, forward_ms_65536=round(ms_fwd, 4),
+               forward_tflops=round(members * 2 * 65536 * sum(i * o for i, o in zip(DIMS[:-1], DIMS[1:])) /
+                                    (ms_fwd * 1e-3) / 1e12, 2))
     if log:
         log(repr(rep))
     return rep
@@ -282,6 +336,7 @@ def main(argv=None) -> int:
     tr.add_argument("--data", type=Path, nargs="+", required=True)
     tr.add_argument("--epochs", type=int, default=400)
     tr.add_argument("--batch", type=int, default=256)
+    tr.add_argument("--members", type=int, default=1, help="ensemble size (networks averaged)")
     tr.add_argument("--out", type=Path, default=WEIGHTS)
     sc = sub.add_parser("screen", help="screen seeded candidates and verify the best by batched FDTD")
     sc.add_argument("--candidates", type=int, default=65536)
@@ -289,7 +344,11 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     if args.cmd == "train":
         X, Z, _ = load_data(args.data)
-        m, rep = train_surrogate(X, Z, epochs=args.epochs, batch=args.batch, log=lambda s: print(s, flush=True))
+        if args.members > 1:
+            m, rep = train_ensemble(X, Z, members=args.members, epochs=args.epochs, batch=args.batch,
+                                    log=lambda s: print(s, flush=True))
+        else:
+            m, rep = train_surrogate(X, Z, epochs=args.epochs, batch=args.batch, log=lambda s: print(s, flush=True))
         save_weights(m, args.out, rep)
         print(f"saved {args.out}", flush=True)
         return 0
