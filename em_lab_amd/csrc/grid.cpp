@@ -659,8 +659,8 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
         } else if (variant == EMFD_KERNEL_TWO_PASS) {
             EMFD_HIP(launch_yee_two_pass(lat, ctx->fields(ctx->cur), ctx->stream));
         } else if (variant == EMFD_KERNEL_TWO_PASS4) {
-            // streaming level: explicit bits 8..9 of the variant, else 2 (measured fastest)
-            const int st = ctx->kernel_variant == EMFD_KERNEL_AUTO ? 2 : (ctx->tile_cfg & 3);
+            // streaming level / block order: explicit bits 8..10 of the variant, else 2 (measured fastest)
+            const int st = ctx->kernel_variant == EMFD_KERNEL_AUTO ? 2 : (ctx->tile_cfg & 7);
             EMFD_HIP(launch_yee_two_pass4(lat, ctx->fields(ctx->cur), st, ctx->stream));
         } else if (variant == EMFD_KERNEL_SLABS) {
             if (!yee_two_pass4_supported(lat))

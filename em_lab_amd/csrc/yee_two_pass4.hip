@@ -20,6 +20,35 @@ __global__ void __launch_bounds__(256) k_update_e4(LatticeArgs a, FieldPtrs f, i
     e_quad<ST>(a, f, x0, y, z);
 }
 
+// XCD-contiguous order: the hardware deals workgroups round-robin to the 8 XCDs; remapping the
+// dispatch index b to tile (b % 8) * per + b / 8 gives each XCD one contiguous run of (y, z)
+// blocks, so the +y / +z neighbour rows a block reads were just read by a block of the same XCD
+// (its L2) instead of one on another XCD.
+template <int ST, bool B>
+__global__ void __launch_bounds__(256) k_update4_xcd(LatticeArgs a, FieldPtrs f, int z_base, int gx, int gy, int n) {
+    const int per = (n + 7) / 8;
+    const int t = (static_cast<int>(blockIdx.x) % 8) * per + static_cast<int>(blockIdx.x) / 8;
+    if (t >= n) return;
+    const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
+    const int x0 = (bx * 64 + static_cast<int>(threadIdx.x) % 64) * 4;
+    const int y = by * 4 + static_cast<int>(threadIdx.x) / 64, z = z_base + bz;
+    if (x0 >= a.nx || y >= a.ny) return;
+    if (B) b_quad<ST>(a, f, x0, y, z);
+    else e_quad<ST>(a, f, x0, y, z);
+}
+
+template <int ST>
+hipError_t launch_st_xcd(const LatticeArgs &a, const FieldPtrs &f, int z0, int nzs, hipStream_t s) {
+    const int gx = (a.nx / 4 + 63) / 64, gy = (a.ny + 3) / 4;
+    const int n = gx * gy * nzs;
+    const int blocks = (n + 7) / 8 * 8;
+    k_update4_xcd<ST, true><<<blocks, 256, 0, s>>>(a, f, z0, gx, gy, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_update4_xcd<ST, false><<<blocks, 256, 0, s>>>(a, f, z0, gx, gy, n);
+    return hipGetLastError();
+}
+
 template <int ST>
 hipError_t launch_st(const LatticeArgs &a, const FieldPtrs &f, int z0, int nzs, hipStream_t s) {
     dim3 block(64, 4, 1), grid((a.nx / 4 + 63) / 64, (a.ny + 3) / 4, nzs);
@@ -31,6 +60,8 @@ hipError_t launch_st(const LatticeArgs &a, const FieldPtrs &f, int z0, int nzs, 
 }
 
 hipError_t launch_range(const LatticeArgs &a, const FieldPtrs &f, int z0, int nzs, int streaming, hipStream_t s) {
+    if (streaming == 6) return launch_st_xcd<2>(a, f, z0, nzs, s);
+    if (streaming == 4) return launch_st_xcd<0>(a, f, z0, nzs, s);
     if (streaming == 1) return launch_st<1>(a, f, z0, nzs, s);
     if (streaming == 2) return launch_st<2>(a, f, z0, nzs, s);
     if (streaming == 3) return launch_st<3>(a, f, z0, nzs, s);

@@ -47,8 +47,9 @@ extern "C" {
 #define EMFD_KERNEL_ROWS 3     /* fused B+E, one wavefront per x-row z-march (nx <= 256) */
 #define EMFD_KERNEL_POINT 4    /* fused B+E, one thread per texel, neighbour B rebuilt */
 #define EMFD_KERNEL_TWO_PASS4 5 /* two passes, four cells per thread with float4 streams (nx % 4 == 0); bits
-                                   8..9 = 1: non-temporal stores, 2: also non-temporal loads of the
-                                   plane each pass reads once (AUTO uses 2) */
+                                   8..10 = 1: non-temporal stores, 2: also non-temporal loads of the
+                                   plane each pass reads once (AUTO uses 2), 3: also class ids;
+                                   4 / 6: 0 / 2 with XCD-contiguous block order */
 #define EMFD_KERNEL_SLABS 6     /* TWO_PASS4 interleaved per z-slab (B, E on slab 0, then slab 1, ...); bits 8..
                                    of the variant = slab thickness in planes (default 32) */
 #define EMFD_KERNEL_ZMARCH 7    /* fused B+E, z-march of 4-row groups with LDS row exchange and a helper

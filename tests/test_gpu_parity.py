@@ -28,8 +28,8 @@ def _fields_match(g, sim, tol=FIELD_TOL):
 
 
 @pytest.mark.parametrize("kernel", [E.KERNEL_FUSED, E.KERNEL_TWO_PASS, E.KERNEL_ROWS, E.KERNEL_POINT,
-                                    E.KERNEL_TWO_PASS4, E.KERNEL_SLABS | (7 << 8), E.KERNEL_ZMARCH,
-                                    E.KERNEL_ZMARCH | (5 << 8)])
+                                    E.KERNEL_TWO_PASS4, E.KERNEL_TWO_PASS4 | (6 << 8),
+                                    E.KERNEL_SLABS | (7 << 8), E.KERNEL_ZMARCH, E.KERNEL_ZMARCH | (5 << 8)])
 def test_hertzian_vacuum_64(kernel):
     """Plain FDTD 64^3 vacuum, 8-cell absorber, Courant 0.99/sqrt(3), z Hertzian pulse at the
     centre (lambda 16), 400 steps: fields, far-field phasors and the directivity pattern."""
@@ -196,7 +196,8 @@ def test_full_size_256_properties():
     sc = spheres_dipole(n=256, pml=10)
     grids = [gpu_grid(sc, k, ports=False, ntff=False)[0]
              for k in (E.KERNEL_TWO_PASS4, E.KERNEL_ROWS, E.KERNEL_FUSED, E.KERNEL_TWO_PASS,
-                       E.KERNEL_POINT, E.KERNEL_SLABS | (24 << 8), E.KERNEL_ZMARCH)]
+                       E.KERNEL_POINT, E.KERNEL_SLABS | (24 << 8), E.KERNEL_ZMARCH,
+                       E.KERNEL_TWO_PASS4 | (6 << 8))]
     sim = oracle_sim(sc, ports=False, ntff=False)
     for g in grids:
         g.step(3)
