@@ -57,6 +57,8 @@ struct NtffState {
     double start_time = 0.0;
     DevBuf d_phasors;  // SoA: 24 planes of n_points floats
     DevBuf d_side;     // x-face side columns (SideCols layout)
+    DevBuf d_samples;  // kNtffRun sample slots of 6 planes of n_points floats (deferred accumulation)
+    int pending = 0;   // sampled steps of the current run not yet applied to the phasors
     std::vector<float> wavelengths;  // accumulated wavelengths; [0] is setup.wavelength
     int n_freq() const { return wavelengths.empty() ? 1 : static_cast<int>(wavelengths.size()); }
     float wavelength(int f) const { return wavelengths.empty() ? setup.wavelength : wavelengths[static_cast<size_t>(f)]; }
@@ -128,5 +130,5 @@ namespace emfd {
 // Per-step monitor hooks used by emfd_grid_step (monitors.cpp).
 int monitors_consts_floats(emfd_ctx *ctx);
 int monitors_fill_consts(emfd_ctx *ctx, float *consts_step, uint64_t step_after);
-int monitors_after_step(emfd_ctx *ctx, const float *consts_step, int64_t k);
+int monitors_after_step(emfd_ctx *ctx, const float *consts_step, int stride, int k, int count);
 }  // namespace emfd

@@ -621,6 +621,7 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
         ctx->side_active = ctx->ntff.enabled && ctx->storage == kStorageF32 && tp4;
     }
     const LatticeArgs lat = ctx->lattice();
+    ctx->ntff.pending = 0;  // a run of deferred far-field samples never spans two calls
     for (int k = 0; k < count; ++k) {
         const float *consts = ctx->d_consts.as<float>() + static_cast<size_t>(k) * stride;
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -695,7 +696,7 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
             }
         }
         ++ctx->step_count;  // observers see the time of the step that just completed
-        if (int rc = monitors_after_step(ctx, consts + nsrc, k)) return rc;
+        if (int rc = monitors_after_step(ctx, consts + nsrc, stride, k, count)) return rc;
         if (ctx->timing && ctx->ev_used >= 4096) {
             double ms;
             int64_t l;

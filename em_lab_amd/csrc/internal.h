@@ -136,6 +136,12 @@ struct NtffFreqs {
 };
 hipError_t launch_ntff_accumulate(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, const float *rot,
                                   const NtffFreqs &freqs, float *phasors, hipStream_t s);
+// deferred far-field accumulation: per-step gather into sample slot (6 planes of n_points), then
+// one pass applying `steps` updates (rotation rows rot_stride floats apart) to the phasors
+constexpr int kNtffRun = 8;
+hipError_t launch_ntff_sample(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, float *slot, hipStream_t s);
+hipError_t launch_ntff_flush(const NtffBox &box, const float *rot, int rot_stride, int steps, const NtffFreqs &freqs,
+                             const float *samples, float *phasors, hipStream_t s);
 hipError_t launch_ntff_pattern(const NtffBox &box, const float *phasors, const float *dirs, int n_dirs, float k,
                                const float center[3], float *partials, int slices, float *out_f2,
                                float *power_partials, hipStream_t s);

@@ -202,8 +202,7 @@ int monitors_fill_consts(emfd_ctx *ctx, float *row, uint64_t step_after) {
     return EMFD_OK;
 }
 
-int monitors_after_step(emfd_ctx *ctx, const float *consts, int64_t k) {
-    (void)k;
+int monitors_after_step(emfd_ctx *ctx, const float *consts, int stride, int k, int count) {
     const LatticeArgs lat = ctx->lattice();
     const FieldPtrs f = ctx->fields(ctx->cur);
     PortsState &P = ctx->ports;
@@ -220,7 +219,23 @@ int monitors_after_step(emfd_ctx *ctx, const float *consts, int64_t k) {
         fr.n = N.n_freq();
         for (int q = 0; q < fr.n; ++q)
             fr.blend[q] = blend_of(N.setup.averaging_periods, N.wavelength(q), ctx->cfg.courant);
-        EMFD_HIP(launch_ntff_accumulate(lat, f, N.box, consts + 4, fr, N.d_phasors.as<float>(), ctx->stream));
+        // runs of up to kNtffRun steps inside one step call are sampled and applied together
+        // (bit-identical to per-step accumulation); a call's last step always completes its run,
+        // so the phasors are current whenever a call returns
+        const bool last = k + 1 == count;
+        if (N.pending == 0 && (last || !N.d_samples.ptr)) {
+            EMFD_HIP(launch_ntff_accumulate(lat, f, N.box, consts + 4, fr, N.d_phasors.as<float>(), ctx->stream));
+        } else {
+            const size_t np = static_cast<size_t>(N.box.n_points);
+            EMFD_HIP(launch_ntff_sample(lat, f, N.box, N.d_samples.as<float>() + 6 * np * N.pending, ctx->stream));
+            ++N.pending;
+            if (N.pending == kNtffRun || last) {
+                const float *rot0 = consts + 4 - static_cast<ptrdiff_t>(N.pending - 1) * stride;
+                EMFD_HIP(launch_ntff_flush(N.box, rot0, stride, N.pending, fr, N.d_samples.as<float>(),
+                                           N.d_phasors.as<float>(), ctx->stream));
+                N.pending = 0;
+            }
+        }
     }
     return EMFD_OK;
 }
@@ -430,6 +445,8 @@ int emfd_ntff_configure(emfd_ctx *ctx, const emfd_far_field_setup *setup, emfd_f
     b.n_points = b.face_offset[6];
     N.wavelengths.assign(1, s.wavelength);  // configure resets the list to the setup's wavelength
     if (int rc = N.d_phasors.ensure(static_cast<size_t>(b.n_points) * 24 * sizeof(float))) return rc;
+    if (int rc = N.d_samples.ensure(static_cast<size_t>(b.n_points) * 6 * kNtffRun * sizeof(float))) return rc;
+    N.pending = 0;
     // side columns x = X-1, X of the two x faces over y, z in [box min - 1, box max]
     N.side = SideCols{};
     N.side.x[0] = b.bmin[0];

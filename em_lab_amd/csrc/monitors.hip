@@ -7,11 +7,14 @@
 namespace emfd {
 namespace {
 
+// S: the field storage format, fixed at compile time so the gathers' loads carry no format branch
+template <int S>
 __global__ void __launch_bounds__(64) k_port_accumulate(LatticeArgs a, FieldPtrs f, const PortDev *ports, int n_ports,
                                                         const float *rot, float blend, float *ph, float *series,
                                                         int64_t rec) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_ports) return;
+    f.storage = S;
     port_sample(a, f, ports[i], rot, blend, ph + 8 * i,
                 (rec >= 0 && series) ? series + 2 * (rec * n_ports + i) : nullptr);
 }
@@ -20,64 +23,137 @@ __global__ void __launch_bounds__(64) k_port_accumulate(LatticeArgs a, FieldPtrs
 // Huygens box point (shaders/ntff_common.glsl:30-53)
 // ---------------------------------------------------------------------------------------------
 __device__ __forceinline__ void ntff_point(const NtffBox &b, int index, int node[3], float n[3], float *w) {
-    int f = 0;
+    int f = 0, first = 0;
 #pragma unroll
     for (int i = 1; i < 6; ++i)
-        if (index >= b.face_offset[i]) f = i;
+        if (index >= b.face_offset[i]) {
+            f = i;
+            first = b.face_offset[i];
+        }
     const int ax = f >> 1, side = f & 1;
     const int u = ax == 0 ? 1 : 0, v = ax == 2 ? 1 : 2;
     const int nu = b.bmax[u] - b.bmin[u], nv = b.bmax[v] - b.bmin[v];
-    const int local = index - b.face_offset[f];
+    const int local = index - first;
     const int iu = local % (nu + 1), iv = local / (nu + 1);
-    node[0] = b.bmin[0];
-    node[1] = b.bmin[1];
-    node[2] = b.bmin[2];
-    node[ax] = side == 1 ? b.bmax[ax] : b.bmin[ax];
-    node[u] = b.bmin[u] + iu;
-    node[v] = b.bmin[v] + iv;
+    // per-component selects (no runtime index into the private arrays, which would live in scratch)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        node[d] = d == ax ? (side == 1 ? b.bmax[d] : b.bmin[d]) : b.bmin[d] + (d == u ? iu : iv);
+        n[d] = d == ax ? (side == 1 ? 1.0f : -1.0f) : 0.0f;
+    }
     *w = ((iu == 0 || iu == nu) ? 0.5f : 1.0f) * ((iv == 0 || iv == nv) ? 0.5f : 1.0f);
-    n[0] = n[1] = n[2] = 0.0f;
-    n[ax] = side == 1 ? 1.0f : -1.0f;
 }
 
 // ntff_accumulate.comp (shaders/ntff_accumulate.comp:30-52). Phasors are stored as 24 planes of
 // n_points floats: stage s (0,1) x quantity q (eRe, eIm, hRe, hIm) x component c -> plane 12 s + 3 q + c.
-__global__ void __launch_bounds__(256) k_ntff_accumulate(LatticeArgs a, FieldPtrs f, NtffBox box, const float *rot,
-                                                         NtffFreqs freqs, float *ph) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= box.n_points) return;
+// Fields at Huygens point i (E, then B): the x-normal faces read the coalesced side columns when
+// the Yee kernel maintains them, the other faces the planes.
+__device__ __forceinline__ void ntff_fields(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, int i,
+                                            float E[3], float H[3]) {
     int node[3];
     float n[3], w;
     ntff_point(box, i, node, n, &w);
-    float E[3], H[3];
     if (a.side.buf && n[0] != 0.0f) {
-        // faces normal to x: the same values from the coalesced side columns
+        // every sample of a point on the face x = X lies in the face's columns (x = X - 1, X;
+        // y, z from box min - 1 to box max), so the offsets are formed without side_index's checks,
+        // which would put each load behind a branch of its own
+        const SideCols &sc = a.side;
         const int face = n[0] > 0.0f ? 1 : 0;
-        const auto side = [&](int c, int x, int y, int z) { return a.side.buf[side_index(a.side, face, c, x, y, z)]; };
+        const float *base = sc.buf + static_cast<size_t>(face) * 12 * sc.nz * sc.ny;
+        const int x1 = sc.x[face] - 1;
+        const auto side = [&](int c, int x, int y, int z) {
+            return base[((2 * c + (x - x1)) * sc.nz + (z - sc.z0)) * sc.ny + (y - sc.y0)];
+        };
         e_at_node_with(side, node[0], node[1], node[2], E);
         b_at_node_with(side, node[0], node[1], node[2], H);
     } else {
         e_at_node(a, f, node[0], node[1], node[2], E);
         b_at_node(a, f, node[0], node[1], node[2], H);
     }
+}
+
+// One step of the two-stage EMA of one wavelength (shaders/ntff_accumulate.comp:40-52) on the 24
+// phasor values of a point, slot 12 s + 3 q + c.
+__device__ __forceinline__ void ntff_update(const float *r, float blend, const float E[3], const float H[3],
+                                            float st[24]) {
+    const float co[4] = {2.0f * r[0], -2.0f * r[1], 2.0f * r[2], -2.0f * r[3]};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float obs = co[q] * (q < 2 ? E[c] : H[c]);
+            float s1 = st[3 * q + c], s2 = st[12 + 3 * q + c];
+            s1 += blend * (obs - s1);
+            s2 += blend * (s1 - s2);
+            st[3 * q + c] = s1;
+            st[12 + 3 * q + c] = s2;
+        }
+}
+
+// The storage format is a template parameter so that the gathers' loads carry no per-load format
+// branch and can all be in flight together.
+template <int S>
+__global__ void __launch_bounds__(256) k_ntff_accumulate(LatticeArgs a, FieldPtrs f, NtffBox box, const float *rot,
+                                                         NtffFreqs freqs, float *ph) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= box.n_points) return;
+    f.storage = S;
+    float E[3], H[3];
+    ntff_fields(a, f, box, i, E, H);
     const size_t np = static_cast<size_t>(box.n_points);
     for (int fq = 0; fq < freqs.n; ++fq) {
-        const float *r = rot + 4 * fq;
-        const float blend = freqs.blend[fq];
-        float *phf = ph + static_cast<size_t>(24 * fq) * np;
-        const float co[4] = {2.0f * r[0], -2.0f * r[1], 2.0f * r[2], -2.0f * r[3]};
+        float *phf = ph + static_cast<size_t>(24 * fq) * np + i;
+        float st[24];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int j = 0; j < 24; ++j) st[j] = phf[j * np];
+        ntff_update(rot + 4 * fq, freqs.blend[fq], E, H, st);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float obs = co[q] * (q < 2 ? E[c] : H[c]);
-                float *p1 = phf + (3 * q + c) * np + i, *p2 = phf + (12 + 3 * q + c) * np + i;
-                float s1 = *p1, s2 = *p2;
-                s1 += blend * (obs - s1);
-                s2 += blend * (s1 - s2);
-                *p1 = s1;
-                *p2 = s2;
-            }
+        for (int j = 0; j < 24; ++j) phf[j * np] = st[j];
+    }
+}
+
+// Deferred accumulation over a run of steps: each step only gathers its six field values into a
+// sample slot (6 planes of n_points floats); k_ntff_flush then applies the run's updates in step
+// order to phasors held in registers. Every phasor goes through the same expressions in the same
+// order as with one k_ntff_accumulate per step, so the result is bit-identical, while each phasor
+// plane is read and written once per run instead of once per step.
+template <int S>
+__global__ void __launch_bounds__(256) k_ntff_sample(LatticeArgs a, FieldPtrs f, NtffBox box, float *slot) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= box.n_points) return;
+    f.storage = S;
+    float E[3], H[3];
+    ntff_fields(a, f, box, i, E, H);
+    const size_t np = static_cast<size_t>(box.n_points);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        __builtin_nontemporal_store(E[c], slot + c * np + i);
+        __builtin_nontemporal_store(H[c], slot + (3 + c) * np + i);
+    }
+}
+
+template <int K>
+__global__ void __launch_bounds__(256) k_ntff_flush(NtffBox box, const float *rot, int rot_stride, NtffFreqs freqs,
+                                                    const float *samples, float *ph) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= box.n_points) return;
+    const size_t np = static_cast<size_t>(box.n_points);
+    // the run's samples are read once and serve every wavelength
+    float smp[K][6];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) smp[k][c] = __builtin_nontemporal_load(samples + (6 * k + c) * np + i);
+    for (int fq = 0; fq < freqs.n; ++fq) {
+        float *phf = ph + static_cast<size_t>(24 * fq) * np + i;
+        float st[24];
+#pragma unroll
+        for (int j = 0; j < 24; ++j) st[j] = phf[j * np];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            ntff_update(rot + static_cast<size_t>(k) * rot_stride + 4 * fq, freqs.blend[fq], smp[k], smp[k] + 3, st);
+#pragma unroll
+        for (int j = 0; j < 24; ++j) phf[j * np] = st[j];
     }
 }
 
@@ -281,15 +357,69 @@ hipError_t launch_port_accumulate(const LatticeArgs &a, const FieldPtrs &f, cons
                                   const float *rot, float blend, float *phasors, float *series, int64_t record_index,
                                   hipStream_t s) {
     if (n_ports <= 0) return hipSuccess;
-    k_port_accumulate<<<(n_ports + 63) / 64, 64, 0, s>>>(a, f, ports, n_ports, rot, blend, phasors, series,
-                                                          record_index);
+    const int g = (n_ports + 63) / 64;
+    switch (f.storage) {
+        case kStorageF32:
+            k_port_accumulate<kStorageF32><<<g, 64, 0, s>>>(a, f, ports, n_ports, rot, blend, phasors, series, record_index);
+            break;
+        case kStorageF16:
+            k_port_accumulate<kStorageF16><<<g, 64, 0, s>>>(a, f, ports, n_ports, rot, blend, phasors, series, record_index);
+            break;
+        case kStorageBF16:
+            k_port_accumulate<kStorageBF16><<<g, 64, 0, s>>>(a, f, ports, n_ports, rot, blend, phasors, series,
+                                                              record_index);
+            break;
+        case kStorageF64:
+            k_port_accumulate<kStorageF64><<<g, 64, 0, s>>>(a, f, ports, n_ports, rot, blend, phasors, series, record_index);
+            break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_ntff_accumulate(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, const float *rot,
                                   const NtffFreqs &freqs, float *phasors, hipStream_t s) {
     if (box.n_points <= 0) return hipSuccess;
-    k_ntff_accumulate<<<(box.n_points + 255) / 256, 256, 0, s>>>(a, f, box, rot, freqs, phasors);
+    const int g = (box.n_points + 255) / 256;
+    switch (f.storage) {
+        case kStorageF32: k_ntff_accumulate<kStorageF32><<<g, 256, 0, s>>>(a, f, box, rot, freqs, phasors); break;
+        case kStorageF16: k_ntff_accumulate<kStorageF16><<<g, 256, 0, s>>>(a, f, box, rot, freqs, phasors); break;
+        case kStorageBF16: k_ntff_accumulate<kStorageBF16><<<g, 256, 0, s>>>(a, f, box, rot, freqs, phasors); break;
+        case kStorageF64: k_ntff_accumulate<kStorageF64><<<g, 256, 0, s>>>(a, f, box, rot, freqs, phasors); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ntff_sample(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, float *slot, hipStream_t s) {
+    if (box.n_points <= 0) return hipSuccess;
+    const int g = (box.n_points + 255) / 256;
+    switch (f.storage) {
+        case kStorageF32: k_ntff_sample<kStorageF32><<<g, 256, 0, s>>>(a, f, box, slot); break;
+        case kStorageF16: k_ntff_sample<kStorageF16><<<g, 256, 0, s>>>(a, f, box, slot); break;
+        case kStorageBF16: k_ntff_sample<kStorageBF16><<<g, 256, 0, s>>>(a, f, box, slot); break;
+        case kStorageF64: k_ntff_sample<kStorageF64><<<g, 256, 0, s>>>(a, f, box, slot); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ntff_flush(const NtffBox &box, const float *rot, int rot_stride, int steps, const NtffFreqs &freqs,
+                             const float *samples, float *phasors, hipStream_t s) {
+    if (box.n_points <= 0 || steps <= 0) return hipSuccess;
+    static_assert(kNtffRun == 8, "one flush instance per run length");
+    const int g = (box.n_points + 255) / 256;
+    switch (steps) {
+        case 1: k_ntff_flush<1><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
+        case 2: k_ntff_flush<2><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
+        case 3: k_ntff_flush<3><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
+        case 4: k_ntff_flush<4><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
+        case 5: k_ntff_flush<5><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
+        case 6: k_ntff_flush<6><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
+        case 7: k_ntff_flush<7><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
+        case 8: k_ntff_flush<8><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
