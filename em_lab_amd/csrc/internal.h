@@ -49,6 +49,9 @@ struct LatticeArgs {
     const float *pml_tab;       // absorber coefficients per (kx, ky, kz) depth class, kPmlStride floats
     int pml_k;                  // depth classes per axis: 2 * pml + 2
     SideCols side;              // far-field x-face columns (buf null when not maintained)
+    const uint32_t *wire_mask;  // 1 bit per texel: its class has a thin-wire factor != 1 (or null);
+                                // rows of mask_wpr words, bit x % 32 of word x / 32
+    int mask_wpr;
 };
 
 // Index of (face, comp, x, y, z) in the side columns, or -1 when (x, y, z) is not a side column.
@@ -110,6 +113,7 @@ hipError_t launch_apply_sources(const LatticeArgs &a, const FieldPtrs &f, const 
                                 const int *lists, const float *packed, const float *signals, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
 hipError_t launch_copy4(const void *src, void *dst, size_t bytes, hipStream_t s);
+hipError_t launch_build_wire_mask(const LatticeArgs &a, uint32_t *mask, int wpr, hipStream_t s);
 hipError_t launch_build_pml_table(const LatticeArgs &a, float *tab, hipStream_t s);
 
 struct PortDev {
@@ -134,12 +138,24 @@ struct NtffFreqs {
     int n;
     float blend[kMaxNtffFreq];
 };
+// A port-monitor step (the arguments of launch_port_accumulate) carried by a far-field launch as
+// extra workgroups, so that both per-step gathers take one launch; n = 0: none.
+struct PortJob {
+    const PortDev *ports = nullptr;
+    int n = 0;
+    const float *rot = nullptr;
+    float blend = 0.0f;
+    float *phasors = nullptr;
+    float *series = nullptr;
+    int64_t rec = -1;
+};
 hipError_t launch_ntff_accumulate(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, const float *rot,
-                                  const NtffFreqs &freqs, float *phasors, hipStream_t s);
+                                  const NtffFreqs &freqs, float *phasors, const PortJob &pj, hipStream_t s);
 // deferred far-field accumulation: per-step gather into sample slot (6 planes of n_points), then
 // one pass applying `steps` updates (rotation rows rot_stride floats apart) to the phasors
 constexpr int kNtffRun = 8;
-hipError_t launch_ntff_sample(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, float *slot, hipStream_t s);
+hipError_t launch_ntff_sample(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, float *slot,
+                              const PortJob &pj, hipStream_t s);
 hipError_t launch_ntff_flush(const NtffBox &box, const float *rot, int rot_stride, int steps, const NtffFreqs &freqs,
                              const float *samples, float *phasors, hipStream_t s);
 hipError_t launch_ntff_pattern(const NtffBox &box, const float *phasors, const float *dirs, int n_dirs, float k,

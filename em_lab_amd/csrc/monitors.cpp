@@ -206,15 +206,26 @@ int monitors_after_step(emfd_ctx *ctx, const float *consts, int stride, int k, i
     const LatticeArgs lat = ctx->lattice();
     const FieldPtrs f = ctx->fields(ctx->cur);
     PortsState &P = ctx->ports;
+    NtffState &N = ctx->ntff;
+    PortJob pj;
     if (P.enabled()) {
         int64_t rec = -1;
         if (P.recording() && P.recorded < P.capacity_steps) rec = P.recorded++;
-        EMFD_HIP(launch_port_accumulate(lat, f, P.d_ports.as<PortDev>(), static_cast<int>(P.ports.size()), consts,
-                                        blend_of(P.averaging, P.wavelength, ctx->cfg.courant), P.d_phasors.as<float>(),
-                                        P.d_series.as<float>(), rec, ctx->stream));
+        pj.ports = P.d_ports.as<PortDev>();
+        pj.n = static_cast<int>(P.ports.size());
+        pj.rot = consts;
+        pj.blend = blend_of(P.averaging, P.wavelength, ctx->cfg.courant);
+        pj.phasors = P.d_phasors.as<float>();
+        pj.series = P.d_series.as<float>();
+        pj.rec = rec;
+        if (!N.enabled || N.box.n_points <= 0) {
+            EMFD_HIP(launch_port_accumulate(lat, f, pj.ports, pj.n, pj.rot, pj.blend, pj.phasors, pj.series, pj.rec,
+                                            ctx->stream));
+            pj = PortJob{};
+        }
     }
-    NtffState &N = ctx->ntff;
     if (N.enabled) {
+        // the port step, when there is one, rides along as extra workgroups of the far-field gather
         NtffFreqs fr{};
         fr.n = N.n_freq();
         for (int q = 0; q < fr.n; ++q)
@@ -224,10 +235,10 @@ int monitors_after_step(emfd_ctx *ctx, const float *consts, int stride, int k, i
         // so the phasors are current whenever a call returns
         const bool last = k + 1 == count;
         if (N.pending == 0 && (last || !N.d_samples.ptr)) {
-            EMFD_HIP(launch_ntff_accumulate(lat, f, N.box, consts + 4, fr, N.d_phasors.as<float>(), ctx->stream));
+            EMFD_HIP(launch_ntff_accumulate(lat, f, N.box, consts + 4, fr, N.d_phasors.as<float>(), pj, ctx->stream));
         } else {
             const size_t np = static_cast<size_t>(N.box.n_points);
-            EMFD_HIP(launch_ntff_sample(lat, f, N.box, N.d_samples.as<float>() + 6 * np * N.pending, ctx->stream));
+            EMFD_HIP(launch_ntff_sample(lat, f, N.box, N.d_samples.as<float>() + 6 * np * N.pending, pj, ctx->stream));
             ++N.pending;
             if (N.pending == kNtffRun || last) {
                 const float *rot0 = consts + 4 - static_cast<ptrdiff_t>(N.pending - 1) * stride;

@@ -92,12 +92,26 @@ __device__ __forceinline__ void ntff_update(const float *r, float blend, const f
 
 // The storage format is a template parameter so that the gathers' loads carry no per-load format
 // branch and can all be in flight together.
+// Workgroups past the far-field points run the port job: one thread per port.
+template <int S>
+__device__ __forceinline__ bool port_tail(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box,
+                                          const PortJob &pj) {
+    const int nb = (box.n_points + 255) / 256;
+    if (static_cast<int>(blockIdx.x) < nb) return false;
+    const int i = (static_cast<int>(blockIdx.x) - nb) * 256 + static_cast<int>(threadIdx.x);
+    if (i < pj.n)
+        port_sample(a, f, pj.ports[i], pj.rot, pj.blend, pj.phasors + 8 * i,
+                    (pj.rec >= 0 && pj.series) ? pj.series + 2 * (pj.rec * pj.n + i) : nullptr);
+    return true;
+}
+
 template <int S>
 __global__ void __launch_bounds__(256) k_ntff_accumulate(LatticeArgs a, FieldPtrs f, NtffBox box, const float *rot,
-                                                         NtffFreqs freqs, float *ph) {
+                                                         NtffFreqs freqs, float *ph, PortJob pj) {
+    f.storage = S;
+    if (port_tail<S>(a, f, box, pj)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= box.n_points) return;
-    f.storage = S;
     float E[3], H[3];
     ntff_fields(a, f, box, i, E, H);
     const size_t np = static_cast<size_t>(box.n_points);
@@ -118,10 +132,11 @@ __global__ void __launch_bounds__(256) k_ntff_accumulate(LatticeArgs a, FieldPtr
 // order as with one k_ntff_accumulate per step, so the result is bit-identical, while each phasor
 // plane is read and written once per run instead of once per step.
 template <int S>
-__global__ void __launch_bounds__(256) k_ntff_sample(LatticeArgs a, FieldPtrs f, NtffBox box, float *slot) {
+__global__ void __launch_bounds__(256) k_ntff_sample(LatticeArgs a, FieldPtrs f, NtffBox box, float *slot, PortJob pj) {
+    f.storage = S;
+    if (port_tail<S>(a, f, box, pj)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= box.n_points) return;
-    f.storage = S;
     float E[3], H[3];
     ntff_fields(a, f, box, i, E, H);
     const size_t np = static_cast<size_t>(box.n_points);
@@ -378,27 +393,28 @@ hipError_t launch_port_accumulate(const LatticeArgs &a, const FieldPtrs &f, cons
 }
 
 hipError_t launch_ntff_accumulate(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, const float *rot,
-                                  const NtffFreqs &freqs, float *phasors, hipStream_t s) {
-    if (box.n_points <= 0) return hipSuccess;
-    const int g = (box.n_points + 255) / 256;
+                                  const NtffFreqs &freqs, float *phasors, const PortJob &pj, hipStream_t s) {
+    const int g = (box.n_points + 255) / 256 + (pj.n + 255) / 256;
+    if (g <= 0) return hipSuccess;
     switch (f.storage) {
-        case kStorageF32: k_ntff_accumulate<kStorageF32><<<g, 256, 0, s>>>(a, f, box, rot, freqs, phasors); break;
-        case kStorageF16: k_ntff_accumulate<kStorageF16><<<g, 256, 0, s>>>(a, f, box, rot, freqs, phasors); break;
-        case kStorageBF16: k_ntff_accumulate<kStorageBF16><<<g, 256, 0, s>>>(a, f, box, rot, freqs, phasors); break;
-        case kStorageF64: k_ntff_accumulate<kStorageF64><<<g, 256, 0, s>>>(a, f, box, rot, freqs, phasors); break;
+        case kStorageF32: k_ntff_accumulate<kStorageF32><<<g, 256, 0, s>>>(a, f, box, rot, freqs, phasors, pj); break;
+        case kStorageF16: k_ntff_accumulate<kStorageF16><<<g, 256, 0, s>>>(a, f, box, rot, freqs, phasors, pj); break;
+        case kStorageBF16: k_ntff_accumulate<kStorageBF16><<<g, 256, 0, s>>>(a, f, box, rot, freqs, phasors, pj); break;
+        case kStorageF64: k_ntff_accumulate<kStorageF64><<<g, 256, 0, s>>>(a, f, box, rot, freqs, phasors, pj); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_ntff_sample(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, float *slot, hipStream_t s) {
-    if (box.n_points <= 0) return hipSuccess;
-    const int g = (box.n_points + 255) / 256;
+hipError_t launch_ntff_sample(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, float *slot,
+                              const PortJob &pj, hipStream_t s) {
+    const int g = (box.n_points + 255) / 256 + (pj.n + 255) / 256;
+    if (g <= 0) return hipSuccess;
     switch (f.storage) {
-        case kStorageF32: k_ntff_sample<kStorageF32><<<g, 256, 0, s>>>(a, f, box, slot); break;
-        case kStorageF16: k_ntff_sample<kStorageF16><<<g, 256, 0, s>>>(a, f, box, slot); break;
-        case kStorageBF16: k_ntff_sample<kStorageBF16><<<g, 256, 0, s>>>(a, f, box, slot); break;
-        case kStorageF64: k_ntff_sample<kStorageF64><<<g, 256, 0, s>>>(a, f, box, slot); break;
+        case kStorageF32: k_ntff_sample<kStorageF32><<<g, 256, 0, s>>>(a, f, box, slot, pj); break;
+        case kStorageF16: k_ntff_sample<kStorageF16><<<g, 256, 0, s>>>(a, f, box, slot, pj); break;
+        case kStorageBF16: k_ntff_sample<kStorageBF16><<<g, 256, 0, s>>>(a, f, box, slot, pj); break;
+        case kStorageF64: k_ntff_sample<kStorageF64><<<g, 256, 0, s>>>(a, f, box, slot, pj); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -88,8 +88,17 @@ __device__ __forceinline__ void b_quad(const LatticeArgs &a, const FieldPtrs &f,
         if (up_z) unpack(ld4(e + p + sz), eZ[c]); else eZ[c][0] = eZ[c][1] = eZ[c][2] = eZ[c][3] = 0.0f;
         unpack(ST >= 2 ? ld4s(f.b[c] + p) : ld4(f.b[c] + p), b[c]);
     }
-    uint32_t id[4];
-    load_ids4<(ST >= 3)>(a, p, id);
+    // the Faraday update reads a class only for its thin-wire factors: no class stream at all
+    // without wires, and with a wire mask only the quads that hold a wire texel load theirs
+    uint32_t id[4] = {0u, 0u, 0u, 0u};
+    if (a.has_wire) {
+        if (a.wire_mask) {
+            const uint32_t w = a.wire_mask[(static_cast<size_t>(z) * a.ny + y) * a.mask_wpr + (x0 >> 5)];
+            if ((w >> (x0 & 31)) & 0xfu) load_ids4<(ST >= 3)>(a, p, id);
+        } else {
+            load_ids4<(ST >= 3)>(a, p, id);
+        }
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const float c0[3] = {e0[0][j], e0[1][j], e0[2][j]};
