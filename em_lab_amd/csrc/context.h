@@ -87,8 +87,8 @@ struct emfd_ctx {
     }
     emfd::DevBuf sigma_cell, eps_cell;  // EM_Lab's alpha channels (probes)
     emfd::DevBuf mat_id, table, prof, pml_tab;
-    emfd::DevBuf wire_mask;  // LatticeArgs::wire_mask, built with the class table in thin-wire scenes
-    bool wire_mask_ok = false;
+    emfd::DevBuf class_masks;  // LatticeArgs::wire_mask then ::class_mask, built with the class table
+    bool masks_ok = false;
     int id_bytes = 0;
     int classes = 0;
     int has_wire = 0;

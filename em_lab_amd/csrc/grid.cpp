@@ -302,9 +302,11 @@ emfd::LatticeArgs emfd_ctx::lattice() const {
     a.has_wire = has_wire;
     a.pml_tab = pml_tab.as<float>();
     a.pml_k = 2 * cfg.pml_thickness + 2;
-    if (has_wire && classes > 0 && wire_mask_ok) {
-        a.wire_mask = wire_mask.as<uint32_t>();
-        a.mask_wpr = (cfg.size[0] + 31) / 32;
+    if (classes > 0 && masks_ok) {
+        const int wpr = (cfg.size[0] + 31) / 32;
+        a.mask_wpr = wpr;
+        a.wire_mask = class_masks.as<uint32_t>();
+        a.class_mask = a.wire_mask + static_cast<size_t>(cfg.size[1]) * cfg.size[2] * wpr;
     }
     if (side_active && ntff.enabled && ntff.d_side.ptr) {
         a.side = ntff.side;
@@ -399,7 +401,7 @@ int emfd_grid_configure(emfd_ctx *ctx, const emfd_grid_config *requested, emfd_g
     ctx->classes = 0;
     ctx->has_wire = 0;
     ctx->id_bytes = 0;
-    ctx->wire_mask_ok = false;
+    ctx->masks_ok = false;
     EMFD_HIP(hipMemsetAsync(ctx->field_buf[0].ptr, 0, ctx->field_bytes(), ctx->stream));
     EMFD_HIP(hipMemsetAsync(ctx->sigma_cell.ptr, 0, n * sizeof(float), ctx->stream));
     EMFD_HIP(launch_fill(ctx->eps_cell.as<float>(), n, 1.0f, ctx->stream));
@@ -439,7 +441,7 @@ int emfd_grid_upload_materials(emfd_ctx *ctx, const float *sigma, const float *e
     const std::vector<ClassKey> &keys = cm.keys;
 
     ctx->has_wire = thin ? 1 : 0;
-    ctx->wire_mask_ok = false;
+    ctx->masks_ok = false;
     if (keys.size() == 1 && !thin) {
         ctx->classes = 0;  // uniform vacuum: kernels skip the class id stream
         ctx->id_bytes = 0;
@@ -460,12 +462,13 @@ int emfd_grid_upload_materials(emfd_ctx *ctx, const float *sigma, const float *e
         EMFD_HIP(hipMemcpy(d_keys.ptr, keys.data(), keys.size() * sizeof(ClassKey), hipMemcpyHostToDevice));
         EMFD_HIP(launch_build_table(d_keys.as<uint16_t>(), ctx->classes, ctx->cfg.courant, ctx->table.as<float>(),
                                     ctx->stream));
-        if (thin) {
+        {
             const int wpr = (ctx->cfg.size[0] + 31) / 32;
             const size_t words = static_cast<size_t>(ctx->cfg.size[1]) * ctx->cfg.size[2] * wpr;
-            if (int rc = ctx->wire_mask.ensure(words * sizeof(uint32_t))) return rc;
-            EMFD_HIP(launch_build_wire_mask(ctx->lattice(), ctx->wire_mask.as<uint32_t>(), wpr, ctx->stream));
-            ctx->wire_mask_ok = true;
+            if (int rc = ctx->class_masks.ensure(2 * words * sizeof(uint32_t))) return rc;
+            EMFD_HIP(launch_build_class_masks(ctx->lattice(), ctx->class_masks.as<uint32_t>(),
+                                              ctx->class_masks.as<uint32_t>() + words, wpr, ctx->stream));
+            ctx->masks_ok = true;
         }
         EMFD_HIP(hipStreamSynchronize(ctx->stream));
         d_keys.release();

@@ -51,6 +51,7 @@ struct LatticeArgs {
     SideCols side;              // far-field x-face columns (buf null when not maintained)
     const uint32_t *wire_mask;  // 1 bit per texel: its class has a thin-wire factor != 1 (or null);
                                 // rows of mask_wpr words, bit x % 32 of word x / 32
+    const uint32_t *class_mask; // same layout: the texel's class is not 0 (vacuum), or null
     int mask_wpr;
 };
 
@@ -113,7 +114,8 @@ hipError_t launch_apply_sources(const LatticeArgs &a, const FieldPtrs &f, const 
                                 const int *lists, const float *packed, const float *signals, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
 hipError_t launch_copy4(const void *src, void *dst, size_t bytes, hipStream_t s);
-hipError_t launch_build_wire_mask(const LatticeArgs &a, uint32_t *mask, int wpr, hipStream_t s);
+hipError_t launch_build_class_masks(const LatticeArgs &a, uint32_t *wire_mask, uint32_t *class_mask, int wpr,
+                                   hipStream_t s);
 hipError_t launch_build_pml_table(const LatticeArgs &a, float *tab, hipStream_t s);
 
 struct PortDev {

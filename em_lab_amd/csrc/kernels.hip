@@ -383,25 +383,29 @@ __global__ void __launch_bounds__(TX *TY) k_yee_fused(LatticeArgs a, FieldPtrs s
     }
 }
 
-// Thin-wire mask: bit x % 32 of word (z ny + y) wpr + x / 32 is set when the class of texel (x, y, z)
-// carries a wire factor other than 1. Only those texels need their class in the Faraday update
-// (a factor of exactly 1 leaves every product unchanged).
-__global__ void __launch_bounds__(256) k_build_wire_mask(LatticeArgs a, uint32_t *mask, int wpr) {
+// Per-texel class masks, bit x % 32 of word (z ny + y) wpr + x / 32: wire_mask marks texels whose
+// class carries a thin-wire factor other than 1 (only those need their class in the Faraday
+// update: a factor of exactly 1 leaves every product unchanged); class_mask marks texels whose
+// class is not 0 (the Ampere update of a class-0 texel needs no class record).
+__global__ void __launch_bounds__(256) k_build_class_masks(LatticeArgs a, uint32_t *wire_mask, uint32_t *class_mask,
+                                                           int wpr) {
     const size_t w = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const size_t rows = static_cast<size_t>(a.ny) * a.nz;
     if (w >= rows * wpr) return;
     const size_t row = w / wpr;
     const int x0 = static_cast<int>(w % wpr) * 32;
-    uint32_t bits = 0u;
+    uint32_t wires = 0u, classes = 0u;
     for (int b = 0; b < 32 && x0 + b < a.nx; ++b) {
         const uint32_t id = load_id(a, row * a.nx + x0 + b);
         if (id == 0u) continue;
+        classes |= 1u << b;
         const float *t = a.table + static_cast<size_t>(id) * kTableStride + kWire;
         bool wire = false;
         for (int k = 0; k < 6; ++k) wire = wire || t[k] != 1.0f;
-        if (wire) bits |= 1u << b;
+        if (wire) wires |= 1u << b;
     }
-    mask[w] = bits;
+    wire_mask[w] = wires;
+    class_mask[w] = classes;
 }
 
 __global__ void __launch_bounds__(64) k_apply_sources(LatticeArgs a, FieldPtrs f, const int *cells, int n_cells,
@@ -422,10 +426,11 @@ hipError_t launch_build_table(const uint16_t *keys, int classes, float dt, float
     return hipGetLastError();
 }
 
-hipError_t launch_build_wire_mask(const LatticeArgs &a, uint32_t *mask, int wpr, hipStream_t s) {
+hipError_t launch_build_class_masks(const LatticeArgs &a, uint32_t *wire_mask, uint32_t *class_mask, int wpr,
+                                   hipStream_t s) {
     const size_t words = static_cast<size_t>(a.ny) * a.nz * wpr;
     if (words == 0 || !a.mat_id) return hipErrorInvalidValue;
-    k_build_wire_mask<<<static_cast<unsigned>((words + 255) / 256), 256, 0, s>>>(a, mask, wpr);
+    k_build_class_masks<<<static_cast<unsigned>((words + 255) / 256), 256, 0, s>>>(a, wire_mask, class_mask, wpr);
     return hipGetLastError();
 }
 

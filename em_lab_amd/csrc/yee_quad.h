@@ -136,8 +136,14 @@ __device__ __forceinline__ void e_quad(const LatticeArgs &a, const FieldPtrs &f,
         if (dn_z) unpack(ld4(bp + p - sz), bZ[c]); else bZ[c][0] = bZ[c][1] = bZ[c][2] = bZ[c][3] = 0.0f;
         unpack(ST >= 2 ? ld4s(f.e[c] + p) : ld4(f.e[c] + p), e[c]);
     }
-    uint32_t id[4];
-    load_ids4<(ST >= 3)>(a, p, id);
+    // class records only where the quad holds a texel of class != 0 (the class mask says which)
+    uint32_t id[4] = {0u, 0u, 0u, 0u};
+    if (a.class_mask) {
+        const uint32_t w = a.class_mask[(static_cast<size_t>(z) * a.ny + y) * a.mask_wpr + (x0 >> 5)];
+        if ((w >> (x0 & 31)) & 0xfu) load_ids4<(ST >= 3)>(a, p, id);
+    } else {
+        load_ids4<(ST >= 3)>(a, p, id);
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const float bxm[3] = {j > 0 ? b0[0][j - 1] : bX[0], j > 0 ? b0[1][j - 1] : bX[1], j > 0 ? b0[2][j - 1] : bX[2]};
