@@ -29,6 +29,7 @@ struct SceneHost {
     size_t n = 0;
     DevBuf fields;  // E x y z, B x y z planes
     DevBuf ids, table, prof;
+    DevBuf masks;  // thin-wire mask then class-nonzero mask (LatticeArgs), when there are classes
     int prof_stride = 0;
     int id_bytes = 0, classes = 0, has_wire = 0;
     int pml_table = 0;  // index into emfd_batch::pml_tabs
@@ -48,7 +49,8 @@ struct SceneHost {
     double residual = 1.0;
 
     void release() {
-        for (DevBuf *b : {&fields, &ids, &table, &prof, &d_packed, &d_cells, &d_lists, &d_ports, &d_phasors, &d_series})
+        for (DevBuf *b : {&fields, &ids, &table, &prof, &masks, &d_packed, &d_cells, &d_lists, &d_ports, &d_phasors,
+                          &d_series})
             b->release();
     }
     LatticeArgs lattice(const float *pml_tab) const {
@@ -66,6 +68,11 @@ struct SceneHost {
         a.has_wire = has_wire;
         a.pml_tab = pml_tab;
         a.pml_k = 2 * cfg.pml_thickness + 2;
+        if (classes > 0 && masks.ptr) {
+            a.mask_wpr = (cfg.size[0] + 31) / 32;
+            a.wire_mask = masks.as<uint32_t>();
+            a.class_mask = a.wire_mask + static_cast<size_t>(cfg.size[1]) * cfg.size[2] * a.mask_wpr;
+        }
         return a;
     }
     FieldPtrs field_ptrs() const {
@@ -285,6 +292,11 @@ int emfd_batch_add_scene(emfd_batch *b, const emfd_batch_scene *sc, int *index, 
         if (int rc = s->table.ensure(cm.keys.size() * kTableStride * sizeof(float))) return rc;
         EMFD_HIP(hipMemcpy(d_keys.ptr, cm.keys.data(), cm.keys.size() * sizeof(ClassKey), hipMemcpyHostToDevice));
         EMFD_HIP(launch_build_table(d_keys.as<uint16_t>(), s->classes, s->cfg.courant, s->table.as<float>(), b->stream));
+        const int wpr = (s->cfg.size[0] + 31) / 32;
+        const size_t words = static_cast<size_t>(s->cfg.size[1]) * s->cfg.size[2] * wpr;
+        if (int rc = s->masks.ensure(2 * words * sizeof(uint32_t))) return rc;
+        EMFD_HIP(launch_build_class_masks(s->lattice(nullptr), s->masks.as<uint32_t>(), s->masks.as<uint32_t>() + words,
+                                          wpr, b->stream));
         EMFD_HIP(hipStreamSynchronize(b->stream));
         d_keys.release();
     }
