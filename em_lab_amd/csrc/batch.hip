@@ -12,6 +12,29 @@
 namespace emfd {
 namespace {
 
+// The tile list and the scene descriptors are written by the host before a launch and only read
+// by the kernels: reading them through the constant address space lets the compiler use scalar
+// loads (the tile index is uniform across the workgroup), so a tile's descriptor arrives in SGPRs
+// in one short scalar round trip. Read through generic pointers, each tile cost three dependent
+// vector-memory round trips before its first field load.
+template <class T>
+using ConstPtr = const __attribute__((address_space(4))) T *;
+template <class T>
+__device__ __forceinline__ ConstPtr<T> as_const(const T *p) {
+    return reinterpret_cast<ConstPtr<T>>(reinterpret_cast<uintptr_t>(p));
+}
+// a copy of *p read dword by dword through the constant address space
+template <class T>
+__device__ __forceinline__ T load_const(ConstPtr<T> p) {
+    static_assert(sizeof(T) % 4 == 0, "dword-sized descriptor");
+    T v;
+    const ConstPtr<uint32_t> src = reinterpret_cast<ConstPtr<uint32_t>>(p);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&v);
+#pragma unroll
+    for (size_t i = 0; i < sizeof(T) / 4; ++i) dst[i] = src[i];
+    return v;
+}
+
 template <bool kB, int Q>
 __global__ void __launch_bounds__(256) k_batch_yee(BatchLaunchArgs args) {
     // tile t of the list is handled by the workgroups of slice t / per; with 8 slices the
@@ -24,9 +47,11 @@ __global__ void __launch_bounds__(256) k_batch_yee(BatchLaunchArgs args) {
     const int per = (args.n_tiles + slices - 1) / slices;
     const int t0 = slice * per;
     const int t1 = min(t0 + per, args.n_tiles);
+    const ConstPtr<int2> tiles = as_const(args.tiles);
+    const ConstPtr<BatchScene> scenes = as_const(args.scenes);
     for (int t = t0 + lb; t < t1; t += nb) {
-        const int2 tl = args.tiles[t];
-        const BatchScene &sc = args.scenes[tl.x];
+        const int2 tl = load_const(tiles + t);
+        const BatchScene sc = load_const(scenes + tl.x);
 #pragma unroll
         for (int j = 0; j < Q; ++j) {
             const uint32_t q = static_cast<uint32_t>(tl.y) + static_cast<uint32_t>(j * 256) + threadIdx.x;
