@@ -15,24 +15,51 @@ namespace {
 // The tile list and the scene descriptors are written by the host before a launch and only read
 // by the kernels: reading them through the constant address space lets the compiler use scalar
 // loads (the tile index is uniform across the workgroup), so a tile's descriptor arrives in SGPRs
-// in one short scalar round trip. Read through generic pointers, each tile cost three dependent
-// vector-memory round trips before its first field load.
+// in one short scalar round trip; and pointers read from constant memory are taken to address
+// global memory, so the field and table accesses through them are global (not flat) loads and
+// stores. Read through generic pointers, each tile cost three dependent vector-memory round trips
+// before its first field load.
 template <class T>
 using ConstPtr = const __attribute__((address_space(4))) T *;
 template <class T>
 __device__ __forceinline__ ConstPtr<T> as_const(const T *p) {
     return reinterpret_cast<ConstPtr<T>>(reinterpret_cast<uintptr_t>(p));
 }
-// a copy of *p read dword by dword through the constant address space
-template <class T>
-__device__ __forceinline__ T load_const(ConstPtr<T> p) {
-    static_assert(sizeof(T) % 4 == 0, "dword-sized descriptor");
-    T v;
-    const ConstPtr<uint32_t> src = reinterpret_cast<ConstPtr<uint32_t>>(p);
-    uint32_t *dst = reinterpret_cast<uint32_t *>(&v);
+
+// Member-wise copies (a struct copy cannot bind to a constant-address-space reference). Batched
+// scenes keep no far-field side columns.
+__device__ __forceinline__ LatticeArgs lattice_from(ConstPtr<LatticeArgs> p) {
+    LatticeArgs a;
+    a.nx = p->nx;
+    a.ny = p->ny;
+    a.nz = p->nz;
+    a.pml = p->pml;
+    a.dt = p->dt;
+    a.prof_stride = p->prof_stride;
+    a.prof = p->prof;
+    a.mat_id = p->mat_id;
+    a.id_bytes = p->id_bytes;
+    a.table = p->table;
+    a.has_wire = p->has_wire;
+    a.pml_tab = p->pml_tab;
+    a.pml_k = p->pml_k;
+    a.side = SideCols{};
+    a.wire_mask = p->wire_mask;
+    a.class_mask = p->class_mask;
+    a.mask_wpr = p->mask_wpr;
+    return a;
+}
+__device__ __forceinline__ FieldPtrs fields_from(ConstPtr<FieldPtrs> p) {
+    FieldPtrs f;
 #pragma unroll
-    for (size_t i = 0; i < sizeof(T) / 4; ++i) dst[i] = src[i];
-    return v;
+    for (int c = 0; c < 3; ++c) {
+        f.e[c] = p->e[c];
+        f.b[c] = p->b[c];
+    }
+    f.storage = p->storage;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) f.scale[c] = p->scale[c];
+    return f;
 }
 
 template <bool kB, int Q>
@@ -50,20 +77,23 @@ __global__ void __launch_bounds__(256) k_batch_yee(BatchLaunchArgs args) {
     const ConstPtr<int2> tiles = as_const(args.tiles);
     const ConstPtr<BatchScene> scenes = as_const(args.scenes);
     for (int t = t0 + lb; t < t1; t += nb) {
-        const int2 tl = load_const(tiles + t);
-        const BatchScene sc = load_const(scenes + tl.x);
+        const int scene = tiles[t].x, first = tiles[t].y;
+        const ConstPtr<BatchScene> sp = scenes + scene;
+        const LatticeArgs lat = lattice_from(&sp->lat);
+        const FieldPtrs f = fields_from(&sp->f);
+        const uint32_t nq = sp->nq, qx = sp->qx, mq = sp->mq, my = sp->my;
 #pragma unroll
         for (int j = 0; j < Q; ++j) {
-            const uint32_t q = static_cast<uint32_t>(tl.y) + static_cast<uint32_t>(j * 256) + threadIdx.x;
-            if (q < sc.nq) {
-                const uint32_t r = __umulhi(q, sc.mq);
-                const uint32_t x0 = 4u * (q - r * sc.qx);
-                const uint32_t z = __umulhi(r, sc.my);
-                const uint32_t y = r - z * static_cast<uint32_t>(sc.lat.ny);
+            const uint32_t q = static_cast<uint32_t>(first) + static_cast<uint32_t>(j * 256) + threadIdx.x;
+            if (q < nq) {
+                const uint32_t r = __umulhi(q, mq);
+                const uint32_t x0 = 4u * (q - r * qx);
+                const uint32_t z = __umulhi(r, my);
+                const uint32_t y = r - z * static_cast<uint32_t>(lat.ny);
                 if (kB)
-                    b_quad<2>(sc.lat, sc.f, static_cast<int>(x0), static_cast<int>(y), static_cast<int>(z));
+                    b_quad<2>(lat, f, static_cast<int>(x0), static_cast<int>(y), static_cast<int>(z));
                 else
-                    e_quad<2>(sc.lat, sc.f, static_cast<int>(x0), static_cast<int>(y), static_cast<int>(z));
+                    e_quad<2>(lat, f, static_cast<int>(x0), static_cast<int>(y), static_cast<int>(z));
             }
         }
     }
