@@ -15,6 +15,14 @@ __device__ __forceinline__ float field_load(const FieldPtrs &f, int comp, size_t
     return static_cast<float>(static_cast<const __bf16 *>(plane)[i]) * f.scale[comp];
 }
 
+// The value field_load returns after field_store(f, comp, i, v): v rounded through the storage
+// format (binary16 / bfloat16 of v / scale, times scale; FP32 and FP64 hold v itself).
+__device__ __forceinline__ float storage_value(const FieldPtrs &f, int comp, float v) {
+    if (f.storage == kStorageF16) return static_cast<float>(static_cast<_Float16>(v / f.scale[comp])) * f.scale[comp];
+    if (f.storage == kStorageBF16) return static_cast<float>(static_cast<__bf16>(v / f.scale[comp])) * f.scale[comp];
+    return v;
+}
+
 __device__ __forceinline__ void field_store(const FieldPtrs &f, int comp, size_t i, float v) {
     void *plane = comp < 3 ? static_cast<void *>(f.e[comp]) : static_cast<void *>(f.b[comp - 3]);
     if (f.storage == kStorageF32) {

@@ -48,7 +48,7 @@ __device__ __forceinline__ void source_cell(const LatticeArgs &a, const FieldPtr
         if (a.side.buf)
             for (int face = 0; face < 2; ++face) {
                 const long k = side_index(a.side, face, c, x, y, z);
-                if (k >= 0) a.side.buf[k] = v;
+                if (k >= 0) a.side.buf[k] = storage_value(f, c, v);  // what the planes now hold
             }
     }
 }

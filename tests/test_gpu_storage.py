@@ -30,6 +30,32 @@ def test_hertzian_storage_bit_exact(storage):
     g.close()
 
 
+@pytest.mark.parametrize("storage", [E.STORAGE_F32, E.STORAGE_F16, E.STORAGE_BF16])
+def test_far_field_box_through_source(storage):
+    """A far-field box whose upper x face (x = 32) runs through the Hertzian source at the centre
+    of a 64^3 grid: the x-face side columns the Yee and source passes keep must hold exactly what
+    the planes hold (the stored, rounded values for 16-bit storage), so the phasors stay equal to
+    the oracle's bit for bit."""
+    print("case: far-field box through the source, storage", storage, flush=True)
+    sc = hertzian_vacuum(64, 8)
+    ff0 = sc.far_field
+    sc.far_field = E.FarFieldSetup((14, 13, 12), (32, 50, 49), ff0.wavelength, ff0.averaging_periods, ff0.map_axis)
+    k = [2, 2, 2, -3, -3, -3]
+    g, _, ff = gpu_grid(sc)
+    sim = oracle_sim(sc, gpu_ntff_setup=ff.setup)
+    if storage != E.STORAGE_F32:
+        g.set_storage(storage, k)
+        sim.set_storage(storage, k)
+    g.step(120)
+    sim.step(120)
+    Eg, Bg = g.fields(E.LAYOUT_TEXEL)
+    assert np.array_equal(Eg[:, :3], sim.E[:, :3]) and np.array_equal(Bg[:, :3], sim.B[:, :3])
+    ph = ff.phasors()
+    assert np.abs(ph).max() > 0
+    assert np.array_equal(ph, sim.ntff["phasors"])
+    g.close()
+
+
 def test_spheres_dipole_f16_ports_bit_exact():
     print("case: spheres+dipole 48^3 f16 storage with ports", flush=True)
     sc = spheres_dipole(n=48, pml=8, n_spheres=4, radius_range=(3.0, 6.0), sweep_points=16)
