@@ -32,6 +32,7 @@ struct SceneHost {
     DevBuf masks;  // thin-wire mask then class-nonzero mask (LatticeArgs), when there are classes
     int prof_stride = 0;
     int id_bytes = 0, classes = 0, has_wire = 0;
+    int pml_materials = 0;  // LatticeArgs::pml_materials
     int pml_table = 0;  // index into emfd_batch::pml_tabs
     std::vector<emfd_source_desc> sources;
     int n_src_cells = 0;
@@ -68,6 +69,7 @@ struct SceneHost {
         a.has_wire = has_wire;
         a.pml_tab = pml_tab;
         a.pml_k = 2 * cfg.pml_thickness + 2;
+        a.pml_materials = classes > 0 ? pml_materials : 0;
         if (classes > 0 && masks.ptr) {
             a.mask_wpr = (cfg.size[0] + 31) / 32;
             a.wire_mask = masks.as<uint32_t>();
@@ -137,6 +139,7 @@ struct emfd_batch {
     }
     int build_tiles();
     int enqueue_steps(int count);
+    BatchLaunchArgs launch_args(const emfd_batch_launch &l) const;
     int collect_timing();
 };
 
@@ -157,13 +160,20 @@ int emfd_batch::build_tiles() {
 
 // Launches `count` steps (B, E, sources, ports per step) then advances the device step counter.
 // Everything here is graph-capturable: kernel launches only.
-int emfd_batch::enqueue_steps(int count) {
+BatchLaunchArgs emfd_batch::launch_args(const emfd_batch_launch &l) const {
     BatchLaunchArgs la{};
     la.scenes = d_scenes.as<BatchScene>();
     la.tiles = d_tiles.as<int2>();
     la.n_tiles = n_tiles;
-    la.slices = launch.xcd_slices;
-    la.quads = launch.quads_per_thread;
+    la.slices = l.xcd_slices;
+    la.quads = l.quads_per_thread;
+    la.pml_materials = 0;
+    for (const auto &sc : scenes) la.pml_materials |= sc->classes > 0 ? sc->pml_materials : 0;
+    return la;
+}
+
+int emfd_batch::enqueue_steps(int count) {
+    const BatchLaunchArgs la = launch_args(launch);
     const float *consts = d_consts.as<float>();
     int64_t *ds = d_step.as<int64_t>();
     for (int k = 0; k < count; ++k) {
@@ -276,6 +286,7 @@ int emfd_batch_add_scene(emfd_batch *b, const emfd_batch_scene *sc, int *index, 
     std::vector<uint32_t> ids(s->n);
     classify_cells(s->cfg, sc->sigma, sc->eps_r, sc->edges, sc->n_edges, sc->wires, sc->n_wires, cm, ids.data());
     s->has_wire = thin ? 1 : 0;
+    s->pml_materials = absorber_has_materials(s->cfg, ids.data()) ? 1 : 0;
     if (!(cm.keys.size() == 1 && !thin)) {
         s->classes = static_cast<int>(cm.keys.size());
         s->id_bytes = cm.keys.size() <= 65536 ? 2 : 4;
@@ -486,8 +497,7 @@ int emfd_batch_autotune(emfd_batch *b, int iterations, emfd_batch_launch *chosen
     int tried = 0;
     for (const emfd_batch_launch &l : cands) {
         if (int rc = emfd_batch_set_launch(b, &l)) return rc;
-        BatchLaunchArgs la{b->d_scenes.as<BatchScene>(), b->d_tiles.as<int2>(), b->n_tiles, l.xcd_slices,
-                           l.quads_per_thread};
+        const BatchLaunchArgs la = b->launch_args(l);
         EMFD_HIP(launch_batch_yee(la, l.persistent_blocks, b->stream));  // warm-up
         EMFD_HIP(hipEventRecord(e0, b->stream));
         for (int i = 0; i < iterations; ++i) EMFD_HIP(launch_batch_yee(la, l.persistent_blocks, b->stream));
@@ -579,8 +589,7 @@ int emfd_batch_step(emfd_batch *b, int count) {
     }
     if (count > done) {
         // direct launches; with timing on, each step's two Yee passes are bracketed
-        BatchLaunchArgs la{b->d_scenes.as<BatchScene>(), b->d_tiles.as<int2>(), b->n_tiles, b->launch.xcd_slices,
-                           b->launch.quads_per_thread};
+        const BatchLaunchArgs la = b->launch_args(b->launch);
         const float *consts = b->d_consts.as<float>();
         int64_t *ds = b->d_step.as<int64_t>();
         const int rest = count - done;

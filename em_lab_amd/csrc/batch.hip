@@ -62,7 +62,7 @@ __device__ __forceinline__ FieldPtrs fields_from(ConstPtr<FieldPtrs> p) {
     return f;
 }
 
-template <bool kB, int Q>
+template <bool kB, int Q, bool PM>
 __global__ void __launch_bounds__(256) k_batch_yee(BatchLaunchArgs args) {
     // tile t of the list is handled by the workgroups of slice t / per; with 8 slices the
     // hardware's round-robin of workgroups over the 8 XCDs keeps each slice's tiles (and their
@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(256) k_batch_yee(BatchLaunchArgs args) {
                 if (kB)
                     b_quad<2>(lat, f, static_cast<int>(x0), static_cast<int>(y), static_cast<int>(z));
                 else
-                    e_quad<2>(lat, f, static_cast<int>(x0), static_cast<int>(y), static_cast<int>(z));
+                    e_quad<2, PM>(lat, f, static_cast<int>(x0), static_cast<int>(y), static_cast<int>(z));
             }
         }
     }
@@ -180,15 +180,22 @@ __global__ void __launch_bounds__(64) k_batch_sweep(const BatchSweepJob *jobs) {
     }
 }
 
-template <bool kB>
-hipError_t launch_pass(const BatchLaunchArgs &a, int grid, hipStream_t s) {
+template <bool kB, bool PM>
+hipError_t launch_pass_pm(const BatchLaunchArgs &a, int grid, hipStream_t s) {
     switch (a.quads) {
-        case 1: k_batch_yee<kB, 1><<<grid, 256, 0, s>>>(a); break;
-        case 2: k_batch_yee<kB, 2><<<grid, 256, 0, s>>>(a); break;
-        case 4: k_batch_yee<kB, 4><<<grid, 256, 0, s>>>(a); break;
+        case 1: k_batch_yee<kB, 1, PM><<<grid, 256, 0, s>>>(a); break;
+        case 2: k_batch_yee<kB, 2, PM><<<grid, 256, 0, s>>>(a); break;
+        case 4: k_batch_yee<kB, 4, PM><<<grid, 256, 0, s>>>(a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// the B pass has no absorber-material case: one instance serves both
+template <bool kB>
+hipError_t launch_pass(const BatchLaunchArgs &a, int grid, hipStream_t s) {
+    if (kB || !a.pml_materials) return launch_pass_pm<kB, false>(a, grid, s);
+    return launch_pass_pm<kB, true>(a, grid, s);
 }
 
 }  // namespace

@@ -92,6 +92,7 @@ struct emfd_ctx {
     int id_bytes = 0;
     int classes = 0;
     int has_wire = 0;
+    int pml_materials = 1;  // LatticeArgs::pml_materials
     int prof_stride = 0;
 
     // sources

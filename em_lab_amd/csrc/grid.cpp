@@ -107,6 +107,19 @@ ClassMap::ClassMap() {
     keys.push_back(vacuum());
 }
 
+bool absorber_has_materials(const emfd_grid_config &c, const uint32_t *ids) {
+    const int nx = c.size[0], ny = c.size[1], nz = c.size[2], L = c.pml_thickness;
+    const auto edge = [L](int i, int n) { return i < L || i >= n - (L + 1); };
+    size_t p = 0;
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y) {
+            const bool yz = edge(y, ny) || edge(z, nz);
+            for (int x = 0; x < nx; ++x, ++p)
+                if (ids[p] != 0u && (yz || edge(x, nx))) return true;
+        }
+    return false;
+}
+
 void classify_cells(const emfd_grid_config &cfg, const float *sigma, const float *eps_r,
                     const emfd_edge_override *edges, int64_t n_edges, const emfd_wire_coefficient *wires,
                     int64_t n_wires, ClassMap &cm, uint32_t *ids) {
@@ -302,6 +315,7 @@ emfd::LatticeArgs emfd_ctx::lattice() const {
     a.has_wire = has_wire;
     a.pml_tab = pml_tab.as<float>();
     a.pml_k = 2 * cfg.pml_thickness + 2;
+    a.pml_materials = classes > 0 ? pml_materials : 0;
     if (classes > 0 && masks_ok) {
         const int wpr = (cfg.size[0] + 31) / 32;
         a.mask_wpr = wpr;
@@ -402,6 +416,7 @@ int emfd_grid_configure(emfd_ctx *ctx, const emfd_grid_config *requested, emfd_g
     ctx->has_wire = 0;
     ctx->id_bytes = 0;
     ctx->masks_ok = false;
+    ctx->pml_materials = 0;  // vacuum everywhere
     EMFD_HIP(hipMemsetAsync(ctx->field_buf[0].ptr, 0, ctx->field_bytes(), ctx->stream));
     EMFD_HIP(hipMemsetAsync(ctx->sigma_cell.ptr, 0, n * sizeof(float), ctx->stream));
     EMFD_HIP(launch_fill(ctx->eps_cell.as<float>(), n, 1.0f, ctx->stream));
@@ -442,6 +457,7 @@ int emfd_grid_upload_materials(emfd_ctx *ctx, const float *sigma, const float *e
 
     ctx->has_wire = thin ? 1 : 0;
     ctx->masks_ok = false;
+    ctx->pml_materials = absorber_has_materials(ctx->cfg, ids.data()) ? 1 : 0;
     if (keys.size() == 1 && !thin) {
         ctx->classes = 0;  // uniform vacuum: kernels skip the class id stream
         ctx->id_bytes = 0;

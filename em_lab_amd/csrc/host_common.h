@@ -45,6 +45,10 @@ void classify_cells(const emfd_grid_config &c, const float *sigma, const float *
                     const emfd_edge_override *edges, int64_t n_edges, const emfd_wire_coefficient *wires,
                     int64_t n_wires, ClassMap &cm, uint32_t *ids);
 
+// Whether some texel of class != 0 lies in the absorber (the texels in_pml() marks): the E pass
+// then needs its general coefficient path.
+bool absorber_has_materials(const emfd_grid_config &c, const uint32_t *ids);
+
 // setSources (src/SimulationGrid.cpp:297-357): packed 16 floats per kept source, per-texel lists
 struct PackedSources {
     std::vector<float> packed;

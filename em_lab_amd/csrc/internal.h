@@ -53,6 +53,7 @@ struct LatticeArgs {
                                 // rows of mask_wpr words, bit x % 32 of word x / 32
     const uint32_t *class_mask; // same layout: the texel's class is not 0 (vacuum), or null
     int mask_wpr;
+    int pml_materials;          // some texel of class != 0 lies in the absorber (0 only when known)
 };
 
 // Index of (face, comp, x, y, z) in the side columns, or -1 when (x, y, z) is not a side column.
@@ -204,6 +205,7 @@ struct BatchLaunchArgs {
     int n_tiles;
     int slices;         // 1 or 8
     int quads;          // quads per thread: 1, 2 or 4
+    int pml_materials;  // some scene has a material texel in its absorber (LatticeArgs::pml_materials)
 };
 hipError_t launch_batch_yee(const BatchLaunchArgs &a, int persistent_blocks, hipStream_t s);
 hipError_t launch_batch_sources(const BatchScene *scenes, const int2 *list, int n, const float *consts,

@@ -75,6 +75,10 @@ __device__ __forceinline__ void faraday(const LatticeArgs &a, int x, int y, int 
 }
 
 // Update coefficients Ca, Cb of the three E edges of texel p (shaders/update_E.comp:46-61).
+// PM = false compiles out the case of a material texel inside the absorber (the exponentials of
+// its summed conductivity): only for lattices known to have none (LatticeArgs::pml_materials 0),
+// where the kernel then needs fewer registers.
+template <bool PM = true>
 __device__ __forceinline__ void edge_coefficients(const LatticeArgs &a, int x, int y, int z, uint32_t id,
                                                   float ca[3], float cb[3]) {
     const float dt = a.dt;
@@ -83,7 +87,7 @@ __device__ __forceinline__ void edge_coefficients(const LatticeArgs &a, int x, i
         const float *t = pml_entry(a, x, y, z);
         ca[0] = t[kPmlEa + 0]; ca[1] = t[kPmlEa + 1]; ca[2] = t[kPmlEa + 2];
         cb[0] = t[kPmlEb + 0]; cb[1] = t[kPmlEb + 1]; cb[2] = t[kPmlEb + 2];
-    } else if (in_pml(a, x, y, z)) {
+    } else if (PM && in_pml(a, x, y, z)) {
         float sig[3] = {0.0f, 0.0f, 0.0f}, eps[3] = {1.0f, 1.0f, 1.0f};
         {
             const float *t = a.table + static_cast<size_t>(id) * kTableStride;

@@ -112,7 +112,8 @@ __global__ void __launch_bounds__(256) k_update_b8h(LatticeArgs a, FieldPtrs f) 
     for (int c = 0; c < 3; ++c) store8<T>(f.b[c], p, 1.0f / f.scale[3 + c], b[c]);
 }
 
-template <typename T>
+// PM: see edge_coefficients (false only for lattices without a material texel in the absorber)
+template <typename T, bool PM>
 __global__ void __launch_bounds__(256) k_update_e8h(LatticeArgs a, FieldPtrs f) {
     const int x0 = (blockIdx.x * 32 + threadIdx.x) * 8, y = blockIdx.y * 8 + threadIdx.y, z = blockIdx.z;
     if (x0 >= a.nx || y >= a.ny) return;
@@ -145,7 +146,7 @@ __global__ void __launch_bounds__(256) k_update_e8h(LatticeArgs a, FieldPtrs f) 
         const float curl[3] = {(b0[2][j] - bY[2][j]) - (b0[1][j] - bZ[1][j]), (b0[0][j] - bZ[0][j]) - (b0[2][j] - bxm[2]),
                                (b0[1][j] - bxm[1]) - (b0[0][j] - bY[0][j])};
         float ca[3], cb[3];
-        edge_coefficients(a, x0 + j, y, z, id[j], ca, cb);
+        edge_coefficients<PM>(a, x0 + j, y, z, id[j], ca, cb);
 #pragma unroll
         for (int c = 0; c < 3; ++c) e[c][j] = ca[c] * e[c][j] + cb[c] * curl[c];
     }
@@ -159,7 +160,10 @@ hipError_t launch_t(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s) {
     k_update_b8h<T><<<grid, block, 0, s>>>(a, f);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_update_e8h<T><<<grid, block, 0, s>>>(a, f);
+    if (a.pml_materials)
+        k_update_e8h<T, true><<<grid, block, 0, s>>>(a, f);
+    else
+        k_update_e8h<T, false><<<grid, block, 0, s>>>(a, f);
     return hipGetLastError();
 }
 

@@ -120,8 +120,8 @@ __device__ __forceinline__ void b_quad(const LatticeArgs &a, const FieldPtrs &f,
 }
 
 // update_E for the four cells x0..x0+3 of row (y, z) (shaders/update_E.comp:31-66); ST as b_quad
-// with the E plane as the once-read plane.
-template <int ST = 0>
+// with the E plane as the once-read plane; PM as edge_coefficients.
+template <int ST = 0, bool PM = true>
 __device__ __forceinline__ void e_quad(const LatticeArgs &a, const FieldPtrs &f, int x0, int y, int z) {
     const size_t sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
     const size_t p = z * sz + y * sy + x0;
@@ -150,7 +150,7 @@ __device__ __forceinline__ void e_quad(const LatticeArgs &a, const FieldPtrs &f,
         const float curl[3] = {(b0[2][j] - bY[2][j]) - (b0[1][j] - bZ[1][j]), (b0[0][j] - bZ[0][j]) - (b0[2][j] - bxm[2]),
                                (b0[1][j] - bxm[1]) - (b0[0][j] - bY[0][j])};
         float ca[3], cb[3];
-        edge_coefficients(a, x0 + j, y, z, id[j], ca, cb);
+        edge_coefficients<PM>(a, x0 + j, y, z, id[j], ca, cb);
 #pragma unroll
         for (int c = 0; c < 3; ++c) e[c][j] = ca[c] * e[c][j] + cb[c] * curl[c];
     }

@@ -13,11 +13,12 @@ __global__ void __launch_bounds__(256) k_update_b4(LatticeArgs a, FieldPtrs f, i
     b_quad<ST>(a, f, x0, y, z);
 }
 
-template <int ST>
+// PM: see edge_coefficients (false only for lattices without a material texel in the absorber)
+template <int ST, bool PM>
 __global__ void __launch_bounds__(256) k_update_e4(LatticeArgs a, FieldPtrs f, int z_base) {
     const int x0 = (blockIdx.x * 64 + threadIdx.x) * 4, y = blockIdx.y * 4 + threadIdx.y, z = z_base + blockIdx.z;
     if (x0 >= a.nx || y >= a.ny) return;
-    e_quad<ST>(a, f, x0, y, z);
+    e_quad<ST, PM>(a, f, x0, y, z);
 }
 
 // XCD-contiguous order: the hardware deals workgroups round-robin to the 8 XCDs; remapping the
@@ -55,7 +56,10 @@ hipError_t launch_st(const LatticeArgs &a, const FieldPtrs &f, int z0, int nzs, 
     k_update_b4<ST><<<grid, block, 0, s>>>(a, f, z0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_update_e4<ST><<<grid, block, 0, s>>>(a, f, z0);
+    if (a.pml_materials)
+        k_update_e4<ST, true><<<grid, block, 0, s>>>(a, f, z0);
+    else
+        k_update_e4<ST, false><<<grid, block, 0, s>>>(a, f, z0);
     return hipGetLastError();
 }
 

@@ -47,3 +47,13 @@ def oracle_sim(scene: Scene, ports=True, ntff=True, gpu_ntff_setup=None, capacit
         st = gpu_ntff_setup or scene.far_field
         sim.configure_ntff(st.box_min, st.box_max, st.wavelength, st.averaging_periods)
     return sim
+
+
+def absorber_material_scene(n: int = 48, pml: int = 8):
+    """Hertzian n^3 vacuum scene with two dielectric boxes that reach into the absorber (one lossy
+    into the lower x layer, one through both y layers): the E pass's material-in-absorber path."""
+    from em_lab_amd.scene import hertzian_vacuum
+    sc = hertzian_vacuum(n, pml)
+    sc.box((2, 10, 10), (20, 30, 30), eps_r=3.0, sigma=0.05)
+    sc.box((26, 2, 20), (30, n - 2, 24), eps_r=2.0)
+    return sc

@@ -7,7 +7,7 @@ import pytest
 from em_lab_amd import emfdtd as E
 from em_lab_amd.scene import draw_array_scene, hertzian_vacuum, spheres_dipole
 
-from gpu_helpers import gpu_grid, oracle_sim, rel_l2
+from gpu_helpers import absorber_material_scene, gpu_grid, oracle_sim, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -86,6 +86,18 @@ def test_batch_matches_single_grids(reference_runs):
             assert gp["best"] == wp["best"]
     b.close()
 
+
+
+def test_batch_material_in_absorber():
+    """A batch holding a scene with materials in its absorber (the E pass's general path for the
+    whole launch) still gives every scene the bits of its single-grid run."""
+    scenes = [absorber_material_scene(), hertzian_vacuum(48, 8), draw_array_scene(0)[1]]
+    print("case: batch with material in the absorber", flush=True)
+    refs = [_single(sc, 120) for sc in scenes]
+    b = _batch(scenes, 120)
+    b.step(120)
+    _check_against(b, scenes, refs, 120)
+    b.close()
 
 @pytest.mark.parametrize("shape", [(2, 0, 8), (4, 0, 1), (1, 2048, 8), (2, 1024, 1)])
 def test_batch_launch_shapes(reference_runs, shape):

@@ -8,7 +8,7 @@ import pytest
 from em_lab_amd import emfdtd as E
 from em_lab_amd.scene import Scene, hertzian_vacuum, spheres_dipole
 
-from gpu_helpers import gpu_grid, oracle_sim, rel_l2
+from gpu_helpers import absorber_material_scene, gpu_grid, oracle_sim, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -228,6 +228,27 @@ def test_abi_errors():
     with pytest.raises(E.EmfdError):
         ff.configure(E.FarFieldSetup((0, 0, 0), (1, 1, 1)))  # degenerate after clamping
     g.close()
+
+
+def test_material_in_absorber():
+    """Dielectric boxes reaching into the absorber (the E pass's summed-conductivity path): the
+    default kernel matches the oracle and, bit for bit, the scalar two-pass kernel; a scene with
+    materials only inside keeps the lighter E pass and matches the scalar kernel too."""
+    print("case: material in the absorber 48^3", flush=True)
+    for sc in (absorber_material_scene(), spheres_dipole(n=48, pml=8, n_spheres=4, radius_range=(3.0, 5.0))):
+        g, _, _ = gpu_grid(sc, ntff=False, ports=False)
+        g2, _, _ = gpu_grid(sc, E.KERNEL_TWO_PASS, ntff=False, ports=False)
+        sim = oracle_sim(sc, ntff=False, ports=False)
+        g.step(80)
+        g2.step(80)
+        sim.step(80)
+        _fields_match(g, sim)
+        e1, b1 = g.fields(E.LAYOUT_SOA)
+        e2, b2 = g2.fields(E.LAYOUT_SOA)
+        assert np.abs(e1).max() > 0
+        assert np.array_equal(e1, e2) and np.array_equal(b1, b2)
+        g.close()
+        g2.close()
 
 
 def test_far_field_runs_bit_identical():

@@ -7,7 +7,7 @@ import pytest
 from em_lab_amd import emfdtd as E
 from em_lab_amd.scene import hertzian_vacuum, spheres_dipole
 
-from gpu_helpers import gpu_grid, oracle_sim, rel_l2
+from gpu_helpers import absorber_material_scene, gpu_grid, oracle_sim, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -53,6 +53,23 @@ def test_far_field_box_through_source(storage):
     ph = ff.phasors()
     assert np.abs(ph).max() > 0
     assert np.array_equal(ph, sim.ntff["phasors"])
+    g.close()
+
+
+@pytest.mark.parametrize("storage", [E.STORAGE_F16, E.STORAGE_BF16])
+def test_material_in_absorber_storage(storage):
+    print("case: material in the absorber 48^3, storage", storage, flush=True)
+    sc = absorber_material_scene()
+    k = [2, 2, 2, -3, -3, -3]
+    g, _, _ = gpu_grid(sc, ntff=False, ports=False)
+    g.set_storage(storage, k)
+    sim = oracle_sim(sc, ntff=False, ports=False)
+    sim.set_storage(storage, k)
+    g.step(80)
+    sim.step(80)
+    Eg, Bg = g.fields(E.LAYOUT_TEXEL)
+    assert np.array_equal(Eg[:, :3], sim.E[:, :3]) and np.array_equal(Bg[:, :3], sim.B[:, :3])
+    assert np.any(Eg[:, :3])
     g.close()
 
 
