@@ -166,7 +166,8 @@ def main() -> int:
         traffic = json.loads(tf.read_text()).get("bytes_per_launch")
     kernel_names = {0: "k_update_b4+k_update_e4", 1: "k_yee_fused", 2: "k_update_b+k_update_e", 3: "k_yee_rows",
                     4: "k_yee_point", 5: "k_update_b4+k_update_e4", 6: "k_update_b4+k_update_e4 (z-slabs)", 7: "k_yee_zmarch"}
-    kname = kernel_names.get(args.kernel & 0xff, "?") if storage == E.STORAGE_F32 else "k_update_b8h+k_update_e8h"
+    kname = (kernel_names.get(args.kernel & 0xff, "?") if storage == E.STORAGE_F32
+             else "k_update_bh+k_update_eh (%d cells per thread)" % ((args.kernel >> 8) & 0xf or 8))
     out = {
         "metric": METRIC,
         "value": round(value, 2),

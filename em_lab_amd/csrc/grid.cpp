@@ -679,7 +679,7 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
             if (ctx->storage == kStorageF64)
                 EMFD_HIP(launch_yee_f64(lat, ctx->fields(ctx->cur), ctx->stream));
             else
-                EMFD_HIP(launch_yee_half(lat, ctx->fields(ctx->cur), ctx->stream));
+                EMFD_HIP(launch_yee_half(lat, ctx->fields(ctx->cur), ctx->tile_cfg & 0xf, ctx->stream));
             variant = -1;
         } else if (variant == EMFD_KERNEL_AUTO)
             variant = yee_two_pass4_supported(lat) ? EMFD_KERNEL_TWO_PASS4 : EMFD_KERNEL_TWO_PASS;

@@ -105,7 +105,7 @@ hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStre
 hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, int streaming, hipStream_t s);
 hipError_t launch_yee_two_pass4_slabs(const LatticeArgs &a, const FieldPtrs &f, int slab, hipStream_t s);
 int yee_two_pass4_supported(const LatticeArgs &a);
-hipError_t launch_yee_half(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_yee_half(const LatticeArgs &a, const FieldPtrs &f, int cells_per_thread, hipStream_t s);
 hipError_t launch_yee_f64(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
 hipError_t launch_sources_f64(const LatticeArgs &a, const FieldPtrs &f, const int *cells, int n_cells,
                               const int *lists, const float *packed, const emfd_source_desc *descs, double t,

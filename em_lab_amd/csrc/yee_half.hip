@@ -1,41 +1,47 @@
 // Two-pass leapfrog on 16-bit field storage: E and B planes hold binary16 or bfloat16 values of
 // field / scale (a power-of-two scale per component), every update runs in FP32 with the same
 // per-texel arithmetic as faraday() / edge_coefficients(), and the result is rounded to nearest
-// even on the way back. Each thread owns eight consecutive x cells so every field stream moves
-// as 16-byte loads and stores (8 values). Requires nx % 8 == 0. Field traffic per cell-update is
-// 24 B (half of FP32), plus the x+-1 neighbours of the edge cells, which hit the same lines.
+// even on the way back. Each thread owns N = 8 (16-byte loads and stores) or N = 4 (8-byte)
+// consecutive x cells. Requires nx % 8 == 0. Field traffic per cell-update is 24 B (half of
+// FP32), plus the x+-1 neighbours of the edge cells, which hit the same lines.
 #include "yee_device.h"
 
 namespace emfd {
 namespace {
 
-template <typename T>
-__device__ __forceinline__ void load8(const void *plane, size_t p, float s, float (&o)[8]) {
-    const uint4 raw = *reinterpret_cast<const uint4 *>(static_cast<const T *>(plane) + p);
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+template <int N>
+struct Raw;  // N 16-bit values
+template <>
+struct Raw<8> {
+    typedef u32x4 type;
+};
+template <>
+struct Raw<4> {
+    typedef u32x2 type;
+};
+
+// N values of a plane; NT: the plane a pass reads exactly once (streaming load)
+template <typename T, int N, bool NT>
+__device__ __forceinline__ void loadn(const void *plane, size_t p, float s, float (&o)[N]) {
+    using R = typename Raw<N>::type;
+    const R *ptr = reinterpret_cast<const R *>(static_cast<const T *>(plane) + p);
+    const R raw = NT ? __builtin_nontemporal_load(ptr) : *ptr;
     const T *h = reinterpret_cast<const T *>(&raw);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) o[j] = static_cast<float>(h[j]) * s;
+    for (int j = 0; j < N; ++j) o[j] = static_cast<float>(h[j]) * s;
 }
 
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-
-// the pass's output plane: streaming (non-temporal) 16-byte store
-template <typename T>
-__device__ __forceinline__ void store8(void *plane, size_t p, float inv, const float (&v)[8]) {
-    u32x4 raw;
+// the pass's output plane: streaming (non-temporal) store
+template <typename T, int N>
+__device__ __forceinline__ void storen(void *plane, size_t p, float inv, const float (&v)[N]) {
+    using R = typename Raw<N>::type;
+    R raw;
     T *h = reinterpret_cast<T *>(&raw);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) h[j] = static_cast<T>(v[j] * inv);
-    __builtin_nontemporal_store(raw, reinterpret_cast<u32x4 *>(static_cast<T *>(plane) + p));
-}
-
-// the plane a pass reads exactly once: streaming load
-template <typename T>
-__device__ __forceinline__ void load8s(const void *plane, size_t p, float s, float (&o)[8]) {
-    const u32x4 raw = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(static_cast<const T *>(plane) + p));
-    const T *h = reinterpret_cast<const T *>(&raw);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) o[j] = static_cast<float>(h[j]) * s;
+    for (int j = 0; j < N; ++j) h[j] = static_cast<T>(v[j] * inv);
+    __builtin_nontemporal_store(raw, reinterpret_cast<R *>(static_cast<T *>(plane) + p));
 }
 
 template <typename T>
@@ -43,63 +49,77 @@ __device__ __forceinline__ float load1(const void *plane, size_t p, float s) {
     return static_cast<float>(static_cast<const T *>(plane)[p]) * s;
 }
 
-__device__ __forceinline__ void load_ids8(const LatticeArgs &a, size_t p, uint32_t (&id)[8]) {
+template <int N>
+__device__ __forceinline__ void load_idsn(const LatticeArgs &a, size_t p, uint32_t (&id)[N]) {
     if (!a.mat_id) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) id[j] = 0u;
+        for (int j = 0; j < N; ++j) id[j] = 0u;
     } else if (a.id_bytes == 2) {
-        const uint4 v = *reinterpret_cast<const uint4 *>(static_cast<const uint16_t *>(a.mat_id) + p);
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            id[2 * j] = w[j] & 0xffffu;
-            id[2 * j + 1] = w[j] >> 16;
+        for (int q = 0; q < N / 4; ++q) {
+            const uint2 v = *reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(a.mat_id) + p + 4 * q);
+            id[4 * q + 0] = v.x & 0xffffu;
+            id[4 * q + 1] = v.x >> 16;
+            id[4 * q + 2] = v.y & 0xffffu;
+            id[4 * q + 3] = v.y >> 16;
         }
     } else {
-        const uint4 v0 = *reinterpret_cast<const uint4 *>(static_cast<const uint32_t *>(a.mat_id) + p);
-        const uint4 v1 = *reinterpret_cast<const uint4 *>(static_cast<const uint32_t *>(a.mat_id) + p + 4);
-        id[0] = v0.x; id[1] = v0.y; id[2] = v0.z; id[3] = v0.w;
-        id[4] = v1.x; id[5] = v1.y; id[6] = v1.z; id[7] = v1.w;
+#pragma unroll
+        for (int q = 0; q < N / 4; ++q) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(static_cast<const uint32_t *>(a.mat_id) + p + 4 * q);
+            id[4 * q + 0] = v.x;
+            id[4 * q + 1] = v.y;
+            id[4 * q + 2] = v.z;
+            id[4 * q + 3] = v.w;
+        }
     }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_update_b8h(LatticeArgs a, FieldPtrs f) {
-    const int x0 = (blockIdx.x * 32 + threadIdx.x) * 8, y = blockIdx.y * 8 + threadIdx.y, z = blockIdx.z;
+// workgroup shape: TX threads along x (TX * N cells), 256 / TX rows
+template <int N>
+constexpr int kTx = N == 8 ? 32 : 64;
+
+template <typename T, int N>
+__global__ void __launch_bounds__(256) k_update_bh(LatticeArgs a, FieldPtrs f) {
+    const int x0 = (blockIdx.x * kTx<N> + threadIdx.x) * N, y = blockIdx.y * (256 / kTx<N>) + threadIdx.y,
+              z = blockIdx.z;
     if (x0 >= a.nx || y >= a.ny) return;
     const size_t sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
     const size_t p = z * sz + y * sy + x0;
-    const bool up_x = x0 + 8 < a.nx, up_y = y + 1 < a.ny, up_z = z + 1 < a.nz;
-    float e0[3][8], eX[3], eY[3][8], eZ[3][8], b[3][8];
+    const bool up_x = x0 + N < a.nx, up_y = y + 1 < a.ny, up_z = z + 1 < a.nz;
+    float e0[3][N], eX[3], eY[3][N], eZ[3][N], b[3][N];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float s = f.scale[c];
-        load8<T>(f.e[c], p, s, e0[c]);
-        eX[c] = up_x ? load1<T>(f.e[c], p + 8, s) : 0.0f;
-        if (up_y) load8<T>(f.e[c], p + sy, s, eY[c]);
+        loadn<T, N, false>(f.e[c], p, s, e0[c]);
+        eX[c] = up_x ? load1<T>(f.e[c], p + N, s) : 0.0f;
+        if (up_y) loadn<T, N, false>(f.e[c], p + sy, s, eY[c]);
         else
 #pragma unroll
-            for (int j = 0; j < 8; ++j) eY[c][j] = 0.0f;
-        if (up_z) load8<T>(f.e[c], p + sz, s, eZ[c]);
+            for (int j = 0; j < N; ++j) eY[c][j] = 0.0f;
+        if (up_z) loadn<T, N, false>(f.e[c], p + sz, s, eZ[c]);
         else
 #pragma unroll
-            for (int j = 0; j < 8; ++j) eZ[c][j] = 0.0f;
-        load8s<T>(f.b[c], p, f.scale[3 + c], b[c]);
+            for (int j = 0; j < N; ++j) eZ[c][j] = 0.0f;
+        loadn<T, N, true>(f.b[c], p, f.scale[3 + c], b[c]);
     }
-    // a class only for thin-wire factors, and only where the wire mask marks one of the 8 texels
-    uint32_t id[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+    // a class only for thin-wire factors, and only where the wire mask marks one of the N texels
+    uint32_t id[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) id[j] = 0u;
     if (a.has_wire) {
         if (a.wire_mask) {
             const uint32_t w = a.wire_mask[(static_cast<size_t>(z) * a.ny + y) * a.mask_wpr + (x0 >> 5)];
-            if ((w >> (x0 & 31)) & 0xffu) load_ids8(a, p, id);
+            if ((w >> (x0 & 31)) & ((1u << N) - 1u)) load_idsn<N>(a, p, id);
         } else {
-            load_ids8(a, p, id);
+            load_idsn<N>(a, p, id);
         }
     }
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < N; ++j) {
         const float c0[3] = {e0[0][j], e0[1][j], e0[2][j]};
-        const float cx[3] = {j < 7 ? e0[0][j + 1] : eX[0], j < 7 ? e0[1][j + 1] : eX[1], j < 7 ? e0[2][j + 1] : eX[2]};
+        const float cx[3] = {j < N - 1 ? e0[0][j + 1] : eX[0], j < N - 1 ? e0[1][j + 1] : eX[1],
+                             j < N - 1 ? e0[2][j + 1] : eX[2]};
         const float cy[3] = {eY[0][j], eY[1][j], eY[2][j]};
         const float cz[3] = {eZ[0][j], eZ[1][j], eZ[2][j]};
         float bb[3] = {b[0][j], b[1][j], b[2][j]};
@@ -109,39 +129,40 @@ __global__ void __launch_bounds__(256) k_update_b8h(LatticeArgs a, FieldPtrs f) 
         b[2][j] = bb[2];
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) store8<T>(f.b[c], p, 1.0f / f.scale[3 + c], b[c]);
+    for (int c = 0; c < 3; ++c) storen<T, N>(f.b[c], p, 1.0f / f.scale[3 + c], b[c]);
 }
 
 // PM: see edge_coefficients (false only for lattices without a material texel in the absorber)
-template <typename T, bool PM>
-__global__ void __launch_bounds__(256) k_update_e8h(LatticeArgs a, FieldPtrs f) {
-    const int x0 = (blockIdx.x * 32 + threadIdx.x) * 8, y = blockIdx.y * 8 + threadIdx.y, z = blockIdx.z;
+template <typename T, int N, bool PM>
+__global__ void __launch_bounds__(256) k_update_eh(LatticeArgs a, FieldPtrs f) {
+    const int x0 = (blockIdx.x * kTx<N> + threadIdx.x) * N, y = blockIdx.y * (256 / kTx<N>) + threadIdx.y,
+              z = blockIdx.z;
     if (x0 >= a.nx || y >= a.ny) return;
     const size_t sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
     const size_t p = z * sz + y * sy + x0;
     const bool dn_x = x0 > 0, dn_y = y > 0, dn_z = z > 0;
-    float b0[3][8], bX[3], bY[3][8], bZ[3][8], e[3][8];
+    float b0[3][N], bX[3], bY[3][N], bZ[3][N], e[3][N];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float s = f.scale[3 + c];
-        load8<T>(f.b[c], p, s, b0[c]);
+        loadn<T, N, false>(f.b[c], p, s, b0[c]);
         bX[c] = dn_x ? load1<T>(f.b[c], p - 1, s) : 0.0f;
-        if (dn_y) load8<T>(f.b[c], p - sy, s, bY[c]);
+        if (dn_y) loadn<T, N, false>(f.b[c], p - sy, s, bY[c]);
         else
 #pragma unroll
-            for (int j = 0; j < 8; ++j) bY[c][j] = 0.0f;
-        if (dn_z) load8<T>(f.b[c], p - sz, s, bZ[c]);
+            for (int j = 0; j < N; ++j) bY[c][j] = 0.0f;
+        if (dn_z) loadn<T, N, false>(f.b[c], p - sz, s, bZ[c]);
         else
 #pragma unroll
-            for (int j = 0; j < 8; ++j) bZ[c][j] = 0.0f;
-        load8s<T>(f.e[c], p, f.scale[c], e[c]);
+            for (int j = 0; j < N; ++j) bZ[c][j] = 0.0f;
+        loadn<T, N, true>(f.e[c], p, f.scale[c], e[c]);
     }
     // (a class-mask gate as in the FP32 pass measured slower here: 71.8 -> 80.5 us at 256^3; the
     // mask-then-class round trip stalls the waves that hold a material texel)
-    uint32_t id[8];
-    load_ids8(a, p, id);
+    uint32_t id[N];
+    load_idsn<N>(a, p, id);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < N; ++j) {
         const float bxm[3] = {j > 0 ? b0[0][j - 1] : bX[0], j > 0 ? b0[1][j - 1] : bX[1], j > 0 ? b0[2][j - 1] : bX[2]};
         const float curl[3] = {(b0[2][j] - bY[2][j]) - (b0[1][j] - bZ[1][j]), (b0[0][j] - bZ[0][j]) - (b0[2][j] - bxm[2]),
                                (b0[1][j] - bxm[1]) - (b0[0][j] - bY[0][j])};
@@ -151,19 +172,20 @@ __global__ void __launch_bounds__(256) k_update_e8h(LatticeArgs a, FieldPtrs f) 
         for (int c = 0; c < 3; ++c) e[c][j] = ca[c] * e[c][j] + cb[c] * curl[c];
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) store8<T>(f.e[c], p, 1.0f / f.scale[c], e[c]);
+    for (int c = 0; c < 3; ++c) storen<T, N>(f.e[c], p, 1.0f / f.scale[c], e[c]);
 }
 
-template <typename T>
+template <typename T, int N>
 hipError_t launch_t(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s) {
-    dim3 block(32, 8, 1), grid((a.nx / 8 + 31) / 32, (a.ny + 7) / 8, a.nz);
-    k_update_b8h<T><<<grid, block, 0, s>>>(a, f);
+    constexpr int tx = kTx<N>, ty = 256 / kTx<N>;
+    dim3 block(tx, ty, 1), grid((a.nx / N + tx - 1) / tx, (a.ny + ty - 1) / ty, a.nz);
+    k_update_bh<T, N><<<grid, block, 0, s>>>(a, f);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (a.pml_materials)
-        k_update_e8h<T, true><<<grid, block, 0, s>>>(a, f);
+        k_update_eh<T, N, true><<<grid, block, 0, s>>>(a, f);
     else
-        k_update_e8h<T, false><<<grid, block, 0, s>>>(a, f);
+        k_update_eh<T, N, false><<<grid, block, 0, s>>>(a, f);
     return hipGetLastError();
 }
 
@@ -171,10 +193,13 @@ hipError_t launch_t(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s) {
 
 int yee_half_supported(const LatticeArgs &a) { return a.nx % 8 == 0 ? 1 : 0; }
 
-hipError_t launch_yee_half(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s) {
+// cells_per_thread: 8 or 4 (0: 8)
+hipError_t launch_yee_half(const LatticeArgs &a, const FieldPtrs &f, int cells_per_thread, hipStream_t s) {
     if (!yee_half_supported(a)) return hipErrorInvalidValue;
-    if (f.storage == kStorageF16) return launch_t<_Float16>(a, f, s);
-    if (f.storage == kStorageBF16) return launch_t<__bf16>(a, f, s);
+    const bool four = cells_per_thread == 4;
+    if (cells_per_thread != 0 && cells_per_thread != 4 && cells_per_thread != 8) return hipErrorInvalidValue;
+    if (f.storage == kStorageF16) return four ? launch_t<_Float16, 4>(a, f, s) : launch_t<_Float16, 8>(a, f, s);
+    if (f.storage == kStorageBF16) return four ? launch_t<__bf16, 4>(a, f, s) : launch_t<__bf16, 8>(a, f, s);
     return hipErrorInvalidValue;
 }
 

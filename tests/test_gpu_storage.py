@@ -12,12 +12,14 @@ from gpu_helpers import absorber_material_scene, gpu_grid, oracle_sim, rel_l2
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("cells", [0, 4])
 @pytest.mark.parametrize("storage", [E.STORAGE_F16, E.STORAGE_BF16])
-def test_hertzian_storage_bit_exact(storage):
-    print("case: hertzian 64^3 storage", storage, flush=True)
+def test_hertzian_storage_bit_exact(storage, cells):
+    """cells: 16-bit kernel cells per thread (bits 8.. of the automatic variant; 0 = 8)."""
+    print("case: hertzian 64^3 storage", storage, "cells per thread", cells, flush=True)
     sc = hertzian_vacuum(64, 8)
     k = [2, 2, 2, -3, -3, -3]
-    g, _, ff = gpu_grid(sc)
+    g, _, ff = gpu_grid(sc, E.KERNEL_AUTO | (cells << 8))
     g.set_storage(storage, k)
     sim = oracle_sim(sc, gpu_ntff_setup=ff.setup)
     sim.set_storage(storage, k)
@@ -73,11 +75,12 @@ def test_material_in_absorber_storage(storage):
     g.close()
 
 
-def test_spheres_dipole_f16_ports_bit_exact():
-    print("case: spheres+dipole 48^3 f16 storage with ports", flush=True)
+@pytest.mark.parametrize("cells", [0, 4])
+def test_spheres_dipole_f16_ports_bit_exact(cells):
+    print("case: spheres+dipole 48^3 f16 storage with ports, cells per thread", cells, flush=True)
     sc = spheres_dipole(n=48, pml=8, n_spheres=4, radius_range=(3.0, 6.0), sweep_points=16)
     k = [0, 0, 0, -4, -4, -4]
-    g, pm, _ = gpu_grid(sc, ntff=False)
+    g, pm, _ = gpu_grid(sc, E.KERNEL_AUTO | (cells << 8), ntff=False)
     g.set_storage(E.STORAGE_F16, k)
     sim = oracle_sim(sc, ntff=False)
     sim.set_storage(1, k)
