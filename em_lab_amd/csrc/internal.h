@@ -156,7 +156,7 @@ hipError_t launch_ntff_accumulate(const LatticeArgs &a, const FieldPtrs &f, cons
                                   const NtffFreqs &freqs, float *phasors, const PortJob &pj, hipStream_t s);
 // deferred far-field accumulation: per-step gather into sample slot (6 planes of n_points), then
 // one pass applying `steps` updates (rotation rows rot_stride floats apart) to the phasors
-constexpr int kNtffRun = 8;
+constexpr int kNtffRun = 16;
 hipError_t launch_ntff_sample(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, float *slot,
                               const PortJob &pj, hipStream_t s);
 hipError_t launch_ntff_flush(const NtffBox &box, const float *rot, int rot_stride, int steps, const NtffFreqs &freqs,

@@ -423,7 +423,7 @@ hipError_t launch_ntff_sample(const LatticeArgs &a, const FieldPtrs &f, const Nt
 hipError_t launch_ntff_flush(const NtffBox &box, const float *rot, int rot_stride, int steps, const NtffFreqs &freqs,
                              const float *samples, float *phasors, hipStream_t s) {
     if (box.n_points <= 0 || steps <= 0) return hipSuccess;
-    static_assert(kNtffRun == 8, "one flush instance per run length");
+    static_assert(kNtffRun == 16, "one flush instance per run length");
     const int g = (box.n_points + 255) / 256;
     switch (steps) {
         case 1: k_ntff_flush<1><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
@@ -434,6 +434,14 @@ hipError_t launch_ntff_flush(const NtffBox &box, const float *rot, int rot_strid
         case 6: k_ntff_flush<6><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
         case 7: k_ntff_flush<7><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
         case 8: k_ntff_flush<8><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
+        case 9: k_ntff_flush<9><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
+        case 10: k_ntff_flush<10><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
+        case 11: k_ntff_flush<11><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
+        case 12: k_ntff_flush<12><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
+        case 13: k_ntff_flush<13><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
+        case 14: k_ntff_flush<14><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
+        case 15: k_ntff_flush<15><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
+        case 16: k_ntff_flush<16><<<g, 256, 0, s>>>(box, rot, rot_stride, freqs, samples, phasors); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

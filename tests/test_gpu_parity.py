@@ -252,16 +252,16 @@ def test_material_in_absorber():
 
 
 def test_far_field_runs_bit_identical():
-    """Far-field phasors accumulated in runs of up to eight steps inside one step call equal, bit
+    """Far-field phasors accumulated in runs of up to sixteen steps inside one step call equal, bit
     for bit, those of one accumulation per step (single-step calls), for two wavelengths and call
-    lengths that leave partial runs (1, 2, 3, 7, 8, 9, 17 steps); the fields stay identical too."""
+    lengths that leave partial runs (1, 2, 3, 7, 8, 9, 16, 17, 24, 33 steps); the fields stay identical too."""
     print("case: far-field deferred runs vs single steps", flush=True)
     sc = hertzian_vacuum(48, 8)
     ga, _, fa = gpu_grid(sc)
     gb, _, fb = gpu_grid(sc)
     for ff in (fa, fb):
         ff.set_wavelengths([16.0, 11.0])
-    calls = [1, 2, 3, 7, 8, 9, 17, 1, 24]
+    calls = [1, 2, 3, 7, 8, 9, 17, 1, 24, 16, 33]
     for c in calls:
         ga.step(c)
     for _ in range(sum(calls)):
