@@ -559,6 +559,79 @@ def draw_array_design(index: int, seed: int = DEFAULT_SEED, lambda0: float = ARR
     return geom, (pick(lo_x, 128), pick(48, 128), pick(96, 160))
 
 
+def draw_array_designs(start: int, count: int, seed: int = DEFAULT_SEED, lambda0: float = ARRAY_LAMBDA0):
+    """draw_array_design(i) for i in start .. start+count-1, with the xoshiro256** streams of all
+    designs stepped together in numpy uint64 lanes: the same draws, the same double arithmetic
+    (the exponentials through math.exp, as the scalar path), the same designs."""
+    lane = np.arange(start, start + count, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        x = np.uint64((seed * 1000003) & _M64) + lane
+        st = []
+        for _ in range(4):
+            x = x + np.uint64(0x9E3779B97F4A7C15)
+            z = x
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            st.append(z ^ (z >> np.uint64(31)))
+        s0, s1, s2, s3 = st
+
+        def rotl(v, k):
+            return (v << np.uint64(k)) | (v >> np.uint64(64 - k))
+
+        draws = []  # 3 n + 3 <= 21 draws per design, as uniform [0, 1) doubles
+        for _ in range(3 * ARRAY_MAX_ELEMENTS + 3):
+            r = rotl(s1 * np.uint64(5), 7) * np.uint64(9)
+            t = s1 << np.uint64(17)
+            s2 = s2 ^ s0
+            s3 = s3 ^ s1
+            s1 = s1 ^ s2
+            s0 = s0 ^ s3
+            s2 = s2 ^ t
+            s3 = rotl(s3, 45)
+            draws.append((r >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53)))
+    u = np.stack(draws, axis=1)
+
+    def uni(col, lo, hi):
+        return lo + (hi - lo) * col
+
+    # draw columns: 0 element count; element 0: length 1, radius 2; element i >= 1: length 3i,
+    # spacing 3i + 1, radius 3i + 2; then the three grid-size picks at 3n, 3n + 1, 3n + 2
+    m = ARRAY_MAX_ELEMENTS
+    n = 1 + np.minimum(np.floor(uni(u[:, 0], 0.0, 6.0)).astype(np.int64), 5)
+    present = np.arange(m)[None, :] < n[:, None]
+    lcol = np.array([1] + [3 * i for i in range(1, m)])
+    rcol = np.array([2] + [3 * i + 2 for i in range(1, m)])
+    L = uni(u[:, lcol], 0.3, 0.55) * lambda0
+    # round_half_away of a positive value: floor(x + 0.5)
+    lengths = np.maximum(1, np.floor(L + 0.5)).astype(np.int64)
+    arm0 = np.maximum(2, np.floor((np.maximum(L[:, 0], 5.0) - 1.0) * 0.5 + 0.5)).astype(np.int64)
+    lengths[:, 0] = 2 * arm0 + 1
+    spacings = np.zeros((count, m), np.int64)
+    spacings[:, 1:] = np.maximum(1, np.floor(uni(u[:, [3 * i + 1 for i in range(1, m)]], 0.1, 0.4) * lambda0 + 0.5))
+    spacings[~present] = 0
+    ru = uni(u[:, rcol], math.log(0.001), math.log(0.01))
+    radii = np.zeros((count, m), np.float64)
+    sel = np.nonzero(present)
+    # the exponential through math.exp, as the scalar generator takes it
+    radii[sel] = np.array([math.exp(v) for v in ru[sel].tolist()]) * lambda0
+    radii = radii.astype(np.float32)
+    span = spacings.sum(axis=1)
+    lo_x = np.maximum(48, ((span + 2 * ARRAY_PML + 8 + 7) // 8) * 8)
+    picks = []
+    for q, (lo, hi) in enumerate(((lo_x, 128), (48, 128), (96, 160))):
+        kk = (hi - lo) // 8 + 1
+        col = np.take_along_axis(u, (3 * n + q)[:, None], axis=1)[:, 0]
+        picks.append(lo + 8 * np.minimum(np.floor(kk * col).astype(np.int64), kk - 1))
+    geoms, sizes = [], []
+    nl, ll, sl, rl = n.tolist(), lengths.tolist(), spacings.tolist(), radii.tolist()
+    p0, p1, p2 = (p.tolist() for p in picks)
+    for d in range(count):
+        nd = nl[d]
+        geoms.append(ArrayGeometry(ll[d][:nd], sl[d][:nd], rl[d][:nd]))
+        sizes.append((p0[d], p1[d], p2[d]))
+    return geoms, sizes
+
+
 def draw_array_scene(index: int, seed: int = DEFAULT_SEED, lambda0: float = ARRAY_LAMBDA0):
     """(geometry, Scene) of design `index` (draw_array_design)."""
     geom, size = draw_array_design(index, seed, lambda0)

@@ -25,7 +25,7 @@ if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
 from em_lab_amd.scene import (ARRAY_LAMBDA0, ARRAY_MAX_ELEMENTS, ArrayGeometry, DEFAULT_SEED,  # noqa: E402
-                              array_scene, draw_array_design, draw_array_scene)
+                              array_scene, draw_array_design, draw_array_designs, draw_array_scene)
 
 N_FEATURES = 3 * ARRAY_MAX_ELEMENTS
 N_BINS = 64
@@ -45,6 +45,27 @@ def features(geom: ArrayGeometry, lambda0: float = ARRAY_LAMBDA0) -> np.ndarray:
         f[3 * i + 0] = geom.lengths[i] / lambda0
         f[3 * i + 1] = geom.spacings[i] / lambda0
         f[3 * i + 2] = math.log10(geom.radii[i] / lambda0) + 3.5
+    return f
+
+
+def features_many(geoms) -> np.ndarray:
+    """features(g) for every geometry, as one (designs, 18) array (the same float32 values)."""
+    m = ARRAY_MAX_ELEMENTS
+    count = len(geoms)
+    n = np.array([g.elements for g in geoms], np.int64)
+    lengths = np.zeros((count, m), np.float64)
+    spacings = np.zeros((count, m), np.float64)
+    logs = np.zeros((count, m), np.float64)
+    present = np.arange(m)[None, :] < n[:, None]
+    rows = np.nonzero(present)
+    lengths[rows] = [v for g in geoms for v in g.lengths]
+    spacings[rows] = [v for g in geoms for v in g.spacings]
+    # log10 through math.log10, as features() takes it
+    logs[rows] = [math.log10(r / ARRAY_LAMBDA0) + 3.5 for g in geoms for r in g.radii]
+    f = np.zeros((count, N_FEATURES), np.float32)
+    f[:, 0::3] = np.where(present, lengths / ARRAY_LAMBDA0, 0.0)
+    f[:, 1::3] = np.where(present, spacings / ARRAY_LAMBDA0, 0.0)
+    f[:, 2::3] = np.where(present, logs, 0.0)
     return f
 
 
@@ -244,13 +265,10 @@ def load_surrogate(path: Path = WEIGHTS, max_batch: int = 65536):
 
 
 def candidates(n: int, seed: int = CANDIDATE_SEED):
-    """n seeded candidate designs: (geometries, sizes, features)."""
-    geoms, sizes = [], []
-    for i in range(n):
-        g, size = draw_array_design(i, seed)
-        geoms.append(g)
-        sizes.append(size)
-    return geoms, sizes, np.stack([features(g) for g in geoms])
+    """n seeded candidate designs: (geometries, sizes, features), drawn with all generator
+    streams stepped together (draw_array_designs gives draw_array_design's designs)."""
+    geoms, sizes = draw_array_designs(0, n, seed)
+    return geoms, sizes, features_many(geoms)
 
 
 def screen(m, X: np.ndarray, top: int = 256):

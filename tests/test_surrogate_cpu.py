@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from em_lab_amd import surrogate as S
-from em_lab_amd.scene import (ARRAY_PML, draw_array_design, draw_array_geometry, draw_array_scene, spheres_array,
+from em_lab_amd.scene import (ARRAY_PML, draw_array_design, draw_array_designs, draw_array_geometry, draw_array_scene, spheres_array,
                               Xoshiro256ss)
 
 
@@ -64,6 +64,18 @@ def test_design_generator_ranges_and_determinism():
     g, sc = draw_array_scene(7)
     assert len(sc.ports) == 1 and len(sc.sources) == 1 and sc.sweep.points == 64
     assert len(sc.subcell.wire_terms) > 0 and not np.any(sc.sigma)
+
+
+@pytest.mark.parametrize("start,count,seed", [(0, 3000, S.CANDIDATE_SEED), (123456, 1000, 7), (5, 1, 1)])
+def test_lane_stepped_generator_equals_scalar(start, count, seed):
+    """draw_array_designs (all xoshiro256** streams stepped together in uint64 lanes) gives the
+    scalar generator's designs exactly, and features_many its features."""
+    geoms, sizes = draw_array_designs(start, count, seed)
+    for k in range(count):
+        g, size = draw_array_design(start + k, seed)
+        assert (geoms[k].lengths, geoms[k].spacings, geoms[k].radii, sizes[k]) == (g.lengths, g.spacings, g.radii, size)
+    ref = np.stack([S.features(g) for g in geoms])
+    assert np.array_equal(S.features_many(geoms), ref)
 
 
 def test_spheres_array_scene():
