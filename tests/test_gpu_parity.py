@@ -148,6 +148,12 @@ def test_rows_kernel_odd_grids(size, zchunk):
         g4.step(50)
         _fields_match(g4, sim)
         g4.close()
+    # the automatic choice (vectorised two-pass with the class masks, or the scalar two-pass when
+    # nx % 4 != 0)
+    ga, _, _ = gpu_grid(sc, ports=False, ntff=False)
+    ga.step(50)
+    _fields_match(ga, sim)
+    ga.close()
 
 
 def test_wide_class_ids():
