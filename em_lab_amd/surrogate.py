@@ -287,6 +287,30 @@ def verify(geoms, sizes, idx, steps: int = STEPS, batch_size: int = 1024):
     return s11_db(z_to_gamma(z)), z, res
 
 
+HARD_SEED = 4242  # candidate pool for hard examples: a generator seed other than the screening's
+
+
+def mine_hard(count: int, pool: int, out: Path, seed: int = HARD_SEED, batch_size: int = 1024, log=None):
+    """Hard examples for the surrogate: of `pool` candidates of generator seed `seed` (not the
+    screening benchmark's CANDIDATE_SEED, so its candidates stay unseen), the `count` the current
+    surrogate ranks best by predicted -10 dB bandwidth, i.e. the extreme designs screening picks
+    and where the surrogate errs most, simulated by batched FDTD and saved in the training-data
+    format (index = position in the pool)."""
+    m = load_surrogate()
+    geoms, sizes = draw_array_designs(0, pool, seed)
+    X = features_many(geoms)
+    chunk = 65536
+    y = np.concatenate([m.forward(X[i:i + chunk]) for i in range(0, pool, chunk)])
+    m.close()
+    s = s11_db(gamma_from_output(y))
+    bw = bandwidth_10db(s, sweep_nu())
+    idx = np.lexsort((s.min(axis=1), -bw))[:count]
+    z, res = run_fdtd(lambda i: array_scene(geoms[idx[i]], sizes[idx[i]]), len(idx), STEPS, batch_size, log=log)
+    np.savez_compressed(out, index=idx, seed=np.array(seed), X=X[idx], z=z, residual=res,
+                        size=np.array([sizes[i] for i in idx]))
+    return X[idx], z
+
+
 def plain_fdtd_rate(n: int = 4, seed: int = CANDIDATE_SEED, steps: int = STEPS) -> float:
     """Designs per second of one-at-a-time FDTD (a single-grid context per design, its sweep)."""
     from em_lab_amd import emfdtd as E
@@ -356,6 +380,11 @@ def main(argv=None) -> int:
     tr.add_argument("--batch", type=int, default=256)
     tr.add_argument("--members", type=int, default=1, help="ensemble size (networks averaged)")
     tr.add_argument("--out", type=Path, default=WEIGHTS)
+    mh = sub.add_parser("mine", help="hard examples: the best-ranked designs of a separate candidate pool, by FDTD")
+    mh.add_argument("--count", type=int, default=2048)
+    mh.add_argument("--pool", type=int, default=262144)
+    mh.add_argument("--seed", type=int, default=HARD_SEED)
+    mh.add_argument("--out", type=Path, required=True)
     sc = sub.add_parser("screen", help="screen seeded candidates and verify the best by batched FDTD")
     sc.add_argument("--candidates", type=int, default=65536)
     sc.add_argument("--top", type=int, default=256)
@@ -372,6 +401,11 @@ def main(argv=None) -> int:
         return 0
     if args.cmd == "screen":
         screening_run(args.candidates, args.top, log=lambda s: print(s, flush=True))
+        return 0
+    if args.cmd == "mine":
+        t = time.time()
+        mine_hard(args.count, args.pool, args.out, seed=args.seed, log=lambda m: print(m, flush=True))
+        print(f"mined {args.count} of {args.pool} designs in {time.time() - t:.1f} s -> {args.out}", flush=True)
         return 0
     if args.cmd == "generate":
         t = time.time()
