@@ -324,6 +324,7 @@ emfd::LatticeArgs emfd_ctx::lattice() const {
     }
     if (side_active && ntff.enabled && ntff.d_side.ptr) {
         a.side = ntff.side;
+        a.side.e_valid = side_e_valid ? 1 : 0;
         a.side.buf = ntff.d_side.as<float>();
     }
     return a;
@@ -650,9 +651,12 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
         const int v = ctx->kernel_variant;
         const bool tp4 = yee_two_pass4_supported(ctx->lattice()) &&
                          (v == EMFD_KERNEL_AUTO || v == EMFD_KERNEL_TWO_PASS4 || v == EMFD_KERNEL_SLABS);
-        // (the 16-bit kernels do not: copying the rounded values cost their E pass a wave per SIMD,
-        // 71.8 -> 91-95 us at 256^3, more than the 15 us it saved the far-field gather)
-        ctx->side_active = ctx->ntff.enabled && ctx->storage == kStorageF32 && tp4;
+        // (the 16-bit kernels keep only the B components: copying the rounded E values cost their
+        // E pass a wave per SIMD, 71.8 -> 91-95 us at 256^3)
+        const bool half = (ctx->storage == kStorageF16 || ctx->storage == kStorageBF16) &&
+                          yee_half_supported(ctx->lattice());
+        ctx->side_active = ctx->ntff.enabled && ((ctx->storage == kStorageF32 && tp4) || half);
+        ctx->side_e_valid = ctx->storage == kStorageF32;
     }
     const LatticeArgs lat = ctx->lattice();
     ctx->ntff.pending = 0;  // a run of deferred far-field samples never spans two calls

@@ -32,6 +32,7 @@ struct SideCols {
     int x[2];     // X of face 0 and face 1
     int y0, ny;   // y range (box min y - 1 .. box max y)
     int z0, nz;   // z range
+    int e_valid;  // the E components are maintained too (else only B: read E from the planes)
 };
 
 // Per-grid constants every lattice kernel needs.

@@ -64,7 +64,8 @@ __device__ __forceinline__ void ntff_fields(const LatticeArgs &a, const FieldPtr
         const auto side = [&](int c, int x, int y, int z) {
             return base[((2 * c + (x - x1)) * sc.nz + (z - sc.z0)) * sc.ny + (y - sc.y0)];
         };
-        e_at_node_with(side, node[0], node[1], node[2], E);
+        if (sc.e_valid) e_at_node_with(side, node[0], node[1], node[2], E);
+        else e_at_node(a, f, node[0], node[1], node[2], E);
         b_at_node_with(side, node[0], node[1], node[2], H);
     } else {
         e_at_node(a, f, node[0], node[1], node[2], E);

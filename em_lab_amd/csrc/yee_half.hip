@@ -49,6 +49,30 @@ __device__ __forceinline__ float load1(const void *plane, size_t p, float s) {
     return static_cast<float>(static_cast<const T *>(plane)[p]) * s;
 }
 
+// Copies the stored B values (rounded through T, times the scale: what a reload returns) of the N
+// cells into the far-field side columns when the run holds one of them (B only: the E pass keeps
+// none, the gather reads E from the planes).
+template <typename T, int N>
+__device__ __forceinline__ void side_write_b(const LatticeArgs &a, const FieldPtrs &f, int x0, int y, int z,
+                                             const float (&v)[3][N]) {
+    const SideCols &s = a.side;
+    if (!s.buf || y < s.y0 || y >= s.y0 + s.ny || z < s.z0 || z >= s.z0 + s.nz) return;
+    if (x0 + N - 1 < s.x[0] - 1 || (x0 > s.x[0] && x0 + N - 1 < s.x[1] - 1) || x0 > s.x[1]) return;
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+#pragma unroll
+        for (int face = 0; face < 2; ++face) {
+            const long k = side_index(s, face, 3, x0 + j, y, z);
+            if (k < 0) continue;
+            const long plane = 2L * s.nz * s.ny;  // component stride
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float sc = f.scale[3 + c];
+                s.buf[k + c * plane] = static_cast<float>(static_cast<T>(v[c][j] * (1.0f / sc))) * sc;
+            }
+        }
+}
+
 template <int N>
 __device__ __forceinline__ void load_idsn(const LatticeArgs &a, size_t p, uint32_t (&id)[N]) {
     if (!a.mat_id) {
@@ -130,6 +154,7 @@ __global__ void __launch_bounds__(256) k_update_bh(LatticeArgs a, FieldPtrs f) {
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) storen<T, N>(f.b[c], p, 1.0f / f.scale[3 + c], b[c]);
+    side_write_b<T, N>(a, f, x0, y, z, b);
 }
 
 // PM: see edge_coefficients (false only for lattices without a material texel in the absorber)
