@@ -78,6 +78,29 @@ __device__ __forceinline__ void faraday(const LatticeArgs &a, int x, int y, int 
 // PM = false compiles out the case of a material texel inside the absorber (the exponentials of
 // its summed conductivity): only for lattices known to have none (LatticeArgs::pml_materials 0),
 // where the kernel then needs fewer registers.
+// Ca, Cb of a material texel (class id != 0) inside the absorber: its edge sigma plus the
+// absorber's at the E stagger (shaders/update_E.comp:46-61).
+__device__ __forceinline__ void absorber_material_coefficients(const LatticeArgs &a, int x, int y, int z, uint32_t id,
+                                                               float ca[3], float cb[3]) {
+    const float dt = a.dt;
+    float sig[3] = {0.0f, 0.0f, 0.0f}, eps[3] = {1.0f, 1.0f, 1.0f};
+    {
+        const float *t = a.table + static_cast<size_t>(id) * kTableStride;
+        sig[0] = t[kSig + 0]; sig[1] = t[kSig + 1]; sig[2] = t[kSig + 2];
+        eps[0] = t[kEps + 0]; eps[1] = t[kEps + 1]; eps[2] = t[kEps + 2];
+    }
+    sig[0] += pml_sigma(a, x, 1, y, 0, z, 0);
+    sig[1] += pml_sigma(a, x, 0, y, 1, z, 0);
+    sig[2] += pml_sigma(a, x, 0, y, 0, z, 1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float dtEps = dt / eps[c];
+        const float xx = sig[c] * dtEps;
+        ca[c] = det_expf(-xx);
+        cb[c] = dtEps * phi_f(xx);
+    }
+}
+
 template <bool PM = true>
 __device__ __forceinline__ void edge_coefficients(const LatticeArgs &a, int x, int y, int z, uint32_t id,
                                                   float ca[3], float cb[3]) {
@@ -88,22 +111,7 @@ __device__ __forceinline__ void edge_coefficients(const LatticeArgs &a, int x, i
         ca[0] = t[kPmlEa + 0]; ca[1] = t[kPmlEa + 1]; ca[2] = t[kPmlEa + 2];
         cb[0] = t[kPmlEb + 0]; cb[1] = t[kPmlEb + 1]; cb[2] = t[kPmlEb + 2];
     } else if (PM && in_pml(a, x, y, z)) {
-        float sig[3] = {0.0f, 0.0f, 0.0f}, eps[3] = {1.0f, 1.0f, 1.0f};
-        {
-            const float *t = a.table + static_cast<size_t>(id) * kTableStride;
-            sig[0] = t[kSig + 0]; sig[1] = t[kSig + 1]; sig[2] = t[kSig + 2];
-            eps[0] = t[kEps + 0]; eps[1] = t[kEps + 1]; eps[2] = t[kEps + 2];
-        }
-        sig[0] += pml_sigma(a, x, 1, y, 0, z, 0);
-        sig[1] += pml_sigma(a, x, 0, y, 1, z, 0);
-        sig[2] += pml_sigma(a, x, 0, y, 0, z, 1);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float dtEps = dt / eps[c];
-            const float xx = sig[c] * dtEps;
-            ca[c] = det_expf(-xx);
-            cb[c] = dtEps * phi_f(xx);
-        }
+        absorber_material_coefficients(a, x, y, z, id, ca, cb);
     } else if (id != 0u) {
         const float *t = a.table + static_cast<size_t>(id) * kTableStride;
         ca[0] = t[kCa + 0]; ca[1] = t[kCa + 1]; ca[2] = t[kCa + 2];

@@ -44,9 +44,26 @@ __device__ __forceinline__ void storen(void *plane, size_t p, float inv, const f
     __builtin_nontemporal_store(raw, reinterpret_cast<R *>(static_cast<T *>(plane) + p));
 }
 
+// loadn() of a neighbour that may lie beyond the lattice: zeros (+0, as the reference's zero
+// boundary) when !in. The load is unconditional, at `own` (the cell's own, always valid, index)
+// when !in, and the raw bits are selected after it: a load inside a branch was converted inside
+// it too, so the wave waited on each such load in turn instead of once for all.
+template <typename T, int N>
+__device__ __forceinline__ void loadn_or_zero(const void *plane, size_t p, size_t own, bool in, float s,
+                                              float (&o)[N]) {
+    using R = typename Raw<N>::type;
+    R raw = *reinterpret_cast<const R *>(static_cast<const T *>(plane) + (in ? p : own));
+#pragma unroll
+    for (int k = 0; k < N / 2; ++k) raw[k] = in ? raw[k] : 0u;
+    const T *h = reinterpret_cast<const T *>(&raw);
+#pragma unroll
+    for (int j = 0; j < N; ++j) o[j] = static_cast<float>(h[j]) * s;
+}
+
 template <typename T>
-__device__ __forceinline__ float load1(const void *plane, size_t p, float s) {
-    return static_cast<float>(static_cast<const T *>(plane)[p]) * s;
+__device__ __forceinline__ float load1_or_zero(const void *plane, size_t p, size_t own, bool in, float s) {
+    const T v = static_cast<const T *>(plane)[in ? p : own];
+    return in ? static_cast<float>(v) * s : 0.0f;
 }
 
 // Copies the stored B values (rounded through T, times the scale: what a reload returns) of the N
@@ -71,6 +88,35 @@ __device__ __forceinline__ void side_write_b(const LatticeArgs &a, const FieldPt
                 s.buf[k + c * plane] = static_cast<float>(static_cast<T>(v[c][j] * (1.0f / sc))) * sc;
             }
         }
+}
+
+// load_idsn() with the id width fixed at compile time (IDB: 0 no class ids, 2 or 4 bytes): a
+// run-time width branch in front of the load made the compiler wait for every field load issued
+// before it, a second round trip per wavefront.
+template <int N, int IDB>
+__device__ __forceinline__ void load_idsn_fixed(const LatticeArgs &a, size_t p, uint32_t (&id)[N]) {
+    if (IDB == 0) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) id[j] = 0u;
+    } else if (IDB == 2) {
+#pragma unroll
+        for (int q = 0; q < N / 4; ++q) {
+            const uint2 v = *reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(a.mat_id) + p + 4 * q);
+            id[4 * q + 0] = v.x & 0xffffu;
+            id[4 * q + 1] = v.x >> 16;
+            id[4 * q + 2] = v.y & 0xffffu;
+            id[4 * q + 3] = v.y >> 16;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < N / 4; ++q) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(static_cast<const uint32_t *>(a.mat_id) + p + 4 * q);
+            id[4 * q + 0] = v.x;
+            id[4 * q + 1] = v.y;
+            id[4 * q + 2] = v.z;
+            id[4 * q + 3] = v.w;
+        }
+    }
 }
 
 template <int N>
@@ -99,32 +145,127 @@ __device__ __forceinline__ void load_idsn(const LatticeArgs &a, size_t p, uint32
     }
 }
 
-// workgroup shape: TX threads along x (TX * N cells), 256 / TX rows
+// workgroup shape: TX threads along x (TX * N cells), 256 / TX rows; a wavefront holds kRw<N> rows
 template <int N>
 constexpr int kTx = N == 8 ? 32 : 64;
+template <int N>
+constexpr int kRw = 64 / kTx<N>;
+
+// Absorber coefficients of the rows a wavefront updates, staged in LDS by that wavefront: entry
+// (row r, x depth class cx) holds the six floats at offset `off` of the depth-class entry
+// (cx, class(y_r), class(z)) (B: exp(-s dt) x3, dt phi(s dt) x3; E: Ca x3, Cb x3 of a vacuum
+// edge). The entry of a texel outside the absorber holds (1, 1, 1, dt, dt, dt): 1 * b - dt * c
+// equals b - dt * c exactly, and (1, dt) are the vacuum edge's Ca, Cb, so every texel of a row
+// takes one branch-free path. A row spans the x layers at both ends, so without this every
+// wavefront ran the absorber branch (and waited on its table load) for each of its cells.
+constexpr int kCoefStride = 8;  // floats per LDS entry (6 used)
+
+// entries per wavefront region: its rows' entries, at least one per lane
+template <int N>
+__host__ __device__ __forceinline__ int coef_entries(const LatticeArgs &a) {
+    return kRw<N> * a.pml_k > 64 ? kRw<N> * a.pml_k : 64;
+}
+
+// One LDS entry of the staged rows: entry i = r * K + cx. The table load is unconditional (at a
+// valid entry) and the interior value is selected after it, so the wavefront issues these loads
+// together with its field loads and waits once.
+struct CoefLoad {
+    float v[6];
+    bool interior;
+};
+
+template <int R>
+__device__ __forceinline__ CoefLoad coef_load(const LatticeArgs &a, int i, int y_first, int z, int off) {
+    const int K = a.pml_k, L = a.pml;
+    CoefLoad c;
+    const int ii = i < R * K ? i : 0;
+    const int r = (R == 2 && ii >= K) ? 1 : 0;
+    const int cx = ii - r * K;
+    const int y = y_first + r;
+    const int cy = y < a.ny ? pml_class(y, a.ny, L) : 0, cz = pml_class(z, a.nz, L);
+    c.interior = (cx | cy | cz) == 0;
+    const float *t = a.pml_tab + static_cast<size_t>(cx + K * (cy + K * cz)) * kPmlStride + off;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) c.v[k] = t[k];
+    return c;
+}
+
+// (unconditional: a wavefront's region holds at least 64 entries, so lanes past the last entry
+// write spare ones; a conditional store let the compiler sink the table load behind the field
+// loads)
+__device__ __forceinline__ void coef_store(const LatticeArgs &a, float *lds, int i, const CoefLoad &c) {
+    float v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = c.interior ? (k < 3 ? 1.0f : a.dt) : c.v[k];
+    float *d = lds + i * kCoefStride;
+    *reinterpret_cast<float4 *>(d) = make_float4(v[0], v[1], v[2], v[3]);
+    *reinterpret_cast<float2 *>(d + 4) = make_float2(v[4], v[5]);
+}
+
+// Entries beyond the first 64 (absorbers thicker than 31 cells), then the wavefront-level
+// ordering of the entry writes before any entry read: LDS operations of one wavefront complete
+// in order, and the fences keep the compiler from moving the reads above the writes.
+template <int R>
+__device__ __forceinline__ void coef_finish(const LatticeArgs &a, float *lds, int lane, int y_first, int z, int off) {
+    for (int i = lane + 64; i < R * a.pml_k; i += 64) coef_store(a, lds, i, coef_load<R>(a, i, y_first, z, off));
+    (void)y_first;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void read_coef(const float *t, float (&lo)[3], float (&hi)[3]) {
+    const float4 u = *reinterpret_cast<const float4 *>(t);
+    const float2 w = *reinterpret_cast<const float2 *>(t + 4);
+    lo[0] = u.x; lo[1] = u.y; lo[2] = u.z;
+    hi[0] = u.w; hi[1] = w.x; hi[2] = w.y;
+}
+
+// faraday() with the texel's absorber coefficients given (ba, bb): the same operations as its
+// absorber branch, which for (1, dt) are those of its interior branch
+__device__ __forceinline__ void faraday_coef(const LatticeArgs &a, const float e0[3], const float exn[3],
+                                             const float eyn[3], const float ezn[3], uint32_t id, const float ba[3],
+                                             const float bb[3], float b[3]) {
+    float t0x = eyn[2] - e0[2], t0y = ezn[0] - e0[0], t0z = exn[1] - e0[1];
+    float t1x = ezn[1] - e0[1], t1y = exn[2] - e0[2], t1z = eyn[0] - e0[0];
+    if (a.has_wire && id != 0u) {
+        const float *w = a.table + static_cast<size_t>(id) * kTableStride + kWire;
+        t0x *= w[0];
+        t1x *= w[1];
+        t0y *= w[2];
+        t1y *= w[3];
+        t0z *= w[4];
+        t1z *= w[5];
+    }
+    const float cx = t0x - t1x, cy = t0y - t1y, cz = t0z - t1z;
+    b[0] = ba[0] * b[0] - bb[0] * cx;
+    b[1] = ba[1] * b[1] - bb[1] * cy;
+    b[2] = ba[2] * b[2] - bb[2] * cz;
+}
 
 template <typename T, int N>
 __global__ void __launch_bounds__(256) k_update_bh(LatticeArgs a, FieldPtrs f) {
+    extern __shared__ float coef_lds[];
     const int x0 = (blockIdx.x * kTx<N> + threadIdx.x) * N, y = blockIdx.y * (256 / kTx<N>) + threadIdx.y,
               z = blockIdx.z;
-    if (x0 >= a.nx || y >= a.ny) return;
+    const int wave = threadIdx.y / kRw<N>, lane = (threadIdx.y % kRw<N>) * kTx<N> + threadIdx.x;
+    const int y_first = blockIdx.y * (256 / kTx<N>) + wave * kRw<N>;
+    float *rows = coef_lds + wave * coef_entries<N>(a) * kCoefStride;
+    const CoefLoad coef0 = coef_load<kRw<N>>(a, lane, y_first, z, kPmlBa);
+    // lanes beyond the lattice load at cell (0, 0, z) and return after the staging
+    const bool active = x0 < a.nx && y < a.ny;
+    const int xa = active ? x0 : 0, ya = active ? y : 0;
     const size_t sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
-    const size_t p = z * sz + y * sy + x0;
-    const bool up_x = x0 + N < a.nx, up_y = y + 1 < a.ny, up_z = z + 1 < a.nz;
+    const size_t p = z * sz + ya * sy + xa;
+    const bool up_x = xa + N < a.nx, up_y = ya + 1 < a.ny, up_z = z + 1 < a.nz;
     float e0[3][N], eX[3], eY[3][N], eZ[3][N], b[3][N];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float s = f.scale[c];
         loadn<T, N, false>(f.e[c], p, s, e0[c]);
-        eX[c] = up_x ? load1<T>(f.e[c], p + N, s) : 0.0f;
-        if (up_y) loadn<T, N, false>(f.e[c], p + sy, s, eY[c]);
-        else
-#pragma unroll
-            for (int j = 0; j < N; ++j) eY[c][j] = 0.0f;
-        if (up_z) loadn<T, N, false>(f.e[c], p + sz, s, eZ[c]);
-        else
-#pragma unroll
-            for (int j = 0; j < N; ++j) eZ[c][j] = 0.0f;
+        eX[c] = load1_or_zero<T>(f.e[c], p + N, p, up_x, s);
+        loadn_or_zero<T, N>(f.e[c], p + sy, p, up_y, s, eY[c]);
+        loadn_or_zero<T, N>(f.e[c], p + sz, p, up_z, s, eZ[c]);
         loadn<T, N, true>(f.b[c], p, f.scale[3 + c], b[c]);
     }
     // a class only for thin-wire factors, and only where the wire mask marks one of the N texels
@@ -133,66 +274,107 @@ __global__ void __launch_bounds__(256) k_update_bh(LatticeArgs a, FieldPtrs f) {
     for (int j = 0; j < N; ++j) id[j] = 0u;
     if (a.has_wire) {
         if (a.wire_mask) {
-            const uint32_t w = a.wire_mask[(static_cast<size_t>(z) * a.ny + y) * a.mask_wpr + (x0 >> 5)];
-            if ((w >> (x0 & 31)) & ((1u << N) - 1u)) load_idsn<N>(a, p, id);
+            const uint32_t w = a.wire_mask[(static_cast<size_t>(z) * a.ny + ya) * a.mask_wpr + (xa >> 5)];
+            if ((w >> (xa & 31)) & ((1u << N) - 1u)) load_idsn<N>(a, p, id);
         } else {
             load_idsn<N>(a, p, id);
         }
     }
+    coef_store(a, rows, lane, coef0);
+    coef_finish<kRw<N>>(a, rows, lane, y_first, z, kPmlBa);
+    if (!active) return;
+    const float *row = rows + (threadIdx.y % kRw<N>) * a.pml_k * kCoefStride;
+    // thin-wire texels (rare: ids are loaded only where the wire mask marks one) take faraday()
+    // with its per-texel branches; every other lane one straight path with staged coefficients
+    bool wired = false;
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-        const float c0[3] = {e0[0][j], e0[1][j], e0[2][j]};
-        const float cx[3] = {j < N - 1 ? e0[0][j + 1] : eX[0], j < N - 1 ? e0[1][j + 1] : eX[1],
-                             j < N - 1 ? e0[2][j + 1] : eX[2]};
-        const float cy[3] = {eY[0][j], eY[1][j], eY[2][j]};
-        const float cz[3] = {eZ[0][j], eZ[1][j], eZ[2][j]};
-        float bb[3] = {b[0][j], b[1][j], b[2][j]};
-        faraday(a, x0 + j, y, z, c0, cx, cy, cz, id[j], bb);
-        b[0][j] = bb[0];
-        b[1][j] = bb[1];
-        b[2][j] = bb[2];
+    for (int j = 0; j < N; ++j) wired |= id[j] != 0u;
+    if (!wired) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const float c0[3] = {e0[0][j], e0[1][j], e0[2][j]};
+            const float cx[3] = {j < N - 1 ? e0[0][j + 1] : eX[0], j < N - 1 ? e0[1][j + 1] : eX[1],
+                                 j < N - 1 ? e0[2][j + 1] : eX[2]};
+            const float cy[3] = {eY[0][j], eY[1][j], eY[2][j]};
+            const float cz[3] = {eZ[0][j], eZ[1][j], eZ[2][j]};
+            float bb[3] = {b[0][j], b[1][j], b[2][j]}, ka[3], kb[3];
+            read_coef(row + pml_class(x0 + j, a.nx, a.pml) * kCoefStride, ka, kb);
+            faraday_coef(a, c0, cx, cy, cz, 0u, ka, kb, bb);
+            b[0][j] = bb[0];
+            b[1][j] = bb[1];
+            b[2][j] = bb[2];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const float c0[3] = {e0[0][j], e0[1][j], e0[2][j]};
+            const float cx[3] = {j < N - 1 ? e0[0][j + 1] : eX[0], j < N - 1 ? e0[1][j + 1] : eX[1],
+                                 j < N - 1 ? e0[2][j + 1] : eX[2]};
+            const float cy[3] = {eY[0][j], eY[1][j], eY[2][j]};
+            const float cz[3] = {eZ[0][j], eZ[1][j], eZ[2][j]};
+            float bb[3] = {b[0][j], b[1][j], b[2][j]};
+            faraday(a, x0 + j, y, z, c0, cx, cy, cz, id[j], bb);
+            b[0][j] = bb[0];
+            b[1][j] = bb[1];
+            b[2][j] = bb[2];
+        }
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) storen<T, N>(f.b[c], p, 1.0f / f.scale[3 + c], b[c]);
     side_write_b<T, N>(a, f, x0, y, z, b);
 }
 
-// PM: see edge_coefficients (false only for lattices without a material texel in the absorber)
-template <typename T, int N, bool PM>
+// PM: see edge_coefficients (false only for lattices without a material texel in the absorber);
+// IDB: class id width (0: no ids)
+template <typename T, int N, bool PM, int IDB>
 __global__ void __launch_bounds__(256) k_update_eh(LatticeArgs a, FieldPtrs f) {
+    extern __shared__ float coef_lds[];
     const int x0 = (blockIdx.x * kTx<N> + threadIdx.x) * N, y = blockIdx.y * (256 / kTx<N>) + threadIdx.y,
               z = blockIdx.z;
-    if (x0 >= a.nx || y >= a.ny) return;
+    const int wave = threadIdx.y / kRw<N>, lane = (threadIdx.y % kRw<N>) * kTx<N> + threadIdx.x;
+    const int y_first = blockIdx.y * (256 / kTx<N>) + wave * kRw<N>;
+    float *rows = coef_lds + wave * coef_entries<N>(a) * kCoefStride;
+    const CoefLoad coef0 = coef_load<kRw<N>>(a, lane, y_first, z, kPmlEa);
+    // lanes beyond the lattice load at cell (0, 0, z) and return after the staging
+    const bool active = x0 < a.nx && y < a.ny;
+    const int xa = active ? x0 : 0, ya = active ? y : 0;
     const size_t sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
-    const size_t p = z * sz + y * sy + x0;
-    const bool dn_x = x0 > 0, dn_y = y > 0, dn_z = z > 0;
+    const size_t p = z * sz + ya * sy + xa;
+    const bool dn_x = xa > 0, dn_y = ya > 0, dn_z = z > 0;
     float b0[3][N], bX[3], bY[3][N], bZ[3][N], e[3][N];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float s = f.scale[3 + c];
         loadn<T, N, false>(f.b[c], p, s, b0[c]);
-        bX[c] = dn_x ? load1<T>(f.b[c], p - 1, s) : 0.0f;
-        if (dn_y) loadn<T, N, false>(f.b[c], p - sy, s, bY[c]);
-        else
-#pragma unroll
-            for (int j = 0; j < N; ++j) bY[c][j] = 0.0f;
-        if (dn_z) loadn<T, N, false>(f.b[c], p - sz, s, bZ[c]);
-        else
-#pragma unroll
-            for (int j = 0; j < N; ++j) bZ[c][j] = 0.0f;
+        bX[c] = load1_or_zero<T>(f.b[c], p - 1, p, dn_x, s);
+        loadn_or_zero<T, N>(f.b[c], p - sy, p, dn_y, s, bY[c]);
+        loadn_or_zero<T, N>(f.b[c], p - sz, p, dn_z, s, bZ[c]);
         loadn<T, N, true>(f.e[c], p, f.scale[c], e[c]);
     }
     // (a class-mask gate as in the FP32 pass measured slower here: 71.8 -> 80.5 us at 256^3; the
     // mask-then-class round trip stalls the waves that hold a material texel)
     uint32_t id[N];
-    load_idsn<N>(a, p, id);
+    load_idsn_fixed<N, IDB>(a, p, id);
+    coef_store(a, rows, lane, coef0);
+    coef_finish<kRw<N>>(a, rows, lane, y_first, z, kPmlEa);
+    if (!active) return;
+    const float *row = rows + (threadIdx.y % kRw<N>) * a.pml_k * kCoefStride;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         const float bxm[3] = {j > 0 ? b0[0][j - 1] : bX[0], j > 0 ? b0[1][j - 1] : bX[1], j > 0 ? b0[2][j - 1] : bX[2]};
         const float curl[3] = {(b0[2][j] - bY[2][j]) - (b0[1][j] - bZ[1][j]), (b0[0][j] - bZ[0][j]) - (b0[2][j] - bxm[2]),
                                (b0[1][j] - bxm[1]) - (b0[0][j] - bY[0][j])};
+        // edge_coefficients() with the vacuum edges' (absorber or interior) coefficients staged
         float ca[3], cb[3];
-        edge_coefficients<PM>(a, x0 + j, y, z, id[j], ca, cb);
+        if (id[j] == 0u) {
+            read_coef(row + pml_class(x0 + j, a.nx, a.pml) * kCoefStride, ca, cb);
+        } else if (PM && in_pml(a, x0 + j, y, z)) {
+            absorber_material_coefficients(a, x0 + j, y, z, id[j], ca, cb);
+        } else {
+            const float *t = a.table + static_cast<size_t>(id[j]) * kTableStride;
+            ca[0] = t[kCa + 0]; ca[1] = t[kCa + 1]; ca[2] = t[kCa + 2];
+            cb[0] = t[kCb + 0]; cb[1] = t[kCb + 1]; cb[2] = t[kCb + 2];
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) e[c][j] = ca[c] * e[c][j] + cb[c] * curl[c];
     }
@@ -204,13 +386,21 @@ template <typename T, int N>
 hipError_t launch_t(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s) {
     constexpr int tx = kTx<N>, ty = 256 / kTx<N>;
     dim3 block(tx, ty, 1), grid((a.nx / N + tx - 1) / tx, (a.ny + ty - 1) / ty, a.nz);
-    k_update_bh<T, N><<<grid, block, 0, s>>>(a, f);
+    // staged absorber rows: 4 wavefronts x coef_entries (pml <= 64: at most 33 KB)
+    const size_t lds = static_cast<size_t>(4 * coef_entries<N>(a) * kCoefStride) * sizeof(float);
+    k_update_bh<T, N><<<grid, block, lds, s>>>(a, f);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (a.pml_materials)
-        k_update_eh<T, N, true><<<grid, block, 0, s>>>(a, f);
-    else
-        k_update_eh<T, N, false><<<grid, block, 0, s>>>(a, f);
+    const int idb = a.mat_id ? a.id_bytes : 0;
+    if (a.pml_materials) {
+        if (idb == 2) k_update_eh<T, N, true, 2><<<grid, block, lds, s>>>(a, f);
+        else if (idb == 4) k_update_eh<T, N, true, 4><<<grid, block, lds, s>>>(a, f);
+        else k_update_eh<T, N, true, 0><<<grid, block, lds, s>>>(a, f);
+    } else {
+        if (idb == 2) k_update_eh<T, N, false, 2><<<grid, block, lds, s>>>(a, f);
+        else if (idb == 4) k_update_eh<T, N, false, 4><<<grid, block, lds, s>>>(a, f);
+        else k_update_eh<T, N, false, 0><<<grid, block, lds, s>>>(a, f);
+    }
     return hipGetLastError();
 }
 
