@@ -4,7 +4,7 @@
 // even on the way back. Each thread owns N = 8 (16-byte loads and stores) or N = 4 (8-byte)
 // consecutive x cells. Requires nx % 8 == 0. Field traffic per cell-update is 24 B (half of
 // FP32), plus the x+-1 neighbours of the edge cells, which hit the same lines.
-#include "yee_device.h"
+#include "coef_rows.h"
 
 namespace emfd {
 namespace {
@@ -151,98 +151,6 @@ constexpr int kTx = N == 8 ? 32 : 64;
 template <int N>
 constexpr int kRw = 64 / kTx<N>;
 
-// Absorber coefficients of the rows a wavefront updates, staged in LDS by that wavefront: entry
-// (row r, x depth class cx) holds the six floats at offset `off` of the depth-class entry
-// (cx, class(y_r), class(z)) (B: exp(-s dt) x3, dt phi(s dt) x3; E: Ca x3, Cb x3 of a vacuum
-// edge). The entry of a texel outside the absorber holds (1, 1, 1, dt, dt, dt): 1 * b - dt * c
-// equals b - dt * c exactly, and (1, dt) are the vacuum edge's Ca, Cb, so every texel of a row
-// takes one branch-free path. A row spans the x layers at both ends, so without this every
-// wavefront ran the absorber branch (and waited on its table load) for each of its cells.
-constexpr int kCoefStride = 8;  // floats per LDS entry (6 used)
-
-// entries per wavefront region: its rows' entries, at least one per lane
-template <int N>
-__host__ __device__ __forceinline__ int coef_entries(const LatticeArgs &a) {
-    return kRw<N> * a.pml_k > 64 ? kRw<N> * a.pml_k : 64;
-}
-
-// One LDS entry of the staged rows: entry i = r * K + cx. The table load is unconditional (at a
-// valid entry) and the interior value is selected after it, so the wavefront issues these loads
-// together with its field loads and waits once.
-struct CoefLoad {
-    float v[6];
-    bool interior;
-};
-
-template <int R>
-__device__ __forceinline__ CoefLoad coef_load(const LatticeArgs &a, int i, int y_first, int z, int off) {
-    const int K = a.pml_k, L = a.pml;
-    CoefLoad c;
-    const int ii = i < R * K ? i : 0;
-    const int r = (R == 2 && ii >= K) ? 1 : 0;
-    const int cx = ii - r * K;
-    const int y = y_first + r;
-    const int cy = y < a.ny ? pml_class(y, a.ny, L) : 0, cz = pml_class(z, a.nz, L);
-    c.interior = (cx | cy | cz) == 0;
-    const float *t = a.pml_tab + static_cast<size_t>(cx + K * (cy + K * cz)) * kPmlStride + off;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) c.v[k] = t[k];
-    return c;
-}
-
-// (unconditional: a wavefront's region holds at least 64 entries, so lanes past the last entry
-// write spare ones; a conditional store let the compiler sink the table load behind the field
-// loads)
-__device__ __forceinline__ void coef_store(const LatticeArgs &a, float *lds, int i, const CoefLoad &c) {
-    float v[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) v[k] = c.interior ? (k < 3 ? 1.0f : a.dt) : c.v[k];
-    float *d = lds + i * kCoefStride;
-    *reinterpret_cast<float4 *>(d) = make_float4(v[0], v[1], v[2], v[3]);
-    *reinterpret_cast<float2 *>(d + 4) = make_float2(v[4], v[5]);
-}
-
-// Entries beyond the first 64 (absorbers thicker than 31 cells), then the wavefront-level
-// ordering of the entry writes before any entry read: LDS operations of one wavefront complete
-// in order, and the fences keep the compiler from moving the reads above the writes.
-template <int R>
-__device__ __forceinline__ void coef_finish(const LatticeArgs &a, float *lds, int lane, int y_first, int z, int off) {
-    for (int i = lane + 64; i < R * a.pml_k; i += 64) coef_store(a, lds, i, coef_load<R>(a, i, y_first, z, off));
-    (void)y_first;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ void read_coef(const float *t, float (&lo)[3], float (&hi)[3]) {
-    const float4 u = *reinterpret_cast<const float4 *>(t);
-    const float2 w = *reinterpret_cast<const float2 *>(t + 4);
-    lo[0] = u.x; lo[1] = u.y; lo[2] = u.z;
-    hi[0] = u.w; hi[1] = w.x; hi[2] = w.y;
-}
-
-// faraday() with the texel's absorber coefficients given (ba, bb): the same operations as its
-// absorber branch, which for (1, dt) are those of its interior branch
-__device__ __forceinline__ void faraday_coef(const LatticeArgs &a, const float e0[3], const float exn[3],
-                                             const float eyn[3], const float ezn[3], uint32_t id, const float ba[3],
-                                             const float bb[3], float b[3]) {
-    float t0x = eyn[2] - e0[2], t0y = ezn[0] - e0[0], t0z = exn[1] - e0[1];
-    float t1x = ezn[1] - e0[1], t1y = exn[2] - e0[2], t1z = eyn[0] - e0[0];
-    if (a.has_wire && id != 0u) {
-        const float *w = a.table + static_cast<size_t>(id) * kTableStride + kWire;
-        t0x *= w[0];
-        t1x *= w[1];
-        t0y *= w[2];
-        t1y *= w[3];
-        t0z *= w[4];
-        t1z *= w[5];
-    }
-    const float cx = t0x - t1x, cy = t0y - t1y, cz = t0z - t1z;
-    b[0] = ba[0] * b[0] - bb[0] * cx;
-    b[1] = ba[1] * b[1] - bb[1] * cy;
-    b[2] = ba[2] * b[2] - bb[2] * cz;
-}
-
 template <typename T, int N>
 __global__ void __launch_bounds__(256) k_update_bh(LatticeArgs a, FieldPtrs f) {
     extern __shared__ float coef_lds[];
@@ -250,7 +158,7 @@ __global__ void __launch_bounds__(256) k_update_bh(LatticeArgs a, FieldPtrs f) {
               z = blockIdx.z;
     const int wave = threadIdx.y / kRw<N>, lane = (threadIdx.y % kRw<N>) * kTx<N> + threadIdx.x;
     const int y_first = blockIdx.y * (256 / kTx<N>) + wave * kRw<N>;
-    float *rows = coef_lds + wave * coef_entries<N>(a) * kCoefStride;
+    float *rows = coef_lds + wave * coef_entries(a, kRw<N>) * kCoefStride;
     const CoefLoad coef0 = coef_load<kRw<N>>(a, lane, y_first, z, kPmlBa);
     // lanes beyond the lattice load at cell (0, 0, z) and return after the staging
     const bool active = x0 < a.nx && y < a.ny;
@@ -333,7 +241,7 @@ __global__ void __launch_bounds__(256) k_update_eh(LatticeArgs a, FieldPtrs f) {
               z = blockIdx.z;
     const int wave = threadIdx.y / kRw<N>, lane = (threadIdx.y % kRw<N>) * kTx<N> + threadIdx.x;
     const int y_first = blockIdx.y * (256 / kTx<N>) + wave * kRw<N>;
-    float *rows = coef_lds + wave * coef_entries<N>(a) * kCoefStride;
+    float *rows = coef_lds + wave * coef_entries(a, kRw<N>) * kCoefStride;
     const CoefLoad coef0 = coef_load<kRw<N>>(a, lane, y_first, z, kPmlEa);
     // lanes beyond the lattice load at cell (0, 0, z) and return after the staging
     const bool active = x0 < a.nx && y < a.ny;
@@ -387,7 +295,7 @@ hipError_t launch_t(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s) {
     constexpr int tx = kTx<N>, ty = 256 / kTx<N>;
     dim3 block(tx, ty, 1), grid((a.nx / N + tx - 1) / tx, (a.ny + ty - 1) / ty, a.nz);
     // staged absorber rows: 4 wavefronts x coef_entries (pml <= 64: at most 33 KB)
-    const size_t lds = static_cast<size_t>(4 * coef_entries<N>(a) * kCoefStride) * sizeof(float);
+    const size_t lds = static_cast<size_t>(4 * coef_entries(a, kRw<N>) * kCoefStride) * sizeof(float);
     k_update_bh<T, N><<<grid, block, lds, s>>>(a, f);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -5,7 +5,7 @@
 // kernel built on these bodies (single grid or batched) gives bit-identical fields.
 #pragma once
 
-#include "yee_device.h"
+#include "coef_rows.h"
 
 namespace emfd {
 
@@ -151,6 +151,132 @@ __device__ __forceinline__ void e_quad(const LatticeArgs &a, const FieldPtrs &f,
                                (b0[1][j] - bxm[1]) - (b0[0][j] - bY[0][j])};
         float ca[3], cb[3];
         edge_coefficients<PM>(a, x0 + j, y, z, id[j], ca, cb);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) e[c][j] = ca[c] * e[c][j] + cb[c] * curl[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (ST >= 1) st4s(f.e[c] + p, e[c][0], e[c][1], e[c][2], e[c][3]);
+        else st4(f.e[c] + p, e[c][0], e[c][1], e[c][2], e[c][3]);
+    }
+    side_write4(a, 0, x0, y, z, e);
+}
+
+// b_quad() for a kernel whose wavefront updates one row (64 lanes x 4 cells) with its absorber
+// coefficients staged in LDS (coef_rows.h): `rows` is the wavefront's LDS region, `lane` its lane.
+// Lanes beyond the lattice (!active) load at cell (0, y0, z) and take part in the staging only.
+// Quads without a thin-wire texel take one branch-free path (faraday_coef); the others faraday().
+template <int ST = 0>
+__device__ __forceinline__ void b_quad_rows(const LatticeArgs &a, const FieldPtrs &f, int x0, int y, int z, bool active,
+                                            float *rows, int lane) {
+    const CoefLoad coef0 = coef_load<1>(a, lane, y, z, kPmlBa);
+    const int xa = active ? x0 : 0, ya = y < a.ny ? y : 0;
+    const size_t sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
+    const size_t p = z * sz + ya * sy + xa;
+    const bool up_x = xa + 4 < a.nx, up_y = ya + 1 < a.ny, up_z = z + 1 < a.nz;
+    float e0[3][4], eX[3], eY[3][4], eZ[3][4], b[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float *e = f.e[c];
+        unpack(ld4(e + p), e0[c]);
+        eX[c] = up_x ? e[p + 4] : 0.0f;
+        if (up_y) unpack(ld4(e + p + sy), eY[c]); else eY[c][0] = eY[c][1] = eY[c][2] = eY[c][3] = 0.0f;
+        if (up_z) unpack(ld4(e + p + sz), eZ[c]); else eZ[c][0] = eZ[c][1] = eZ[c][2] = eZ[c][3] = 0.0f;
+        unpack(ST >= 2 ? ld4s(f.b[c] + p) : ld4(f.b[c] + p), b[c]);
+    }
+    uint32_t id[4] = {0u, 0u, 0u, 0u};
+    if (a.has_wire) {
+        if (a.wire_mask) {
+            const uint32_t w = a.wire_mask[(static_cast<size_t>(z) * a.ny + ya) * a.mask_wpr + (xa >> 5)];
+            if ((w >> (xa & 31)) & 0xfu) load_ids4<(ST >= 3)>(a, p, id);
+        } else {
+            load_ids4<(ST >= 3)>(a, p, id);
+        }
+    }
+    coef_store(a, rows, lane, coef0);
+    coef_finish<1>(a, rows, lane, y, z, kPmlBa);
+    if (!active) return;
+    if ((id[0] | id[1] | id[2] | id[3]) == 0u) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float c0[3] = {e0[0][j], e0[1][j], e0[2][j]};
+            const float cx[3] = {j < 3 ? e0[0][j + 1] : eX[0], j < 3 ? e0[1][j + 1] : eX[1], j < 3 ? e0[2][j + 1] : eX[2]};
+            const float cy[3] = {eY[0][j], eY[1][j], eY[2][j]};
+            const float cz[3] = {eZ[0][j], eZ[1][j], eZ[2][j]};
+            float bb[3] = {b[0][j], b[1][j], b[2][j]}, ka[3], kb[3];
+            read_coef(rows + pml_class(x0 + j, a.nx, a.pml) * kCoefStride, ka, kb);
+            faraday_coef(a, c0, cx, cy, cz, 0u, ka, kb, bb);
+            b[0][j] = bb[0];
+            b[1][j] = bb[1];
+            b[2][j] = bb[2];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float c0[3] = {e0[0][j], e0[1][j], e0[2][j]};
+            const float cx[3] = {j < 3 ? e0[0][j + 1] : eX[0], j < 3 ? e0[1][j + 1] : eX[1], j < 3 ? e0[2][j + 1] : eX[2]};
+            const float cy[3] = {eY[0][j], eY[1][j], eY[2][j]};
+            const float cz[3] = {eZ[0][j], eZ[1][j], eZ[2][j]};
+            float bb[3] = {b[0][j], b[1][j], b[2][j]};
+            faraday(a, x0 + j, y, z, c0, cx, cy, cz, id[j], bb);
+            b[0][j] = bb[0];
+            b[1][j] = bb[1];
+            b[2][j] = bb[2];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (ST >= 1) st4s(f.b[c] + p, b[c][0], b[c][1], b[c][2], b[c][3]);
+        else st4(f.b[c] + p, b[c][0], b[c][1], b[c][2], b[c][3]);
+    }
+    side_write4(a, 3, x0, y, z, b);
+}
+
+// e_quad() with staged absorber coefficients, as b_quad_rows(): vacuum edges read theirs from
+// the wavefront's LDS rows; material edges as edge_coefficients().
+template <int ST = 0, bool PM = true>
+__device__ __forceinline__ void e_quad_rows(const LatticeArgs &a, const FieldPtrs &f, int x0, int y, int z, bool active,
+                                            float *rows, int lane) {
+    const CoefLoad coef0 = coef_load<1>(a, lane, y, z, kPmlEa);
+    const int xa = active ? x0 : 0, ya = y < a.ny ? y : 0;
+    const size_t sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
+    const size_t p = z * sz + ya * sy + xa;
+    const bool dn_x = xa > 0, dn_y = ya > 0, dn_z = z > 0;
+    float b0[3][4], bX[3], bY[3][4], bZ[3][4], e[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float *bp = f.b[c];
+        unpack(ld4(bp + p), b0[c]);
+        bX[c] = dn_x ? bp[p - 1] : 0.0f;
+        if (dn_y) unpack(ld4(bp + p - sy), bY[c]); else bY[c][0] = bY[c][1] = bY[c][2] = bY[c][3] = 0.0f;
+        if (dn_z) unpack(ld4(bp + p - sz), bZ[c]); else bZ[c][0] = bZ[c][1] = bZ[c][2] = bZ[c][3] = 0.0f;
+        unpack(ST >= 2 ? ld4s(f.e[c] + p) : ld4(f.e[c] + p), e[c]);
+    }
+    uint32_t id[4] = {0u, 0u, 0u, 0u};
+    if (a.class_mask) {
+        const uint32_t w = a.class_mask[(static_cast<size_t>(z) * a.ny + ya) * a.mask_wpr + (xa >> 5)];
+        if ((w >> (xa & 31)) & 0xfu) load_ids4<(ST >= 3)>(a, p, id);
+    } else {
+        load_ids4<(ST >= 3)>(a, p, id);
+    }
+    coef_store(a, rows, lane, coef0);
+    coef_finish<1>(a, rows, lane, y, z, kPmlEa);
+    if (!active) return;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float bxm[3] = {j > 0 ? b0[0][j - 1] : bX[0], j > 0 ? b0[1][j - 1] : bX[1], j > 0 ? b0[2][j - 1] : bX[2]};
+        const float curl[3] = {(b0[2][j] - bY[2][j]) - (b0[1][j] - bZ[1][j]), (b0[0][j] - bZ[0][j]) - (b0[2][j] - bxm[2]),
+                               (b0[1][j] - bxm[1]) - (b0[0][j] - bY[0][j])};
+        float ca[3], cb[3];
+        if (id[j] == 0u) {
+            read_coef(rows + pml_class(x0 + j, a.nx, a.pml) * kCoefStride, ca, cb);
+        } else if (PM && in_pml(a, x0 + j, y, z)) {
+            absorber_material_coefficients(a, x0 + j, y, z, id[j], ca, cb);
+        } else {
+            const float *t = a.table + static_cast<size_t>(id[j]) * kTableStride;
+            ca[0] = t[kCa + 0]; ca[1] = t[kCa + 1]; ca[2] = t[kCa + 2];
+            cb[0] = t[kCb + 0]; cb[1] = t[kCb + 1]; cb[2] = t[kCb + 2];
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) e[c][j] = ca[c] * e[c][j] + cb[c] * curl[c];
     }

@@ -6,19 +6,23 @@
 namespace emfd {
 namespace {
 
+// Block (64, 4): wavefront w = row y0 + w, 256 cells; its absorber coefficients are staged in
+// its own LDS region (coef_rows.h).
 template <int ST>
 __global__ void __launch_bounds__(256) k_update_b4(LatticeArgs a, FieldPtrs f, int z_base) {
+    extern __shared__ float coef_lds[];
     const int x0 = (blockIdx.x * 64 + threadIdx.x) * 4, y = blockIdx.y * 4 + threadIdx.y, z = z_base + blockIdx.z;
-    if (x0 >= a.nx || y >= a.ny) return;
-    b_quad<ST>(a, f, x0, y, z);
+    float *rows = coef_lds + threadIdx.y * coef_entries(a, 1) * kCoefStride;
+    b_quad_rows<ST>(a, f, x0, y, z, x0 < a.nx && y < a.ny, rows, threadIdx.x);
 }
 
 // PM: see edge_coefficients (false only for lattices without a material texel in the absorber)
 template <int ST, bool PM>
 __global__ void __launch_bounds__(256) k_update_e4(LatticeArgs a, FieldPtrs f, int z_base) {
+    extern __shared__ float coef_lds[];
     const int x0 = (blockIdx.x * 64 + threadIdx.x) * 4, y = blockIdx.y * 4 + threadIdx.y, z = z_base + blockIdx.z;
-    if (x0 >= a.nx || y >= a.ny) return;
-    e_quad<ST, PM>(a, f, x0, y, z);
+    float *rows = coef_lds + threadIdx.y * coef_entries(a, 1) * kCoefStride;
+    e_quad_rows<ST, PM>(a, f, x0, y, z, x0 < a.nx && y < a.ny, rows, threadIdx.x);
 }
 
 // XCD-contiguous order: the hardware deals workgroups round-robin to the 8 XCDs; remapping the
@@ -53,13 +57,14 @@ hipError_t launch_st_xcd(const LatticeArgs &a, const FieldPtrs &f, int z0, int n
 template <int ST>
 hipError_t launch_st(const LatticeArgs &a, const FieldPtrs &f, int z0, int nzs, hipStream_t s) {
     dim3 block(64, 4, 1), grid((a.nx / 4 + 63) / 64, (a.ny + 3) / 4, nzs);
-    k_update_b4<ST><<<grid, block, 0, s>>>(a, f, z0);
+    const size_t lds = static_cast<size_t>(4 * coef_entries(a, 1) * kCoefStride) * sizeof(float);
+    k_update_b4<ST><<<grid, block, lds, s>>>(a, f, z0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (a.pml_materials)
-        k_update_e4<ST, true><<<grid, block, 0, s>>>(a, f, z0);
+        k_update_e4<ST, true><<<grid, block, lds, s>>>(a, f, z0);
     else
-        k_update_e4<ST, false><<<grid, block, 0, s>>>(a, f, z0);
+        k_update_e4<ST, false><<<grid, block, lds, s>>>(a, f, z0);
     return hipGetLastError();
 }
 
