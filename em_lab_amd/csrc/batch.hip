@@ -1,8 +1,8 @@
 // Batched stepper kernels: many independent scenes of uneven sizes advance in one launch per
 // pass. The Yee passes walk a flat list of (scene, first quad) tiles; each tile is 256 threads x
-// `quads` 4-cell x runs of one scene, and every quad goes through the same b_quad / e_quad bodies
-// as the single-grid vectorised kernel, so each scene's fields are bit-identical to a run of its
-// own. Sources and ports walk flat (scene, item) lists with the single-grid device bodies. The
+// `quads` 4-cell x runs of one scene, and every quad goes through b_quad_pre / e_quad_pre (the
+// per-texel arithmetic of the single-grid vectorised kernel), so each scene's fields are
+// bit-identical to a run of its own. Sources and ports walk flat (scene, item) lists with the single-grid device bodies. The
 // step index lives on the device (d_step) so that a captured HIP graph can be replayed.
 #include "internal.h"
 #include "port_device.h"
@@ -91,9 +91,9 @@ __global__ void __launch_bounds__(256) k_batch_yee(BatchLaunchArgs args) {
                 const uint32_t z = __umulhi(r, my);
                 const uint32_t y = r - z * static_cast<uint32_t>(lat.ny);
                 if (kB)
-                    b_quad<2>(lat, f, static_cast<int>(x0), static_cast<int>(y), static_cast<int>(z));
+                    b_quad_pre<2>(lat, f, static_cast<int>(x0), static_cast<int>(y), static_cast<int>(z));
                 else
-                    e_quad<2, PM>(lat, f, static_cast<int>(x0), static_cast<int>(y), static_cast<int>(z));
+                    e_quad_pre<2, PM>(lat, f, static_cast<int>(x0), static_cast<int>(y), static_cast<int>(z));
             }
         }
     }

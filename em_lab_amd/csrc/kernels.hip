@@ -82,6 +82,17 @@ __global__ void k_build_pml_table(LatticeArgs a, float *tab) {
         t[kPmlEb + c] = dtEps * phi_f(xx);
     }
     t[12] = t[13] = t[14] = t[15] = 0.0f;
+    if (e == 0) {
+        // no absorber on any axis: the interior coefficients, read by kernels that load every
+        // texel's entry (1 * b - dt * c is b - dt * c bit for bit; (1, dt) are the vacuum edge's)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            t[kPmlBa + c] = 1.0f;
+            t[kPmlBb + c] = dt;
+            t[kPmlEa + c] = 1.0f;
+            t[kPmlEb + c] = dt;
+        }
+    }
 }
 
 // Plain float4 copy: the bandwidth and counter calibration reference for the roofline figures.

@@ -288,4 +288,135 @@ __device__ __forceinline__ void e_quad_rows(const LatticeArgs &a, const FieldPtr
     side_write4(a, 0, x0, y, z, e);
 }
 
+// The four cells' absorber table entries, the B half (OFF = kPmlBa: exp(-s dt), dt phi(s dt)) or
+// the E half (OFF = kPmlEa: Ca, Cb of a vacuum edge), loaded unconditionally with the field
+// loads. Entry 0 (no absorber on any axis) holds the interior coefficients (1, dt) (the table
+// builder writes it so), which give the interior update bit for bit; behind a per-cell branch
+// each cell of an absorber quad waited for its own table load in turn.
+template <int OFF>
+__device__ __forceinline__ void quad_coefs(const LatticeArgs &a, int x0, int y, int z, float (&lo)[4][3],
+                                           float (&hi)[4][3]) {
+    const int K = a.pml_k, L = a.pml;
+    const int cyz = K * (pml_class(y, a.ny, L) + K * pml_class(z, a.nz, L));
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float *t = a.pml_tab + static_cast<size_t>(pml_class(x0 + j, a.nx, L) + cyz) * kPmlStride + OFF;
+        if (OFF == kPmlBa) {
+            const float4 u = *reinterpret_cast<const float4 *>(t);
+            const float2 w = *reinterpret_cast<const float2 *>(t + 4);
+            lo[j][0] = u.x; lo[j][1] = u.y; lo[j][2] = u.z;
+            hi[j][0] = u.w; hi[j][1] = w.x; hi[j][2] = w.y;
+        } else {
+            const float2 u = *reinterpret_cast<const float2 *>(t);
+            const float2 v = *reinterpret_cast<const float2 *>(t + 2);
+            const float2 w = *reinterpret_cast<const float2 *>(t + 4);
+            lo[j][0] = u.x; lo[j][1] = u.y; lo[j][2] = v.x;
+            hi[j][0] = v.y; hi[j][1] = w.x; hi[j][2] = w.y;
+        }
+    }
+}
+
+// b_quad() with the absorber coefficients loaded up front (quad_coefs): quads without a thin-wire
+// texel take one branch-free path, the others faraday(). For kernels whose wavefronts do not
+// cover whole rows (the batched stepper).
+template <int ST = 0>
+__device__ __forceinline__ void b_quad_pre(const LatticeArgs &a, const FieldPtrs &f, int x0, int y, int z) {
+    const size_t sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
+    const size_t p = z * sz + y * sy + x0;
+    const bool up_x = x0 + 4 < a.nx, up_y = y + 1 < a.ny, up_z = z + 1 < a.nz;
+    float e0[3][4], eX[3], eY[3][4], eZ[3][4], b[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float *e = f.e[c];
+        unpack(ld4(e + p), e0[c]);
+        eX[c] = up_x ? e[p + 4] : 0.0f;
+        if (up_y) unpack(ld4(e + p + sy), eY[c]); else eY[c][0] = eY[c][1] = eY[c][2] = eY[c][3] = 0.0f;
+        if (up_z) unpack(ld4(e + p + sz), eZ[c]); else eZ[c][0] = eZ[c][1] = eZ[c][2] = eZ[c][3] = 0.0f;
+        unpack(ST >= 2 ? ld4s(f.b[c] + p) : ld4(f.b[c] + p), b[c]);
+    }
+    float ka[4][3], kb[4][3];
+    quad_coefs<kPmlBa>(a, x0, y, z, ka, kb);
+    uint32_t id[4] = {0u, 0u, 0u, 0u};
+    if (a.has_wire) {
+        if (a.wire_mask) {
+            const uint32_t w = a.wire_mask[(static_cast<size_t>(z) * a.ny + y) * a.mask_wpr + (x0 >> 5)];
+            if ((w >> (x0 & 31)) & 0xfu) load_ids4<(ST >= 3)>(a, p, id);
+        } else {
+            load_ids4<(ST >= 3)>(a, p, id);
+        }
+    }
+    const bool wired = (id[0] | id[1] | id[2] | id[3]) != 0u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float c0[3] = {e0[0][j], e0[1][j], e0[2][j]};
+        const float cx[3] = {j < 3 ? e0[0][j + 1] : eX[0], j < 3 ? e0[1][j + 1] : eX[1], j < 3 ? e0[2][j + 1] : eX[2]};
+        const float cy[3] = {eY[0][j], eY[1][j], eY[2][j]};
+        const float cz[3] = {eZ[0][j], eZ[1][j], eZ[2][j]};
+        float bb[3] = {b[0][j], b[1][j], b[2][j]};
+        if (!wired) faraday_coef(a, c0, cx, cy, cz, 0u, ka[j], kb[j], bb);
+        else faraday(a, x0 + j, y, z, c0, cx, cy, cz, id[j], bb);
+        b[0][j] = bb[0];
+        b[1][j] = bb[1];
+        b[2][j] = bb[2];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (ST >= 1) st4s(f.b[c] + p, b[c][0], b[c][1], b[c][2], b[c][3]);
+        else st4(f.b[c] + p, b[c][0], b[c][1], b[c][2], b[c][3]);
+    }
+    side_write4(a, 3, x0, y, z, b);
+}
+
+// e_quad() with the vacuum edges' absorber (or interior) coefficients loaded up front; material
+// edges as edge_coefficients().
+template <int ST = 0, bool PM = true>
+__device__ __forceinline__ void e_quad_pre(const LatticeArgs &a, const FieldPtrs &f, int x0, int y, int z) {
+    const size_t sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
+    const size_t p = z * sz + y * sy + x0;
+    const bool dn_x = x0 > 0, dn_y = y > 0, dn_z = z > 0;
+    float b0[3][4], bX[3], bY[3][4], bZ[3][4], e[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float *bp = f.b[c];
+        unpack(ld4(bp + p), b0[c]);
+        bX[c] = dn_x ? bp[p - 1] : 0.0f;
+        if (dn_y) unpack(ld4(bp + p - sy), bY[c]); else bY[c][0] = bY[c][1] = bY[c][2] = bY[c][3] = 0.0f;
+        if (dn_z) unpack(ld4(bp + p - sz), bZ[c]); else bZ[c][0] = bZ[c][1] = bZ[c][2] = bZ[c][3] = 0.0f;
+        unpack(ST >= 2 ? ld4s(f.e[c] + p) : ld4(f.e[c] + p), e[c]);
+    }
+    float ka[4][3], kb[4][3];
+    quad_coefs<kPmlEa>(a, x0, y, z, ka, kb);
+    uint32_t id[4] = {0u, 0u, 0u, 0u};
+    if (a.class_mask) {
+        const uint32_t w = a.class_mask[(static_cast<size_t>(z) * a.ny + y) * a.mask_wpr + (x0 >> 5)];
+        if ((w >> (x0 & 31)) & 0xfu) load_ids4<(ST >= 3)>(a, p, id);
+    } else {
+        load_ids4<(ST >= 3)>(a, p, id);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float bxm[3] = {j > 0 ? b0[0][j - 1] : bX[0], j > 0 ? b0[1][j - 1] : bX[1], j > 0 ? b0[2][j - 1] : bX[2]};
+        const float curl[3] = {(b0[2][j] - bY[2][j]) - (b0[1][j] - bZ[1][j]), (b0[0][j] - bZ[0][j]) - (b0[2][j] - bxm[2]),
+                               (b0[1][j] - bxm[1]) - (b0[0][j] - bY[0][j])};
+        float ca[3] = {ka[j][0], ka[j][1], ka[j][2]}, cb[3] = {kb[j][0], kb[j][1], kb[j][2]};
+        if (id[j] != 0u) {
+            if (PM && in_pml(a, x0 + j, y, z)) {
+                absorber_material_coefficients(a, x0 + j, y, z, id[j], ca, cb);
+            } else {
+                const float *t = a.table + static_cast<size_t>(id[j]) * kTableStride;
+                ca[0] = t[kCa + 0]; ca[1] = t[kCa + 1]; ca[2] = t[kCa + 2];
+                cb[0] = t[kCb + 0]; cb[1] = t[kCb + 1]; cb[2] = t[kCb + 2];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) e[c][j] = ca[c] * e[c][j] + cb[c] * curl[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (ST >= 1) st4s(f.e[c] + p, e[c][0], e[c][1], e[c][2], e[c][3]);
+        else st4(f.e[c] + p, e[c][0], e[c][1], e[c][2], e[c][3]);
+    }
+    side_write4(a, 0, x0, y, z, e);
+}
+
 }  // namespace emfd
