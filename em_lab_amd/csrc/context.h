@@ -81,7 +81,7 @@ struct emfd_ctx {
     emfd::DevBuf d_src_descs;  // kept sources (FP64 stepper evaluates their signals on the device)
     int storage = emfd::kStorageF32;  // field storage (EMFD_STORAGE_*)
     bool side_active = false;         // the current step's Yee kernel writes the far-field side columns
-    bool side_e_valid = true;         // ... including the E components (the 16-bit kernels write only B)
+    bool side_e_valid = true;         // ... including the E components
     int scale_log2[6] = {0, 0, 0, 0, 0, 0};
     size_t field_bytes() const {
         return n * 6 * (storage == emfd::kStorageF32 ? 4 : storage == emfd::kStorageF64 ? 8 : 2);

@@ -647,16 +647,15 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
     EMFD_HIP(hipEventRecord(ctx->h_consts_ev[hb], ctx->stream));
 
     {
-        // the vectorised FP32 two-pass kernels copy the far-field x-face columns as they go
+        // the vectorised FP32 two-pass kernels and the 16-bit kernels copy the far-field x-face
+        // columns as they go (the 16-bit ones the stored, rounded values)
         const int v = ctx->kernel_variant;
         const bool tp4 = yee_two_pass4_supported(ctx->lattice()) &&
                          (v == EMFD_KERNEL_AUTO || v == EMFD_KERNEL_TWO_PASS4 || v == EMFD_KERNEL_SLABS);
-        // (the 16-bit kernels keep only the B components: copying the rounded E values cost their
-        // E pass a wave per SIMD, 71.8 -> 91-95 us at 256^3)
         const bool half = (ctx->storage == kStorageF16 || ctx->storage == kStorageBF16) &&
                           yee_half_supported(ctx->lattice());
         ctx->side_active = ctx->ntff.enabled && ((ctx->storage == kStorageF32 && tp4) || half);
-        ctx->side_e_valid = ctx->storage == kStorageF32;
+        ctx->side_e_valid = true;
     }
     const LatticeArgs lat = ctx->lattice();
     ctx->ntff.pending = 0;  // a run of deferred far-field samples never spans two calls

@@ -66,11 +66,10 @@ __device__ __forceinline__ float load1_or_zero(const void *plane, size_t p, size
     return in ? static_cast<float>(v) * s : 0.0f;
 }
 
-// Copies the stored B values (rounded through T, times the scale: what a reload returns) of the N
-// cells into the far-field side columns when the run holds one of them (B only: the E pass keeps
-// none, the gather reads E from the planes).
+// Copies the stored values of components comp0..comp0+2 (rounded through T, times the scale: what
+// a reload returns) of the N cells into the far-field side columns when the run holds one of them.
 template <typename T, int N>
-__device__ __forceinline__ void side_write_b(const LatticeArgs &a, const FieldPtrs &f, int x0, int y, int z,
+__device__ __forceinline__ void side_write_h(const LatticeArgs &a, const FieldPtrs &f, int comp0, int x0, int y, int z,
                                              const float (&v)[3][N]) {
     const SideCols &s = a.side;
     if (!s.buf || y < s.y0 || y >= s.y0 + s.ny || z < s.z0 || z >= s.z0 + s.nz) return;
@@ -79,12 +78,12 @@ __device__ __forceinline__ void side_write_b(const LatticeArgs &a, const FieldPt
     for (int j = 0; j < N; ++j)
 #pragma unroll
         for (int face = 0; face < 2; ++face) {
-            const long k = side_index(s, face, 3, x0 + j, y, z);
+            const long k = side_index(s, face, comp0, x0 + j, y, z);
             if (k < 0) continue;
             const long plane = 2L * s.nz * s.ny;  // component stride
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float sc = f.scale[3 + c];
+                const float sc = f.scale[comp0 + c];
                 s.buf[k + c * plane] = static_cast<float>(static_cast<T>(v[c][j] * (1.0f / sc))) * sc;
             }
         }
@@ -229,7 +228,7 @@ __global__ void __launch_bounds__(256) k_update_bh(LatticeArgs a, FieldPtrs f) {
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) storen<T, N>(f.b[c], p, 1.0f / f.scale[3 + c], b[c]);
-    side_write_b<T, N>(a, f, x0, y, z, b);
+    side_write_h<T, N>(a, f, 3, x0, y, z, b);
 }
 
 // PM: see edge_coefficients (false only for lattices without a material texel in the absorber);
@@ -288,6 +287,7 @@ __global__ void __launch_bounds__(256) k_update_eh(LatticeArgs a, FieldPtrs f) {
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) storen<T, N>(f.e[c], p, 1.0f / f.scale[c], e[c]);
+    side_write_h<T, N>(a, f, 0, x0, y, z, e);
 }
 
 template <typename T, int N>
