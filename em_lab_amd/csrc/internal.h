@@ -57,6 +57,29 @@ struct LatticeArgs {
     int pml_materials;          // some texel of class != 0 lies in the absorber (0 only when known)
 };
 
+// Writes value(c, j) as component comp0 + c (c = 0..2) of the cells x0 + j (j = 0..N-1) of row
+// (y, z) that are side columns: the index arithmetic of side_index() hoisted out of the cell loop,
+// since every wavefront whose rows cross the box runs this (the lanes at the faces divergently).
+template <int N, class F>
+__device__ __forceinline__ void side_write_cells(const SideCols &s, int comp0, int x0, int y, int z, F value) {
+    if (!s.buf || y < s.y0 || y >= s.y0 + s.ny || z < s.z0 || z >= s.z0 + s.nz) return;
+    const long col = static_cast<long>(s.nz) * s.ny;  // one x column of one component
+    const long yz = static_cast<long>(z - s.z0) * s.ny + (y - s.y0);
+#pragma unroll
+    for (int face = 0; face < 2; ++face) {
+        const int xo0 = x0 - (s.x[face] - 1);  // column offset of cell j: xo0 + j, kept for 0 or 1
+        if (xo0 > 1 || xo0 + N - 1 < 0) continue;
+        const long kf = static_cast<long>(face * 6 + comp0) * 2 * col + yz;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const int xo = xo0 + j;
+            if (xo < 0 || xo > 1) continue;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s.buf[kf + (2 * c + xo) * col] = value(c, j);
+        }
+    }
+}
+
 // Index of (face, comp, x, y, z) in the side columns, or -1 when (x, y, z) is not a side column.
 __host__ __device__ inline long side_index(const SideCols &s, int face, int comp, int x, int y, int z) {
     const int xo = x - (s.x[face] - 1);

@@ -71,22 +71,10 @@ __device__ __forceinline__ float load1_or_zero(const void *plane, size_t p, size
 template <typename T, int N>
 __device__ __forceinline__ void side_write_h(const LatticeArgs &a, const FieldPtrs &f, int comp0, int x0, int y, int z,
                                              const float (&v)[3][N]) {
-    const SideCols &s = a.side;
-    if (!s.buf || y < s.y0 || y >= s.y0 + s.ny || z < s.z0 || z >= s.z0 + s.nz) return;
-    if (x0 + N - 1 < s.x[0] - 1 || (x0 > s.x[0] && x0 + N - 1 < s.x[1] - 1) || x0 > s.x[1]) return;
-#pragma unroll
-    for (int j = 0; j < N; ++j)
-#pragma unroll
-        for (int face = 0; face < 2; ++face) {
-            const long k = side_index(s, face, comp0, x0 + j, y, z);
-            if (k < 0) continue;
-            const long plane = 2L * s.nz * s.ny;  // component stride
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float sc = f.scale[comp0 + c];
-                s.buf[k + c * plane] = static_cast<float>(static_cast<T>(v[c][j] * (1.0f / sc))) * sc;
-            }
-        }
+    side_write_cells<N>(a.side, comp0, x0, y, z, [&](int c, int j) {
+        const float sc = f.scale[comp0 + c];
+        return static_cast<float>(static_cast<T>(v[c][j] * (1.0f / sc))) * sc;
+    });
 }
 
 // load_idsn() with the id width fixed at compile time (IDB: 0 no class ids, 2 or 4 bytes): a
