@@ -58,7 +58,10 @@ __device__ __forceinline__ FieldPtrs fields_from(ConstPtr<FieldPtrs> p) {
     }
     f.storage = p->storage;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) f.scale[c] = p->scale[c];
+    for (int c = 0; c < 6; ++c) {
+        f.scale[c] = p->scale[c];
+        f.inv_scale[c] = p->inv_scale[c];
+    }
     return f;
 }
 

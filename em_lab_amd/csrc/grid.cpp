@@ -296,7 +296,10 @@ emfd::FieldPtrs emfd_ctx::fields(int which) const {
         f.b[c] = reinterpret_cast<float *>(base + static_cast<size_t>(3 + c) * n);
     }
     f.storage = storage;
-    for (int c = 0; c < 6; ++c) f.scale[c] = std::ldexp(1.0f, scale_log2[c]);
+    for (int c = 0; c < 6; ++c) {
+        f.scale[c] = std::ldexp(1.0f, scale_log2[c]);
+        f.inv_scale[c] = std::ldexp(1.0f, -scale_log2[c]);
+    }
     return f;
 }
 

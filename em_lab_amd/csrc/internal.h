@@ -103,6 +103,7 @@ struct FieldPtrs {
     float *b[3];
     int storage = kStorageF32;
     float scale[6] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f};  // E x y z, B x y z: value = stored * scale
+    float inv_scale[6] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f};  // 1 / scale (exact: powers of two)
 };
 
 // Device-side step constants: per step k, nsrc signals, then port rotation (4), then the

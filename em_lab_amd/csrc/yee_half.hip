@@ -60,6 +60,31 @@ __device__ __forceinline__ void loadn_or_zero(const void *plane, size_t p, size_
     for (int j = 0; j < N; ++j) o[j] = static_cast<float>(h[j]) * s;
 }
 
+// loadn_or_zero() without the conversion: the N stored values themselves (+0 when !in)
+template <typename T, int N>
+__device__ __forceinline__ void loadn_raw_or_zero(const void *plane, size_t p, size_t own, bool in, T (&o)[N]) {
+    using R = typename Raw<N>::type;
+    R raw = *reinterpret_cast<const R *>(static_cast<const T *>(plane) + (in ? p : own));
+#pragma unroll
+    for (int k = 0; k < N / 2; ++k) raw[k] = in ? raw[k] : 0u;
+    const T *h = reinterpret_cast<const T *>(&raw);
+#pragma unroll
+    for (int j = 0; j < N; ++j) o[j] = h[j];
+}
+
+// c - h * s as the separate operations round it (h * s is exact: s is a power of two), in one
+// mixed-precision fused multiply-add
+template <typename T>
+__device__ __forceinline__ float minus_scaled(float c, T h, float s) {
+    return __builtin_fmaf(-static_cast<float>(h), s, c);
+}
+
+// h * s - c as the separate operations round it, in one mixed-precision fused multiply-add
+template <typename T>
+__device__ __forceinline__ float scaled_minus(T h, float s, float c) {
+    return __builtin_fmaf(static_cast<float>(h), s, -c);
+}
+
 template <typename T>
 __device__ __forceinline__ float load1_or_zero(const void *plane, size_t p, size_t own, bool in, float s) {
     const T v = static_cast<const T *>(plane)[in ? p : own];
@@ -72,8 +97,7 @@ template <typename T, int N>
 __device__ __forceinline__ void side_write_h(const LatticeArgs &a, const FieldPtrs &f, int comp0, int x0, int y, int z,
                                              const float (&v)[3][N]) {
     side_write_cells<N>(a.side, comp0, x0, y, z, [&](int c, int j) {
-        const float sc = f.scale[comp0 + c];
-        return static_cast<float>(static_cast<T>(v[c][j] * (1.0f / sc))) * sc;
+        return static_cast<float>(static_cast<T>(v[c][j] * f.inv_scale[comp0 + c])) * f.scale[comp0 + c];
     });
 }
 
@@ -153,16 +177,21 @@ __global__ void __launch_bounds__(256) k_update_bh(LatticeArgs a, FieldPtrs f) {
     const size_t sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
     const size_t p = z * sz + ya * sy + xa;
     const bool up_x = xa + N < a.nx, up_y = ya + 1 < a.ny, up_z = z + 1 < a.nz;
-    float e0[3][N], eX[3], eY[3][N], eZ[3][N], b[3][N];
+    // the +y and +z neighbours enter the curl once each, as neighbour - e0: kept as stored values
+    // and folded into scaled_minus(); the +y neighbour's y component and the +z neighbour's z
+    // component are not used
+    float e0[3][N], eX[3], b[3][N];
+    T hY[3][N], hZ[3][N];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float s = f.scale[c];
         loadn<T, N, false>(f.e[c], p, s, e0[c]);
         eX[c] = load1_or_zero<T>(f.e[c], p + N, p, up_x, s);
-        loadn_or_zero<T, N>(f.e[c], p + sy, p, up_y, s, eY[c]);
-        loadn_or_zero<T, N>(f.e[c], p + sz, p, up_z, s, eZ[c]);
+        if (c != 1) loadn_raw_or_zero<T, N>(f.e[c], p + sy, p, up_y, hY[c]);
+        if (c != 2) loadn_raw_or_zero<T, N>(f.e[c], p + sz, p, up_z, hZ[c]);
         loadn<T, N, true>(f.b[c], p, f.scale[3 + c], b[c]);
     }
+    const float sE[3] = {f.scale[0], f.scale[1], f.scale[2]};
     // a class only for thin-wire factors, and only where the wire mask marks one of the N texels
     uint32_t id[N];
 #pragma unroll
@@ -187,17 +216,18 @@ __global__ void __launch_bounds__(256) k_update_bh(LatticeArgs a, FieldPtrs f) {
     if (!wired) {
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-            const float c0[3] = {e0[0][j], e0[1][j], e0[2][j]};
-            const float cx[3] = {j < N - 1 ? e0[0][j + 1] : eX[0], j < N - 1 ? e0[1][j + 1] : eX[1],
-                                 j < N - 1 ? e0[2][j + 1] : eX[2]};
-            const float cy[3] = {eY[0][j], eY[1][j], eY[2][j]};
-            const float cz[3] = {eZ[0][j], eZ[1][j], eZ[2][j]};
-            float bb[3] = {b[0][j], b[1][j], b[2][j]}, ka[3], kb[3];
+            // faraday_coef() without wire factors: the same terms, in the same order
+            const float xn1 = j < N - 1 ? e0[1][j + 1] : eX[1], xn2 = j < N - 1 ? e0[2][j + 1] : eX[2];
+            const float t0x = scaled_minus(hY[2][j], sE[2], e0[2][j]), t0y = scaled_minus(hZ[0][j], sE[0], e0[0][j]),
+                        t0z = xn1 - e0[1][j];
+            const float t1x = scaled_minus(hZ[1][j], sE[1], e0[1][j]), t1y = xn2 - e0[2][j],
+                        t1z = scaled_minus(hY[0][j], sE[0], e0[0][j]);
+            const float cx = t0x - t1x, cy = t0y - t1y, cz = t0z - t1z;
+            float ka[3], kb[3];
             read_coef(row + pml_class(x0 + j, a.nx, a.pml) * kCoefStride, ka, kb);
-            faraday_coef(a, c0, cx, cy, cz, 0u, ka, kb, bb);
-            b[0][j] = bb[0];
-            b[1][j] = bb[1];
-            b[2][j] = bb[2];
+            b[0][j] = ka[0] * b[0][j] - kb[0] * cx;
+            b[1][j] = ka[1] * b[1][j] - kb[1] * cy;
+            b[2][j] = ka[2] * b[2][j] - kb[2] * cz;
         }
     } else {
 #pragma unroll
@@ -205,8 +235,8 @@ __global__ void __launch_bounds__(256) k_update_bh(LatticeArgs a, FieldPtrs f) {
             const float c0[3] = {e0[0][j], e0[1][j], e0[2][j]};
             const float cx[3] = {j < N - 1 ? e0[0][j + 1] : eX[0], j < N - 1 ? e0[1][j + 1] : eX[1],
                                  j < N - 1 ? e0[2][j + 1] : eX[2]};
-            const float cy[3] = {eY[0][j], eY[1][j], eY[2][j]};
-            const float cz[3] = {eZ[0][j], eZ[1][j], eZ[2][j]};
+            const float cy[3] = {static_cast<float>(hY[0][j]) * sE[0], 0.0f, static_cast<float>(hY[2][j]) * sE[2]};
+            const float cz[3] = {static_cast<float>(hZ[0][j]) * sE[0], static_cast<float>(hZ[1][j]) * sE[1], 0.0f};
             float bb[3] = {b[0][j], b[1][j], b[2][j]};
             faraday(a, x0 + j, y, z, c0, cx, cy, cz, id[j], bb);
             b[0][j] = bb[0];
@@ -215,7 +245,7 @@ __global__ void __launch_bounds__(256) k_update_bh(LatticeArgs a, FieldPtrs f) {
         }
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) storen<T, N>(f.b[c], p, 1.0f / f.scale[3 + c], b[c]);
+    for (int c = 0; c < 3; ++c) storen<T, N>(f.b[c], p, f.inv_scale[3 + c], b[c]);
     side_write_h<T, N>(a, f, 3, x0, y, z, b);
 }
 
@@ -236,16 +266,21 @@ __global__ void __launch_bounds__(256) k_update_eh(LatticeArgs a, FieldPtrs f) {
     const size_t sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
     const size_t p = z * sz + ya * sy + xa;
     const bool dn_x = xa > 0, dn_y = ya > 0, dn_z = z > 0;
-    float b0[3][N], bX[3], bY[3][N], bZ[3][N], e[3][N];
+    // the -y and -z neighbours enter the curl once each, as b0 - neighbour: kept as stored values
+    // and folded into minus_scaled(); the -y neighbour's y component and the -z neighbour's z
+    // component are not used
+    float b0[3][N], bX[3], e[3][N];
+    T hY[3][N], hZ[3][N];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float s = f.scale[3 + c];
         loadn<T, N, false>(f.b[c], p, s, b0[c]);
         bX[c] = load1_or_zero<T>(f.b[c], p - 1, p, dn_x, s);
-        loadn_or_zero<T, N>(f.b[c], p - sy, p, dn_y, s, bY[c]);
-        loadn_or_zero<T, N>(f.b[c], p - sz, p, dn_z, s, bZ[c]);
+        if (c != 1) loadn_raw_or_zero<T, N>(f.b[c], p - sy, p, dn_y, hY[c]);
+        if (c != 2) loadn_raw_or_zero<T, N>(f.b[c], p - sz, p, dn_z, hZ[c]);
         loadn<T, N, true>(f.e[c], p, f.scale[c], e[c]);
     }
+    const float sB[3] = {f.scale[3], f.scale[4], f.scale[5]};
     // (a class-mask gate as in the FP32 pass measured slower here: 71.8 -> 80.5 us at 256^3; the
     // mask-then-class round trip stalls the waves that hold a material texel)
     uint32_t id[N];
@@ -257,8 +292,9 @@ __global__ void __launch_bounds__(256) k_update_eh(LatticeArgs a, FieldPtrs f) {
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         const float bxm[3] = {j > 0 ? b0[0][j - 1] : bX[0], j > 0 ? b0[1][j - 1] : bX[1], j > 0 ? b0[2][j - 1] : bX[2]};
-        const float curl[3] = {(b0[2][j] - bY[2][j]) - (b0[1][j] - bZ[1][j]), (b0[0][j] - bZ[0][j]) - (b0[2][j] - bxm[2]),
-                               (b0[1][j] - bxm[1]) - (b0[0][j] - bY[0][j])};
+        const float curl[3] = {minus_scaled(b0[2][j], hY[2][j], sB[2]) - minus_scaled(b0[1][j], hZ[1][j], sB[1]),
+                               minus_scaled(b0[0][j], hZ[0][j], sB[0]) - (b0[2][j] - bxm[2]),
+                               (b0[1][j] - bxm[1]) - minus_scaled(b0[0][j], hY[0][j], sB[0])};
         // edge_coefficients() with the vacuum edges' (absorber or interior) coefficients staged
         float ca[3], cb[3];
         if (id[j] == 0u) {
@@ -274,7 +310,7 @@ __global__ void __launch_bounds__(256) k_update_eh(LatticeArgs a, FieldPtrs f) {
         for (int c = 0; c < 3; ++c) e[c][j] = ca[c] * e[c][j] + cb[c] * curl[c];
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) storen<T, N>(f.e[c], p, 1.0f / f.scale[c], e[c]);
+    for (int c = 0; c < 3; ++c) storen<T, N>(f.e[c], p, f.inv_scale[c], e[c]);
     side_write_h<T, N>(a, f, 0, x0, y, z, e);
 }
 
