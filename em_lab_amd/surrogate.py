@@ -93,14 +93,30 @@ def bandwidth_10db(s11: np.ndarray, nu: np.ndarray) -> np.ndarray:
     return best * (nu[1] - nu[0])
 
 
-def targets_from_z(z: np.ndarray) -> np.ndarray:
-    """(designs, 64) complex Z -> (designs, 128) network targets: Re Gamma then Im Gamma."""
+def targets_from_z(z: np.ndarray, power: float = 0.0) -> np.ndarray:
+    """(designs, 64) complex Z -> (designs, 128) network targets: Re then Im of the companded
+    reflection coefficient Gamma |Gamma|^-power (power 0: Gamma itself). With power > 0 the
+    squared error weighs a deviation at small |Gamma| (a deep resonance) more, closer to the
+    screening test, which is in dB: a deviation d at |Gamma| costs about 8.69 d / |Gamma| dB."""
     g = z_to_gamma(z)
+    if power:
+        a = np.abs(g)
+        g = np.where(a > 0, g * np.power(np.maximum(a, 1e-300), -power), 0)
     return np.concatenate([g.real, g.imag], axis=1).astype(np.float32)
 
 
-def gamma_from_output(y: np.ndarray) -> np.ndarray:
-    return y[:, :N_BINS].astype(np.float64) + 1j * y[:, N_BINS:].astype(np.float64)
+def gamma_from_output(y: np.ndarray, power: float = 0.0) -> np.ndarray:
+    """Inverse of targets_from_z: |output| = |Gamma|^(1 - power), same phase."""
+    o = y[:, :N_BINS].astype(np.float64) + 1j * y[:, N_BINS:].astype(np.float64)
+    if power:
+        a = np.abs(o)
+        o = o * np.power(np.maximum(a, 1e-300), power / (1.0 - power))
+    return o
+
+
+def output_power(m) -> float:
+    """The target companding power a loaded or trained surrogate was trained with."""
+    return float(getattr(m, "target_power", 0.0))
 
 
 def s11_error_db(s11_a: np.ndarray, s11_b: np.ndarray) -> np.ndarray:
@@ -170,15 +186,16 @@ def load_data(paths):
     return np.concatenate(X), np.concatenate(Z), np.concatenate(R)
 
 
-def train_surrogate(X, Z, epochs=400, batch=256, lr=(2e-3, 2e-5), seed=1, holdout=1024, log=None):
-    """Trains the surrogate on the GPU (emfd_mlp_*) with the last `holdout` designs held out.
-    Returns (Mlp, report dict)."""
+def train_surrogate(X, Z, epochs=400, batch=256, lr=(2e-3, 2e-5), seed=1, holdout=1024, log=None, power=0.0):
+    """Trains the surrogate on the GPU (emfd_mlp_*) with the last `holdout` designs held out, on
+    targets companded with `power` (targets_from_z). Returns (Mlp, report dict)."""
     from em_lab_amd import emfdtd as E
-    T = targets_from_z(Z)
+    T = targets_from_z(Z, power)
     ntr = X.shape[0] - holdout
     m = E.Mlp(DIMS, max_batch=65536)
     m.init_params(seed=seed)
     m.set_adam(0.9, 0.999, 1e-8, 0.0)
+    m.target_power = float(power)
     m.set_dataset(X[:ntr], T[:ntr])
     t0 = time.time()
     losses = m.train(batch=batch, epochs=epochs, lr_start=lr[0], lr_end=lr[1], seed=seed)
@@ -187,7 +204,7 @@ def train_surrogate(X, Z, epochs=400, batch=256, lr=(2e-3, 2e-5), seed=1, holdou
                train_designs=int(ntr))
     if holdout:
         y = m.forward(X[ntr:])
-        s_pred = s11_db(gamma_from_output(y))
+        s_pred = s11_db(gamma_from_output(y, power))
         s_true = s11_db(z_to_gamma(Z[ntr:]))
         err = s11_error_db(s_pred, s_true)
         rep.update(holdout=int(holdout), holdout_mse=float(np.mean((y - T[ntr:]) ** 2)),
@@ -204,6 +221,7 @@ class Ensemble:
     def __init__(self, members):
         self.members = list(members)
         self.dims = self.members[0].dims
+        self.target_power = output_power(self.members[0])
 
     def forward(self, X):
         y = self.members[0].forward(X).astype(np.float64)
@@ -222,12 +240,13 @@ class Ensemble:
             m.close()
 
 
-def train_ensemble(X, Z, members=4, epochs=600, batch=256, lr=(2e-3, 2e-5), holdout=1024, log=None):
+def train_ensemble(X, Z, members=4, epochs=600, batch=256, lr=(2e-3, 2e-5), holdout=1024, log=None, power=0.0):
     """`members` networks trained from different seeds on the same data; the report scores their
     averaged prediction on the held-out designs."""
     nets, reps = [], []
     for k in range(members):
-        m, rep = train_surrogate(X, Z, epochs=epochs, batch=batch, lr=lr, seed=1 + k, holdout=holdout, log=log)
+        m, rep = train_surrogate(X, Z, epochs=epochs, batch=batch, lr=lr, seed=1 + k, holdout=holdout, log=log,
+                                 power=power)
         nets.append(m)
         reps.append(rep)
     ens = Ensemble(nets)
@@ -236,7 +255,7 @@ def train_ensemble(X, Z, members=4, epochs=600, batch=256, lr=(2e-3, 2e-5), hold
     if holdout:
         ntr = X.shape[0] - holdout
         y = ens.forward(X[ntr:])
-        err = s11_error_db(s11_db(gamma_from_output(y)), s11_db(z_to_gamma(Z[ntr:])))
+        err = s11_error_db(s11_db(gamma_from_output(y, power)), s11_db(z_to_gamma(Z[ntr:])))
         rep.update(holdout_s11_err_median_db=float(np.median(err)), holdout_s11_err_p90_db=float(np.percentile(err, 90)),
                    holdout_within_0p5db=float(np.mean(err <= 0.5)))
     if log:
@@ -247,7 +266,7 @@ def train_ensemble(X, Z, members=4, epochs=600, batch=256, lr=(2e-3, 2e-5), hold
 def save_weights(m, path: Path = WEIGHTS, report=None) -> None:
     """Parameters of one network (params: n) or of an ensemble (params: members x n)."""
     path.parent.mkdir(parents=True, exist_ok=True)
-    np.savez_compressed(path, params=m.params(), dims=np.array(m.dims),
+    np.savez_compressed(path, params=m.params(), dims=np.array(m.dims), target_power=np.array(output_power(m)),
                         report=np.array(repr(report or {})))
 
 
@@ -256,10 +275,12 @@ def load_surrogate(path: Path = WEIGHTS, max_batch: int = 65536):
     d = np.load(path)
     dims = tuple(int(v) for v in d["dims"])
     params = np.atleast_2d(d["params"])
+    power = float(d["target_power"]) if "target_power" in d.files else 0.0
     nets = []
     for p in params:
         m = E.Mlp(dims, max_batch=max_batch)
         m.set_params(p)
+        m.target_power = power
         nets.append(m)
     return nets[0] if len(nets) == 1 else Ensemble(nets)
 
@@ -275,7 +296,7 @@ def screen(m, X: np.ndarray, top: int = 256):
     """Surrogate S11 of every candidate and the `top` by predicted -10 dB bandwidth (ties broken
     by the lowest predicted S11)."""
     y = m.forward(X)
-    s = s11_db(gamma_from_output(y))
+    s = s11_db(gamma_from_output(y, output_power(m)))
     bw = bandwidth_10db(s, sweep_nu())
     order = np.lexsort((s.min(axis=1), -bw))
     return order[:top], s, bw
@@ -301,8 +322,9 @@ def mine_hard(count: int, pool: int, out: Path, seed: int = HARD_SEED, batch_siz
     X = features_many(geoms)
     chunk = 65536
     y = np.concatenate([m.forward(X[i:i + chunk]) for i in range(0, pool, chunk)])
+    power = output_power(m)
     m.close()
-    s = s11_db(gamma_from_output(y))
+    s = s11_db(gamma_from_output(y, power))
     bw = bandwidth_10db(s, sweep_nu())
     idx = np.lexsort((s.min(axis=1), -bw))[:count]
     z, res = run_fdtd(lambda i: array_scene(geoms[idx[i]], sizes[idx[i]]), len(idx), STEPS, batch_size, log=log)
@@ -331,13 +353,14 @@ def plain_fdtd_rate(n: int = 4, seed: int = CANDIDATE_SEED, steps: int = STEPS) 
     return n / (time.time() - t)
 
 
-def screening_run(n_candidates: int = 65536, top: int = 256, plain_designs: int = 4, log=None):
+def screening_run(n_candidates: int = 65536, top: int = 256, plain_designs: int = 4, log=None,
+                  weights: Path = WEIGHTS):
     """The learned screening pipeline end to end: candidates -> surrogate -> top designs ->
     batched FDTD verification. Returns a report with designs/s against one-at-a-time FDTD."""
     t0 = time.time()
     geoms, sizes, X = candidates(n_candidates)
     t1 = time.time()
-    m = load_surrogate()
+    m = load_surrogate(weights)
     t2 = time.time()
     idx, s_pred, bw_pred = screen(m, X, top)
     t3 = time.time()
@@ -380,6 +403,8 @@ def main(argv=None) -> int:
     tr.add_argument("--batch", type=int, default=256)
     tr.add_argument("--members", type=int, default=1, help="ensemble size (networks averaged)")
     tr.add_argument("--out", type=Path, default=WEIGHTS)
+    tr.add_argument("--target-power", type=float, default=0.0,
+                    help="train on Gamma |Gamma|^-p (0: Gamma itself; see targets_from_z)")
     mh = sub.add_parser("mine", help="hard examples: the best-ranked designs of a separate candidate pool, by FDTD")
     mh.add_argument("--count", type=int, default=2048)
     mh.add_argument("--pool", type=int, default=262144)
@@ -388,19 +413,21 @@ def main(argv=None) -> int:
     sc = sub.add_parser("screen", help="screen seeded candidates and verify the best by batched FDTD")
     sc.add_argument("--candidates", type=int, default=65536)
     sc.add_argument("--top", type=int, default=256)
+    sc.add_argument("--weights", type=Path, default=WEIGHTS)
     args = ap.parse_args(argv)
     if args.cmd == "train":
         X, Z, _ = load_data(args.data)
         if args.members > 1:
             m, rep = train_ensemble(X, Z, members=args.members, epochs=args.epochs, batch=args.batch,
-                                    log=lambda s: print(s, flush=True))
+                                    log=lambda s: print(s, flush=True), power=args.target_power)
         else:
-            m, rep = train_surrogate(X, Z, epochs=args.epochs, batch=args.batch, log=lambda s: print(s, flush=True))
+            m, rep = train_surrogate(X, Z, epochs=args.epochs, batch=args.batch, log=lambda s: print(s, flush=True),
+                                     power=args.target_power)
         save_weights(m, args.out, rep)
         print(f"saved {args.out}", flush=True)
         return 0
     if args.cmd == "screen":
-        screening_run(args.candidates, args.top, log=lambda s: print(s, flush=True))
+        screening_run(args.candidates, args.top, log=lambda s: print(s, flush=True), weights=args.weights)
         return 0
     if args.cmd == "mine":
         t = time.time()

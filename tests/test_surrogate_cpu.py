@@ -21,6 +21,26 @@ def test_gamma_z_round_trip_and_s11():
     assert S.s11_db(np.array([0.1]))[0] == pytest.approx(-20.0)
 
 
+@pytest.mark.parametrize("power", [0.0, 0.5, 0.25])
+def test_companded_targets_invert(power):
+    """targets_from_z / gamma_from_output are inverse at every companding power, keep the phase,
+    map |Gamma| to |Gamma|^(1 - power) and Gamma = 0 to 0."""
+    rng = np.random.default_rng(3)
+    z = rng.uniform(1, 400, (5, S.N_BINS)) + 1j * rng.uniform(-600, 600, (5, S.N_BINS))
+    z[0, 0] = S.R0  # Gamma exactly 0
+    t = S.targets_from_z(z, power)
+    assert t.shape == (5, 2 * S.N_BINS) and t.dtype == np.float32
+    g = S.z_to_gamma(z)
+    o = t[:, :S.N_BINS] + 1j * t[:, S.N_BINS:]
+    assert o[0, 0] == 0
+    m = np.abs(g) > 1e-6
+    assert np.allclose(np.abs(o[m]), np.abs(g[m]) ** (1.0 - power), rtol=1e-5)
+    assert np.allclose(np.angle(o[m]), np.angle(g[m]), atol=1e-5)
+    back = S.gamma_from_output(t, power)
+    assert np.allclose(back, g, rtol=2e-5, atol=1e-6)
+    assert back[0, 0] == 0
+
+
 def test_bandwidth_matches_brute_force():
     rng = np.random.default_rng(1)
     s = rng.uniform(-20, 0, (200, 64))
