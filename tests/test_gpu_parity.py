@@ -257,6 +257,23 @@ def test_material_in_absorber():
         g2.close()
 
 
+def test_thick_absorber_partial_rows():
+    """A 36-cell absorber (74 depth classes per axis: the per-wavefront coefficient staging takes
+    more than one pass over the lanes) on a 96-wide lattice (a row fills 24 of a wavefront's 64
+    lanes), with dielectric boxes reaching into the absorber: the default kernel matches the
+    oracle bit for bit."""
+    print("case: 96^3, 36-cell absorber, materials in it", flush=True)
+    sc = absorber_material_scene(n=96, pml=36)
+    g, _, _ = gpu_grid(sc, ntff=False, ports=False)
+    sim = oracle_sim(sc, ntff=False, ports=False)
+    g.step(30)
+    sim.step(30)
+    _fields_match(g, sim)
+    e1, _ = g.fields(E.LAYOUT_SOA)
+    assert np.abs(e1).max() > 0
+    g.close()
+
+
 def test_far_field_runs_bit_identical():
     """Far-field phasors accumulated in runs of up to sixteen steps inside one step call equal, bit
     for bit, those of one accumulation per step (single-step calls), for two wavelengths and call

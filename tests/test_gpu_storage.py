@@ -75,6 +75,26 @@ def test_material_in_absorber_storage(storage):
     g.close()
 
 
+@pytest.mark.parametrize("storage", [E.STORAGE_F16, E.STORAGE_BF16])
+def test_thick_absorber_partial_rows_storage(storage):
+    """16-bit kernels with a 36-cell absorber (two rows of 74 depth classes staged per wavefront:
+    more than one pass over the lanes) on a 96-wide lattice (a row fills 12 of 32 lanes) and
+    dielectric boxes reaching into the absorber: bit for bit against the oracle."""
+    print("case: 96^3, 36-cell absorber, storage", storage, flush=True)
+    sc = absorber_material_scene(n=96, pml=36)
+    k = [2, 2, 2, -3, -3, -3]
+    g, _, _ = gpu_grid(sc, ntff=False, ports=False)
+    g.set_storage(storage, k)
+    sim = oracle_sim(sc, ntff=False, ports=False)
+    sim.set_storage(storage, k)
+    g.step(30)
+    sim.step(30)
+    Eg, Bg = g.fields(E.LAYOUT_TEXEL)
+    assert np.array_equal(Eg[:, :3], sim.E[:, :3]) and np.array_equal(Bg[:, :3], sim.B[:, :3])
+    assert np.any(Eg[:, :3])
+    g.close()
+
+
 @pytest.mark.parametrize("cells", [0, 4])
 def test_spheres_dipole_f16_ports_bit_exact(cells):
     print("case: spheres+dipole 48^3 f16 storage with ports, cells per thread", cells, flush=True)
