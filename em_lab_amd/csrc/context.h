@@ -100,6 +100,11 @@ struct emfd_ctx {
     std::vector<emfd_source_desc> sources;  // kept sources, in order
     emfd::DevBuf d_packed, d_cells, d_lists;
     int n_cells = 0;
+    // per source texel records (launch_build_source_records), rebuilt before a step after the
+    // sources or the materials change
+    emfd::DevBuf d_src_recs, d_src_pairs;
+    size_t sources_lists_len = 0;  // entries of d_lists (one SourcePair each)
+    bool src_recs_dirty = true;
 
     // per-call step constants: pinned host staging (two buffers, each released by an event once
     // its copy has run) and the device copy the kernels read

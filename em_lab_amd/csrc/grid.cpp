@@ -367,7 +367,7 @@ void emfd_destroy(emfd_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &b : ctx->field_buf) b.release();
     DevBuf *bufs[] = {&ctx->d_src_descs, &ctx->sigma_cell, &ctx->eps_cell, &ctx->mat_id, &ctx->table, &ctx->prof, &ctx->pml_tab, &ctx->d_packed,
-                      &ctx->d_cells, &ctx->d_lists, &ctx->d_consts, &ctx->ports.d_ports, &ctx->ports.d_phasors,
+                      &ctx->d_cells, &ctx->d_lists, &ctx->d_src_recs, &ctx->d_src_pairs, &ctx->d_consts, &ctx->ports.d_ports, &ctx->ports.d_phasors,
                       &ctx->ports.d_series, &ctx->ntff.d_phasors, &ctx->ntff.d_side, &ctx->ntff.d_dirs, &ctx->ntff.d_partials,
                       &ctx->ntff.d_out, &ctx->ntff.d_power, &ctx->scratch_partials, &ctx->scratch_misc};
     for (DevBuf *b : bufs) b->release();
@@ -384,6 +384,7 @@ int emfd_grid_configure(emfd_ctx *ctx, const emfd_grid_config *requested, emfd_g
     if (!ctx || !requested) return fail(EMFD_ERR_INVALID, "emfd_grid_configure: null argument");
     EMFD_HIP(hipSetDevice(ctx->device));
     EMFD_HIP(hipStreamSynchronize(ctx->stream));  // buffers may be reallocated below
+    ctx->src_recs_dirty = true;
     const emfd_grid_config c = clamp_config(*requested);
     // The reference rejects grids beyond GL_MAX_3D_TEXTURE_SIZE; here the limit is the 32-bit
     // plane index used by the kernels for one component plane.
@@ -447,6 +448,7 @@ int emfd_grid_upload_materials(emfd_ctx *ctx, const float *sigma, const float *e
         return fail(EMFD_ERR_INVALID, "emfd_grid_upload_materials: null override array");
     EMFD_HIP(hipSetDevice(ctx->device));
     EMFD_HIP(hipStreamSynchronize(ctx->stream));  // buffers may be reallocated below
+    ctx->src_recs_dirty = true;
     const size_t n = ctx->n;
 
     // cell materials (alpha channels of the reference's field textures)
@@ -514,6 +516,7 @@ int emfd_grid_set_sources(emfd_ctx *ctx, const emfd_source_desc *sources, int n,
     EMFD_HIP(hipSetDevice(ctx->device));
     PackedSources ps = pack_sources(ctx->cfg, sources, n);
     ctx->n_cells = ps.n_cells;
+    ctx->src_recs_dirty = true;
     std::vector<float> &packed = ps.packed;
     std::vector<int> &cells = ps.cells, &lists = ps.lists;
     const std::vector<emfd_source_desc> &kept = ps.kept;
@@ -521,6 +524,7 @@ int emfd_grid_set_sources(emfd_ctx *ctx, const emfd_source_desc *sources, int n,
     if (int rc = ctx->d_src_descs.ensure(std::max<size_t>(kept.size(), 1) * sizeof(emfd_source_desc))) return rc;
     if (int rc = ctx->d_cells.ensure(cells.size() * sizeof(int))) return rc;
     if (int rc = ctx->d_lists.ensure(lists.size() * sizeof(int))) return rc;
+    ctx->sources_lists_len = lists.size();
     EMFD_HIP(hipStreamSynchronize(ctx->stream));
     EMFD_HIP(hipMemcpy(ctx->d_packed.ptr, packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice));
     if (!kept.empty())
@@ -661,6 +665,15 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
         ctx->side_e_valid = true;
     }
     const LatticeArgs lat = ctx->lattice();
+    if (ctx->src_recs_dirty && ctx->n_cells > 0 && ctx->storage != kStorageF64) {
+        const size_t pairs = ctx->sources_lists_len;
+        if (int rc = ctx->d_src_recs.ensure(static_cast<size_t>(ctx->n_cells) * kSourceRecBytes)) return rc;
+        if (int rc = ctx->d_src_pairs.ensure(std::max<size_t>(pairs, 1) * kSourcePairBytes)) return rc;
+        EMFD_HIP(launch_build_source_records(lat, ctx->d_cells.as<int>(), ctx->n_cells, ctx->d_lists.as<int>(),
+                                             ctx->d_packed.as<float>(), ctx->d_src_recs.ptr, ctx->d_src_pairs.ptr,
+                                             ctx->stream));
+        ctx->src_recs_dirty = false;
+    }
     ctx->ntff.pending = 0;  // a run of deferred far-field samples never spans two calls
     for (int k = 0; k < count; ++k) {
         const float *consts = ctx->d_consts.as<float>() + static_cast<size_t>(k) * stride;
@@ -731,8 +744,9 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
                                             ctx->d_lists.as<int>(), ctx->d_packed.as<float>(),
                                             ctx->d_src_descs.as<emfd_source_desc>(), t, ctx->stream));
             } else {
-                EMFD_HIP(launch_apply_sources(lat, ctx->fields(ctx->cur), ctx->d_cells.as<int>(), ctx->n_cells,
-                                              ctx->d_lists.as<int>(), ctx->d_packed.as<float>(), consts, ctx->stream));
+                EMFD_HIP(launch_apply_source_records(lat, ctx->fields(ctx->cur), ctx->d_src_recs.ptr,
+                                                     ctx->d_src_pairs.ptr, ctx->n_cells, ctx->d_packed.as<float>(),
+                                                     consts, ctx->stream));
             }
         }
         ++ctx->step_count;  // observers see the time of the step that just completed

@@ -138,6 +138,13 @@ hipError_t launch_sources_f64(const LatticeArgs &a, const FieldPtrs &f, const in
 int yee_half_supported(const LatticeArgs &a);
 hipError_t launch_apply_sources(const LatticeArgs &a, const FieldPtrs &f, const int *cells, int n_cells,
                                 const int *lists, const float *packed, const float *signals, hipStream_t s);
+// The source pass from per-texel records (SourceRec, SourcePair in sources_device.h, 64 and 16
+// bytes each): launch_build_source_records fills them after sources or materials change.
+constexpr size_t kSourceRecBytes = 64, kSourcePairBytes = 16;
+hipError_t launch_build_source_records(const LatticeArgs &a, const int *cells, int n_cells, const int *lists,
+                                       const float *packed, void *recs, void *pairs, hipStream_t s);
+hipError_t launch_apply_source_records(const LatticeArgs &a, const FieldPtrs &f, const void *recs, const void *pairs,
+                                       int n_cells, const float *packed, const float *signals, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
 hipError_t launch_copy4(const void *src, void *dst, size_t bytes, hipStream_t s);
 hipError_t launch_build_class_masks(const LatticeArgs &a, uint32_t *wire_mask, uint32_t *class_mask, int wpr,

@@ -426,6 +426,25 @@ __global__ void __launch_bounds__(64) k_apply_sources(LatticeArgs a, FieldPtrs f
     source_cell(a, f, cells, lists, packed, signals, id);
 }
 
+__global__ void __launch_bounds__(64) k_build_source_records(LatticeArgs a, const int *cells, int n_cells,
+                                                              const int *lists, const float *packed, SourceRec *recs,
+                                                              SourcePair *pairs) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n_cells) return;
+    source_record(a, cells, lists, packed, id, recs, pairs);
+}
+
+// S: the field storage, fixed at compile time so the field loads carry no format branch
+template <int S>
+__global__ void __launch_bounds__(64) k_apply_source_records(LatticeArgs a, FieldPtrs f, const SourceRec *recs,
+                                                             const SourcePair *pairs, int n_cells,
+                                                             const float *packed, const float *signals) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n_cells) return;
+    f.storage = S;
+    source_apply(a, f, recs, pairs, packed, signals, id);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------------
@@ -524,6 +543,33 @@ hipError_t launch_apply_sources(const LatticeArgs &a, const FieldPtrs &f, const 
                                 const int *lists, const float *packed, const float *signals, hipStream_t s) {
     if (n_cells <= 0) return hipSuccess;
     k_apply_sources<<<(n_cells + 63) / 64, 64, 0, s>>>(a, f, cells, n_cells, lists, packed, signals);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_source_records(const LatticeArgs &a, const int *cells, int n_cells, const int *lists,
+                                       const float *packed, void *recs, void *pairs, hipStream_t s) {
+    static_assert(sizeof(SourceRec) == kSourceRecBytes && sizeof(SourcePair) == kSourcePairBytes, "record layout");
+    if (n_cells <= 0) return hipSuccess;
+    k_build_source_records<<<(n_cells + 63) / 64, 64, 0, s>>>(a, cells, n_cells, lists, packed,
+                                                              static_cast<SourceRec *>(recs),
+                                                              static_cast<SourcePair *>(pairs));
+    return hipGetLastError();
+}
+
+hipError_t launch_apply_source_records(const LatticeArgs &a, const FieldPtrs &f, const void *recs, const void *pairs,
+                                       int n_cells, const float *packed, const float *signals, hipStream_t s) {
+    if (n_cells <= 0) return hipSuccess;
+    const dim3 g((n_cells + 63) / 64);
+    const SourceRec *r = static_cast<const SourceRec *>(recs);
+    const SourcePair *q = static_cast<const SourcePair *>(pairs);
+    switch (f.storage) {
+        case kStorageF32: k_apply_source_records<kStorageF32><<<g, 64, 0, s>>>(a, f, r, q, n_cells, packed, signals); break;
+        case kStorageF16: k_apply_source_records<kStorageF16><<<g, 64, 0, s>>>(a, f, r, q, n_cells, packed, signals); break;
+        case kStorageBF16:
+            k_apply_source_records<kStorageBF16><<<g, 64, 0, s>>>(a, f, r, q, n_cells, packed, signals);
+            break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
