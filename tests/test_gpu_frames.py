@@ -1,0 +1,49 @@
+"""The headless `--frames` driver on the GPU: a Hertzian scene run to completion prints the
+reference's report (a header every 100 frames and at the end, the far-field summary with its three
+cuts, the timing line) and exits 0; non-finite fields, loaded from the host before the run, give
+the reference's error line and exit status 2 (src/Instruments.cpp:271-275, src/Main.cpp:1157)."""
+import math
+
+import numpy as np
+import pytest
+
+from em_lab_amd import emfdtd as E
+from em_lab_amd import frames as F
+
+pytestmark = pytest.mark.gpu
+
+
+def test_frames_run_reports_and_exits_zero():
+    print("case: frames hertzian 48^3, 120 frames", flush=True)
+    lines = []
+    rc = F.run(["--scene", "hertzian", "--n", "48", "--frames", "120", "--probe", "centre:24,24,30"],
+               out=lines.append)
+    assert rc == 0
+    headers = [ln for ln in lines if ln.startswith("[t=")]
+    assert len(headers) == 2  # frame 100 and the end of the run
+    assert headers[-1] == "[t=%8.1f step %7d]" % (240 * F.build_scene("hertzian", 48).courant, 240)
+    probe = [ln for ln in lines if ln.startswith("  > probe centre")]
+    assert len(probe) == 2 and "nan" not in probe[-1]
+    far = [ln for ln in lines if ln.startswith("    far field [lambda=16.0 cells")]
+    assert len(far) == 2
+    d_max = float(far[-1].split("D_max=")[1].split(" dBi")[0])
+    assert 0.5 < d_max < 3.0  # a short z dipole: about 1.76 dBi once settled
+    assert sum(1 for ln in lines if "-cut: peak" in ln) == 6
+    assert lines[-1].startswith("GPU timings: sim ") and lines[-1].endswith("ms/frame (2 steps), render 0.000 ms")
+
+
+def test_frames_non_finite_fields_exit_2(capsys):
+    print("case: frames hertzian 48^3 with a NaN field texel", flush=True)
+
+    def poison(grid):
+        e, _ = grid.fields(E.LAYOUT_SOA)
+        nx, ny, nz = grid.size
+        e[:, (24 * ny + 24) * nx + 24] = math.nan
+        grid.set_fields(e, None, E.LAYOUT_SOA)
+
+    lines = []
+    rc = F.run(["--scene", "hertzian", "--n", "48", "--frames", "4", "--probe", "c:24,24,24"], after_setup=poison,
+               out=lines.append)
+    assert rc == 2
+    assert "ERROR: non-finite field values detected at step 2" in capsys.readouterr().err
+    assert lines[-1].startswith("GPU timings: sim ")
