@@ -47,3 +47,18 @@ def test_frames_non_finite_fields_exit_2(capsys):
     assert rc == 2
     assert "ERROR: non-finite field values detected at step 2" in capsys.readouterr().err
     assert lines[-1].startswith("GPU timings: sim ")
+
+
+def test_frames_fed_dipole_sweep_and_flux():
+    """The fed-dipole scene (ports with a sweep, a flux box, the far field) at 64^3: the sweep
+    summary replaces the continuous-wave port lines, the flux line appears, and the run ends with
+    status 0."""
+    print("case: frames spheres_dipole 64^3, 100 frames", flush=True)
+    lines = []
+    rc = F.run(["--scene", "spheres_dipole", "--n", "64", "--frames", "100"], out=lines.append)
+    assert rc == 0
+    assert any(ln.startswith("    flux  ") for ln in lines)
+    assert any(ln.startswith("    sweep: ") and "steps recorded" in ln for ln in lines)
+    assert any(ln.startswith("    sweep dipole") and "best match" in ln for ln in lines)
+    assert not any(ln.startswith("    port  ") for ln in lines)  # sweeping: no continuous-wave lines
+
