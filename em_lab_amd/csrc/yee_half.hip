@@ -297,14 +297,17 @@ __global__ void __launch_bounds__(256) k_update_eh(LatticeArgs a, FieldPtrs f) {
                                (b0[1][j] - bxm[1]) - minus_scaled(b0[0][j], hY[0][j], sB[0])};
         // edge_coefficients() with the vacuum edges' (absorber or interior) coefficients staged
         float ca[3], cb[3];
-        if (id[j] == 0u) {
-            read_coef(row + pml_class(x0 + j, a.nx, a.pml) * kCoefStride, ca, cb);
-        } else if (PM && in_pml(a, x0 + j, y, z)) {
-            absorber_material_coefficients(a, x0 + j, y, z, id[j], ca, cb);
-        } else {
-            const float *t = a.table + static_cast<size_t>(id[j]) * kTableStride;
-            ca[0] = t[kCa + 0]; ca[1] = t[kCa + 1]; ca[2] = t[kCa + 2];
-            cb[0] = t[kCb + 0]; cb[1] = t[kCb + 1]; cb[2] = t[kCb + 2];
+        // every texel reads its staged entry; a material texel then replaces it (read first and
+        // unconditionally, the vacuum entry does not wait behind the material texels' class loads)
+        read_coef(row + pml_class(x0 + j, a.nx, a.pml) * kCoefStride, ca, cb);
+        if (id[j] != 0u) {
+            if (PM && in_pml(a, x0 + j, y, z)) {
+                absorber_material_coefficients(a, x0 + j, y, z, id[j], ca, cb);
+            } else {
+                const float *t = a.table + static_cast<size_t>(id[j]) * kTableStride;
+                ca[0] = t[kCa + 0]; ca[1] = t[kCa + 1]; ca[2] = t[kCa + 2];
+                cb[0] = t[kCb + 0]; cb[1] = t[kCb + 1]; cb[2] = t[kCb + 2];
+            }
         }
 #pragma unroll
         for (int c = 0; c < 3; ++c) e[c][j] = ca[c] * e[c][j] + cb[c] * curl[c];

@@ -270,14 +270,17 @@ __device__ __forceinline__ void e_quad_rows(const LatticeArgs &a, const FieldPtr
         const float curl[3] = {(b0[2][j] - bY[2][j]) - (b0[1][j] - bZ[1][j]), (b0[0][j] - bZ[0][j]) - (b0[2][j] - bxm[2]),
                                (b0[1][j] - bxm[1]) - (b0[0][j] - bY[0][j])};
         float ca[3], cb[3];
-        if (id[j] == 0u) {
-            read_coef(rows + pml_class(x0 + j, a.nx, a.pml) * kCoefStride, ca, cb);
-        } else if (PM && in_pml(a, x0 + j, y, z)) {
-            absorber_material_coefficients(a, x0 + j, y, z, id[j], ca, cb);
-        } else {
-            const float *t = a.table + static_cast<size_t>(id[j]) * kTableStride;
-            ca[0] = t[kCa + 0]; ca[1] = t[kCa + 1]; ca[2] = t[kCa + 2];
-            cb[0] = t[kCb + 0]; cb[1] = t[kCb + 1]; cb[2] = t[kCb + 2];
+        // every texel reads its staged entry; a material texel then replaces it (read first and
+        // unconditionally, the vacuum entry does not wait behind the material texels' class loads)
+        read_coef(rows + pml_class(x0 + j, a.nx, a.pml) * kCoefStride, ca, cb);
+        if (id[j] != 0u) {
+            if (PM && in_pml(a, x0 + j, y, z)) {
+                absorber_material_coefficients(a, x0 + j, y, z, id[j], ca, cb);
+            } else {
+                const float *t = a.table + static_cast<size_t>(id[j]) * kTableStride;
+                ca[0] = t[kCa + 0]; ca[1] = t[kCa + 1]; ca[2] = t[kCa + 2];
+                cb[0] = t[kCb + 0]; cb[1] = t[kCb + 1]; cb[2] = t[kCb + 2];
+            }
         }
 #pragma unroll
         for (int c = 0; c < 3; ++c) e[c][j] = ca[c] * e[c][j] + cb[c] * curl[c];
