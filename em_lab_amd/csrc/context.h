@@ -118,8 +118,8 @@ struct emfd_ctx {
     int kernel_variant = EMFD_KERNEL_AUTO;
     int tile_cfg = 0;
 
-    // kernel timing
-    bool timing = false;
+    // kernel timing: the Yee launch of every timing_every-th step is bracketed (0 = off)
+    int timing_every = 0;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     double timing_ms = 0.0;

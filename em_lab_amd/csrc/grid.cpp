@@ -602,11 +602,11 @@ int emfd_grid_kernel_timing(emfd_ctx *ctx, int enable, double *total_ms, int64_t
     if (total_ms) *total_ms = ctx->timing_ms;
     if (launches) *launches = ctx->timing_launches;
     if (enable >= 0) {
-        if (enable && !ctx->timing) {
+        if (enable && !ctx->timing_every) {
             ctx->timing_ms = 0.0;
             ctx->timing_launches = 0;
         }
-        ctx->timing = enable != 0;
+        ctx->timing_every = enable;
     }
     return EMFD_OK;
 }
@@ -678,7 +678,7 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
     for (int k = 0; k < count; ++k) {
         const float *consts = ctx->d_consts.as<float>() + static_cast<size_t>(k) * stride;
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
-        if (ctx->timing) {
+        if (ctx->timing_every && ctx->step_count % static_cast<uint64_t>(ctx->timing_every) == 0) {
             if (ctx->ev_used + 2 > ctx->ev_pool.size()) {
                 hipEvent_t a, b;
                 EMFD_HIP(hipEventCreate(&a));
@@ -751,7 +751,7 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
         }
         ++ctx->step_count;  // observers see the time of the step that just completed
         if (int rc = monitors_after_step(ctx, consts + nsrc, stride, k, count)) return rc;
-        if (ctx->timing && ctx->ev_used >= 4096) {
+        if (ctx->ev_used >= 4096) {
             double ms;
             int64_t l;
             if (int rc = emfd_grid_kernel_timing(ctx, -1, &ms, &l)) return rc;

@@ -446,6 +446,9 @@ class SimulationGrid:
         return c.value
 
     def kernel_timing(self, enable: int = -1):
+        """Live Yee-launch timing: enable = k > 0 brackets the Yee launch of every later step whose
+        step count is a multiple of k with HIP events, 0 stops, -1 only reads. Returns the summed
+        milliseconds and the number of bracketed launches since timing was switched on."""
         ms = C.c_double(0)
         n = C.c_int64(0)
         _check(self._lib.emfd_grid_kernel_timing(self._h, enable, C.byref(ms), C.byref(n)))

@@ -243,9 +243,11 @@ int emfd_grid_set_storage(emfd_ctx *ctx, int storage, const int *scale_log2);
 int emfd_grid_download_fields_f64(emfd_ctx *ctx, double *E, double *B);
 /* Number of distinct coefficient classes after the last material upload (0 = uniform vacuum). */
 int emfd_grid_material_classes(emfd_ctx *ctx, int *classes);
-/* Live timing of the Yee kernel: with enable != 0, every Yee launch of later steps is bracketed
- * by HIP events on the context stream; *total_ms / *launches (may be NULL) report the sum and
- * count since the last enable. */
+/* Live timing of the Yee kernel: with enable = k > 0, the Yee launch of every later step whose
+ * step count is a multiple of k (k = 1: every step) is bracketed by HIP events on the context
+ * stream; enable = 0 stops, enable < 0 only reads. *total_ms / *launches (may be NULL) report
+ * the sum and count of the bracketed launches since timing was last switched on. Each bracket
+ * costs the stream a few microseconds, so long timed runs sample with k > 1. */
 int emfd_grid_kernel_timing(emfd_ctx *ctx, int enable, double *total_ms, int64_t *launches);
 
 /* ---------------------------------------------------------------------------------------

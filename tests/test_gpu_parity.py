@@ -236,6 +236,30 @@ def test_abi_errors():
     g.close()
 
 
+def test_sampled_kernel_timing():
+    """Yee launch timing brackets the launches of the steps whose count is a multiple of the
+    period, and bracketed steps give the same fields bit for bit."""
+    print("case: kernel timing period 4 and 1, hertzian 32^3", flush=True)
+    scene = hertzian_vacuum(32, 4)
+    g, _, _ = gpu_grid(scene, ports=False, ntff=False)
+    g.kernel_timing(4)
+    g.step(10)  # step counts 0..9 at launch: 0, 4, 8 are bracketed
+    ms, n = g.kernel_timing(0)
+    assert n == 3 and ms > 0.0
+    g.step(3)
+    assert g.kernel_timing(-1)[1] == 3  # off: nothing more bracketed
+    g.kernel_timing(1)
+    g.step(5)
+    assert g.kernel_timing(0)[1] == 5
+    e_timed, b_timed = g.fields(E.LAYOUT_SOA)
+    g.close()
+    h, _, _ = gpu_grid(scene, ports=False, ntff=False)
+    h.step(18)
+    e_plain, b_plain = h.fields(E.LAYOUT_SOA)
+    h.close()
+    assert np.array_equal(e_timed, e_plain) and np.array_equal(b_timed, b_plain)
+
+
 def test_material_in_absorber():
     """Dielectric boxes reaching into the absorber (the E pass's summed-conductivity path): the
     default kernel matches the oracle and, bit for bit, the scalar two-pass kernel; a scene with
