@@ -46,6 +46,15 @@ struct PortsState {
     bool recording() const { return has_sweep || record_steps > 0; }
 };
 
+struct LinesState {
+    int n = 0;                 // quantities
+    float wavelength = 16.0f;  // drive wavelength as given (the rotation uses max(wavelength, 2))
+    float averaging = 4.0f;
+    double start_time = 0.0;
+    DevBuf d_quant, d_terms, d_weights, d_phasors, d_now;
+    bool enabled() const { return n > 0; }
+};
+
 struct NtffState {
     bool enabled = false;
     emfd_far_field_setup setup{};
@@ -126,6 +135,7 @@ struct emfd_ctx {
     int64_t timing_launches = 0;
 
     emfd::PortsState ports;
+    emfd::LinesState lines;
     emfd::NtffState ntff;
 
     emfd::DevBuf scratch_partials, scratch_misc;

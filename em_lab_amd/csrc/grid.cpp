@@ -369,7 +369,9 @@ void emfd_destroy(emfd_ctx *ctx) {
     DevBuf *bufs[] = {&ctx->d_src_descs, &ctx->sigma_cell, &ctx->eps_cell, &ctx->mat_id, &ctx->table, &ctx->prof, &ctx->pml_tab, &ctx->d_packed,
                       &ctx->d_cells, &ctx->d_lists, &ctx->d_src_recs, &ctx->d_src_pairs, &ctx->d_consts, &ctx->ports.d_ports, &ctx->ports.d_phasors,
                       &ctx->ports.d_series, &ctx->ntff.d_phasors, &ctx->ntff.d_side, &ctx->ntff.d_dirs, &ctx->ntff.d_partials,
-                      &ctx->ntff.d_out, &ctx->ntff.d_power, &ctx->scratch_partials, &ctx->scratch_misc};
+                      &ctx->ntff.d_out, &ctx->ntff.d_power, &ctx->scratch_partials, &ctx->scratch_misc,
+                      &ctx->lines.d_quant, &ctx->lines.d_terms, &ctx->lines.d_weights, &ctx->lines.d_phasors,
+                      &ctx->lines.d_now};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t ev : ctx->ev_pool) (void)hipEventDestroy(ev);
     for (int i = 0; i < 2; ++i) {

@@ -186,6 +186,10 @@ struct PortJob {
 };
 hipError_t launch_ntff_accumulate(const LatticeArgs &a, const FieldPtrs &f, const NtffBox &box, const float *rot,
                                   const NtffFreqs &freqs, float *phasors, const PortJob &pj, hipStream_t s);
+// transmission-line monitor: quantities and terms 4 ints each, phasors 4 floats per quantity
+hipError_t launch_line_accumulate(const LatticeArgs &a, const FieldPtrs &f, const int *quantities, int n,
+                                  const int *terms, const float *weights, const float *rot, float blend, float *phasors,
+                                  float *now, hipStream_t s);
 // deferred far-field accumulation: per-step gather into sample slot (6 planes of n_points), then
 // one pass applying `steps` updates (rotation rows rot_stride floats apart) to the phasors
 constexpr int kNtffRun = 16;

@@ -191,7 +191,8 @@ float to_dbi(double d) { return d > 1e-12 ? static_cast<float>(10.0 * std::log10
 
 // ---- hooks used by emfd_grid_step ----
 int monitors_consts_floats(emfd_ctx *ctx) {
-    return 4 + 4 * ctx->ntff.n_freq();  // port rotation (4) + far-field rotation (4 per wavelength)
+    // port rotation (4), far-field rotation (4 per wavelength), line-monitor rotation (4)
+    return 4 + 4 * ctx->ntff.n_freq() + 4;
 }
 
 int monitors_fill_consts(emfd_ctx *ctx, float *row, uint64_t step_after) {
@@ -199,6 +200,8 @@ int monitors_fill_consts(emfd_ctx *ctx, float *row, uint64_t step_after) {
     if (ctx->ntff.enabled)
         for (int f = 0; f < ctx->ntff.n_freq(); ++f)
             rotation(ctx->ntff.wavelength(f), ctx->cfg.courant, step_after, row + 4 + 4 * f);
+    if (ctx->lines.enabled())  // src/Coax.cpp:177-180
+        rotation(std::max(ctx->lines.wavelength, 2.0f), ctx->cfg.courant, step_after, row + 4 + 4 * ctx->ntff.n_freq());
     return EMFD_OK;
 }
 
@@ -248,6 +251,14 @@ int monitors_after_step(emfd_ctx *ctx, const float *consts, int stride, int k, i
             }
         }
     }
+    if (ctx->lines.enabled()) {
+        LinesState &Ls = ctx->lines;
+        // src/Coax.cpp:181: tau from the wavelength as given
+        EMFD_HIP(launch_line_accumulate(lat, f, Ls.d_quant.as<int>(), Ls.n, Ls.d_terms.as<int>(),
+                                        Ls.d_weights.as<float>(), consts + 4 + 4 * N.n_freq(),
+                                        blend_of(Ls.averaging, Ls.wavelength, ctx->cfg.courant),
+                                        Ls.d_phasors.as<float>(), Ls.d_now.as<float>(), ctx->stream));
+    }
     return EMFD_OK;
 }
 
@@ -256,6 +267,106 @@ int monitors_after_step(emfd_ctx *ctx, const float *consts, int stride, int k, i
 using namespace emfd;
 
 extern "C" {
+
+// ============================================================================================
+// CoaxMonitor's line monitor (src/Coax.cpp:95-198)
+// ============================================================================================
+int emfd_lines_configure(emfd_ctx *ctx, const int *quantities, int n, const int *terms, const float *weights,
+                         int n_terms, float wavelength, float averaging_periods) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
+    if (n < 0 || n_terms < 0 || (n > 0 && (!quantities || (n_terms > 0 && (!terms || !weights)))))
+        return fail(EMFD_ERR_INVALID, "null or negative line-monitor arrays");
+    for (int q = 0; q < n; ++q) {
+        const int first = quantities[4 * q], count = quantities[4 * q + 1], kind = quantities[4 * q + 2];
+        if (first < 0 || count < 0 || first > n_terms - count || (kind != 0 && kind != 1))
+            return fail(EMFD_ERR_INVALID, "line-monitor quantity outside the term list or of unknown kind");
+    }
+    for (int k = 0; k < n_terms; ++k)
+        if (terms[4 * k + 3] < 0 || terms[4 * k + 3] > 5)
+            return fail(EMFD_ERR_INVALID, "line-monitor term component outside 0..5");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    LinesState &Ls = ctx->lines;
+    Ls.n = 0;
+    Ls.wavelength = wavelength;
+    Ls.averaging = std::max(averaging_periods, 0.5f);
+    Ls.start_time = ctx->time();
+    if (n == 0) return EMFD_OK;
+    const size_t nt = static_cast<size_t>(std::max(n_terms, 1));
+    if (int rc = Ls.d_quant.ensure(static_cast<size_t>(n) * 4 * sizeof(int))) return rc;
+    if (int rc = Ls.d_terms.ensure(nt * 4 * sizeof(int))) return rc;
+    if (int rc = Ls.d_weights.ensure(nt * sizeof(float))) return rc;
+    if (int rc = Ls.d_phasors.ensure(static_cast<size_t>(n) * 4 * sizeof(float))) return rc;
+    if (int rc = Ls.d_now.ensure(static_cast<size_t>(n) * sizeof(float))) return rc;
+    EMFD_HIP(hipMemcpy(Ls.d_quant.ptr, quantities, static_cast<size_t>(n) * 4 * sizeof(int), hipMemcpyHostToDevice));
+    if (n_terms > 0) {
+        EMFD_HIP(hipMemcpy(Ls.d_terms.ptr, terms, static_cast<size_t>(n_terms) * 4 * sizeof(int),
+                           hipMemcpyHostToDevice));
+        EMFD_HIP(hipMemcpy(Ls.d_weights.ptr, weights, static_cast<size_t>(n_terms) * sizeof(float),
+                           hipMemcpyHostToDevice));
+    }
+    EMFD_HIP(hipMemset(Ls.d_phasors.ptr, 0, static_cast<size_t>(n) * 4 * sizeof(float)));
+    EMFD_HIP(hipMemset(Ls.d_now.ptr, 0, static_cast<size_t>(n) * sizeof(float)));
+    Ls.n = n;
+    return EMFD_OK;
+}
+
+int emfd_lines_read(emfd_ctx *ctx, float *phasors, float *now, int n) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    const LinesState &Ls = ctx->lines;
+    if (n < Ls.n) return fail(EMFD_ERR_INVALID, "line-monitor output smaller than its quantity count");
+    if (Ls.n == 0) return EMFD_OK;
+    EMFD_HIP(hipSetDevice(ctx->device));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    if (phasors)
+        EMFD_HIP(hipMemcpy(phasors, Ls.d_phasors.ptr, static_cast<size_t>(Ls.n) * 4 * sizeof(float),
+                           hipMemcpyDeviceToHost));
+    if (now) EMFD_HIP(hipMemcpy(now, Ls.d_now.ptr, static_cast<size_t>(Ls.n) * sizeof(float), hipMemcpyDeviceToHost));
+    return EMFD_OK;
+}
+
+double emfd_coax_geometry_factor(int inner, int outer) {
+    // CoaxMonitor::geometryFactor (src/Coax.cpp:42-83): Laplace's equation on the (outer + 1)^2
+    // nodes of the cross-section, inner conductor nodes at 1 and the shield ring at 0, by
+    // successive over-relaxation in row order with the optimal factor; G = the sum over the
+    // lattice edges of the squared potential difference
+    const int n = outer + 1;
+    if (n < 2) return 0.0;
+    const int lo = (outer - inner) / 2, hi = lo + inner;
+    std::vector<double> phi(static_cast<size_t>(n) * n, 0.0);
+    std::vector<char> fixed(phi.size(), 0);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            const size_t k = static_cast<size_t>(j) * n + i;
+            if (i == 0 || j == 0 || i == n - 1 || j == n - 1) fixed[k] = 1;
+            if (i >= lo && i <= hi && j >= lo && j <= hi) {
+                fixed[k] = 1;
+                phi[k] = 1.0;
+            }
+        }
+    const double omega = 2.0 / (1.0 + std::sin(3.14159265358979323846 / n));
+    for (int it = 0; it < 20000; ++it) {
+        double change = 0.0;
+        for (int j = 1; j < n - 1; ++j)
+            for (int i = 1; i < n - 1; ++i) {
+                const size_t k = static_cast<size_t>(j) * n + i;
+                if (fixed[k]) continue;
+                const double d = omega * (0.25 * (phi[k - 1] + phi[k + 1] + phi[k - n] + phi[k + n]) - phi[k]);
+                phi[k] += d;
+                change = std::max(change, std::abs(d));
+            }
+        if (change < 1e-13) break;
+    }
+    double g = 0.0;
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            const size_t k = static_cast<size_t>(j) * n + i;
+            if (i + 1 < n) g += (phi[k + 1] - phi[k]) * (phi[k + 1] - phi[k]);
+            if (j + 1 < n) g += (phi[k + n] - phi[k]) * (phi[k + n] - phi[k]);
+        }
+    return g;
+}
 
 // ============================================================================================
 // PortMonitor

@@ -367,7 +367,43 @@ __global__ void __launch_bounds__(64) k_probe_gather(LatticeArgs a, FieldPtrs f,
     }
 }
 
+// line_accumulate.comp (shaders/line_accumulate.comp:30-48): one thread per quantity sums its
+// weighted texel components (clamped indices; 0-2 from E, 3-5 from B), keeps the sum and folds it
+// into the two-stage running DFT with the E (kind 0) or B (kind 1) rotation
+__global__ void __launch_bounds__(64) k_line_accumulate(LatticeArgs a, FieldPtrs f, const int4 *quantities, int n,
+                                                        const int4 *terms, const float *weights, const float *rot,
+                                                        float blend, float4 *phasors, float *now) {
+    const int q = blockIdx.x * 64 + threadIdx.x;
+    if (q >= n) return;
+    const int4 info = quantities[q];
+    float sum = 0.0f;
+    for (int k = info.x; k < info.x + info.y; ++k) {
+        const int4 t = terms[k];
+        sum += weights[k] * fetch_c(a, f, t.w, t.x, t.y, t.z);
+    }
+    now[q] = sum;
+    const float rc = info.z == 0 ? rot[0] : rot[2], rs = info.z == 0 ? rot[1] : rot[3];
+    const float two = 2.0f * sum;
+    const float ox = two * rc, oy = two * -rs;
+    float4 ph = phasors[q];
+    ph.x += blend * (ox - ph.x);
+    ph.y += blend * (oy - ph.y);
+    ph.z += blend * (ph.x - ph.z);
+    ph.w += blend * (ph.y - ph.w);
+    phasors[q] = ph;
+}
+
 }  // namespace
+
+hipError_t launch_line_accumulate(const LatticeArgs &a, const FieldPtrs &f, const int *quantities, int n,
+                                  const int *terms, const float *weights, const float *rot, float blend, float *phasors,
+                                  float *now, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_line_accumulate<<<(n + 63) / 64, 64, 0, s>>>(a, f, reinterpret_cast<const int4 *>(quantities), n,
+                                                   reinterpret_cast<const int4 *>(terms), weights, rot, blend,
+                                                   reinterpret_cast<float4 *>(phasors), now);
+    return hipGetLastError();
+}
 
 hipError_t launch_port_accumulate(const LatticeArgs &a, const FieldPtrs &f, const PortDev *ports, int n_ports,
                                   const float *rot, float blend, float *phasors, float *series, int64_t record_index,

@@ -135,6 +135,9 @@ _SIGNATURES = {
     "emfd_grid_set_storage": (C.c_int, [_P, C.c_int, _P]),
     "emfd_grid_download_fields_f64": (C.c_int, [_P, _P, _P]),
     "emfd_grid_kernel_timing": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "emfd_lines_configure": (C.c_int, [_P, _P, C.c_int, _P, _P, C.c_int, C.c_float, C.c_float]),
+    "emfd_lines_read": (C.c_int, [_P, _P, _P, C.c_int]),
+    "emfd_coax_geometry_factor": (C.c_double, [C.c_int, C.c_int]),
     "emfd_ports_configure": (C.c_int, [_P, _P, C.c_int, C.c_float, _P, C.c_float, C.c_int64]),
     "emfd_ports_disable": (C.c_int, [_P]),
     "emfd_ports_read_phasors": (C.c_int, [_P, _P, C.c_int]),
@@ -519,6 +522,42 @@ class PortReading:
     return_loss_db: float
     accepted_power: float
     available_power: float
+
+
+class LineMonitor:
+    """The device side of CoaxMonitor (src/Coax.cpp:95-198): weighted sums of field texel
+    components (quantities of terms), accumulated after every step into running-DFT phasors."""
+
+    def __init__(self, grid: SimulationGrid):
+        self.grid = grid
+        self._lib = grid._lib
+        self.n = 0
+
+    def configure(self, quantities, terms, weights, wavelength: float, averaging_periods: float = 4.0) -> None:
+        """quantities: (n, 4) int (first term, term count, 0 = E / 1 = B integral, unused);
+        terms: (m, 4) int (x, y, z, component 0-2 E / 3-5 B); weights: (m,) float."""
+        q = np.ascontiguousarray(np.asarray(quantities, np.int32).reshape(-1, 4))
+        t = np.ascontiguousarray(np.asarray(terms, np.int32).reshape(-1, 4))
+        w = np.ascontiguousarray(np.asarray(weights, np.float32).reshape(-1))
+        if w.shape[0] != t.shape[0]:
+            raise ValueError("one weight per term")
+        _check(self._lib.emfd_lines_configure(self.grid.handle, _ptr(q) if q.size else None, q.shape[0],
+                                              _ptr(t) if t.size else None, _ptr(w) if w.size else None, t.shape[0],
+                                              float(wavelength), float(averaging_periods)))
+        self.n = q.shape[0]
+
+    def read(self):
+        """(phasors (n, 4): stage-1 re, im, stage-2 re, im; instantaneous values (n,))."""
+        ph = np.zeros((max(self.n, 1), 4), np.float32)
+        now = np.zeros(max(self.n, 1), np.float32)
+        _check(self._lib.emfd_lines_read(self.grid.handle, _ptr(ph), _ptr(now), ph.shape[0]))
+        return ph[: self.n], now[: self.n]
+
+
+def coax_geometry_factor(inner: int, outer: int) -> float:
+    """CoaxMonitor::geometryFactor (src/Coax.cpp:42-83): lattice capacitance per unit length of the
+    square coax in units of eps; Z0 = eta0 / (sqrt(eps_r) G)."""
+    return float(load_library().emfd_coax_geometry_factor(int(inner), int(outer)))
 
 
 class PortMonitor:

@@ -250,6 +250,24 @@ int emfd_grid_material_classes(emfd_ctx *ctx, int *classes);
  * costs the stream a few microseconds, so long timed runs sample with k > 1. */
 int emfd_grid_kernel_timing(emfd_ctx *ctx, int enable, double *total_ms, int64_t *launches);
 
+/* CoaxMonitor's device side (src/Coax.cpp:95-198, shaders/line_accumulate.comp:30-48): n
+ * quantities of 4 ints (first term, term count, kind 0 = E line integral / 1 = B loop integral,
+ * unused), n_terms terms of 4 ints (x, y, z, component 0-2 E / 3-5 B) with one weight each.
+ * After every step each quantity's weighted sum (indices clamped to the grid) is kept as its
+ * instantaneous value and folded into a two-stage running DFT at the drive wavelength (E
+ * sampled at t, B at t - dt/2), as the ports. Phasors and values start at 0; n = 0 disables.
+ * Quantities reaching outside the term list, kinds other than 0 / 1 and components outside
+ * 0..5 are rejected (EMFD_ERR_INVALID). */
+int emfd_lines_configure(emfd_ctx *ctx, const int *quantities, int n, const int *terms, const float *weights,
+                         int n_terms, float wavelength, float averaging_periods);
+/* Phasors (stage-1 re, im, stage-2 re, im per quantity; may be NULL) and instantaneous values
+ * (may be NULL) of the n >= configured quantities. */
+int emfd_lines_read(emfd_ctx *ctx, float *phasors, float *now, int n);
+/* CoaxMonitor::geometryFactor (src/Coax.cpp:42-83): the square coax's lattice capacitance per
+ * unit length in units of eps (Z0 = eta0 / (sqrt(eps_r) G)), from an SOR solve of the
+ * cross-section; host-only. */
+double emfd_coax_geometry_factor(int inner, int outer);
+
 /* ---------------------------------------------------------------------------------------
  * PortMonitor (src/Ports.hpp:84-140)
  * --------------------------------------------------------------------------------------- */

@@ -676,6 +676,38 @@ void or_monitor_constants(double wavelength, float averaging, float dt, uint64_t
     *blend = (float)(1.0 - exp(-(double)dt / (0.5 * tau)));
 }
 
+/* line_accumulate.comp (shaders/line_accumulate.comp:30-48): per quantity, the weighted sum of
+   its terms' texel components (indices clamped to the grid; components 0-2 from E, 3-5 from B),
+   kept as the instantaneous value and folded into the two-stage running DFT with the E rotation
+   (kind 0) or the B rotation (kind 1). quantities: 4 ints each (first, count, kind, unused);
+   terms: 4 ints each (x, y, z, component). */
+void or_line_accumulate(const or_grid *g, const float *E, const float *B, const int *quantities, int n,
+                        const int *terms, const float *weights, const float rot[4], float blend, float *phasors,
+                        float *now) {
+    for (int q = 0; q < n; ++q) {
+        const int first = quantities[4 * q], count = quantities[4 * q + 1], kind = quantities[4 * q + 2];
+        float sum = 0.0f;
+        for (int k = first; k < first + count; ++k) {
+            const int *t = terms + 4 * k;
+            const int x = t[0] < 0 ? 0 : (t[0] > g->nx - 1 ? g->nx - 1 : t[0]);
+            const int y = t[1] < 0 ? 0 : (t[1] > g->ny - 1 ? g->ny - 1 : t[1]);
+            const int z = t[2] < 0 ? 0 : (t[2] > g->nz - 1 ? g->nz - 1 : t[2]);
+            const float *F = t[3] < 3 ? E : B;
+            const size_t i = ((size_t)z * g->ny + y) * g->nx + x;
+            sum += weights[k] * F[4 * i + t[3] % 3];
+        }
+        now[q] = sum;
+        const float rc = kind == 0 ? rot[0] : rot[2], rs = kind == 0 ? rot[1] : rot[3];
+        const float two = 2.0f * sum;
+        const float ox = two * rc, oy = two * -rs;
+        float *ph = phasors + 4 * q;
+        ph[0] += blend * (ox - ph[0]);
+        ph[1] += blend * (oy - ph[1]);
+        ph[2] += blend * (ph[0] - ph[2]);
+        ph[3] += blend * (ph[1] - ph[3]);
+    }
+}
+
 /* ntff_common.glsl ntffPoint (shaders/ntff_common.glsl:30-53) */
 void or_ntff_point(const int bmin[3], const int bmax[3], const int fo[7], int index, int node[3], float normal[3],
                    float *weight) {

@@ -113,6 +113,9 @@ void or_port_accumulate(const or_grid *g, const float *E, const float *B, const 
 
 /* PortMonitor::accumulate host constants (src/Ports.cpp:253-266): rot and blend for the step
  * whose completion leaves the grid at step_count. */
+void or_line_accumulate(const or_grid *g, const float *E, const float *B, const int *quantities, int n,
+                        const int *terms, const float *weights, const float rot[4], float blend, float *phasors,
+                        float *now);
 void or_monitor_constants(double wavelength, float averaging, float dt, uint64_t step_count, float rot[4],
                           float *blend);
 

@@ -64,6 +64,7 @@ _SIG = {
     "or_round_bf16": (C.c_float, [C.c_float]),
     "or_port_accumulate": (None, [C.POINTER(OrGrid), _P, _P, _P, _I, _P, _F, _I, _P, _P]),
     "or_monitor_constants": (None, [_D, _F, _F, C.c_uint64, _P, C.POINTER(_F)]),
+    "or_line_accumulate": (None, [C.POINTER(OrGrid), _P, _P, _P, _I, _P, _P, _P, _F, _P, _P]),
     "or_ntff_point": (None, [_P, _P, _P, _I, _P, _P, C.POINTER(_F)]),
     "or_ntff_face_offsets": (_I, [_P, _P, _P]),
     "or_ntff_accumulate": (None, [C.POINTER(OrGrid), _P, _P, _P, _P, _P, _I, _P, _F, _P]),
@@ -254,6 +255,29 @@ class OracleSim:
                                        C.byref(blend))
                 L.or_ntff_accumulate(C.byref(self.g), ptr(self.E), ptr(self.B), ptr(N["bmin"]), ptr(N["bmax"]),
                                      ptr(N["fo"]), N["n"], ptr(rot), blend.value, ptr(X["phasors"]))
+        if getattr(self, "lines", None):
+            Q = self.lines
+            rot = np.zeros(4, np.float32)
+            scratch = np.zeros(4, np.float32)
+            blend = C.c_float()
+            unused = C.c_float()
+            # src/Coax.cpp:177-182: the rotation at max(wavelength, 2), the blend at the wavelength
+            L.or_monitor_constants(max(Q["wavelength"], 2.0), Q["averaging"], self.g.dt, self.step_count, ptr(rot),
+                                   C.byref(unused))
+            L.or_monitor_constants(float(Q["wavelength"]), Q["averaging"], self.g.dt, self.step_count, ptr(scratch),
+                                   C.byref(blend))
+            L.or_line_accumulate(C.byref(self.g), ptr(self.E), ptr(self.B), ptr(Q["quantities"]), Q["n"],
+                                 ptr(Q["terms"]), ptr(Q["weights"]), ptr(rot), blend.value, ptr(Q["phasors"]),
+                                 ptr(Q["now"]))
+
+    def configure_lines(self, quantities, terms, weights, wavelength, averaging):
+        """CoaxMonitor::configure's device state (src/Coax.cpp:95-188): quantities (n, 4) int32,
+        terms (m, 4) int32, weights (m,) float32; phasors and instantaneous values start at 0."""
+        q = np.ascontiguousarray(quantities, np.int32).reshape(-1, 4)
+        self.lines = dict(quantities=q, n=q.shape[0], terms=np.ascontiguousarray(terms, np.int32).reshape(-1, 4),
+                          weights=np.ascontiguousarray(weights, np.float32), wavelength=float(wavelength),
+                          averaging=max(float(averaging), 0.5), phasors=np.zeros((q.shape[0], 4), np.float32),
+                          now=np.zeros(q.shape[0], np.float32))
 
     def ntff_eval_f32(self, dirs, k=None, extra=None):
         """Pattern |N_perp - r x L|^2 and power of the primary accumulator, or of the extra
