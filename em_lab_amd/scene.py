@@ -105,6 +105,7 @@ class Scene:
     far_field: Optional[FarFieldSetup] = None
     fluxes: list = field(default_factory=list)
     warnings: list = field(default_factory=list)
+    coaxes: list = field(default_factory=list)
 
     def __post_init__(self):
         nx, ny, nz = self.size
@@ -248,6 +249,119 @@ class Scene:
         self.ports.append(PortSetup(name=name, texel=tuple(int(t) for t in texel), axis=axis, sign=float(sign),
                                     edges=edges, wavelength=wave.wavelength, voltage_feed=voltage,
                                     impedance_ohm=float(f32(impedance)), drive=float(f32(drive))))
+
+    def coax(self, start, axis=0, sign=1, length=None, outer=5, wall=1, z0=50.0, eps_r=None, termination="load",
+             load=None, balun=False, balun_length=None, balun_gap=2, amplitude=1.0, impedance=None, name="cable",
+             **wave):
+        """api_coax (src/ScriptEngine.cpp:1961-2127): a square coaxial cable along `axis` (0-2,
+        `sign` toward the far end) from the lattice plane through `start` (a point; its
+        transverse coordinates pick the cell-centre line). Shield (with end plates unless the
+        termination is "terminals"), a dielectric giving `z0` from the lattice geometry factor
+        (or `eps_r` as given), a 1x1-cell inner conductor, a generator across the gap cell at the
+        start, a resistor across the last cell for termination "load" (also "open", "short",
+        "terminals"), an optional quarter-wave sleeve balun; registers the line monitor's
+        CoaxSetup in `coaxes`. Returns (CoaxSetup, inner terminal node, outer terminal node)."""
+        from .coax import CoaxSetup
+        from .emfdtd import coax_geometry_factor
+        wv = self.waveform(**wave)
+        ax, s = int(axis), (1 if sign >= 0 else -1)
+        u, v = (ax + 1) % 3, (ax + 2) % 3
+        ln = round_half_away(4.0 * wv.wavelength if length is None else float(length))
+        inner = 1
+        if outer < 3 or outer % 2 == 0:
+            raise ValueError("coax: outer (the inside of the shield, cells) must be odd and >= 3")
+        if wall < 1:
+            raise ValueError("coax: wall must be >= 1 cell")
+        term = termination.lower()
+        if term not in ("load", "open", "short", "terminals"):
+            raise ValueError('coax: termination must be "load", "open", "short" or "terminals"')
+        if ln < 2 * outer + 12:
+            raise ValueError(f"coax: length must be at least {2 * outer + 12} cells for this cross-section")
+        g = coax_geometry_factor(inner, outer)
+        z0_air = ETA0 / g
+        if eps_r is None:
+            er = (z0_air / z0) * (z0_air / z0)
+            if er < 1.0:
+                self.warnings.append(f"coax: {z0:.1f} ohm needs a wider shield; with outer = {outer} the "
+                                     f"air-filled line is {z0_air:.1f} ohm")
+                er = 1.0
+            diel_eps = f32(er)
+        else:
+            diel_eps = max(f32(eps_r), f32(1.0))
+        z0_line = z0_air / math.sqrt(max(float(diel_eps), 1.0))
+        st = round_half_away(float(start[ax]))
+        centre = [0.0, 0.0, 0.0]
+        centre[ax] = float(st)
+        centre[u] = math.floor(float(f32(start[u]))) + 0.5
+        centre[v] = math.floor(float(f32(start[v]))) + 0.5
+
+        def put(t0, t1, size, eps=1.0, sigma=0.0):
+            lo, hi = [0.0] * 3, [0.0] * 3
+            lo[ax] = float(st + t0 if s > 0 else st - t1)
+            hi[ax] = float(st + t1 if s > 0 else st - t0)
+            for k in (u, v):
+                lo[k] = float(f32(centre[k]) - f32(0.5) * f32(size))
+                hi[k] = float(f32(centre[k]) + f32(0.5) * f32(size))
+            self.box(lo, hi, eps_r=eps, sigma=sigma)
+
+        metal = float(f32(4.0) * PERFECT_EDGE_SIGMA)  # every edge touching it reaches the clamp
+        shield = outer + 2 * wall
+        capped = term != "terminals"
+        c = CoaxSetup(name=name, axis=ax, sign=s, start=st, center=tuple(centre), length=ln, inner=inner, outer=outer,
+                      wall=wall, epsilon_r=float(diel_eps), z0_ohm=float(f32(z0_line)), wavelength=wv.wavelength,
+                      cell_size=float(f32(self.cell_size)), termination=term, pulsed=wv.waveform == WAVE_PULSE)
+        if balun:
+            bl = round_half_away(0.25 * wv.wavelength if balun_length is None else float(balun_length))
+            gap = round_half_away(float(balun_gap))
+            sleeve = shield + 2 * (gap + wall)
+            if bl + wall >= ln - 4:
+                raise ValueError("coax: balun_length is too long for this cable")
+            put(ln - bl - wall, ln, sleeve, sigma=metal)
+            put(ln - bl, ln, shield + 2 * gap)
+            c.sleeve_start, c.sleeve_end, c.sleeve_outer = ln - bl - wall, ln, sleeve
+        put(-wall, ln + wall if capped else ln, shield, sigma=metal)
+        put(0, ln, outer, eps=float(diel_eps))
+        inner_end = {"terminals": ln + 3, "short": ln, "load": ln - 1}.get(term, ln - 2)
+        put(1, inner_end, inner, sigma=metal)
+        gen_ohm = float(f32(z0_line if impedance is None else impedance))
+        gap_cell = [0, 0, 0]
+        gap_cell[ax] = st if s > 0 else st - 1
+        gap_cell[u] = int(math.floor(centre[u]))
+        gap_cell[v] = int(math.floor(centre[v]))
+        self.add_feed(wv, tuple(gap_cell), ax, float(s), 4, impedance=gen_ohm, drive=amplitude, voltage=True,
+                      name=name)
+        c.source_ohm, c.drive = gen_ohm, float(f32(amplitude))
+        c.load_ohm = float(f32(z0_line if load is None else load))
+        if term == "load":
+            quiet = SourceDesc(**{k: getattr(wv, k) for k in wv.__dataclass_fields__})
+            quiet.enabled = True
+            cell = list(gap_cell)
+            cell[ax] = st + ln - 1 if s > 0 else st - ln
+            self.add_feed(quiet, tuple(cell), ax, float(-s), 4, impedance=c.load_ohm, drive=0.0, voltage=True,
+                          name=name + " load")
+        elif load is not None:
+            self.warnings.append("coax: 'load' only applies to termination = \"load\"")
+        inner_node = outer_node = None
+        if term == "terminals":
+            tax = 1 if ax == 2 else 2
+            ts = 1
+            tu = v if tax == u else u
+            lo, hi = [0.0] * 3, [0.0] * 3
+            lo[ax] = float(st + ln if s > 0 else st - ln - 3)
+            hi[ax] = float(st + ln + 3 if s > 0 else st - ln)
+            lo[tu], hi[tu] = centre[tu] - 0.5, centre[tu] + 0.5
+            wall_in = float(f32(0.5) * f32(outer))
+            wall_out = wall_in + float(wall)
+            lo[tax] = centre[tax] - wall_out if ts > 0 else centre[tax] + wall_in
+            hi[tax] = centre[tax] - wall_in if ts > 0 else centre[tax] + wall_out
+            self.box(lo, hi, sigma=metal)
+            inner_node, outer_node = [0.0] * 3, [0.0] * 3
+            inner_node[ax] = outer_node[ax] = float(st + s * (ln + 2))
+            inner_node[tu] = outer_node[tu] = centre[tu] - 0.5
+            inner_node[tax] = centre[tax] + 0.5 * ts
+            outer_node[tax] = centre[tax] - ts * wall_out
+        self.coaxes.append(c)
+        return c, inner_node, outer_node
 
     def dipole_thin(self, center, axis=2, length=11.0, radius=None, impedance=50.0, amplitude=1.0, sign=1.0,
                     name="dipole", **wave):

@@ -181,3 +181,22 @@ def test_two_wave_fit_recovers_waves():
     assert r.forward_fraction == pytest.approx(abs(A) ** 2 / (2 * z0n) / (1.0 / (8 * z0n)), rel=1e-5)
     # a pulsed line or fewer than 8 stations: no fit
     assert not X.analyse(X.CoaxSetup(length=60, pulsed=True), n, ph.astype(np.float32), np.zeros(3 * n), dt).valid
+
+
+def test_scene_coax_builder():
+    """Scene.coax (src/ScriptEngine.cpp:1961-2127): dielectric from the geometry factor, the
+    inner conductor's extent per termination, generator and load feeds, the sleeve balun."""
+    from em_lab_amd.scene import Scene
+    g = coax_geometry_factor(1, 5)
+    for term, inner_cells in (("load", 62), ("short", 63), ("open", 61)):
+        sc = Scene(size=(96, 32, 32), pml=6, wavelength=16.0)
+        c, _, _ = sc.coax((10, 16, 16), axis=0, termination=term)
+        assert c.epsilon_r == pytest.approx((X.ETA0 / g / 50.0) ** 2, rel=1e-6) and c.z0_ohm == pytest.approx(50.0)
+        assert sc.eps[16, 16, 11] == 1.0 and sc.sigma[16, 16, 11] > 1e5  # inner conductor
+        assert int((sc.sigma[16, 16, :] > 1e5).sum()) == inner_cells + 2  # + both end plates
+        assert len(sc.ports) == (2 if term == "load" else 1)
+    sc = Scene(size=(96, 40, 40), pml=6, wavelength=16.0)
+    c, _, _ = sc.coax((10, 20, 20), axis=0, balun=True, termination="open")
+    assert (c.sleeve_start, c.sleeve_end, c.sleeve_outer) == (64 - 4 - 1, 64, 7 + 2 * 3)
+    with pytest.raises(ValueError):
+        Scene(size=(96, 32, 32)).coax((10, 16, 16), outer=4)

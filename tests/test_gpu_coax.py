@@ -70,3 +70,43 @@ def test_lines_configure_errors_and_disable():
     g.step(2)
     assert dev.read()[0].shape == (0, 4)
     g.close()
+
+
+# The reference's validation scene scenes/validation/coax_line.lua: 232 x 64 x 64 cells, absorber
+# 16, lambda 48, a 150-cell 50 ohm coax{} from (36, 32, 32) along +x. Its quoted measurements
+# (the scene's header, README.md:208): Z0 50.0 ohm; matched load Gamma 0.057 (return loss
+# 25-31 dB); open |Gamma| 0.999; short 1.000 at 180 deg; 100 ohm load Gamma 0.338, Z 94 - j22 ohm.
+COAX_VALIDATION = [("load", None), ("open", None), ("short", None), ("load", 100.0)]
+
+
+@pytest.mark.parametrize("term,load", COAX_VALIDATION, ids=["load", "open", "short", "100ohm"])
+def test_coax_validation_scene(term, load):
+    from em_lab_amd.scene import Scene
+    print(f"case: coax validation scene, termination {term} load {load}, 4000 steps", flush=True)
+    sc = Scene(size=(232, 64, 64), pml=16, wavelength=48.0)
+    c, _, _ = sc.coax((36, 32, 32), axis=0, sign=1, length=150, z0=50.0, termination=term, load=load)
+    g = E.SimulationGrid(0)
+    g.configure(sc.grid_config())
+    s, e = sc.flat_materials()
+    g.upload_materials(s, e, sc.subcell)
+    g.set_sources(sc.sources)
+    mon = X.CoaxMonitor(g)
+    mon.configure(sc.coaxes)
+    g.step(4000)
+    r = mon.readings()[0]
+    gam = r.gamma_end
+    print(f"{term} {load}: z0 {r.z0_measured:.3f} |gamma| {abs(gam):.4f} @ {np.degrees(np.angle(gam)):.1f} deg "
+          f"RL {r.return_loss_db:.1f} dB Z {r.z_end.real:.1f}{r.z_end.imag:+.1f}j fwd {r.forward_fraction:.4f} "
+          f"residual {r.fit_residual:.2e}", flush=True)
+    g.close()
+    assert r.valid
+    assert r.z0_measured == pytest.approx(50.0, abs=1.0)
+    if term == "load" and load is None:
+        assert abs(gam) < 0.08  # reference 0.057
+    elif term == "open":
+        assert abs(gam) > 0.98
+    elif term == "short":
+        assert abs(gam) > 0.98 and abs(abs(np.degrees(np.angle(gam))) - 180.0) < 5.0
+    else:
+        assert abs(gam) == pytest.approx(0.338, abs=0.03)
+        assert abs(r.z_end - (94 - 22j)) < 8.0
