@@ -242,6 +242,8 @@ class CoaxMonitor:
         self.start_time = 0.0
         self._dev = LineMonitor(grid)
         self._peaks: List[float] = []
+        self._pending = None
+        self.polled: List[CoaxReading] = []
 
     def configure(self, lines: Sequence[CoaxSetup], averaging_periods: float = 4.0) -> None:
         self.lines = list(lines)
@@ -251,6 +253,8 @@ class CoaxMonitor:
         wl = self.lines[0].wavelength if self.lines else 16.0
         self._dev.configure(q, t, w, wl, self.averaging)
         self._peaks = [0.0] * len(self.lines)
+        self._pending = None
+        self.polled = [CoaxReading() for _ in self.lines]
 
     def restart(self) -> None:
         self.configure(self.lines, self.averaging)
@@ -266,8 +270,18 @@ class CoaxMonitor:
             return 0.0
         return max(self.grid.time - self.start_time, 0.0) / max(self.lines[0].wavelength, 2.0)
 
+    def poll(self) -> None:
+        """CoaxMonitor::poll (src/Coax.cpp:199-214), once per frame: the readback taken at the
+        previous poll is analysed into `polled`, then the current phasors are read back."""
+        if self._pending is not None:
+            self.polled = self._analyse(*self._pending)
+        self._pending = self._dev.read()
+
     def readings(self) -> List[CoaxReading]:
-        ph, now = self._dev.read()
+        """The analysis of the phasors as they are now."""
+        return self._analyse(*self._dev.read())
+
+    def _analyse(self, ph, now) -> List[CoaxReading]:
         out = []
         for i, c in enumerate(self.lines):
             first, stations = self.layout[i]

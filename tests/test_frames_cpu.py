@@ -106,3 +106,23 @@ def test_probe_parsing_and_scene_errors():
     # a scene that cannot be built ends the run with status 3 before any device work
     assert F.run(["--scene", "no-such-scene", "--frames", "1"]) == 3
     assert F.run(["--scene", "hertzian", "--probe", "bad", "--frames", "1"]) == 3
+
+
+def test_coax_report_lines():
+    """printCoax (src/Instruments.cpp:393-410) on a hand-chosen reading."""
+    from em_lab_amd.coax import CoaxReading, CoaxSetup
+    inst = F.Instruments(SimpleNamespace(time=480.0, step_count=842, dt=0.57))
+    c = CoaxSetup(name="cable", z0_ohm=50.0, epsilon_r=2.25, termination="load", wavelength=48.0)
+    r = CoaxReading(valid=True, z0_measured=49.96, gamma_end=complex(0.0, -0.0583), return_loss_db=24.69, vswr=1.1238,
+                    z_end=complex(48.9, -5.7), forward_fraction=0.9948, common_mode_db=-250.0, fit_residual=1.6e-3)
+    inst.coax = SimpleNamespace(enabled=lambda: True, periods=lambda: 10.0, lines=[c, c],
+                                polled=[r, CoaxReading()])
+    assert inst.coax_lines() == [
+        "    cable: Z0 50.0 ohm (eps_r 2.25, velocity factor 0.67), load end, 10 periods",
+        "      waves: Z0 measured 50.0 ohm, end Gamma 0.058 @ -90 deg (RL 24.7 dB), VSWR 1.12, Z_end 48.9 -5.7j ohm",
+        "      forward 99% of available, delivered 100% of forward; shield outside current -200.0 dB vs line current; "
+        "fit residual 1.6e-03",
+        "    cable: Z0 50.0 ohm (eps_r 2.25, velocity factor 0.67), load end, 10 periods",
+    ]
+    with pytest.raises(ValueError):
+        F.build_scene("coax_line:lossy")

@@ -62,3 +62,18 @@ def test_frames_fed_dipole_sweep_and_flux():
     assert any(ln.startswith("    sweep dipole") and "best match" in ln for ln in lines)
     assert not any(ln.startswith("    port  ") for ln in lines)  # sweeping: no continuous-wave lines
 
+
+
+def test_frames_coax_line():
+    """The reference's coax validation scene headless (8 steps per frame, as its view block):
+    the cable lines appear with the measured wave impedance and the matched load's reflection."""
+    print("case: frames coax_line 232x64x64, 400 frames of 8 steps", flush=True)
+    lines = []
+    rc = F.run(["--scene", "coax_line", "--frames", "400", "--steps", "8"], out=lines.append)
+    assert rc == 0
+    head = [ln for ln in lines if ln.startswith("    cable: Z0 50.0 ohm (eps_r 2.46")]
+    dt = float(np.float32(F.build_scene("coax_line").courant))
+    assert len(head) == 5 and head[-1].endswith("load end, %.0f periods" % (3200 * dt / 48.0))  # 4 + the end
+    waves = [ln for ln in lines if ln.startswith("      waves: Z0 measured ")]
+    assert waves and float(waves[-1].split("Z0 measured ")[1].split(" ohm")[0]) == pytest.approx(50.0, abs=1.0)
+    assert float(waves[-1].split("end Gamma ")[1].split(" @")[0]) < 0.08
