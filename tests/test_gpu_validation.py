@@ -1,0 +1,66 @@
+"""Two of the reference's validation scenes run on the GPU, checked against the figures the
+reference quotes as measured with its own code (each scene's header, README.md:194-196):
+- scenes/validation/dipole_impedance.lua: 160^3 cells, lambda 40, a centre-fed z thin-wire
+  dipole of radius 0.02 cells, CW, far field on: the port impedance at the four lengths listed;
+- scenes/validation/hertzian_dipole.lua: 128^3 cells, lambda 16, a z point dipole of amplitude 7,
+  CW, far field on: directivity 1.87 dBi and HPBW 87.5 deg in the elevation cuts."""
+import numpy as np
+import pytest
+
+from em_lab_amd import emfdtd as E
+from em_lab_amd.scene import Scene
+
+pytestmark = pytest.mark.gpu
+
+# length (cells) -> the reference's measured feed impedance (ohm)
+DIPOLES = [(7, 5.9 - 812j), (15, 32.6 - 222j), (19, 62.9 - 37j), (21, 87.1 + 54j)]
+
+
+@pytest.mark.parametrize("length,z_ref", DIPOLES, ids=[str(d[0]) for d in DIPOLES])
+def test_dipole_impedance_scene(length, z_ref):
+    print(f"case: dipole impedance scene 160^3, length {length}, 4000 steps", flush=True)
+    sc = Scene(size=(160, 160, 160), wavelength=40.0)
+    sc.dipole_thin((80.0, 80.0, 80.0), axis=2, length=float(length), radius=0.02, amplitude=1.0)
+    sc.set_far_field()
+    sc.emit_thin_wires()
+    g = E.SimulationGrid(0)
+    g.configure(sc.grid_config())
+    s, e = sc.flat_materials()
+    g.upload_materials(s, e, sc.subcell)
+    g.set_sources(sc.sources)
+    pm = E.PortMonitor(g)
+    pm.configure(sc.ports, sc.far_field.averaging_periods, None, sc.cell_size)
+    ff = E.FarFieldMonitor(g)
+    ff.configure(sc.far_field)
+    g.step(4000)
+    z = pm.evaluate()[0].z_ohm
+    print(f"length {length}: Z {z.real:.1f} {z.imag:+.1f}j ohm (reference {z_ref.real:.1f} {z_ref.imag:+.1f}j)",
+          flush=True)
+    g.close()
+    # the reference quotes R to 0.1 ohm and X to 1 ohm
+    assert abs(z.real - z_ref.real) < 0.5
+    assert abs(z.imag - z_ref.imag) < 1.5
+
+
+def test_hertzian_dipole_scene():
+    print("case: hertzian dipole scene 128^3, lambda 16, 3000 steps", flush=True)
+    sc = Scene(size=(128, 128, 128), wavelength=16.0)
+    sc.hertzian((64.0, 64.0, 64.0), (0, 0, 1), 7.0)
+    sc.set_far_field()
+    g = E.SimulationGrid(0)
+    g.configure(sc.grid_config())
+    s, e = sc.flat_materials()
+    g.upload_materials(s, e, sc.subcell)
+    g.set_sources(sc.sources)
+    ff = E.FarFieldMonitor(g)
+    ff.configure(sc.far_field)
+    g.step(3000)
+    pat = ff.evaluate()
+    g.close()
+    d = pat["max_directivity_dbi"]
+    bw = [pat["cuts"][k]["beamwidth_deg"] for k in ("xz", "yz")]
+    print(f"hertzian: D {d:.3f} dBi, HPBW xz {bw[0]:.2f} yz {bw[1]:.2f} deg (reference 1.87 dBi, 87.5 deg)", flush=True)
+    assert pat["valid"]
+    assert d == pytest.approx(1.87, abs=0.02)
+    for b in bw:
+        assert b == pytest.approx(87.5, abs=0.3)
