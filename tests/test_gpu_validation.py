@@ -64,3 +64,33 @@ def test_hertzian_dipole_scene():
     assert d == pytest.approx(1.87, abs=0.02)
     for b in bw:
         assert b == pytest.approx(87.5, abs=0.3)
+
+
+def test_phased_array_beamwidth():
+    """scenes/03_phased_array.lua at its defaults: 8 centre-fed thin-wire z dipoles of 9 cells
+    spaced lambda/2 = 9 cells along y around the grid centre, each with its own 50 ohm feed,
+    uniform, unsteered, far field about +x, on 176 x 176 x 96 cells at lambda 18. The reference
+    quotes a 12.7 deg beamwidth for this 8-element array (README.md:200); the array's cut is xy."""
+    print("case: phased array scene 176x176x96, 8 elements, 3000 steps", flush=True)
+    sc = Scene(size=(176, 176, 96), wavelength=18.0)
+    for i in range(8):
+        sc.dipole_thin((88.0, 88.0 + (i - 3.5) * 9.0, 48.0), axis=2, length=9.0, amplitude=1.0, name=f"element {i + 1}")
+    sc.set_far_field(axis=(1, 0, 0))
+    sc.emit_thin_wires()
+    g = E.SimulationGrid(0)
+    g.configure(sc.grid_config())
+    s, e = sc.flat_materials()
+    g.upload_materials(s, e, sc.subcell)
+    g.set_sources(sc.sources)
+    pm = E.PortMonitor(g)
+    pm.configure(sc.ports, sc.far_field.averaging_periods, None, sc.cell_size)
+    ff = E.FarFieldMonitor(g)
+    ff.configure(sc.far_field)
+    g.step(3000)
+    pat = ff.evaluate()
+    g.close()
+    bw = pat["cuts"]["xy"]["beamwidth_deg"]
+    print(f"phased array: xy-cut HPBW {bw:.2f} deg, D {pat['max_directivity_dbi']:.2f} dBi (reference 12.7 deg)",
+          flush=True)
+    assert pat["valid"]
+    assert bw == pytest.approx(12.7, abs=0.1)
