@@ -94,3 +94,35 @@ def test_phased_array_beamwidth():
           flush=True)
     assert pat["valid"]
     assert bw == pytest.approx(12.7, abs=0.1)
+
+
+def test_planar_array_over_reflector():
+    """scenes/11_planar_array.lua at its defaults: a 4 x 4 array of 9-cell thin-wire z dipoles
+    (spacing 9 cells along y, 13 along z) around the centre of 160 x 176 x 176 cells at lambda 18,
+    a PEC plate (sigma 1e7, src/ScriptEngine.cpp:425) one cell thick and 4.5 lambda square a
+    quarter wavelength behind it, far field about +x. Quoted: 18.7 dBi (README.md:201)."""
+    print("case: planar array scene 160x176x176, 16 elements over a reflector, 3000 steps", flush=True)
+    sc = Scene(size=(160, 176, 176), wavelength=18.0)
+    for i in range(4):
+        for j in range(4):
+            sc.dipole_thin((80.0, 88.0 + (i - 1.5) * 9.0, 88.0 + (j - 1.5) * 13.0), axis=2, length=9.0, amplitude=1.0,
+                           name=f"element {4 * i + j + 1}")
+    sc.box((80.0 - 4.5 - 0.5, 88.0 - 40.5, 88.0 - 40.5), (80.0 - 4.5 + 0.5, 88.0 + 40.5, 88.0 + 40.5), sigma=1.0e7)
+    sc.set_far_field(axis=(1, 0, 0))
+    sc.emit_thin_wires()
+    g = E.SimulationGrid(0)
+    g.configure(sc.grid_config())
+    s, e = sc.flat_materials()
+    g.upload_materials(s, e, sc.subcell)
+    g.set_sources(sc.sources)
+    pm = E.PortMonitor(g)
+    pm.configure(sc.ports, sc.far_field.averaging_periods, None, sc.cell_size)
+    ff = E.FarFieldMonitor(g)
+    ff.configure(sc.far_field)
+    g.step(3000)
+    pat = ff.evaluate()
+    g.close()
+    d = pat["max_directivity_dbi"]
+    print(f"planar array: D {d:.2f} dBi (reference 18.7 dBi)", flush=True)
+    assert pat["valid"]
+    assert d == pytest.approx(18.7, abs=0.1)
