@@ -1,7 +1,8 @@
 """Pins for the CPU oracle: each test checks an oracle function against something the oracle
 does not compute itself — binary16 known answers, closed forms of the discrete Yee scheme,
 conservation laws, and exact sums. A plausible mistake (wrong stagger, sign, index or operand)
-fails at least one of them; test_oracle_mutations.py plants such mistakes to show it."""
+fails at least one of them; tests/test_oracle_mutations.py compiles copies of the oracle with such
+mistakes planted and shows that these pins reject each one."""
 import ctypes as C
 import math
 
@@ -459,3 +460,142 @@ def test_ntff_hertzian_directivity(oracle):
                        _ptr(center), _ptr(f2), C.byref(P64))
     np.testing.assert_allclose(out[0::2][::53], f2[:d64.shape[0]], rtol=2e-4, atol=1e-7 * f2.max())
     assert P == pytest.approx(P64.value, rel=1e-5)
+
+
+# ---------------------------------------------------------------------------------------------
+# edge averaging of uploadMaterials (src/SimulationGrid.cpp:224-248)
+# ---------------------------------------------------------------------------------------------
+def _unpack_edges(oracle, edge):
+    """(n, 3) edge sigma and (n, 3) edge eps_r from the packed half2 edge texels."""
+    L = oracle.lib()
+    lo = (edge & 0xFFFF).astype(np.uint16).view(np.float16).astype(np.float64)
+    hi = (edge >> 16).astype(np.uint16).view(np.float16).astype(np.float64)
+    sig = np.stack([lo[:, 0], hi[:, 0], lo[:, 1]], 1)
+    eps = np.stack([hi[:, 1], lo[:, 2], hi[:, 2]], 1)
+    assert L.or_f16_to_f32(L.or_f32_to_f16(1.0)) == 1.0
+    return sig, eps
+
+
+# Yee edge positions relative to the texel (i, j, k): Ex (i+1/2, j, k), Ey (i, j+1/2, k), Ez (i, j, k+1/2)
+EDGE_AT = np.array([[0.5, 0, 0], [0, 0.5, 0], [0, 0, 0.5]])
+
+
+def test_edge_average_linear_medium(oracle):
+    """sigma and eps_r linear in (x, y, z), sampled at the cell centres (i+1/2, j+1/2, k+1/2): the
+    4-cell average on a Yee edge is the field at the edge midpoint, exactly (values and sums are
+    binary16- and float-exact). At the lower faces the neighbour index is clamped to 0, so the
+    transverse coordinate reads 1/2 instead of 0 there."""
+    size = (9, 8, 10)  # the grid is at least 8 texels wide (configure clamp)
+    nx, ny, nz = size
+    n = nx * ny * nz
+    z, y, x = np.unravel_index(np.arange(n), (nz, ny, nx))
+    pos = np.stack([x, y, z], -1).astype(np.float64)
+    cs, ce = np.array([0.25, 0.125, 0.5]), np.array([0.5, 0.25, 0.125])
+    sig = (0.5 + (pos + 0.5) @ cs).astype(np.float32)
+    eps = (1.0 + (pos + 0.5) @ ce).astype(np.float32)
+    sim = oracle.OracleSim(size, 0.5, 0, sigma_max=0.0)
+    sim.upload_materials(sig, eps)
+    es, ee = _unpack_edges(oracle, sim.edge)
+    for c in range(3):
+        at = pos + EDGE_AT[c]
+        # transverse axes: clamped neighbour at index 0 -> both cells sit at coordinate 1/2
+        for a in range(3):
+            if a != c:
+                at[:, a] = np.maximum(at[:, a], 0.5)
+        np.testing.assert_array_equal(es[:, c], 0.5 + at @ cs, err_msg=f"sigma, component {c}")
+        np.testing.assert_array_equal(ee[:, c], 1.0 + at @ ce, err_msg=f"eps_r, component {c}")
+
+
+@pytest.mark.parametrize("cell", [(4, 3, 5), (0, 3, 2), (3, 0, 0), (0, 0, 0)])
+def test_edge_average_isolated_cell(oracle, cell):
+    """One non-vacuum cell touches exactly the 4 Ex, 4 Ey and 4 Ez edges its cube has (a quarter
+    of its value each); an edge on a lower face at index 0 counts the cell once more for each
+    clamped neighbour (1/2 or all of it)."""
+    size = (8, 9, 10)
+    nx, ny, nz = size
+    n = nx * ny * nz
+    a, b, c0 = cell
+    sig = np.zeros(n, np.float32)
+    eps = np.ones(n, np.float32)
+    idx = (c0 * ny + b) * nx + a
+    sig[idx] = 8.0
+    eps[idx] = 5.0
+    sim = oracle.OracleSim(size, 0.5, 0, sigma_max=0.0)
+    sim.upload_materials(sig, eps)
+    es, ee = _unpack_edges(oracle, sim.edge)
+    es = es.reshape(nz, ny, nx, 3)
+    ee = ee.reshape(nz, ny, nx, 3)
+    for comp in range(3):
+        want = np.zeros((nz, ny, nx))
+        # the edge of component comp at texel t averages cells t - {0,1} on the two other axes
+        for dz in (0, 1):
+            for dy in (0, 1):
+                for dx in (0, 1):
+                    d = (dx, dy, dz)
+                    if d[comp] != 0:
+                        continue
+                    # cell (a, b, c0) is the neighbour t - d of texel t = cell + d (clamped at 0)
+                    for tz in range(nz):
+                        for ty in range(ny):
+                            for tx in range(nx):
+                                t = (tx, ty, tz)
+                                nb = tuple(max(t[i] - d[i], 0) for i in range(3))
+                                if nb == (a, b, c0):
+                                    want[tz, ty, tx] += 0.25
+        assert np.count_nonzero(want) > 0
+        np.testing.assert_array_equal(es[..., comp], 8.0 * want, err_msg=f"sigma comp {comp}")
+        np.testing.assert_array_equal(ee[..., comp], 1.0 + 4.0 * want, err_msg=f"eps comp {comp}")
+        if cell == (4, 3, 5):
+            # interior cell: exactly four edges per component, each holding a quarter
+            assert np.count_nonzero(want) == 4 and np.all(want[want > 0] == 0.25)
+            touched = set(zip(*np.nonzero(want)))
+            axes = [i for i in range(3) if i != comp]
+            expect = set()
+            for u in (0, 1):
+                for v in (0, 1):
+                    t = [a, b, c0]
+                    t[axes[0]] += u
+                    t[axes[1]] += v
+                    expect.add((t[2], t[1], t[0]))
+            assert touched == expect
+
+
+# ---------------------------------------------------------------------------------------------
+# Huygens-node sampling eAtNode / bAtNode (shaders/field_sampling.glsl:17-32)
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("seed", [0, 1])
+def test_node_sampling_linear_fields(oracle, seed):
+    """A field linear in position, stored at each component's Yee stagger, is returned at the
+    integer node exactly: the E means run along the component's own axis, the B means over the two
+    transverse axes. At index 0 the clamped neighbour is the texel itself, so the node value is
+    the staggered sample's own (coordinate 1/2 on the averaged axes)."""
+    L = oracle.lib()
+    size = (10, 9, 8)
+    nx, ny, nz = size
+    n = nx * ny * nz
+    rng = np.random.default_rng(seed)
+    ce = rng.integers(-8, 9, (3, 3)) / 16.0
+    cb = rng.integers(-8, 9, (3, 3)) / 16.0
+    z, y, x = np.unravel_index(np.arange(n), (nz, ny, nx))
+    pos = np.stack([x, y, z], -1).astype(np.float64)
+    sim = oracle.OracleSim(size, 0.5, 0)
+    for c in range(3):
+        sim.E[:, c] = (pos + STAG_E[c]) @ ce[c] + c
+        sim.B[:, c] = (pos + STAG_B[c]) @ cb[c] - c
+    out = np.zeros(3, np.float32)
+    for p in [(4, 5, 3), (9, 8, 7), (1, 1, 1), (0, 4, 2), (3, 0, 0), (0, 0, 0)]:
+        L.or_e_at_node(C.byref(sim.g), _ptr(sim.E), *p, _ptr(out))
+        e = out.copy()
+        L.or_b_at_node(C.byref(sim.g), _ptr(sim.B), *p, _ptr(out))
+        b = out.copy()
+        q = np.array(p, np.float64)
+        for c in range(3):
+            qe = q.copy()
+            if q[c] == 0:
+                qe[c] = 0.5
+            assert e[c] == np.dot(qe, ce[c]) + c, (p, c, "E")
+            qb = q.copy()
+            for a in range(3):
+                if a != c and q[a] == 0:
+                    qb[a] = 0.5
+            assert b[c] == np.dot(qb, cb[c]) - c, (p, c, "B")
