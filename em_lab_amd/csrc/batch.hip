@@ -131,7 +131,7 @@ __global__ void k_batch_advance(int64_t *d_step, int count) {
 }
 
 // One workgroup (one wavefront) per scene, one lane per sweep point. The rotation recurrence is
-// PortMonitor::evaluateSweep's (src/Ports.cpp:352-366) in FP64 with products rounded as written.
+// PortMonitor::evaluateSweep's (src/Ports.cpp:217-233) in FP64 with products rounded as written.
 __global__ void __launch_bounds__(64) k_batch_sweep(const BatchSweepJob *jobs) {
     const BatchSweepJob jb = jobs[blockIdx.x];
     const int lane = threadIdx.x;
@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(64) k_batch_sweep(const BatchSweepJob *jobs) {
             o[1] = make_double2(sir, sii);
         }
     }
-    // ring-down residual of port 0 (src/Ports.cpp:340-348): max is order-independent
+    // ring-down residual of port 0 (src/Ports.cpp:206-213): max is order-independent
     __shared__ double s_peak[64], s_late[64];
     double peak = 0.0, late = 0.0;
     const int64_t tail = jb.n - max(jb.n / 20, static_cast<int64_t>(1));

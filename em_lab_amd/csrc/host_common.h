@@ -59,7 +59,7 @@ struct PackedSources {
 };
 PackedSources pack_sources(const emfd_grid_config &c, const emfd_source_desc *src, int n);
 
-// PortMonitor sweep (src/Ports.cpp:336-414): nu evenly spaced over [1/lambda_max, 1/lambda_min]
+// PortMonitor sweep (src/Ports.cpp:187-218): nu evenly spaced over [1/lambda_max, 1/lambda_min]
 struct SweepBand {
     int points = 2;
     double nu_lo = 0.0, nu_hi = 0.0;

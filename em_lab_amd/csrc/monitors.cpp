@@ -53,7 +53,7 @@ SweepBand sweep_band(const emfd_sweep_setup &sw) {
 
 void sweep_finish(const SweepBand &band, const std::complex<double> *sumV, const std::complex<double> *sumI,
                   double impedance, emfd_sweep_point *points, emfd_port_sweep *summary) {
-    // src/Ports.cpp:370-414
+    // src/Ports.cpp:234-278
     const double z0 = std::max(impedance, 1e-6);
     const int npts = band.points;
     std::vector<emfd_sweep_point> pts;
@@ -275,7 +275,7 @@ int emfd_lines_configure(emfd_ctx *ctx, const int *quantities, int n, const int 
                          int n_terms, float wavelength, float averaging_periods) {
     if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
     if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
-    if (n < 0 || n_terms < 0 || (n > 0 && (!quantities || (n_terms > 0 && (!terms || !weights)))))
+    if (n < 0 || n_terms < 0 || (n > 0 && !quantities) || (n_terms > 0 && (!terms || !weights)))
         return fail(EMFD_ERR_INVALID, "null or negative line-monitor arrays");
     for (int q = 0; q < n; ++q) {
         const int first = quantities[4 * q], count = quantities[4 * q + 1], kind = quantities[4 * q + 2];
@@ -440,7 +440,7 @@ int emfd_ports_evaluate(emfd_ctx *ctx, emfd_port_reading *out, int n) {
     std::vector<float> ph(static_cast<size_t>(np) * 8);
     if (np > 0)
         if (int rc = emfd_ports_read_phasors(ctx, ph.data(), np)) return rc;
-    // PortMonitor::computeReadings (src/Ports.cpp:299-325)
+    // PortMonitor::computeReadings (src/Ports.cpp:159-185)
     for (int i = 0; i < np; ++i) {
         const emfd_port_setup &p = ctx->ports.ports[i];
         const float *s2 = &ph[8 * i + 4];
@@ -492,7 +492,7 @@ int emfd_ports_evaluate_sweep(emfd_ctx *ctx, double dt, emfd_sweep_point *points
     const size_t n = static_cast<size_t>(P.recorded);
     std::vector<float> series(n * nPorts * 2);
     if (int rc = emfd_ports_read_series(ctx, series.data(), static_cast<int64_t>(series.size()))) return rc;
-    // PortMonitor::evaluateSweep (src/Ports.cpp:336-414)
+    // PortMonitor::evaluateSweep (src/Ports.cpp:187-278)
     const SweepBand band = sweep_band(P.sweep);
     double peak = 0.0, late = 0.0;
     for (size_t k = 0; k < n; ++k) {
@@ -715,7 +715,7 @@ int emfd_ntff_evaluate(emfd_ctx *ctx, int *valid, double *radiated_power, float 
     std::vector<float> f2(static_cast<size_t>(nd));
     double P = 0.0;
     if (int rc = emfd_ntff_eval(ctx, N.dirs.data(), nd, f2.data(), &P)) return rc;
-    // FarFieldMonitor::finishEvaluation (src/FarField.cpp:427-468)
+    // FarFieldMonitor::finishEvaluation (src/FarField.cpp:294-335)
     const bool ok = std::isfinite(P) && P > 1e-12;
     if (valid) *valid = ok ? 1 : 0;
     if (radiated_power) *radiated_power = P;

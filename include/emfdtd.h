@@ -278,14 +278,14 @@ int emfd_ports_configure(emfd_ctx *ctx, const emfd_port_setup *ports, int n, flo
 int emfd_ports_disable(emfd_ctx *ctx);
 /* Raw EMA phasors: 8 floats per port (stage1 Vre Vim Ire Iim, stage2 ...). */
 int emfd_ports_read_phasors(emfd_ctx *ctx, float *out, int n);
-/* PortMonitor::evaluate + computeReadings (src/Ports.cpp:285-325): one reading per port. */
+/* PortMonitor::evaluate + computeReadings (src/Ports.cpp:145-185): one reading per port. */
 int emfd_ports_evaluate(emfd_ctx *ctx, emfd_port_reading *out, int n);
 /* Recorded series steps so far (PortMonitor::recordedSteps). */
 int64_t emfd_ports_recorded_steps(emfd_ctx *ctx);
 /* PortMonitor::readSeries (src/Ports.cpp:278-283): 2 floats (V, I) per [step][port];
  * cap_floats bounds the write. */
 int emfd_ports_read_series(emfd_ctx *ctx, float *out, int64_t cap_floats);
-/* PortMonitor::evaluateSweep (src/Ports.cpp:327-418). points: n_ports * sweep points entries,
+/* PortMonitor::evaluateSweep (src/Ports.cpp:187-278). points: n_ports * sweep points entries,
  * port-major; summary: n_ports entries. Any output pointer may be NULL. Returns EMFD_ERR_STATE
  * when no sweep is configured or fewer than 16 steps are recorded (the reference returns). */
 int emfd_ports_evaluate_sweep(emfd_ctx *ctx, double dt, emfd_sweep_point *points, emfd_port_sweep *summary,
@@ -320,7 +320,7 @@ int emfd_ntff_set_wavelengths(emfd_ctx *ctx, const float *wavelengths, int n);
 /* emfd_ntff_eval / emfd_ntff_read_phasors for wavelength `freq` of the list. */
 int emfd_ntff_eval_at(emfd_ctx *ctx, int freq, const float *dirs, int n_dirs, float *f2, double *power);
 int emfd_ntff_read_phasors_at(emfd_ctx *ctx, int freq, float *out, int64_t n_points);
-/* FarFieldMonitor::evaluate + finishEvaluation (src/FarField.cpp:411-468) on the reference
+/* FarFieldMonitor::evaluate + finishEvaluation (src/FarField.cpp:278-335) on the reference
  * direction set. directivity_dbi: n_dirs floats (may be NULL). cuts: 3 x 6 floats per cut
  * (peak dBi, peak angle, beamwidth, front/back, unused, unused) (may be NULL). Returns
  * *valid = 0 when P <= 1e-12 or non-finite. */
@@ -330,9 +330,9 @@ int emfd_ntff_evaluate(emfd_ctx *ctx, int *valid, double *radiated_power, float 
 /* ---------------------------------------------------------------------------------------
  * FieldAnalyzer (src/FieldAnalyzer.hpp:73-130)
  * --------------------------------------------------------------------------------------- */
-/* FieldAnalyzer::measureFlux (src/FieldAnalyzer.cpp:447-505) for n surfaces. */
+/* FieldAnalyzer::measureFlux (src/FieldAnalyzer.cpp:317-375) for n surfaces. */
 int emfd_flux_measure(emfd_ctx *ctx, const emfd_flux_surface *surfaces, int n, double *out);
-/* FieldAnalyzer::probe / submit+collect (src/FieldAnalyzer.cpp:274-303,367-388,404-445):
+/* FieldAnalyzer::probe / submit+collect (src/FieldAnalyzer.cpp:126-272,274-315):
  * positions: 3 floats each. */
 int emfd_probe_measure(emfd_ctx *ctx, const float *positions, int n, emfd_probe_sample *out);
 

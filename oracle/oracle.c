@@ -797,7 +797,7 @@ static inline void crossf(const float a[3], const float b[3], float o[3]) {
 static inline float dotf(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
 /* ntff_pattern.comp + ntff_reduce.comp + FarFieldMonitor::evaluate chunking
- * (shaders/ntff_pattern.comp:45-110, shaders/ntff_reduce.comp:19-39, src/FarField.cpp:297,354-376,411-425) */
+ * (shaders/ntff_pattern.comp:45-110, shaders/ntff_reduce.comp:19-39, src/FarField.cpp:226-243,256-276,294-297) */
 void or_ntff_eval_f32(const int bmin[3], const int bmax[3], const int fo[7], int n_points, const float *ph,
                       const float *dirs, int n_dirs, float k, const float center[3], float *out) {
     int slices = (n_points + 4095) / 4096;
@@ -1010,7 +1010,7 @@ int or_build_directions(const float axis[3], float *dirs, int cap, int *map_inde
 }
 
 /* poynting_flux.comp + FieldAnalyzer::planeFlux (shaders/poynting_flux.comp:32-63,
- * src/FieldAnalyzer.cpp:447-489) */
+ * src/FieldAnalyzer.cpp:317-359) */
 double or_plane_flux(const or_grid *g, const float *E, const float *B, int axis, int index, int rmin_u, int rmin_v,
                      int rmax_u, int rmax_v) {
     const int size[3] = {g->nx, g->ny, g->nz};
@@ -1129,7 +1129,7 @@ void or_probe(const or_grid *g, const float *E, const float *B, const float lp[3
     out[10] = bb[4 * ci + 3];
 }
 
-/* PortMonitor::evaluateSweep (src/Ports.cpp:327-418) */
+/* PortMonitor::evaluateSweep (src/Ports.cpp:187-278) */
 double or_sweep(const float *series, size_t n, int n_ports, int port, double dt, double wl_min, double wl_max,
                 int points, double z0, double *out, double *band) {
     const double kEta0 = 376.730313668;

@@ -147,13 +147,13 @@ int or_build_directions(const float axis[3], float *dirs, int cap, int *map_inde
 void or_e_at_node(const or_grid *g, const float *E, int x, int y, int z, float out[3]);
 void or_b_at_node(const or_grid *g, const float *B, int x, int y, int z, float out[3]);
 
-/* poynting_flux.comp + FieldAnalyzer::planeFlux (src/FieldAnalyzer.cpp:447-489) */
+/* poynting_flux.comp + FieldAnalyzer::planeFlux (src/FieldAnalyzer.cpp:317-359) */
 double or_plane_flux(const or_grid *g, const float *E, const float *B, int axis, int index, int rmin_u,
                      int rmin_v, int rmax_u, int rmax_v);
-/* FieldAnalyzer::measureFlux for a closed box (src/FieldAnalyzer.cpp:491-505) */
+/* FieldAnalyzer::measureFlux for a closed box (src/FieldAnalyzer.cpp:361-375) */
 double or_box_flux(const or_grid *g, const float *E, const float *B, const int box_min[3], const int box_max[3]);
 
-/* probe: probe_gather + sampleBlock (src/FieldAnalyzer.cpp:19-40,404-445).
+/* probe: probe_gather + sampleBlock (src/FieldAnalyzer.cpp:19-40,274-315).
  * out: E[3], B[3], S[3], sigma, eps (11 floats) */
 void or_probe(const or_grid *g, const float *E, const float *B, const float pos[3], float out[11]);
 

@@ -62,6 +62,8 @@ def test_lines_configure_errors_and_disable():
         dev.configure([[0, 1, 2, 0]], [[1, 1, 1, 0]], [1.0], 16.0)  # unknown kind
     with pytest.raises(E.EmfdError):
         dev.configure([[0, 1, 0, 0]], [[1, 1, 1, 6]], [1.0], 16.0)  # component outside 0..5
+    # no quantities but a term count with null term and weight arrays: rejected, not dereferenced
+    assert g._lib.emfd_lines_configure(g.handle, None, 0, None, None, 3, 16.0, 4.0) == 1  # EMFD_ERR_INVALID
     dev.configure([[0, 1, 0, 0]], [[12, 12, 12, 2]], [1.0], 16.0)
     g.step(20)
     ph, now = dev.read()

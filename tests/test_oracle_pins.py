@@ -351,7 +351,7 @@ def test_flux_uniform_fields(oracle):
 
 def test_probe_linear_fields_exact(oracle):
     """Trilinear interpolation reproduces a linear field exactly when each component is sampled
-    at its own Yee stagger (src/FieldAnalyzer.cpp:376-381)."""
+    at its own Yee stagger (src/FieldAnalyzer.cpp:300-306)."""
     size = (12, 11, 10)
     sim = oracle.OracleSim(size, 0.5, 0)
     n = np.prod(size)
