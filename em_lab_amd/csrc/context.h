@@ -103,6 +103,11 @@ struct emfd_ctx {
     int classes = 0;
     int has_wire = 0;
     int pml_materials = 1;  // LatticeArgs::pml_materials
+    // texels the one-pass kernel (EMFD_KERNEL_MARCH) recomputes after its vacuum update, sorted
+    // lattice indices: fix_b thin-wire classes (B), fix_e classes != 0 and the E texels whose curl
+    // reads a thin-wire texel's B (texel itself, +x, +y, +z)
+    emfd::DevBuf fix_b, fix_e;
+    int n_fix_b = 0, n_fix_e = 0;
     int prof_stride = 0;
 
     // sources
@@ -133,6 +138,7 @@ struct emfd_ctx {
     size_t ev_used = 0;
     double timing_ms = 0.0;
     int64_t timing_launches = 0;
+    std::vector<float> timing_samples;  // each bracketed launch's milliseconds since switched on
 
     emfd::PortsState ports;
     emfd::LinesState lines;

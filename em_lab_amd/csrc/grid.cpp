@@ -421,6 +421,7 @@ int emfd_grid_configure(emfd_ctx *ctx, const emfd_grid_config *requested, emfd_g
     // vacuum everywhere: E = (0,0,0, sigma 0), B = (0,0,0, eps 1) (src/SimulationGrid.cpp:185-192)
     ctx->classes = 0;
     ctx->has_wire = 0;
+    ctx->n_fix_b = ctx->n_fix_e = 0;
     ctx->id_bytes = 0;
     ctx->masks_ok = false;
     ctx->pml_materials = 0;  // vacuum everywhere
@@ -438,6 +439,53 @@ int emfd_grid_get_config(emfd_ctx *ctx, emfd_grid_config *out) {
     *out = ctx->cfg;
     return EMFD_OK;
 }
+
+namespace {
+// The texels the one-pass kernel recomputes after its vacuum update (EMFD_KERNEL_MARCH): B of the
+// thin-wire classes (a wire factor other than 1), E of every class other than vacuum and of the
+// texels whose curl reads a thin-wire texel's B (the texel itself and its +x, +y, +z neighbours).
+// Sorted lattice indices.
+int build_fix_lists(emfd_ctx *ctx, const std::vector<ClassKey> &keys, const uint32_t *ids) {
+    ctx->n_fix_b = ctx->n_fix_e = 0;
+    const int nx = ctx->cfg.size[0], ny = ctx->cfg.size[1], nz = ctx->cfg.size[2];
+    const size_t n = ctx->n;
+    std::vector<char> wire_class(keys.size(), 0);
+    for (size_t k = 0; k < keys.size(); ++k)
+        for (int f = 6; f < 12; ++f) wire_class[k] |= keys[k][f] != 0x3C00u;  // binary16 1.0
+    std::vector<uint32_t> fb, fe, near;
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t id = ids[i];
+        if (id == 0u) continue;
+        fe.push_back(static_cast<uint32_t>(i));
+        if (!wire_class[id]) continue;
+        fb.push_back(static_cast<uint32_t>(i));
+        const size_t x = i % nx, y = (i / nx) % ny, z = i / (static_cast<size_t>(nx) * ny);
+        if (x + 1 < static_cast<size_t>(nx)) near.push_back(static_cast<uint32_t>(i + 1));
+        if (y + 1 < static_cast<size_t>(ny)) near.push_back(static_cast<uint32_t>(i + nx));
+        if (z + 1 < static_cast<size_t>(nz)) near.push_back(static_cast<uint32_t>(i + static_cast<size_t>(nx) * ny));
+    }
+    if (!near.empty()) {
+        std::sort(near.begin(), near.end());
+        std::vector<uint32_t> merged;
+        merged.reserve(fe.size() + near.size());
+        std::merge(fe.begin(), fe.end(), near.begin(), near.end(), std::back_inserter(merged));
+        merged.erase(std::unique(merged.begin(), merged.end()), merged.end());
+        fe.swap(merged);
+    }
+    if (fb.size() > 0x7fffffffu || fe.size() > 0x7fffffffu) return fail(EMFD_ERR_INVALID, "too many material texels");
+    if (!fb.empty()) {
+        if (int rc = ctx->fix_b.ensure(fb.size() * sizeof(uint32_t))) return rc;
+        EMFD_HIP(hipMemcpy(ctx->fix_b.ptr, fb.data(), fb.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    if (!fe.empty()) {
+        if (int rc = ctx->fix_e.ensure(fe.size() * sizeof(uint32_t))) return rc;
+        EMFD_HIP(hipMemcpy(ctx->fix_e.ptr, fe.data(), fe.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    ctx->n_fix_b = static_cast<int>(fb.size());
+    ctx->n_fix_e = static_cast<int>(fe.size());
+    return EMFD_OK;
+}
+}  // namespace
 
 int emfd_grid_upload_materials(emfd_ctx *ctx, const float *sigma, const float *eps_r, int64_t n_cells,
                                const emfd_edge_override *edges, int64_t n_edges, const emfd_wire_coefficient *wires,
@@ -466,6 +514,7 @@ int emfd_grid_upload_materials(emfd_ctx *ctx, const float *sigma, const float *e
     ctx->has_wire = thin ? 1 : 0;
     ctx->masks_ok = false;
     ctx->pml_materials = absorber_has_materials(ctx->cfg, ids.data()) ? 1 : 0;
+    if (int rc = build_fix_lists(ctx, keys, ids.data())) return rc;
     if (keys.size() == 1 && !thin) {
         ctx->classes = 0;  // uniform vacuum: kernels skip the class id stream
         ctx->id_bytes = 0;
@@ -582,7 +631,7 @@ int emfd_grid_set_storage(emfd_ctx *ctx, int storage, const int *scale_log2) {
 int emfd_grid_set_kernel(emfd_ctx *ctx, int variant) {
     if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
     const int v = variant & 0xff;
-    if (v < 0 || v > 7) return fail(EMFD_ERR_INVALID, "unknown kernel variant");
+    if (v < 0 || v > 8) return fail(EMFD_ERR_INVALID, "unknown kernel variant");
     ctx->kernel_variant = v;
     ctx->tile_cfg = variant >> 8;  // optional tile shape / z-chunk selector for the fused kernel
     return EMFD_OK;
@@ -598,6 +647,7 @@ int emfd_grid_kernel_timing(emfd_ctx *ctx, int enable, double *total_ms, int64_t
             EMFD_HIP(hipEventElapsedTime(&ms, ctx->ev_pool[i], ctx->ev_pool[i + 1]));
             ctx->timing_ms += ms;
             ctx->timing_launches += 1;
+            if (ctx->timing_samples.size() < (1u << 20)) ctx->timing_samples.push_back(ms);
         }
         ctx->ev_used = 0;
     }
@@ -607,11 +657,32 @@ int emfd_grid_kernel_timing(emfd_ctx *ctx, int enable, double *total_ms, int64_t
         if (enable && !ctx->timing_every) {
             ctx->timing_ms = 0.0;
             ctx->timing_launches = 0;
+            ctx->timing_samples.clear();
         }
         ctx->timing_every = enable;
     }
     return EMFD_OK;
 }
+
+int emfd_grid_kernel_timing_samples(emfd_ctx *ctx, float *out, int cap, int *n) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    if (cap < 0 || (cap > 0 && !out)) return fail(EMFD_ERR_INVALID, "null or negative sample buffer");
+    if (int rc = emfd_grid_kernel_timing(ctx, -1, nullptr, nullptr)) return rc;
+    const int have = static_cast<int>(ctx->timing_samples.size());
+    const int m = std::min(have, cap);
+    for (int i = 0; i < m; ++i) out[i] = ctx->timing_samples[static_cast<size_t>(i)];
+    if (n) *n = have;
+    return EMFD_OK;
+}
+
+namespace {
+// EMFD_KERNEL_AUTO runs the one-pass march on FP32 lattices of at least 200^3 cells (8e6; at 256^3
+// 0.159 against 0.174 ms per Yee step for the two passes, at 768^3 3.83 against 5.76 ms); below
+// that its few long workgroups leave the GPU part idle (128^3: 0.046 against 0.036 ms).
+bool march_auto(const emfd_ctx *ctx) {
+    return ctx->storage == kStorageF32 && ctx->n >= 8000000u && yee_march_supported(ctx->lattice());
+}
+}  // namespace
 
 int emfd_grid_step(emfd_ctx *ctx, int count) {
     if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
@@ -663,7 +734,9 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
                          (v == EMFD_KERNEL_AUTO || v == EMFD_KERNEL_TWO_PASS4 || v == EMFD_KERNEL_SLABS);
         const bool half = (ctx->storage == kStorageF16 || ctx->storage == kStorageBF16) &&
                           yee_half_supported(ctx->lattice());
-        ctx->side_active = ctx->ntff.enabled && ((ctx->storage == kStorageF32 && tp4) || half);
+        const bool march = (v == EMFD_KERNEL_MARCH && yee_march_supported(ctx->lattice())) ||
+                           (v == EMFD_KERNEL_AUTO && march_auto(ctx));
+        ctx->side_active = ctx->ntff.enabled && ((ctx->storage == kStorageF32 && (tp4 || march)) || half);
         ctx->side_e_valid = true;
     }
     const LatticeArgs lat = ctx->lattice();
@@ -703,13 +776,16 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
                 EMFD_HIP(launch_yee_half(lat, ctx->fields(ctx->cur), ctx->tile_cfg & 0xf, ctx->stream));
             variant = -1;
         } else if (variant == EMFD_KERNEL_AUTO)
-            variant = yee_two_pass4_supported(lat) ? EMFD_KERNEL_TWO_PASS4 : EMFD_KERNEL_TWO_PASS;
+            variant = march_auto(ctx) ? EMFD_KERNEL_MARCH
+                                      : yee_two_pass4_supported(lat) ? EMFD_KERNEL_TWO_PASS4 : EMFD_KERNEL_TWO_PASS;
         if (variant == EMFD_KERNEL_TWO_PASS4 && !yee_two_pass4_supported(lat))
             return fail(EMFD_ERR_INVALID, "the vectorised two-pass kernel needs nx % 4 == 0");
         if (variant == EMFD_KERNEL_ZMARCH && !yee_zmarch_supported(lat))
             return fail(EMFD_ERR_INVALID, "the z-march kernel supports nx <= 256 with nx % 4 == 0");
         if (variant == EMFD_KERNEL_ROWS && !yee_rows_supported(lat))
             return fail(EMFD_ERR_INVALID, "the row-wavefront kernel supports nx <= 256");
+        if (variant == EMFD_KERNEL_MARCH && !yee_march_supported(lat))
+            return fail(EMFD_ERR_INVALID, "the one-pass march kernel needs nx % 4 == 0 and planes of at most 2 GB");
         if (variant == -1) {
             // launched above
         } else if (variant == EMFD_KERNEL_TWO_PASS) {
@@ -729,6 +805,11 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
         } else if (variant == EMFD_KERNEL_ZMARCH) {
             EMFD_HIP(launch_yee_zmarch(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->tile_cfg,
                                        ctx->stream));
+            ctx->cur ^= 1;
+        } else if (variant == EMFD_KERNEL_MARCH) {
+            EMFD_HIP(launch_yee_march(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->tile_cfg,
+                                      ctx->fix_b.as<uint32_t>(), ctx->n_fix_b, ctx->fix_e.as<uint32_t>(), ctx->n_fix_e,
+                                      ctx->stream));
             ctx->cur ^= 1;
         } else if (variant == EMFD_KERNEL_ROWS) {
             EMFD_HIP(launch_yee_rows(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->tile_cfg, ctx->stream));

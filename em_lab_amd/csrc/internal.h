@@ -125,6 +125,13 @@ int yee_rows_supported(const LatticeArgs &a);
 hipError_t launch_yee_zmarch(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int zchunk,
                              hipStream_t s);
 int yee_zmarch_supported(const LatticeArgs &a);
+// Fused one-pass z-march (yee_march.hip): src -> dst, then the sparse recomputation of the texels
+// in fix_b (B, thin-wire classes) and fix_e (E, non-vacuum classes and thin-wire neighbours).
+// cfg: z-chunk (0 = automatic).
+hipError_t launch_yee_march(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int cfg,
+                            const uint32_t *fix_b, int n_fix_b, const uint32_t *fix_e, int n_fix_e,
+                            hipStream_t s);
+int yee_march_supported(const LatticeArgs &a);
 hipError_t launch_yee_point(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, hipStream_t s);
 hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
 hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, int streaming, hipStream_t s);

@@ -1,0 +1,457 @@
+// Fused leapfrog in one pass: B^{n+1/2} and E^{n+1} of a whole lattice from one read of E^n and
+// B^{n-1/2} and one write of each (ping-pong buffers: src holds the state, dst receives the next).
+//
+// Work split. A workgroup owns an x segment of 256 cells (one wavefront per row: lane l holds the
+// four cells x0 = xs + 4l .. x0 + 3, float4 streams) of NW - 1 consecutive rows y0 .. y0 + NW - 2,
+// over one chunk of z planes, which it walks in order (z-march). Wave w holds row y0 - 1 + w: wave
+// 0 is the row below the group, whose B^{n+1/2} x, z components the first row's E update needs;
+// it computes them and stores nothing. Per plane z:
+//   B^{n+1/2}(z) needs E^n(z) and E^n(z+1) of its own row (registers: the march keeps three E
+//   planes and two B planes in flight), E^n(y+1) x, z (LDS: published by the wave above one
+//   plane earlier; the top wave loads its row above itself) and E^n(x+1) y, z (the next lane,
+//   DPP wave shift; lane 63 loads the cell past the segment).
+//   E^{n+1}(z) needs B^{n+1/2} of its own row at z and z-1 (registers), of the row below at z
+//   (LDS, published before the plane's barrier) and of x-1 y, z (the previous lane; lane 0 of a
+//   segment that does not start at x = 0 rebuilds B^{n+1/2}(xs - 1) from its own loads).
+// One workgroup barrier per plane: each wave publishes B^{n+1/2}(z) x, z and E^n(z+1) x, z into
+// buffers that alternate with the plane's parity. A chunk starts one plane early to rebuild
+// B^{n+1/2}(z0 - 1), which it does not store.
+//
+// Coefficients. Every texel is updated with the coefficients of a vacuum edge: absorber entries
+// per depth class staged in LDS per wavefront (as the two-pass kernels stage them), (1, dt)
+// outside the absorber; no class ids are read. Texels whose class is not vacuum (materials, feed
+// and wire edges) and the E texels next to a thin-wire texel are then recomputed from the same
+// src and dst planes by two sparse passes (k_march_fix_b, k_march_fix_e) with faraday() and
+// edge_coefficients(), so the result is bit-identical to the two-pass kernels.
+#include "yee_quad.h"
+
+namespace emfd {
+namespace {
+
+constexpr int kSeg = 256;  // x cells per wavefront
+
+__device__ __forceinline__ float lane_next(float v) {  // lane i <- lane i + 1 (lane 63 <- 0)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float lane_prev(float v) {  // lane i <- lane i - 1 (lane 0 <- 0)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
+}
+
+struct Row4 {
+    float v[3][4];
+};
+
+// Plane reads go through buffer resources: a byte offset per lane against the plane's 32-bit
+// range, where an offset at or past the range (kOob) reads 0 without touching memory. That gives
+// the lattice's zero boundaries and idle lanes without a select the compiler could turn into a
+// branch (and a wait for every load in flight). Writes use the plane pointer plus the offset.
+// Planes are at most 2 GB.
+constexpr uint32_t kOob = 0x80000000u;
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ __forceinline__ Rsrc plane_rsrc(const float *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), static_cast<short>(0), static_cast<int>(bytes),
+                                             0x00020000);
+}
+__device__ __forceinline__ void bld4(Rsrc r, uint32_t off, float (&o)[4]) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    o[0] = __uint_as_float(v[0]); o[1] = __uint_as_float(v[1]); o[2] = __uint_as_float(v[2]); o[3] = __uint_as_float(v[3]);
+}
+__device__ __forceinline__ float bld1(Rsrc r, uint32_t off) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void gst4(float *p, uint32_t off, const float (&v)[4]) {
+    // non-temporal: each value is written once per step
+    typedef float f32x4_t __attribute__((ext_vector_type(4)));
+    const f32x4_t d = {v[0], v[1], v[2], v[3]};
+    __builtin_nontemporal_store(d, reinterpret_cast<f32x4_t *>(reinterpret_cast<char *>(p) + off));
+}
+
+// LDS layout (floats): xe[2][NW][2][kSeg] | xb[2][NW][2][kSeg] | coef[NW][2][K * kCoefStride]
+//   xe[p & 1][w]: E^n(row of wave w, plane p) z, x components
+//   xb[p & 1][w]: B^{n+1/2}(row of wave w, plane p) x, z components
+template <int NW>
+__host__ __device__ constexpr size_t march_xch_floats() {
+    return static_cast<size_t>(2) * 2 * NW * 2 * kSeg;
+}
+
+template <int NW>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16 / NW, 16 / NW))) k_yee_march(LatticeArgs a, FieldPtrs src, FieldPtrs dst, int zchunk,
+                                                        int nseg, int nyt, int n_tiles) {
+    extern __shared__ float lds[];
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    // tiles in dispatch order: x segment fastest, then the row group, then the z-chunk, so the
+    // workgroups in flight cover one thin z-slab of the whole lattice at a time (handing each XCD
+    // a contiguous run of tiles instead measured 4-8% slower at 768^3)
+    const int tile = static_cast<int>(blockIdx.x);
+    if (tile >= n_tiles) return;  // uniform over the workgroup, before any barrier
+    const int nx = a.nx, ny = a.ny, nz = a.nz, L = a.pml, K = a.pml_k;
+    const int seg = tile % nseg;
+    const int yt = (tile / nseg) % nyt;
+    const int zc = tile / (nseg * nyt);
+    const int y = yt * (NW - 1) - 1 + w;
+    const int z0 = zc * zchunk;
+    const int z1 = min(z0 + zchunk, nz);
+    const int zs = z0 > 0 ? z0 - 1 : 0;
+    const int xs = seg * kSeg;
+    const int x0 = xs + 4 * lane;
+    const bool row_ok = y >= 0 && y < ny;
+    const bool cell_ok = row_ok && x0 < nx;
+    const bool out_row = w > 0 && row_ok;
+    const bool top = w == NW - 1;
+    const bool up_ok = top && y + 1 < ny;
+    const bool left = xs > 0;
+    const bool hl_ok = left && lane == 0 && row_ok;               // lane 0 rebuilds B^{n+1/2}(xs - 1)
+    const bool hr_ok = xs + kSeg < nx && lane == 63 && row_ok;    // lane 63 reads E^n(xs + 256)
+    const uint32_t plane = static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny) * 4u;  // bytes per z plane
+    const uint32_t yrow = static_cast<uint32_t>(nx) * 4u;
+    // byte offsets within a plane, kOob for an access outside the lattice
+    const uint32_t o_own = cell_ok ? (static_cast<uint32_t>(y) * nx + x0) * 4u : kOob;
+    const uint32_t o_hl = hl_ok ? (static_cast<uint32_t>(y) * nx + xs - 1) * 4u : kOob;
+    const uint32_t o_hu = hl_ok && y + 1 < ny ? o_hl + yrow : kOob;
+    const uint32_t o_hr = hr_ok ? (static_cast<uint32_t>(y) * nx + xs + kSeg) * 4u : kOob;
+    const uint32_t o_up = up_ok && x0 < nx ? (static_cast<uint32_t>(y + 1) * nx + x0) * 4u : kOob;
+    // The per-lane offsets above stay fixed for the whole march; each load's descriptor covers
+    // the one z plane it reads (base + z plane, range one plane), a plane past the chunk's last
+    // needed one or past the lattice has range 0 and reads zeros. Every load is issued
+    // unconditionally and no address is computed per plane in vector registers, so that the
+    // compiler's wait counts stay exact (a skipped load, or a register reused while its load is
+    // in flight, makes it wait for every load in flight).
+    auto plane_of = [&](const float *p, int z, int zmax) {
+        const int zc = z <= zmax ? z : 0;
+        return plane_rsrc(p + static_cast<size_t>(zc) * nx * ny, z <= zmax ? plane : 0u);
+    };
+
+    float *de[3] = {dst.e[0], dst.e[1], dst.e[2]}, *db[3] = {dst.b[0], dst.b[1], dst.b[2]};
+
+    float *xe = lds;
+    float *xb = lds + 2 * NW * 2 * kSeg;
+    float *coef = lds + march_xch_floats<NW>() + static_cast<size_t>(w) * 2 * K * kCoefStride;
+    float *coefB = coef, *coefE = coef + K * kCoefStride;
+    auto xe_at = [&](int p, int wave, int comp) { return xe + ((((p & 1) * NW + wave) * 2 + comp) * kSeg); };
+    auto xb_at = [&](int p, int wave, int comp) { return xb + ((((p & 1) * NW + wave) * 2 + comp) * kSeg); };
+
+    // depth class per cell of the lane (0 outside the lattice: any finite entry will do there)
+    int cx[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) cx[j] = x0 + j < nx ? pml_class(x0 + j, nx, L) : 0;
+    const int cxl = left ? pml_class(xs - 1, nx, L) : 0;
+
+    // E^n of the row at plane z, and lane 0's E^n x, y, z at xs - 1
+    const int ze_max = min(z1, nz - 1), zb_max = z1 - 1;  // last E and B^n planes the chunk reads
+    auto load_e = [&](int z, Row4 &r, float (&hl)[3]) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const Rsrc q = plane_of(src.e[c], z, ze_max);
+            bld4(q, o_own, r.v[c]);
+            hl[c] = bld1(q, o_hl);
+        }
+    };
+    struct BSet {
+        Row4 b;
+        float hb[2];     // lane 0: B^n y, z at xs - 1
+        float hu;        // lane 0: E^n x at (xs - 1, y + 1)
+        float hr[2];     // lane 63: E^n y, z at xs + 256
+        float eu[2][4];  // top wave: E^n z, x of row y + 1
+    };
+    auto load_b = [&](int z, BSet &s) {
+        const Rsrc b0 = plane_of(src.b[0], z, zb_max), b1 = plane_of(src.b[1], z, zb_max),
+                   b2 = plane_of(src.b[2], z, zb_max);
+        const Rsrc e0 = plane_of(src.e[0], z, zb_max), e1 = plane_of(src.e[1], z, zb_max),
+                   e2 = plane_of(src.e[2], z, zb_max);
+        bld4(b0, o_own, s.b.v[0]);
+        bld4(b1, o_own, s.b.v[1]);
+        bld4(b2, o_own, s.b.v[2]);
+        s.hb[0] = bld1(b1, o_hl);
+        s.hb[1] = bld1(b2, o_hl);
+        s.hu = bld1(e0, o_hu);
+        s.hr[0] = bld1(e1, o_hr);
+        s.hr[1] = bld1(e2, o_hr);
+        bld4(e2, o_up, s.eu[0]);  // kOob but for the top wave
+        bld4(e0, o_up, s.eu[1]);
+    };
+    // absorber coefficients of this wave's row at plane z: entry cx for (cx, class(y), class(z));
+    // the wave issues the table loads, waits, and stores them (one entry per lane, more when the
+    // absorber is thicker than 31 cells)
+    auto stage = [&](int z) {
+        const int yy = row_ok ? y : 0;
+        const CoefLoad cb = coef_load<1>(a, lane, yy, z, kPmlBa);
+        const CoefLoad ce = coef_load<1>(a, lane, yy, z, kPmlEa);
+        if (lane < K) {
+            coef_store(a, coefB, lane, cb);
+            coef_store(a, coefE, lane, ce);
+        }
+        for (int i = lane + 64; i < K; i += 64) {
+            coef_store(a, coefB, i, coef_load<1>(a, i, yy, z, kPmlBa));
+            coef_store(a, coefE, i, coef_load<1>(a, i, yy, z, kPmlEa));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+
+    // rings: E^n planes z, z + 1 (ec, en) and two in flight (P, Q: z + 2, z + 3); B^n plane z (bc)
+    // and two in flight (Pb, Qb: z + 1, z + 2). The loop is unrolled by two so that the in-flight
+    // slots trade names instead of being copied: a slot is read (copied into en / bc) one plane
+    // after its loads were issued, never in the iteration that issued them.
+    Row4 ec, en, eP, eQ;
+    float hlc[3], hln[3], hlP[3], hlQ[3];
+    BSet bc, bP, bQ;
+    float bm[2][4];  // B^{n+1/2}(z - 1) x, y of the row
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bm[0][j] = bm[1][j] = 0.0f;
+
+    load_e(zs, ec, hlc);
+    load_e(zs + 1, en, hln);
+    load_e(zs + 2, eP, hlP);
+    load_b(zs, bc);
+    load_b(zs + 1, bP);
+    // publish E^n(zs) z, x of the row for the wave below
+    *reinterpret_cast<float4 *>(xe_at(zs, w, 0) + 4 * lane) = make_float4(ec.v[2][0], ec.v[2][1], ec.v[2][2], ec.v[2][3]);
+    *reinterpret_cast<float4 *>(xe_at(zs, w, 1) + 4 * lane) = make_float4(ec.v[0][0], ec.v[0][1], ec.v[0][2], ec.v[0][3]);
+    __syncthreads();
+
+    int cz_prev = -1;
+    auto plane_step = [&](int z, Row4 &P, Row4 &Q, float (&hP)[3], float (&hQ)[3], BSet &Pb, BSet &Qb) {
+        const int czz = pml_class(z, nz, L);
+        if (czz != cz_prev) {  // uniform over the workgroup
+            stage(z);
+            cz_prev = czz;
+        }
+        // prefetch E^n(z + 3) and B^n(z + 2): two planes ahead of their use
+        load_e(z + 3, Q, hQ);
+        load_b(z + 2, Qb);
+
+        // ---- B^{n+1/2}(z) ----
+        // E^n(y + 1, z) z, x: the wave above's row from LDS, the top wave's own loads (a select,
+        // not a branch: the loaded registers stay in use every plane, so they are never reused
+        // while their loads are in flight)
+        float eyu[4], exu[4];
+        {
+            const int wu = top ? w : w + 1;
+            const float4 u0 = *reinterpret_cast<const float4 *>(xe_at(z, wu, 0) + 4 * lane);
+            const float4 u1 = *reinterpret_cast<const float4 *>(xe_at(z, wu, 1) + 4 * lane);
+            const float l0[4] = {u0.x, u0.y, u0.z, u0.w}, l1[4] = {u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                eyu[j] = top ? bc.eu[0][j] : l0[j];
+                exu[j] = top ? bc.eu[1][j] : l1[j];
+            }
+        }
+        // E^n(x + 1) y, z for the lane's last cell: the next lane's first, or past the segment
+        float eyx = lane_next(ec.v[1][0]), ezx = lane_next(ec.v[2][0]);
+        if (lane == 63) { eyx = bc.hr[0]; ezx = bc.hr[1]; }
+        float bn[3][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float e0x = ec.v[0][j], e0y = ec.v[1][j], e0z = ec.v[2][j];
+            const float exn_y = j < 3 ? ec.v[1][j + 1] : eyx, exn_z = j < 3 ? ec.v[2][j + 1] : ezx;
+            // shaders/update_B.comp:41-47 term order
+            const float t0x = eyu[j] - e0z, t0y = en.v[0][j] - e0x, t0z = exn_y - e0y;
+            const float t1x = en.v[1][j] - e0y, t1y = exn_z - e0z, t1z = exu[j] - e0x;
+            const float cxv = t0x - t1x, cyv = t0y - t1y, czv = t0z - t1z;
+            float ba[3], bb[3];
+            read_coef(coefB + cx[j] * kCoefStride, ba, bb);
+            bn[0][j] = ba[0] * bc.b.v[0][j] - bb[0] * cxv;
+            bn[1][j] = ba[1] * bc.b.v[1][j] - bb[1] * cyv;
+            bn[2][j] = ba[2] * bc.b.v[2][j] - bb[2] * czv;
+        }
+        if (!row_ok) {
+            // a row outside the lattice (the group below row 0): B there is the boundary's 0, which
+            // the first row's E update reads (shaders/update_E.comp:27-29)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bn[0][j] = bn[1][j] = bn[2][j] = 0.0f;
+        }
+        // lane 0 of a segment past x = 0: B^{n+1/2}(xs - 1) y, z from its own loads
+        float hby = 0.0f, hbz = 0.0f;
+        if (left && lane == 0) {
+            float ba[3], bb[3];
+            read_coef(coefB + cxl * kCoefStride, ba, bb);
+            const float t0y = hln[0] - hlc[0], t1y = ec.v[2][0] - hlc[2];
+            const float t0z = ec.v[1][0] - hlc[1], t1z = bc.hu - hlc[0];
+            hby = ba[1] * bc.hb[0] - bb[1] * (t0y - t1y);
+            hbz = ba[2] * bc.hb[1] - bb[2] * (t0z - t1z);
+        }
+        // publish B^{n+1/2}(z) x, z and E^n(z + 1) z, x
+        *reinterpret_cast<float4 *>(xb_at(z, w, 0) + 4 * lane) = make_float4(bn[0][0], bn[0][1], bn[0][2], bn[0][3]);
+        *reinterpret_cast<float4 *>(xb_at(z, w, 1) + 4 * lane) = make_float4(bn[2][0], bn[2][1], bn[2][2], bn[2][3]);
+        *reinterpret_cast<float4 *>(xe_at(z + 1, w, 0) + 4 * lane) =
+            make_float4(en.v[2][0], en.v[2][1], en.v[2][2], en.v[2][3]);
+        *reinterpret_cast<float4 *>(xe_at(z + 1, w, 1) + 4 * lane) =
+            make_float4(en.v[0][0], en.v[0][1], en.v[0][2], en.v[0][3]);
+        // B^{n+1/2}(x - 1) y, z for the lane's first cell (all lanes take part in the shift)
+        float byx = lane_prev(bn[1][3]), bzx = lane_prev(bn[2][3]);
+        if (lane == 0) { byx = hby; bzx = hbz; }
+        __syncthreads();
+
+        // ---- E^{n+1}(z) ----
+        if (out_row && z >= z0) {
+            const float4 d0 = *reinterpret_cast<const float4 *>(xb_at(z, w - 1, 0) + 4 * lane);
+            const float4 d1 = *reinterpret_cast<const float4 *>(xb_at(z, w - 1, 1) + 4 * lane);
+            const float bxd[4] = {d0.x, d0.y, d0.z, d0.w}, bzd[4] = {d1.x, d1.y, d1.z, d1.w};
+            float e[3][4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float bxm_y = j > 0 ? bn[1][j - 1] : byx, bxm_z = j > 0 ? bn[2][j - 1] : bzx;
+                // shaders/update_E.comp:35-40 curl order
+                const float curl0 = (bn[2][j] - bzd[j]) - (bn[1][j] - bm[1][j]);
+                const float curl1 = (bn[0][j] - bm[0][j]) - (bn[2][j] - bxm_z);
+                const float curl2 = (bn[1][j] - bxm_y) - (bn[0][j] - bxd[j]);
+                float ca[3], cb[3];
+                read_coef(coefE + cx[j] * kCoefStride, ca, cb);
+                e[0][j] = ca[0] * ec.v[0][j] + cb[0] * curl0;
+                e[1][j] = ca[1] * ec.v[1][j] + cb[1] * curl1;
+                e[2][j] = ca[2] * ec.v[2][j] + cb[2] * curl2;
+            }
+            const uint32_t o = o_own + static_cast<uint32_t>(z) * plane;
+            if (cell_ok) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    gst4(db[c], o, bn[c]);
+                    gst4(de[c], o, e[c]);
+                }
+            }
+            if (a.side.buf && cell_ok) {
+                side_write4(a, 3, x0, y, z, bn);
+                side_write4(a, 0, x0, y, z, e);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) { bm[0][j] = bn[0][j]; bm[1][j] = bn[1][j]; }
+        ec = en;
+        en = P;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { hlc[c] = hln[c]; hln[c] = hP[c]; }
+        bc = Pb;
+    };
+    for (int z = zs; z < z1; z += 2) {
+        plane_step(z, eP, eQ, hlP, hlQ, bP, bQ);
+        if (z + 1 < z1) plane_step(z + 1, eQ, eP, hlQ, hlP, bQ, bP);
+    }
+}
+
+// B^{n+1/2} of the listed texels (thin-wire classes), from src as update_B.comp computes it
+// (faraday(): wire factors and absorber from the class and absorber tables).
+__global__ void __launch_bounds__(256) k_march_fix_b(LatticeArgs a, FieldPtrs src, FieldPtrs dst,
+                                                     const uint32_t *list, int n) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i >= n) return;
+    const size_t p = list[i];
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int x = static_cast<int>(p % nx), y = static_cast<int>((p / nx) % ny), z = static_cast<int>(p / (static_cast<size_t>(nx) * ny));
+    const size_t sy = nx, sz = static_cast<size_t>(nx) * ny;
+    float e0[3], exn[3], eyn[3], ezn[3], b[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float *e = src.e[c];
+        e0[c] = e[p];
+        exn[c] = x + 1 < nx ? e[p + 1] : 0.0f;
+        eyn[c] = y + 1 < ny ? e[p + sy] : 0.0f;
+        ezn[c] = z + 1 < nz ? e[p + sz] : 0.0f;
+        b[c] = src.b[c][p];
+    }
+    faraday(a, x, y, z, e0, exn, eyn, ezn, load_id(a, p), b);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dst.b[c][p] = b[c];
+    if (a.side.buf) {
+        for (int face = 0; face < 2; ++face) {
+            const long k = side_index(a.side, face, 3, x, y, z);
+            if (k < 0) continue;
+            const long plane = 2L * a.side.nz * a.side.ny;
+            for (int c = 0; c < 3; ++c) a.side.buf[k + c * plane] = b[c];
+        }
+    }
+}
+
+// E^{n+1} of the listed texels (classes other than vacuum, and the E texels whose curl reads a
+// thin-wire texel's B), from src E and dst B as update_E.comp computes it (edge_coefficients()).
+__global__ void __launch_bounds__(256) k_march_fix_e(LatticeArgs a, FieldPtrs src, FieldPtrs dst,
+                                                     const uint32_t *list, int n) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i >= n) return;
+    const size_t p = list[i];
+    const int nx = a.nx, ny = a.ny;
+    const int x = static_cast<int>(p % nx), y = static_cast<int>((p / nx) % ny), z = static_cast<int>(p / (static_cast<size_t>(nx) * ny));
+    const size_t sy = nx, sz = static_cast<size_t>(nx) * ny;
+    float b0[3], bX[3], bY[3], bZ[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float *bp = dst.b[c];
+        b0[c] = bp[p];
+        bX[c] = x > 0 ? bp[p - 1] : 0.0f;
+        bY[c] = y > 0 ? bp[p - sy] : 0.0f;
+        bZ[c] = z > 0 ? bp[p - sz] : 0.0f;
+    }
+    const float curl[3] = {(b0[2] - bY[2]) - (b0[1] - bZ[1]), (b0[0] - bZ[0]) - (b0[2] - bX[2]),
+                           (b0[1] - bX[1]) - (b0[0] - bY[0])};
+    float ca[3], cb[3];
+    edge_coefficients<true>(a, x, y, z, load_id(a, p), ca, cb);
+    float e[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        e[c] = ca[c] * src.e[c][p] + cb[c] * curl[c];
+        dst.e[c][p] = e[c];
+    }
+    if (a.side.buf) {
+        for (int face = 0; face < 2; ++face) {
+            const long k = side_index(a.side, face, 0, x, y, z);
+            if (k < 0) continue;
+            const long plane = 2L * a.side.nz * a.side.ny;
+            for (int c = 0; c < 3; ++c) a.side.buf[k + c * plane] = e[c];
+        }
+    }
+}
+
+template <int NW>
+hipError_t launch_nw(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int zchunk, hipStream_t s) {
+    const int nseg = (a.nx + kSeg - 1) / kSeg;
+    const int nyt = (a.ny + (NW - 2)) / (NW - 1);
+    if (zchunk <= 0) {
+        // thin z-chunks: the workgroups in flight then cover a thin slab, which streams best (at
+        // 768^3: chunks of 8 planes 3.83 ms per step, 12 planes 3.87, 16 planes 3.91, 48 planes
+        // 4.00, 96 planes 4.21); each chunk recomputes one leading plane of B
+        zchunk = 8;
+    }
+    if (zchunk > a.nz) zchunk = a.nz;
+    const int nzc = (a.nz + zchunk - 1) / zchunk;
+    const int n_tiles = nseg * nyt * nzc;
+    const size_t lds = (march_xch_floats<NW>() + static_cast<size_t>(NW) * 2 * a.pml_k * kCoefStride) * sizeof(float);
+    if (lds > 65536) {
+        static size_t raised = 0;  // dynamic LDS past 64 KB must be allowed per kernel
+        if (lds > raised) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_yee_march<NW>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+            if (e != hipSuccess) return e;
+            raised = lds;
+        }
+    }
+    k_yee_march<NW><<<n_tiles, 64 * NW, lds, s>>>(a, src, dst, zchunk, nseg, nyt, n_tiles);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int yee_march_supported(const LatticeArgs &a) {
+    const size_t bytes = static_cast<size_t>(a.nx) * a.ny * a.nz * 4;
+    return a.nx % 4 == 0 && bytes <= kOob ? 1 : 0;
+}
+
+hipError_t launch_yee_march(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int cfg,
+                            const uint32_t *fix_b, int n_fix_b, const uint32_t *fix_e, int n_fix_e,
+                            hipStream_t s) {
+    if (!yee_march_supported(a)) return hipErrorInvalidValue;
+    const int zchunk = cfg;  // 0: automatic
+    hipError_t e = launch_nw<8>(a, src, dst, zchunk, s);
+    if (e != hipSuccess) return e;
+    if (n_fix_b > 0) {
+        k_march_fix_b<<<(n_fix_b + 255) / 256, 256, 0, s>>>(a, src, dst, fix_b, n_fix_b);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (n_fix_e > 0) {
+        k_march_fix_e<<<(n_fix_e + 255) / 256, 256, 0, s>>>(a, src, dst, fix_e, n_fix_e);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace emfd

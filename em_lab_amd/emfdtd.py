@@ -29,6 +29,7 @@ KERNEL_POINT = 4
 KERNEL_TWO_PASS4 = 5
 KERNEL_SLABS = 6
 KERNEL_ZMARCH = 7
+KERNEL_MARCH = 8
 STORAGE_F32 = 0
 STORAGE_F16 = 1
 STORAGE_BF16 = 2
@@ -135,6 +136,7 @@ _SIGNATURES = {
     "emfd_grid_set_storage": (C.c_int, [_P, C.c_int, _P]),
     "emfd_grid_download_fields_f64": (C.c_int, [_P, _P, _P]),
     "emfd_grid_kernel_timing": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "emfd_grid_kernel_timing_samples": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_int)]),
     "emfd_lines_configure": (C.c_int, [_P, _P, C.c_int, _P, _P, C.c_int, C.c_float, C.c_float]),
     "emfd_lines_read": (C.c_int, [_P, _P, _P, C.c_int]),
     "emfd_coax_geometry_factor": (C.c_double, [C.c_int, C.c_int]),
@@ -377,6 +379,8 @@ class SimulationGrid:
         self._lib = lib
         self._config: Optional[GridConfig] = None
         self._sources: list = []
+        self._variant = KERNEL_AUTO
+        self._storage = STORAGE_F32
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -427,12 +431,14 @@ class SimulationGrid:
 
     def set_kernel(self, variant: int) -> None:
         _check(self._lib.emfd_grid_set_kernel(self._h, int(variant)))
+        self._variant = int(variant)
 
     def set_storage(self, storage: int, scale_log2=None) -> None:
         """Field storage STORAGE_F32 / STORAGE_F16 / STORAGE_BF16 with a power-of-two scale per
         component (E x y z, B x y z). Clears the fields and rewinds time."""
         k = None if scale_log2 is None else (C.c_int * 6)(*[int(v) for v in scale_log2])
         _check(self._lib.emfd_grid_set_storage(self._h, int(storage), None if k is None else C.cast(k, C.c_void_p)))
+        self._storage = int(storage)
 
     def fields_f64(self):
         """(E, B) as (3, n) float64 planes (exact for STORAGE_F64)."""
@@ -456,6 +462,31 @@ class SimulationGrid:
         n = C.c_int64(0)
         _check(self._lib.emfd_grid_kernel_timing(self._h, enable, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def kernel_timing_samples(self):
+        """The bracketed Yee launches one by one (milliseconds, step order)."""
+        n = C.c_int(0)
+        _check(self._lib.emfd_grid_kernel_timing_samples(self._h, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), np.float32)
+        _check(self._lib.emfd_grid_kernel_timing_samples(self._h, _ptr(out), out.size, C.byref(n)))
+        return [float(v) for v in out[: n.value]]
+
+    def yee_kernel_name(self) -> str:
+        """The Yee kernel(s) a step launches with the current variant and storage."""
+        v = self._variant & 0xFF
+        if self._storage == STORAGE_F64:
+            return "k_update_b_f64+k_update_e_f64"
+        if self._storage in (STORAGE_F16, STORAGE_BF16):
+            return "k_update_bh+k_update_eh"
+        nx, ny, nz = self.size
+        if v == KERNEL_MARCH or (v == KERNEL_AUTO and nx * ny * nz >= 8_000_000 and nx % 4 == 0
+                                 and nx * ny * nz * 4 <= 0x80000000):
+            return "k_yee_march (+k_march_fix_b, k_march_fix_e)"
+        if v in (KERNEL_AUTO, KERNEL_TWO_PASS4) and self.size[0] % 4 == 0:
+            return "k_update_b4+k_update_e4"
+        return {KERNEL_FUSED: "k_yee_fused", KERNEL_TWO_PASS: "k_update_b+k_update_e", KERNEL_ROWS: "k_yee_rows",
+                KERNEL_POINT: "k_yee_point", KERNEL_SLABS: "k_update_b4+k_update_e4 (z-slabs)",
+                KERNEL_ZMARCH: "k_yee_zmarch"}.get(v, "k_update_b+k_update_e")
 
     @property
     def config(self) -> GridConfig:

@@ -41,7 +41,8 @@ extern "C" {
 #define EMFD_LAYOUT_TEXEL 0 /* float[n][4]: (x, y, z, alpha) per cell, EM_Lab's RGBA32F texel */
 #define EMFD_LAYOUT_SOA 1   /* float[3][n]: component planes */
 
-#define EMFD_KERNEL_AUTO 0     /* fastest variant that supports the grid (default) */
+#define EMFD_KERNEL_AUTO 0     /* fastest variant that supports the grid (default): EMFD_KERNEL_MARCH for FP32
+                                   lattices of at least 8e6 cells, else the vectorised two-pass kernel */
 #define EMFD_KERNEL_FUSED 1    /* fused B+E, LDS-tiled z-march (ping-pong buffers) */
 #define EMFD_KERNEL_TWO_PASS 2 /* separate update_B / update_E passes, in place */
 #define EMFD_KERNEL_ROWS 3     /* fused B+E, one wavefront per x-row z-march (nx <= 256) */
@@ -54,6 +55,11 @@ extern "C" {
                                    of the variant = slab thickness in planes (default 32) */
 #define EMFD_KERNEL_ZMARCH 7    /* fused B+E, z-march of 4-row groups with LDS row exchange and a helper
                                    wave (nx <= 256, nx % 4 == 0); bits 8.. = z-chunk (0 = automatic) */
+#define EMFD_KERNEL_MARCH 8     /* fused B+E in one pass (ping-pong buffers, nx % 4 == 0, planes of at most
+                                   2 GB): z-march of 256-cell x segments of 7 rows with one workgroup barrier
+                                   per plane, vacuum and absorber coefficients in the march, then sparse
+                                   passes recompute the texels of non-vacuum classes and next to thin wires;
+                                   bits 8.. of the variant = z-chunk (0 = automatic) */
 
 #define EMFD_STORAGE_F32 0  /* FP32 field planes (default; EM_Lab's RGBA32F) */
 #define EMFD_STORAGE_F16 1  /* binary16 planes of value / 2^k per component, FP32 updates */
@@ -249,6 +255,11 @@ int emfd_grid_material_classes(emfd_ctx *ctx, int *classes);
  * the sum and count of the bracketed launches since timing was last switched on. Each bracket
  * costs the stream a few microseconds, so long timed runs sample with k > 1. */
 int emfd_grid_kernel_timing(emfd_ctx *ctx, int enable, double *total_ms, int64_t *launches);
+/* The bracketed launches of emfd_grid_kernel_timing one by one (milliseconds, in step order,
+ * since timing was last switched on; at most 2^20 kept): the first min(cap, *n) are written to
+ * out, *n receives how many there are. No counterpart in the reference (its timing line is an
+ * exponential mean of GPU timer queries, src/Main.cpp:1151); used for medians and spreads. */
+int emfd_grid_kernel_timing_samples(emfd_ctx *ctx, float *out, int cap, int *n);
 
 /* CoaxMonitor's device side (src/Coax.cpp:95-198, shaders/line_accumulate.comp:30-48): n
  * quantities of 4 ints (first term, term count, kind 0 = E line integral / 1 = B loop integral,
