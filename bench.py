@@ -225,8 +225,12 @@ def main() -> int:
         g = E.SimulationGrid(0)
         nbytes = args.copy_probe * 1000 * 1000
         ms = g.copy_probe(nbytes, max(args.steps, 1))
-        print(json.dumps({"copy_probe_bytes": nbytes, "avg_ms": round(ms, 5),
-                          "read_plus_write_GBs": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}), flush=True)
+        res = {"copy_probe_bytes": nbytes, "avg_ms": round(ms, 5),
+               "read_plus_write_GBs": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+        for planes in (1, 2, 6):
+            sms, moved = g.stream_probe(nbytes, planes, max(args.steps, 1))
+            res[f"stream_{planes}in_{planes}out_GBs"] = round(moved / (sms * 1e-3) / 1e9, 1)
+        print(json.dumps(res), flush=True)
         g.close()
         return 0
 
@@ -257,6 +261,18 @@ def main() -> int:
     }
     if "ntff_pattern" in head:
         out["ntff_pattern"] = head["ntff_pattern"]
+    # the GPU's own streaming ceiling for the Yee pass's access pattern (6 planes read, 6 written
+    # at one index, no stencil), against which the achieved figure can also be read
+    g = E.SimulationGrid(0)
+    try:
+        plane = args.size ** 3 * 4
+        ms, moved = g.stream_probe(plane, 6, 5)
+        stream = moved / (ms * 1e-3) / 1e9
+        out["roofline"]["stream_probe"] = {"GBs": round(stream, 1), "frac_of_peak": round(stream / HBM_PEAK_GBS, 4),
+                                           "pattern": "6 planes read + 6 written per index, float4, no stencil",
+                                           "yee_achieved_over_stream": round(head["roofline"]["achieved"] / stream, 4)}
+    finally:
+        g.close()
     if not args.no_extras:
         extras = {}
         n_ex = max(8, min(args.steps, 40))

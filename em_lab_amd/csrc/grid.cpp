@@ -1032,6 +1032,32 @@ int emfd_copy_probe(emfd_ctx *ctx, int64_t bytes, int iterations, double *avg_ms
     return EMFD_OK;
 }
 
+int emfd_stream_probe(emfd_ctx *ctx, int64_t plane_bytes, int planes, int iterations, double *avg_ms) {
+    if (!ctx || plane_bytes < (1 << 23) || iterations < 1 || (planes != 1 && planes != 2 && planes != 3 && planes != 6))
+        return fail(EMFD_ERR_INVALID, "emfd_stream_probe: bad arguments");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    const size_t unit = static_cast<size_t>(256) * 2 * 4096;  // float4 per block-round of the probe kernel
+    const size_t plane_f4 = static_cast<size_t>(plane_bytes) / 16 / unit * unit;
+    DevBuf a, b;
+    if (int rc = a.ensure(plane_f4 * 16 * planes)) return rc;
+    if (int rc = b.ensure(plane_f4 * 16 * planes)) return rc;
+    EMFD_HIP(launch_fill(a.as<float>(), plane_f4 * 4 * planes, 1.0f, ctx->stream));
+    EMFD_HIP(launch_stream_probe(a.ptr, b.ptr, plane_f4, planes, ctx->stream));  // warm-up
+    hipEvent_t e0, e1;
+    EMFD_HIP(hipEventCreate(&e0));
+    EMFD_HIP(hipEventCreate(&e1));
+    EMFD_HIP(hipEventRecord(e0, ctx->stream));
+    for (int i = 0; i < iterations; ++i) EMFD_HIP(launch_stream_probe(a.ptr, b.ptr, plane_f4, planes, ctx->stream));
+    EMFD_HIP(hipEventRecord(e1, ctx->stream));
+    EMFD_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    EMFD_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (avg_ms) *avg_ms = ms / iterations;
+    return EMFD_OK;
+}
+
 int64_t emfd_grid_memory_bytes(emfd_ctx *ctx) {
     if (!ctx) return -1;
     int64_t total = 0;

@@ -154,6 +154,8 @@ hipError_t launch_apply_source_records(const LatticeArgs &a, const FieldPtrs &f,
                                        int n_cells, const float *packed, const float *signals, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
 hipError_t launch_copy4(const void *src, void *dst, size_t bytes, hipStream_t s);
+// R = W = planes (1, 2, 3 or 6) planes of plane_f4 float4 each, read and written at one index
+hipError_t launch_stream_probe(const void *src, void *dst, size_t plane_f4, int planes, hipStream_t s);
 hipError_t launch_build_class_masks(const LatticeArgs &a, uint32_t *wire_mask, uint32_t *class_mask, int wpr,
                                    hipStream_t s);
 hipError_t launch_build_pml_table(const LatticeArgs &a, float *tab, hipStream_t s);

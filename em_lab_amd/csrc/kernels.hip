@@ -95,11 +95,38 @@ __global__ void k_build_pml_table(LatticeArgs a, float *tab) {
     }
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
 // Plain float4 copy: the bandwidth and counter calibration reference for the roofline figures.
 __global__ void __launch_bounds__(256) k_copy4(const float4 *__restrict__ a, float4 *__restrict__ b, size_t n) {
     size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     for (; i < n; i += stride) b[i] = a[i];
+}
+
+// Stream probe: R planes read and W planes written at the same index, two float4 of each plane
+// per thread (every load issued before any store), non-temporal stores; the access pattern of the
+// fused Yee pass (R = W = 6) with the stencil taken out. n: float4 per plane, a multiple of
+// 256 * 2 * gridDim.x.
+template <int R, int W>
+__global__ void __launch_bounds__(256) k_stream(const f32x4 *__restrict__ a, f32x4 *__restrict__ b, size_t n,
+                                                size_t plane) {
+    constexpr int U = 2;
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256 * U;
+    for (size_t base = static_cast<size_t>(blockIdx.x) * 256 * U + threadIdx.x; base < n; base += stride) {
+        f32x4 acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc[u] = a[base + u * 256];
+#pragma unroll
+        for (int r = 1; r < R; ++r)
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc[u] += a[r * plane + base + u * 256];
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                __builtin_nontemporal_store(acc[u] * static_cast<float>(w + 1), &b[w * plane + base + u * 256]);
+    }
 }
 
 __global__ void k_fill(float *p, size_t n, float v) {
@@ -473,6 +500,20 @@ hipError_t launch_build_pml_table(const LatticeArgs &a, float *tab, hipStream_t 
 hipError_t launch_copy4(const void *src, void *dst, size_t bytes, hipStream_t s) {
     const size_t n = bytes / 16;
     k_copy4<<<16384, 256, 0, s>>>(static_cast<const float4 *>(src), static_cast<float4 *>(dst), n);
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_probe(const void *src, void *dst, size_t plane_f4, int planes, hipStream_t s) {
+    // plane_f4 is a multiple of 256 * 2 * 4096 (the caller rounds it)
+    const unsigned blocks = 4096;
+    const f32x4 *a = static_cast<const f32x4 *>(src);
+    f32x4 *b = static_cast<f32x4 *>(dst);
+    switch (planes) {
+    case 1: k_stream<1, 1><<<blocks, 256, 0, s>>>(a, b, plane_f4, plane_f4); break;
+    case 2: k_stream<2, 2><<<blocks, 256, 0, s>>>(a, b, plane_f4, plane_f4); break;
+    case 3: k_stream<3, 3><<<blocks, 256, 0, s>>>(a, b, plane_f4, plane_f4); break;
+    default: k_stream<6, 6><<<blocks, 256, 0, s>>>(a, b, plane_f4, plane_f4); break;
+    }
     return hipGetLastError();
 }
 

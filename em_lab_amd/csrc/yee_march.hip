@@ -41,6 +41,37 @@ struct Row4 {
     float v[3][4];
 };
 
+// Far-field side columns (SideCols): bit f * 4 + j is set when cell x0 + j of a lane is one of the
+// two columns of face f (x = X_f - 1 or X_f). Fixed for the march, so a plane's copy is a test of
+// these bits and a few stores on the one or two lanes that hold the columns.
+__device__ __forceinline__ uint32_t side_bits4(const SideCols &s, int x0, int nx) {
+    uint32_t bits = 0u;
+    if (!s.buf) return bits;
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int xo = x0 + j - (s.x[f] - 1);
+            if (xo >= 0 && xo <= 1 && x0 + j < nx) bits |= 1u << (f * 4 + j);
+        }
+    return bits;
+}
+__device__ __forceinline__ void side_put4(const SideCols &s, uint32_t bits, int comp0, int x0, int y, int z,
+                                          const float (&v)[3][4]) {
+    const long col = static_cast<long>(s.nz) * s.ny;
+    const long yz = static_cast<long>(z - s.z0) * s.ny + (y - s.y0);
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (bits & (1u << (f * 4 + j))) {
+                const int xo = x0 + j - (s.x[f] - 1);
+                float *d = s.buf + (static_cast<long>(f * 6 + comp0) * 2 + xo) * col + yz;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) d[c * 2 * col] = v[c][j];
+            }
+}
+
 // Plane reads go through buffer resources: a byte offset per lane against the plane's 32-bit
 // range, where an offset at or past the range (kOob) reads 0 without touching memory. That gives
 // the lattice's zero boundaries and idle lanes without a select the compiler could turn into a
@@ -131,6 +162,10 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
     auto xe_at = [&](int p, int wave, int comp) { return xe + ((((p & 1) * NW + wave) * 2 + comp) * kSeg); };
     auto xb_at = [&](int p, int wave, int comp) { return xb + ((((p & 1) * NW + wave) * 2 + comp) * kSeg); };
 
+    // far-field side columns this wave's output row keeps (lanes with column cells, rows in range)
+    const uint32_t sbits = out_row ? side_bits4(a.side, x0, nx) : 0u;
+    const bool side_row = out_row && a.side.buf && y >= a.side.y0 && y < a.side.y0 + a.side.ny &&
+                          __builtin_amdgcn_ballot_w64(sbits != 0u) != 0;
     // depth class per cell of the lane (0 outside the lattice: any finite entry will do there)
     int cx[4];
 #pragma unroll
@@ -311,9 +346,9 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
                     gst4(de[c], o, e[c]);
                 }
             }
-            if (a.side.buf && cell_ok) {
-                side_write4(a, 3, x0, y, z, bn);
-                side_write4(a, 0, x0, y, z, e);
+            if (side_row && z >= a.side.z0 && z < a.side.z0 + a.side.nz && sbits != 0u) {
+                side_put4(a.side, sbits, 3, x0, y, z, bn);
+                side_put4(a.side, sbits, 0, x0, y, z, e);
             }
         }
 #pragma unroll

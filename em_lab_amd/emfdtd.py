@@ -162,6 +162,7 @@ _SIGNATURES = {
     "emfd_flux_measure": (C.c_int, [_P, _P, C.c_int, _P]),
     "emfd_probe_measure": (C.c_int, [_P, _P, C.c_int, _P]),
     "emfd_copy_probe": (C.c_int, [_P, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
+    "emfd_stream_probe": (C.c_int, [_P, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "emfd_batch_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "emfd_batch_destroy": (C.c_int, [_P]),
     "emfd_batch_add_scene": (C.c_int, [_P, C.POINTER(BatchSceneC), C.POINTER(C.c_int), C.POINTER(GridConfigC)]),
@@ -522,6 +523,14 @@ class SimulationGrid:
         ms = C.c_double()
         _check(self._lib.emfd_copy_probe(self._h, int(nbytes), int(iterations), C.byref(ms)))
         return ms.value
+
+    def stream_probe(self, plane_bytes: int, planes: int = 6, iterations: int = 5):
+        """(mean ms, bytes moved) of the stream probe: planes in, planes out at one index."""
+        ms = C.c_double()
+        _check(self._lib.emfd_stream_probe(self._h, int(plane_bytes), int(planes), int(iterations), C.byref(ms)))
+        unit = 256 * 2 * 4096 * 16
+        moved = 2 * planes * (int(plane_bytes) // unit * unit)
+        return ms.value, moved
 
     def gpu_memory_bytes(self) -> int:
         return int(self._lib.emfd_grid_memory_bytes(self._h))

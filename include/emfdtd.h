@@ -469,6 +469,11 @@ int emfd_mlp_time_forward(emfd_mlp *m, int64_t batch, int iterations, double *av
 /* Copies `bytes` (rounded down to 16) between two scratch buffers `iterations` times on the
  * context stream with a plain float4 kernel; *avg_ms = mean time per copy from HIP events. */
 int emfd_copy_probe(emfd_ctx *ctx, int64_t bytes, int iterations, double *avg_ms);
+/* Stream probe (no counterpart in the reference): `planes` planes of about plane_bytes each read
+ * and as many written at the same index (planes = 1, 2, 3 or 6; 6 is the fused Yee pass's access
+ * pattern without its stencil), mean milliseconds per pass over `iterations`. Moves
+ * 2 * planes * plane_bytes (rounded down to 32 MiB per plane) per pass. */
+int emfd_stream_probe(emfd_ctx *ctx, int64_t plane_bytes, int planes, int iterations, double *avg_ms);
 
 #ifdef __cplusplus
 }
