@@ -73,6 +73,11 @@ struct NtffState {
     float wavelength(int f) const { return wavelengths.empty() ? setup.wavelength : wavelengths[static_cast<size_t>(f)]; }
     SideCols side{};   // geometry of d_side (buf set when the Yee kernel maintains it)
     DevBuf d_dirs, d_partials, d_out, d_power;
+    // radiation integrals: operand geometry of the current box and its FP16 operand buffers
+    FfGeom ff{};
+    size_t ff_halves = 0;
+    bool ff_zeroed = false;
+    DevBuf d_ahi, d_alo, d_absmax;
 };
 
 }  // namespace emfd

@@ -371,7 +371,8 @@ void emfd_destroy(emfd_ctx *ctx) {
                       &ctx->ports.d_series, &ctx->ntff.d_phasors, &ctx->ntff.d_side, &ctx->ntff.d_dirs, &ctx->ntff.d_partials,
                       &ctx->ntff.d_out, &ctx->ntff.d_power, &ctx->scratch_partials, &ctx->scratch_misc,
                       &ctx->lines.d_quant, &ctx->lines.d_terms, &ctx->lines.d_weights, &ctx->lines.d_phasors,
-                      &ctx->lines.d_now};
+                      &ctx->lines.d_now, &ctx->ntff.d_ahi, &ctx->ntff.d_alo, &ctx->ntff.d_absmax, &ctx->fix_b,
+                      &ctx->fix_e};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t ev : ctx->ev_pool) (void)hipEventDestroy(ev);
     for (int i = 0; i < 2; ++i) {

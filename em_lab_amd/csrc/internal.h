@@ -206,10 +206,30 @@ hipError_t launch_ntff_sample(const LatticeArgs &a, const FieldPtrs &f, const Nt
                               const PortJob &pj, hipStream_t s);
 hipError_t launch_ntff_flush(const NtffBox &box, const float *rot, int rot_stride, int steps, const NtffFreqs &freqs,
                              const float *samples, float *phasors, hipStream_t s);
-hipError_t launch_ntff_pattern(const NtffBox &box, const float *phasors, const float *dirs, int n_dirs, float k,
-                               const float center[3], float *partials, int slices, float *out_f2,
-                               float *power_partials, hipStream_t s);
-int ntff_pattern_slices(int n_points);
+// Far-field radiation integrals as matrix products (farfield.hip). Per box face: normal axis ax
+// (side 0: lower face, 1: upper), in-plane axes u (summed in the epilogue) and v (the product's
+// contraction axis), U x V points from phasor index `first` (u fastest), 16-u blocks, contraction
+// length kp (2 V rounded up to 16), coordinates relative to the box centre of the face plane and of
+// the first u and v, and the face's offset (halves) in the FP16 operand buffers.
+struct FfFace {
+    int ax, side, u, v;
+    int U, V, first;
+    int ublocks, kp;
+    float a0, u0, v0;
+    size_t a_off;
+};
+struct FfGeom {
+    FfFace f[6];
+};
+FfGeom ff_geometry(const NtffBox &box, size_t *a_halves);
+int ff_absmax_blocks(int n_points);
+// |F|^2 per direction into out_f2 and, when power_partials is not null, (n_points + 255) / 256
+// FP64 partial sums of w Re(E x H*).n (the radiated power is half their sum). Scratch: absmax
+// (ff_absmax_blocks floats), a_hi / a_lo (the geometry's halves each, zeroed once), face_partials
+// (6 x 12 x n_dirs floats).
+hipError_t launch_far_field(const NtffBox &box, const FfGeom &g, const float *phasors, const float *dirs, int n_dirs,
+                            float k, float *absmax, _Float16 *a_hi, _Float16 *a_lo, float *face_partials,
+                            float *out_f2, double *power_partials, hipStream_t s);
 
 struct FluxPlane {
     int axis, index;

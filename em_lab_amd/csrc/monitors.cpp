@@ -580,6 +580,8 @@ int emfd_ntff_configure(emfd_ctx *ctx, const emfd_far_field_setup *setup, emfd_f
     if (int rc = N.d_side.ensure(static_cast<size_t>(2 * 6 * 2) * N.side.ny * N.side.nz * sizeof(float))) return rc;
     EMFD_HIP(hipMemset(N.d_phasors.ptr, 0, static_cast<size_t>(b.n_points) * 24 * sizeof(float)));
     N.box = b;
+    N.ff = ff_geometry(b, &N.ff_halves);
+    N.ff_zeroed = false;
     N.setup = s;
     N.enabled = true;
     N.start_time = ctx->time();
@@ -675,32 +677,36 @@ int emfd_ntff_eval_at(emfd_ctx *ctx, int freq, const float *dirs, int n_dirs, fl
     if (freq < 0 || freq >= N.n_freq()) return fail(EMFD_ERR_INVALID, "no such far-field wavelength");
     if (n_dirs < 0 || (n_dirs > 0 && (!dirs || !f2))) return fail(EMFD_ERR_INVALID, "bad direction arguments");
     EMFD_HIP(hipSetDevice(ctx->device));
-    const int slices = ntff_pattern_slices(N.box.n_points);
     const int pgroups = (N.box.n_points + 255) / 256;
     if (int rc = N.d_dirs.ensure(static_cast<size_t>(std::max(n_dirs, 1)) * 4 * sizeof(float))) return rc;
-    if (int rc = N.d_partials.ensure(static_cast<size_t>(std::max(n_dirs, 1)) * slices * 12 * sizeof(float)))
-        return rc;
+    if (int rc = N.d_partials.ensure(static_cast<size_t>(std::max(n_dirs, 1)) * 6 * 12 * sizeof(float))) return rc;
     if (int rc = N.d_out.ensure(static_cast<size_t>(std::max(n_dirs, 1)) * sizeof(float))) return rc;
-    if (int rc = N.d_power.ensure(static_cast<size_t>(pgroups) * sizeof(float))) return rc;
+    if (int rc = N.d_power.ensure(static_cast<size_t>(pgroups) * sizeof(double))) return rc;
+    if (int rc = N.d_absmax.ensure(static_cast<size_t>(ff_absmax_blocks(N.box.n_points)) * sizeof(float))) return rc;
+    if (!N.ff_zeroed) {
+        // operand buffers of the box's geometry; their padding must stay zero
+        if (int rc = N.d_ahi.ensure(std::max<size_t>(N.ff_halves, 8) * 2)) return rc;
+        if (int rc = N.d_alo.ensure(std::max<size_t>(N.ff_halves, 8) * 2)) return rc;
+        EMFD_HIP(launch_fill(N.d_ahi.as<float>(), N.ff_halves / 2, 0.0f, ctx->stream));
+        EMFD_HIP(launch_fill(N.d_alo.as<float>(), N.ff_halves / 2, 0.0f, ctx->stream));
+        N.ff_zeroed = true;
+    }
     if (n_dirs > 0)
         EMFD_HIP(hipMemcpyAsync(N.d_dirs.ptr, dirs, static_cast<size_t>(n_dirs) * 4 * sizeof(float),
                                 hipMemcpyHostToDevice, ctx->stream));
     const float k = static_cast<float>(kTwoPi / N.wavelength(freq));  // src/FarField.cpp:293
-    const float center[3] = {0.5f * static_cast<float>(N.box.bmin[0] + N.box.bmax[0]),
-                             0.5f * static_cast<float>(N.box.bmin[1] + N.box.bmax[1]),
-                             0.5f * static_cast<float>(N.box.bmin[2] + N.box.bmax[2])};
     const float *ph = N.d_phasors.as<float>() + static_cast<size_t>(N.box.n_points) * 24 * static_cast<size_t>(freq);
-    EMFD_HIP(launch_ntff_pattern(N.box, ph, N.d_dirs.as<float>(), n_dirs, k, center,
-                                 N.d_partials.as<float>(), slices, N.d_out.as<float>(),
-                                 power ? N.d_power.as<float>() : nullptr, ctx->stream));
+    EMFD_HIP(launch_far_field(N.box, N.ff, ph, N.d_dirs.as<float>(), n_dirs, k, N.d_absmax.as<float>(),
+                              N.d_ahi.as<_Float16>(), N.d_alo.as<_Float16>(), N.d_partials.as<float>(),
+                              N.d_out.as<float>(), power ? N.d_power.as<double>() : nullptr, ctx->stream));
     EMFD_HIP(hipStreamSynchronize(ctx->stream));
     if (n_dirs > 0)
         EMFD_HIP(hipMemcpy(f2, N.d_out.ptr, static_cast<size_t>(n_dirs) * sizeof(float), hipMemcpyDeviceToHost));
     if (power) {
-        std::vector<float> pp(static_cast<size_t>(pgroups));
-        EMFD_HIP(hipMemcpy(pp.data(), N.d_power.ptr, pp.size() * sizeof(float), hipMemcpyDeviceToHost));
+        std::vector<double> pp(static_cast<size_t>(pgroups));
+        EMFD_HIP(hipMemcpy(pp.data(), N.d_power.ptr, pp.size() * sizeof(double), hipMemcpyDeviceToHost));
         double P = 0.0;
-        for (float v : pp) P += static_cast<double>(v);
+        for (double v : pp) P += v;
         *power = 0.5 * P;
     }
     return EMFD_OK;

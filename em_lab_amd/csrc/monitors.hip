@@ -174,143 +174,6 @@ __global__ void __launch_bounds__(256) k_ntff_flush(NtffBox box, const float *ro
 }
 
 // ---------------------------------------------------------------------------------------------
-// Radiation integrals: stage 1 (per direction x surface slice partial N, L) and stage 2 (sum of
-// slices and |N_perp - r x L|^2), shaders/ntff_pattern.comp:45-110 and ntff_reduce.comp:19-39.
-// Each thread owns one direction; the slice's surface points stream through LDS in tiles.
-// ---------------------------------------------------------------------------------------------
-constexpr int kPatTile = 256;
-
-__global__ void __launch_bounds__(kPatTile) k_ntff_pattern(NtffBox box, const float *ph, const float *dirs, int n_dirs,
-                                                           float k, float cx, float cy, float cz, int slices,
-                                                           float *partials) {
-    __shared__ float sPos[3][kPatTile];
-    __shared__ float sCur[12][kPatTile];
-    const int lid = threadIdx.x;
-    const int d = blockIdx.x * kPatTile + lid;
-    const int slice = blockIdx.y;
-    const bool in_range = d < n_dirs;
-    const float rx = in_range ? dirs[4 * d + 0] : 0.0f, ry = in_range ? dirs[4 * d + 1] : 0.0f,
-                rz = in_range ? dirs[4 * d + 2] : 1.0f;
-    const int np = box.n_points;
-    const int per = (np + slices - 1) / slices;
-    const int first = slice * per, last = min(first + per, np);
-    const size_t stride = static_cast<size_t>(np);
-    float Nr[3] = {0, 0, 0}, Ni[3] = {0, 0, 0}, Lr[3] = {0, 0, 0}, Li[3] = {0, 0, 0};
-    for (int base = first; base < last; base += kPatTile) {
-        const int i = base + lid;
-        if (i < last) {
-            int node[3];
-            float n[3], w;
-            ntff_point(box, i, node, n, &w);
-            sPos[0][lid] = static_cast<float>(node[0]) - cx;
-            sPos[1][lid] = static_cast<float>(node[1]) - cy;
-            sPos[2][lid] = static_cast<float>(node[2]) - cz;
-            float v[4][3];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) v[q][c] = ph[(12 + 3 * q + c) * stride + i];
-            // J = n x H, M = -n x E, weighted (cross order as GLSL cross())
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float *h = v[q];
-                const float cxv = n[1] * h[2] - h[1] * n[2];
-                const float cyv = n[2] * h[0] - h[2] * n[0];
-                const float czv = n[0] * h[1] - h[0] * n[1];
-                const int slot = q < 2 ? 6 + 3 * q : 3 * (q - 2);  // eRe->Mr, eIm->Mi, hRe->Jr, hIm->Ji
-                const float sw = q < 2 ? -w : w;
-                sCur[slot + 0][lid] = sw * cxv;
-                sCur[slot + 1][lid] = sw * cyv;
-                sCur[slot + 2][lid] = sw * czv;
-            }
-        }
-        __syncthreads();
-        const int count = min(kPatTile, last - base);
-        for (int kk = 0; kk < count; ++kk) {
-            const float psi = k * (rx * sPos[0][kk] + ry * sPos[1][kk] + rz * sPos[2][kk]);
-            float s, c;
-            sincosf(psi, &s, &c);
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const float jr = sCur[j][kk], ji = sCur[3 + j][kk], mr = sCur[6 + j][kk], mi = sCur[9 + j][kk];
-                Nr[j] += jr * c - ji * s;
-                Ni[j] += jr * s + ji * c;
-                Lr[j] += mr * c - mi * s;
-                Li[j] += mr * s + mi * c;
-            }
-        }
-        __syncthreads();
-    }
-    if (in_range) {
-        float *o = partials + (static_cast<size_t>(d) * slices + slice) * 12;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            o[j] = Nr[j];
-            o[3 + j] = Ni[j];
-            o[6 + j] = Lr[j];
-            o[9 + j] = Li[j];
-        }
-    }
-}
-
-__global__ void __launch_bounds__(64) k_ntff_reduce(const float *dirs, int n_dirs, int slices, const float *partials,
-                                                    float *out) {
-    const int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= n_dirs) return;
-    const float r[3] = {dirs[4 * d + 0], dirs[4 * d + 1], dirs[4 * d + 2]};
-    float acc[12];
-#pragma unroll
-    for (int j = 0; j < 12; ++j) acc[j] = 0.0f;
-    for (int s = 0; s < slices; ++s) {
-        const float *p = partials + (static_cast<size_t>(d) * slices + s) * 12;
-#pragma unroll
-        for (int j = 0; j < 12; ++j) acc[j] += p[j];
-    }
-    const float *Nr = acc, *Ni = acc + 3, *Lr = acc + 6, *Li = acc + 9;
-    const float dr = Nr[0] * r[0] + Nr[1] * r[1] + Nr[2] * r[2];
-    const float di = Ni[0] * r[0] + Ni[1] * r[1] + Ni[2] * r[2];
-    const float crr[3] = {r[1] * Lr[2] - Lr[1] * r[2], r[2] * Lr[0] - Lr[2] * r[0], r[0] * Lr[1] - Lr[0] * r[1]};
-    const float cri[3] = {r[1] * Li[2] - Li[1] * r[2], r[2] * Li[0] - Li[2] * r[0], r[0] * Li[1] - Li[0] * r[1]};
-    float fr[3], fi[3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        fr[j] = (Nr[j] - dr * r[j]) - crr[j];
-        fi[j] = (Ni[j] - di * r[j]) - cri[j];
-    }
-    out[d] = (fr[0] * fr[0] + fr[1] * fr[1] + fr[2] * fr[2]) + (fi[0] * fi[0] + fi[1] * fi[1] + fi[2] * fi[2]);
-}
-
-// Radiated power through the box: P = 1/2 sum w Re(E x H*) . n; one float partial per block
-// (tree reduction), summed in double on the host.
-__global__ void __launch_bounds__(256) k_ntff_power(NtffBox box, const float *ph, float *partials) {
-    __shared__ float sP[256];
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    float P = 0.0f;
-    if (i < box.n_points) {
-        int node[3];
-        float n[3], w;
-        ntff_point(box, i, node, n, &w);
-        const size_t np = static_cast<size_t>(box.n_points);
-        float v[4][3];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) v[q][c] = ph[(12 + 3 * q + c) * np + i];
-        const float *eR = v[0], *eI = v[1], *hR = v[2], *hI = v[3];
-        const float c1[3] = {eR[1] * hR[2] - hR[1] * eR[2], eR[2] * hR[0] - hR[2] * eR[0], eR[0] * hR[1] - hR[0] * eR[1]};
-        const float c2[3] = {eI[1] * hI[2] - hI[1] * eI[2], eI[2] * hI[0] - hI[2] * eI[0], eI[0] * hI[1] - hI[0] * eI[1]};
-        P = w * ((c1[0] + c2[0]) * n[0] + (c1[1] + c2[1]) * n[1] + (c1[2] + c2[2]) * n[2]);
-    }
-    sP[threadIdx.x] = P;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sP[threadIdx.x] += sP[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partials[blockIdx.x] = sP[0];
-}
-
-// ---------------------------------------------------------------------------------------------
 // poynting_flux.comp (shaders/poynting_flux.comp:32-63): 16x16 tiles, shared-memory tree, one
 // float partial per tile. Several planes in one launch: blockIdx.y selects the plane.
 // ---------------------------------------------------------------------------------------------
@@ -482,32 +345,6 @@ hipError_t launch_ntff_flush(const NtffBox &box, const float *rot, int rot_strid
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
-}
-
-int ntff_pattern_slices(int n_points) {
-    // enough slices that even one tile of directions spreads over the CUs
-    int slices = (n_points + 2047) / 2048;
-    return slices < 1 ? 1 : (slices > 256 ? 256 : slices);
-}
-
-hipError_t launch_ntff_pattern(const NtffBox &box, const float *phasors, const float *dirs, int n_dirs, float k,
-                               const float center[3], float *partials, int slices, float *out_f2,
-                               float *power_partials, hipStream_t s) {
-    if (n_dirs > 0) {
-        dim3 grid((n_dirs + kPatTile - 1) / kPatTile, slices, 1);
-        k_ntff_pattern<<<grid, kPatTile, 0, s>>>(box, phasors, dirs, n_dirs, k, center[0], center[1], center[2],
-                                                 slices, partials);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        k_ntff_reduce<<<(n_dirs + 63) / 64, 64, 0, s>>>(dirs, n_dirs, slices, partials, out_f2);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    if (power_partials && box.n_points > 0) {
-        k_ntff_power<<<(box.n_points + 255) / 256, 256, 0, s>>>(box, phasors, power_partials);
-        return hipGetLastError();
-    }
-    return hipSuccess;
 }
 
 hipError_t launch_flux(const LatticeArgs &a, const FieldPtrs &f, const FluxPlane *planes, int n_planes,
