@@ -42,6 +42,8 @@ struct PortsState {
     int64_t recorded = 0;
     double start_time = 0.0;
     DevBuf d_ports, d_phasors, d_series;
+    std::vector<double> dft_wavelengths;  // FP64 running DFT of V, I at these wavelengths (opt-in)
+    DevBuf d_dft;                         // [port][frequency][V re, V im, I re, I im]
     bool enabled() const { return !ports.empty(); }
     bool recording() const { return has_sweep || record_steps > 0; }
 };
@@ -78,6 +80,10 @@ struct NtffState {
     size_t ff_halves = 0;
     bool ff_zeroed = false;
     DevBuf d_ahi, d_alo, d_absmax;
+    // FP64 running DFT of the Huygens points at every accumulated wavelength (opt-in):
+    // per wavelength 12 planes of n_points doubles; d_dft_ph: FP32 planes for the pattern
+    bool dft = false;
+    DevBuf d_dft, d_dft_ph;
 };
 
 }  // namespace emfd
@@ -128,6 +134,9 @@ struct emfd_ctx {
     // per-call step constants: pinned host staging (two buffers, each released by an event once
     // its copy has run) and the device copy the kernels read
     emfd::DevBuf d_consts;
+    // FP64 rotations of the running DFTs for one step call: [step][frequency][4], ports' first
+    std::vector<double> h_consts64;
+    emfd::DevBuf d_consts64;
     float *h_consts[2] = {nullptr, nullptr};
     size_t h_consts_cap[2] = {0, 0};
     hipEvent_t h_consts_ev[2] = {nullptr, nullptr};
@@ -161,4 +170,6 @@ namespace emfd {
 int monitors_consts_floats(emfd_ctx *ctx);
 int monitors_fill_consts(emfd_ctx *ctx, float *consts_step, uint64_t step_after);
 int monitors_after_step(emfd_ctx *ctx, const float *consts_step, int stride, int k, int count);
+int monitors_dft_freqs(const emfd_ctx *ctx);
+int monitors_upload_dft(emfd_ctx *ctx, int count);
 }  // namespace emfd

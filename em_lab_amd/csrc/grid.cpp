@@ -372,7 +372,7 @@ void emfd_destroy(emfd_ctx *ctx) {
                       &ctx->ntff.d_out, &ctx->ntff.d_power, &ctx->scratch_partials, &ctx->scratch_misc,
                       &ctx->lines.d_quant, &ctx->lines.d_terms, &ctx->lines.d_weights, &ctx->lines.d_phasors,
                       &ctx->lines.d_now, &ctx->ntff.d_ahi, &ctx->ntff.d_alo, &ctx->ntff.d_absmax, &ctx->fix_b,
-                      &ctx->fix_e};
+                      &ctx->fix_e, &ctx->ports.d_dft, &ctx->ntff.d_dft, &ctx->ntff.d_dft_ph, &ctx->d_consts64};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t ev : ctx->ev_pool) (void)hipEventDestroy(ev);
     for (int i = 0; i < 2; ++i) {
@@ -726,6 +726,7 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
     }
     EMFD_HIP(hipMemcpyAsync(ctx->d_consts.ptr, hc, nfl * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     EMFD_HIP(hipEventRecord(ctx->h_consts_ev[hb], ctx->stream));
+    if (int rc = monitors_upload_dft(ctx, count)) return rc;
 
     {
         // the vectorised FP32 two-pass kernels and the 16-bit kernels copy the far-field x-face

@@ -206,6 +206,16 @@ hipError_t launch_ntff_sample(const LatticeArgs &a, const FieldPtrs &f, const Nt
                               const PortJob &pj, hipStream_t s);
 hipError_t launch_ntff_flush(const NtffBox &box, const float *rot, int rot_stride, int steps, const NtffFreqs &freqs,
                              const float *samples, float *phasors, hipStream_t s);
+// FP64 running DFTs (dft.hip): ports [port][frequency][V re, V im, I re, I im], one launch per
+// step with that step's rotations rot[frequency][cos wtE, sin wtE, cos wtH, sin wtH]; Huygens
+// points per frequency 12 planes of n_points doubles (E re, E im, H re, H im, x y z each), over a
+// run of deferred samples (rotation of step k, frequency q at rot[k stride + 4 q]).
+hipError_t launch_port_dft(const LatticeArgs &a, const FieldPtrs &f, const PortDev *ports, int n_ports,
+                           const double *rot, int n_freq, double *acc, hipStream_t s);
+hipError_t launch_ntff_dft(const NtffBox &box, const double *rot, int stride, int steps, int n_freq,
+                           const float *samples, double *acc, hipStream_t s);
+hipError_t launch_ntff_dft_planes(const double *acc, int n_points, float *ph, hipStream_t s);
+
 // Far-field radiation integrals as matrix products (farfield.hip). Per box face: normal axis ax
 // (side 0: lower face, 1: upper), in-plane axes u (summed in the epilogue) and v (the product's
 // contraction axis), U x V points from phasor index `first` (u fastest), 16-u blocks, contraction
@@ -231,13 +241,15 @@ hipError_t launch_far_field(const NtffBox &box, const FfGeom &g, const float *ph
                             float k, float *absmax, _Float16 *a_hi, _Float16 *a_lo, float *face_partials,
                             float *out_f2, double *power_partials, hipStream_t s);
 
+// One rectangle of a flux surface: normal axis, plane index, u / v origin and extent, and the
+// index of its first row partial (one float per row of rsize[1]).
 struct FluxPlane {
     int axis, index;
     int rmin[2], rsize[2];
     int first_group;
 };
 hipError_t launch_flux(const LatticeArgs &a, const FieldPtrs &f, const FluxPlane *planes, int n_planes,
-                       int total_groups, float *partials, hipStream_t s);
+                       int max_rows, float *partials, hipStream_t s);
 hipError_t launch_probe_gather(const LatticeArgs &a, const FieldPtrs &f, const int *origins, int n_probes,
                                const float *sigma_cell, const float *eps_cell, float *blocks, hipStream_t s);
 

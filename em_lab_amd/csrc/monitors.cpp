@@ -205,6 +205,46 @@ int monitors_fill_consts(emfd_ctx *ctx, float *row, uint64_t step_after) {
     return EMFD_OK;
 }
 
+int monitors_dft_freqs(const emfd_ctx *ctx) {
+    const int np = ctx->ports.enabled() ? static_cast<int>(ctx->ports.dft_wavelengths.size()) : 0;
+    const int nf = ctx->ntff.enabled && ctx->ntff.dft ? ctx->ntff.n_freq() : 0;
+    return np + nf;
+}
+
+// FP64 rotations of the running DFTs for the `count` steps of one step call: per step, per
+// frequency (ports' first, then the far-field wavelengths) cos w tE, sin w tE, cos w tH, sin w tH
+// with tE = n dt (n = the step count after the step) and tH = tE - dt / 2.
+int monitors_upload_dft(emfd_ctx *ctx, int count) {
+    const int nd = monitors_dft_freqs(ctx);
+    if (nd == 0) return EMFD_OK;
+    std::vector<double> wl;
+    if (ctx->ports.enabled())
+        for (double w : ctx->ports.dft_wavelengths) wl.push_back(w);
+    if (ctx->ntff.enabled && ctx->ntff.dft)
+        for (int f = 0; f < ctx->ntff.n_freq(); ++f) wl.push_back(static_cast<double>(ctx->ntff.wavelength(f)));
+    const double dt = static_cast<double>(ctx->cfg.courant);
+    std::vector<double> &h = ctx->h_consts64;
+    h.assign(static_cast<size_t>(count) * nd * 4, 0.0);
+    for (int k = 0; k < count; ++k)
+        for (int q = 0; q < nd; ++q) {
+            const double w = kTwoPi / wl[static_cast<size_t>(q)];
+            const double tE = static_cast<double>(ctx->step_count + static_cast<uint64_t>(k) + 1) * dt;
+            const double tH = tE - 0.5 * dt;
+            double *r = &h[(static_cast<size_t>(k) * nd + q) * 4];
+            r[0] = std::cos(w * tE);
+            r[1] = std::sin(w * tE);
+            r[2] = std::cos(w * tH);
+            r[3] = std::sin(w * tH);
+        }
+    if (ctx->d_consts64.bytes < h.size() * sizeof(double)) {
+        EMFD_HIP(hipStreamSynchronize(ctx->stream));  // earlier steps may still read the old buffer
+        if (int rc = ctx->d_consts64.ensure(h.size() * sizeof(double))) return rc;
+    }
+    EMFD_HIP(hipMemcpyAsync(ctx->d_consts64.ptr, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                            ctx->stream));
+    return EMFD_OK;
+}
+
 int monitors_after_step(emfd_ctx *ctx, const float *consts, int stride, int k, int count) {
     const LatticeArgs lat = ctx->lattice();
     const FieldPtrs f = ctx->fields(ctx->cur);
@@ -226,6 +266,12 @@ int monitors_after_step(emfd_ctx *ctx, const float *consts, int stride, int k, i
                                             ctx->stream));
             pj = PortJob{};
         }
+        if (!P.dft_wavelengths.empty()) {
+            const int nd = monitors_dft_freqs(ctx);
+            EMFD_HIP(launch_port_dft(lat, f, P.d_ports.as<PortDev>(), static_cast<int>(P.ports.size()),
+                                     ctx->d_consts64.as<double>() + static_cast<size_t>(k) * nd * 4,
+                                     static_cast<int>(P.dft_wavelengths.size()), P.d_dft.as<double>(), ctx->stream));
+        }
     }
     if (N.enabled) {
         // the port step, when there is one, rides along as extra workgroups of the far-field gather
@@ -237,7 +283,7 @@ int monitors_after_step(emfd_ctx *ctx, const float *consts, int stride, int k, i
         // (bit-identical to per-step accumulation); a call's last step always completes its run,
         // so the phasors are current whenever a call returns
         const bool last = k + 1 == count;
-        if (N.pending == 0 && (last || !N.d_samples.ptr)) {
+        if (N.pending == 0 && (last || !N.d_samples.ptr) && !N.dft) {
             EMFD_HIP(launch_ntff_accumulate(lat, f, N.box, consts + 4, fr, N.d_phasors.as<float>(), pj, ctx->stream));
         } else {
             const size_t np = static_cast<size_t>(N.box.n_points);
@@ -247,6 +293,14 @@ int monitors_after_step(emfd_ctx *ctx, const float *consts, int stride, int k, i
                 const float *rot0 = consts + 4 - static_cast<ptrdiff_t>(N.pending - 1) * stride;
                 EMFD_HIP(launch_ntff_flush(N.box, rot0, stride, N.pending, fr, N.d_samples.as<float>(),
                                            N.d_phasors.as<float>(), ctx->stream));
+                if (N.dft) {
+                    const int nd = monitors_dft_freqs(ctx);
+                    const int np_dft = ctx->ports.enabled() ? static_cast<int>(ctx->ports.dft_wavelengths.size()) : 0;
+                    const double *r0 = ctx->d_consts64.as<double>() +
+                                       (static_cast<size_t>(k + 1 - N.pending) * nd + np_dft) * 4;
+                    EMFD_HIP(launch_ntff_dft(N.box, r0, nd * 4, N.pending, N.n_freq(), N.d_samples.as<float>(),
+                                             N.d_dft.as<double>(), ctx->stream));
+                }
                 N.pending = 0;
             }
         }
@@ -390,6 +444,7 @@ int emfd_ports_configure(emfd_ctx *ctx, const emfd_port_setup *ports, int n, flo
     P.recorded = 0;
     P.capacity_steps = 0;
     P.record_steps = std::max<int64_t>(record_steps, 0);
+    P.dft_wavelengths.clear();
     if (P.ports.empty()) return EMFD_OK;
     std::vector<PortDev> packed;
     for (const emfd_port_setup &p : P.ports) {
@@ -419,6 +474,36 @@ int emfd_ports_disable(emfd_ctx *ctx) {
     ctx->ports.ports.clear();
     ctx->ports.has_sweep = false;
     ctx->ports.recorded = 0;
+    return EMFD_OK;
+}
+
+int emfd_ports_set_dft(emfd_ctx *ctx, const double *wavelengths, int n) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    PortsState &P = ctx->ports;
+    if (n < 0 || n > 4096 || (n > 0 && !wavelengths)) return fail(EMFD_ERR_INVALID, "0 to 4096 wavelengths");
+    if (n > 0 && !P.enabled()) return fail(EMFD_ERR_STATE, "port monitor not configured");
+    for (int q = 0; q < n; ++q)
+        if (!(wavelengths[q] > 0.0)) return fail(EMFD_ERR_INVALID, "running-DFT wavelengths must be positive");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    P.dft_wavelengths.assign(wavelengths, wavelengths + n);
+    if (n == 0) return EMFD_OK;
+    const size_t dbl = P.ports.size() * static_cast<size_t>(n) * 4;
+    if (int rc = P.d_dft.ensure(dbl * sizeof(double))) return rc;
+    EMFD_HIP(launch_fill(P.d_dft.as<float>(), 2 * dbl, 0.0f, ctx->stream));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    return EMFD_OK;
+}
+
+int emfd_ports_read_dft(emfd_ctx *ctx, double *out, int64_t n_values) {
+    if (!ctx || !out) return fail(EMFD_ERR_INVALID, "null argument");
+    PortsState &P = ctx->ports;
+    if (P.dft_wavelengths.empty()) return fail(EMFD_ERR_STATE, "port running DFT not enabled");
+    const size_t dbl = P.ports.size() * P.dft_wavelengths.size() * 4;
+    if (n_values < static_cast<int64_t>(dbl)) return fail(EMFD_ERR_INVALID, "output too small");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    EMFD_HIP(hipMemcpy(out, P.d_dft.ptr, dbl * sizeof(double), hipMemcpyDeviceToHost));
     return EMFD_OK;
 }
 
@@ -582,6 +667,7 @@ int emfd_ntff_configure(emfd_ctx *ctx, const emfd_far_field_setup *setup, emfd_f
     N.box = b;
     N.ff = ff_geometry(b, &N.ff_halves);
     N.ff_zeroed = false;
+    N.dft = false;
     N.setup = s;
     N.enabled = true;
     N.start_time = ctx->time();
@@ -619,7 +705,45 @@ int emfd_ntff_set_wavelengths(emfd_ctx *ctx, const float *wavelengths, int n) {
     if (int rc = N.d_phasors.ensure(floats * sizeof(float))) return rc;
     EMFD_HIP(launch_fill(N.d_phasors.as<float>(), floats, 0.0f, ctx->stream));
     N.start_time = ctx->time();
+    if (N.dft) {
+        // the running DFTs follow the wavelength list, restarted
+        const size_t dbl = static_cast<size_t>(N.box.n_points) * 12 * n;
+        if (int rc = N.d_dft.ensure(dbl * sizeof(double))) return rc;
+        EMFD_HIP(launch_fill(N.d_dft.as<float>(), 2 * dbl, 0.0f, ctx->stream));
+    }
     EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    return EMFD_OK;
+}
+
+int emfd_ntff_set_dft(emfd_ctx *ctx, int enable) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    NtffState &N = ctx->ntff;
+    if (!N.enabled) return fail(EMFD_ERR_STATE, "far-field monitor not configured");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    N.dft = enable != 0;
+    if (!N.dft) return EMFD_OK;
+    const size_t dbl = static_cast<size_t>(N.box.n_points) * 12 * N.n_freq();
+    if (int rc = N.d_dft.ensure(dbl * sizeof(double))) return rc;
+    EMFD_HIP(launch_fill(N.d_dft.as<float>(), 2 * dbl, 0.0f, ctx->stream));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    return EMFD_OK;
+}
+
+int emfd_ntff_read_dft(emfd_ctx *ctx, int freq, double *out, int64_t n_points) {
+    if (!ctx || !out) return fail(EMFD_ERR_INVALID, "null argument");
+    NtffState &N = ctx->ntff;
+    if (!N.enabled || !N.dft) return fail(EMFD_ERR_STATE, "far-field running DFT not enabled");
+    if (freq < 0 || freq >= N.n_freq()) return fail(EMFD_ERR_INVALID, "no such far-field wavelength");
+    const size_t np = static_cast<size_t>(N.box.n_points);
+    if (n_points < static_cast<int64_t>(np)) return fail(EMFD_ERR_INVALID, "output too small");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<double> soa(np * 12);
+    EMFD_HIP(hipMemcpy(soa.data(), N.d_dft.as<double>() + np * 12 * static_cast<size_t>(freq),
+                       soa.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < np; ++i)
+        for (size_t j = 0; j < 12; ++j) out[i * 12 + j] = soa[j * np + i];
     return EMFD_OK;
 }
 
@@ -670,10 +794,24 @@ int emfd_ntff_eval(emfd_ctx *ctx, const float *dirs, int n_dirs, float *f2, doub
     return emfd_ntff_eval_at(ctx, 0, dirs, n_dirs, f2, power);
 }
 
+namespace {
+int ntff_eval_impl(emfd_ctx *ctx, int freq, bool from_dft, const float *dirs, int n_dirs, float *f2, double *power);
+}
+
 int emfd_ntff_eval_at(emfd_ctx *ctx, int freq, const float *dirs, int n_dirs, float *f2, double *power) {
+    return ntff_eval_impl(ctx, freq, false, dirs, n_dirs, f2, power);
+}
+
+int emfd_ntff_eval_dft(emfd_ctx *ctx, int freq, const float *dirs, int n_dirs, float *f2, double *power) {
+    return ntff_eval_impl(ctx, freq, true, dirs, n_dirs, f2, power);
+}
+
+namespace {
+int ntff_eval_impl(emfd_ctx *ctx, int freq, bool from_dft, const float *dirs, int n_dirs, float *f2, double *power) {
     if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
     NtffState &N = ctx->ntff;
     if (!N.enabled) return fail(EMFD_ERR_STATE, "far-field monitor not configured");
+    if (from_dft && !N.dft) return fail(EMFD_ERR_STATE, "far-field running DFT not enabled");
     if (freq < 0 || freq >= N.n_freq()) return fail(EMFD_ERR_INVALID, "no such far-field wavelength");
     if (n_dirs < 0 || (n_dirs > 0 && (!dirs || !f2))) return fail(EMFD_ERR_INVALID, "bad direction arguments");
     EMFD_HIP(hipSetDevice(ctx->device));
@@ -696,6 +834,14 @@ int emfd_ntff_eval_at(emfd_ctx *ctx, int freq, const float *dirs, int n_dirs, fl
                                 hipMemcpyHostToDevice, ctx->stream));
     const float k = static_cast<float>(kTwoPi / N.wavelength(freq));  // src/FarField.cpp:293
     const float *ph = N.d_phasors.as<float>() + static_cast<size_t>(N.box.n_points) * 24 * static_cast<size_t>(freq);
+    if (from_dft) {
+        // the FP64 sums as FP32 stage-2 planes (the pattern's inputs); |F|^2 and P scale alike
+        const size_t np = static_cast<size_t>(N.box.n_points);
+        if (int rc = N.d_dft_ph.ensure(np * 24 * sizeof(float))) return rc;
+        EMFD_HIP(launch_ntff_dft_planes(N.d_dft.as<double>() + np * 12 * static_cast<size_t>(freq), N.box.n_points,
+                                        N.d_dft_ph.as<float>(), ctx->stream));
+        ph = N.d_dft_ph.as<float>();
+    }
     EMFD_HIP(launch_far_field(N.box, N.ff, ph, N.d_dirs.as<float>(), n_dirs, k, N.d_absmax.as<float>(),
                               N.d_ahi.as<_Float16>(), N.d_alo.as<_Float16>(), N.d_partials.as<float>(),
                               N.d_out.as<float>(), power ? N.d_power.as<double>() : nullptr, ctx->stream));
@@ -711,6 +857,7 @@ int emfd_ntff_eval_at(emfd_ctx *ctx, int freq, const float *dirs, int n_dirs, fl
     }
     return EMFD_OK;
 }
+}  // namespace
 
 int emfd_ntff_evaluate(emfd_ctx *ctx, int *valid, double *radiated_power, float *max_directivity_dbi,
                        float *max_direction, float *directivity_dbi, float *cuts) {
@@ -786,7 +933,7 @@ int emfd_flux_measure(emfd_ctx *ctx, const emfd_flux_surface *surfaces, int n, d
         p.rsize[0] = mxu - mnu;
         p.rsize[1] = mxv - mnv;
         p.first_group = total;
-        const int groups = ((p.rsize[0] + 15) / 16) * ((p.rsize[1] + 15) / 16);
+        const int groups = p.rsize[1];  // one partial per row of the rectangle
         total += groups;
         max_groups = std::max(max_groups, groups);
         planes.push_back(p);
@@ -823,7 +970,7 @@ int emfd_flux_measure(emfd_ctx *ctx, const emfd_flux_surface *surfaces, int n, d
     EMFD_HIP(hipMemcpy(partials.data(), ctx->scratch_partials.ptr, partials.size() * sizeof(float),
                        hipMemcpyDeviceToHost));
     for (size_t i = 0; i < planes.size(); ++i) {
-        const int groups = ((planes[i].rsize[0] + 15) / 16) * ((planes[i].rsize[1] + 15) / 16);
+        const int groups = planes[i].rsize[1];
         double sum = 0.0;
         for (int k = 0; k < groups; ++k) sum += static_cast<double>(partials[static_cast<size_t>(planes[i].first_group + k)]);
         out[refs[i].surface] += refs[i].sign * sum;

@@ -174,41 +174,36 @@ __global__ void __launch_bounds__(256) k_ntff_flush(NtffBox box, const float *ro
 }
 
 // ---------------------------------------------------------------------------------------------
-// poynting_flux.comp (shaders/poynting_flux.comp:32-63): 16x16 tiles, shared-memory tree, one
-// float partial per tile. Several planes in one launch: blockIdx.y selects the plane.
+// Flux through axis-aligned rectangles: one wavefront per row of a plane (four rows per
+// workgroup), its lanes stepping along the row; each lane sums S_axis = (E x B)_axis at its nodes
+// (field_sampling.glsl node values), the wavefront adds its lanes by butterfly shuffles and
+// writes one float per row, summed in FP64 on the host (FieldAnalyzer::planeFlux,
+// src/FieldAnalyzer.cpp:317-359, defines the quantity). blockIdx.y selects the plane.
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_flux(LatticeArgs a, FieldPtrs f, const FluxPlane *planes, float *partials) {
-    __shared__ float sPart[256];
     const FluxPlane pl = planes[blockIdx.y];
-    const int gx = (pl.rsize[0] + 15) / 16, gy = (pl.rsize[1] + 15) / 16;
-    const int group = blockIdx.x;
-    const int lid = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int lv = static_cast<int>(blockIdx.x) * 4 + (static_cast<int>(threadIdx.x) >> 6);
+    if (lv >= pl.rsize[1]) return;  // whole wavefronts: no barrier follows
+    const int vv = pl.rmin[1] + lv;
     float flux = 0.0f;
-    const bool active = group < gx * gy;
-    if (active) {
-        const int lu = (group % gx) * 16 + (lid & 15), lv = (group / gx) * 16 + (lid >> 4);
-        if (lu < pl.rsize[0] && lv < pl.rsize[1]) {
-            const int uu = pl.rmin[0] + lu, vv = pl.rmin[1] + lv;
-            int p[3];
-            if (pl.axis == 0) { p[0] = pl.index; p[1] = uu; p[2] = vv; }
-            else if (pl.axis == 1) { p[0] = uu; p[1] = pl.index; p[2] = vv; }
-            else { p[0] = uu; p[1] = vv; p[2] = pl.index; }
-            if (p[0] >= 0 && p[1] >= 0 && p[2] >= 0 && p[0] < a.nx && p[1] < a.ny && p[2] < a.nz) {
-                float e[3], b[3];
-                e_at_node(a, f, p[0], p[1], p[2], e);
-                b_at_node(a, f, p[0], p[1], p[2], b);
-                const float S[3] = {e[1] * b[2] - b[1] * e[2], e[2] * b[0] - b[2] * e[0], e[0] * b[1] - b[0] * e[1]};
-                flux = S[pl.axis];
-            }
+    for (int lu = lane; lu < pl.rsize[0]; lu += 64) {
+        const int uu = pl.rmin[0] + lu;
+        int p[3];
+        if (pl.axis == 0) { p[0] = pl.index; p[1] = uu; p[2] = vv; }
+        else if (pl.axis == 1) { p[0] = uu; p[1] = pl.index; p[2] = vv; }
+        else { p[0] = uu; p[1] = vv; p[2] = pl.index; }
+        if (p[0] >= 0 && p[1] >= 0 && p[2] >= 0 && p[0] < a.nx && p[1] < a.ny && p[2] < a.nz) {
+            float e[3], b[3];
+            e_at_node(a, f, p[0], p[1], p[2], e);
+            b_at_node(a, f, p[0], p[1], p[2], b);
+            const int a1 = (pl.axis + 1) % 3, a2 = (pl.axis + 2) % 3;
+            flux += e[a1] * b[a2] - e[a2] * b[a1];
         }
     }
-    sPart[lid] = flux;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (lid < s) sPart[lid] += sPart[lid + s];
-        __syncthreads();
-    }
-    if (lid == 0 && active) partials[pl.first_group + group] = sPart[0];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) flux += __shfl_xor(flux, o);
+    if (lane == 0) partials[pl.first_group + lv] = flux;
 }
 
 // probe_gather.comp (shaders/probe_gather.comp:20-26): 27 E texels then 27 B texels per probe,
@@ -348,9 +343,9 @@ hipError_t launch_ntff_flush(const NtffBox &box, const float *rot, int rot_strid
 }
 
 hipError_t launch_flux(const LatticeArgs &a, const FieldPtrs &f, const FluxPlane *planes, int n_planes,
-                       int max_groups, float *partials, hipStream_t s) {
-    if (n_planes <= 0 || max_groups <= 0) return hipSuccess;
-    dim3 grid(max_groups, n_planes, 1);
+                       int max_rows, float *partials, hipStream_t s) {
+    if (n_planes <= 0 || max_rows <= 0) return hipSuccess;
+    dim3 grid((max_rows + 3) / 4, n_planes, 1);
     k_flux<<<grid, 256, 0, s>>>(a, f, planes, partials);
     return hipGetLastError();
 }

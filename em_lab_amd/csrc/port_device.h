@@ -81,10 +81,10 @@ __device__ __forceinline__ float edge_current(const LatticeArgs &a, const FieldP
     return (b0v - buv) - (b0u - bvu);
 }
 
-// One port's gap sample and two-stage EMA update (shaders/port_accumulate.comp:46-83).
-// ph: the port's 8 phasor floats; series_vi: where to record (V, I), or null.
-__device__ __forceinline__ void port_sample(const LatticeArgs &a, const FieldPtrs &f, const PortDev &pt,
-                                            const float *rot, float blend, float *ph, float *series_vi) {
+// A port's gap voltage and loop current (shaders/port_accumulate.comp:46-70): the thin gap's E
+// and B loop, or the rod gap's four edges (V averaged, I summed); signed by the port's sign.
+__device__ __forceinline__ void port_gap(const LatticeArgs &a, const FieldPtrs &f, const PortDev &pt, float *Vo,
+                                         float *Io) {
     int p[3] = {pt.texel[0], pt.texel[1], pt.texel[2]};
     const int ax = pt.axis;
     float V = 0.0f, I = 0.0f;
@@ -102,8 +102,16 @@ __device__ __forceinline__ void port_sample(const LatticeArgs &a, const FieldPtr
         I = edge_current(a, f, p, ax);
     }
     const float s = static_cast<float>(pt.sign);
-    V *= s;
-    I *= s;
+    *Vo = V * s;
+    *Io = I * s;
+}
+
+// One port's gap sample and two-stage EMA update (shaders/port_accumulate.comp:46-83).
+// ph: the port's 8 phasor floats; series_vi: where to record (V, I), or null.
+__device__ __forceinline__ void port_sample(const LatticeArgs &a, const FieldPtrs &f, const PortDev &pt,
+                                            const float *rot, float blend, float *ph, float *series_vi) {
+    float V, I;
+    port_gap(a, f, pt, &V, &I);
     if (series_vi) {
         series_vi[0] = V;
         series_vi[1] = I;

@@ -143,6 +143,8 @@ _SIGNATURES = {
     "emfd_ports_configure": (C.c_int, [_P, _P, C.c_int, C.c_float, _P, C.c_float, C.c_int64]),
     "emfd_ports_disable": (C.c_int, [_P]),
     "emfd_ports_read_phasors": (C.c_int, [_P, _P, C.c_int]),
+    "emfd_ports_set_dft": (C.c_int, [_P, _P, C.c_int]),
+    "emfd_ports_read_dft": (C.c_int, [_P, _P, C.c_int64]),
     "emfd_ports_evaluate": (C.c_int, [_P, _P, C.c_int]),
     "emfd_ports_recorded_steps": (C.c_int64, [_P]),
     "emfd_ports_read_series": (C.c_int, [_P, _P, C.c_int64]),
@@ -157,6 +159,9 @@ _SIGNATURES = {
     "emfd_ntff_set_wavelengths": (C.c_int, [_P, _P, C.c_int]),
     "emfd_ntff_eval_at": (C.c_int, [_P, C.c_int, _P, C.c_int, _P, C.POINTER(C.c_double)]),
     "emfd_ntff_read_phasors_at": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
+    "emfd_ntff_set_dft": (C.c_int, [_P, C.c_int]),
+    "emfd_ntff_read_dft": (C.c_int, [_P, C.c_int, _P, C.c_int64]),
+    "emfd_ntff_eval_dft": (C.c_int, [_P, C.c_int, _P, C.c_int, _P, C.POINTER(C.c_double)]),
     "emfd_ntff_evaluate": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_float), _P, _P,
                                      _P]),
     "emfd_flux_measure": (C.c_int, [_P, _P, C.c_int, _P]),
@@ -628,6 +633,19 @@ class PortMonitor:
         _check(self._lib.emfd_ports_read_phasors(self.grid.handle, _ptr(out), out.shape[0]))
         return out[: len(self.ports)]
 
+    def set_dft(self, wavelengths) -> None:
+        """FP64 running DFTs of every port's V and I at these wavelengths (empty: off)."""
+        w = np.ascontiguousarray(wavelengths, np.float64).ravel()
+        _check(self._lib.emfd_ports_set_dft(self.grid.handle, _ptr(w) if w.size else None, w.size))
+        self.dft_wavelengths = [float(v) for v in w]
+
+    def dft(self) -> np.ndarray:
+        """(n_ports, n_wavelengths, 2) complex128: V(w), I(w)."""
+        nw = len(getattr(self, "dft_wavelengths", []))
+        out = np.zeros((len(self.ports), nw, 4), np.float64)
+        _check(self._lib.emfd_ports_read_dft(self.grid.handle, _ptr(out), out.size))
+        return out[..., 0::2] + 1j * out[..., 1::2]
+
     def evaluate(self) -> list:
         n = len(self.ports)
         arr = (PortReadingC * max(n, 1))()
@@ -724,6 +742,26 @@ class FarFieldMonitor:
         out = np.zeros(d.shape[0], np.float32)
         P = C.c_double()
         _check(self._lib.emfd_ntff_eval_at(self.grid.handle, int(freq), _ptr(d), d.shape[0], _ptr(out), C.byref(P)))
+        return out, P.value
+
+    def set_dft(self, enable: bool = True) -> None:
+        """FP64 running DFTs of the Huygens-node fields at every far-field wavelength."""
+        _check(self._lib.emfd_ntff_set_dft(self.grid.handle, 1 if enable else 0))
+
+    def dft(self, freq: int = 0):
+        """(E, H) running DFTs of wavelength `freq`: two (n_points, 3) complex128 arrays."""
+        out = np.zeros((self.n_points, 12), np.float64)
+        _check(self._lib.emfd_ntff_read_dft(self.grid.handle, int(freq), _ptr(out), self.n_points))
+        return out[:, 0:3] + 1j * out[:, 3:6], out[:, 6:9] + 1j * out[:, 9:12]
+
+    def radiation_dft(self, dirs: np.ndarray, freq: int = 0):
+        """radiation() on the running DFTs."""
+        d = np.ascontiguousarray(dirs, np.float32)
+        if d.ndim == 2 and d.shape[1] == 3:
+            d = np.ascontiguousarray(np.concatenate([d, np.zeros((d.shape[0], 1), np.float32)], 1))
+        out = np.zeros(d.shape[0], np.float32)
+        P = C.c_double()
+        _check(self._lib.emfd_ntff_eval_dft(self.grid.handle, int(freq), _ptr(d), d.shape[0], _ptr(out), C.byref(P)))
         return out, P.value
 
     @staticmethod

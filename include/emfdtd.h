@@ -289,6 +289,14 @@ int emfd_ports_configure(emfd_ctx *ctx, const emfd_port_setup *ports, int n, flo
 int emfd_ports_disable(emfd_ctx *ctx);
 /* Raw EMA phasors: 8 floats per port (stage1 Vre Vim Ire Iim, stage2 ...). */
 int emfd_ports_read_phasors(emfd_ctx *ctx, float *out, int n);
+/* Running DFTs of the port gap V and I in FP64 (no counterpart in the reference, which keeps the
+ * FP32 EMA and transforms recorded series on the host): from the next step on, every step adds
+ * V_n e^{-i w n dt} and I_n e^{-i w (n - 1/2) dt} at each of n wavelengths (0 to 4096, > 0; n = 0
+ * switches them off), the rotation computed in FP64. Clears the sums; emfd_ports_configure
+ * switches them off. EMFD_ERR_STATE without configured ports. */
+int emfd_ports_set_dft(emfd_ctx *ctx, const double *wavelengths, int n);
+/* The sums: 4 doubles (V re, V im, I re, I im) per [port][wavelength]. */
+int emfd_ports_read_dft(emfd_ctx *ctx, double *out, int64_t n_values);
 /* PortMonitor::evaluate + computeReadings (src/Ports.cpp:145-185): one reading per port. */
 int emfd_ports_evaluate(emfd_ctx *ctx, emfd_port_reading *out, int n);
 /* Recorded series steps so far (PortMonitor::recordedSteps). */
@@ -331,6 +339,17 @@ int emfd_ntff_set_wavelengths(emfd_ctx *ctx, const float *wavelengths, int n);
 /* emfd_ntff_eval / emfd_ntff_read_phasors for wavelength `freq` of the list. */
 int emfd_ntff_eval_at(emfd_ctx *ctx, int freq, const float *dirs, int n_dirs, float *f2, double *power);
 int emfd_ntff_read_phasors_at(emfd_ctx *ctx, int freq, float *out, int64_t n_points);
+/* Running DFTs of the Huygens-node E and H in FP64 at every far-field wavelength, beside the EMA
+ * phasors (no counterpart in the reference): enable != 0 clears and starts them, 0 stops them.
+ * E_n is taken at n dt, H_n at (n - 1/2) dt, the rotation stepped in FP64. emfd_ntff_configure
+ * switches them off; emfd_ntff_set_wavelengths restarts them. */
+int emfd_ntff_set_dft(emfd_ctx *ctx, int enable);
+/* The sums of wavelength `freq`: 12 doubles per point (E re xyz, E im xyz, H re xyz, H im xyz). */
+int emfd_ntff_read_dft(emfd_ctx *ctx, int freq, double *out, int64_t n_points);
+/* emfd_ntff_eval_at on the running DFTs (rounded to FP32) instead of the EMA phasors. *power is
+ * then the frequency-domain flux 1/2 Re of (E x H*) through the Huygens box, up to the same
+ * constant as the EMA's. */
+int emfd_ntff_eval_dft(emfd_ctx *ctx, int freq, const float *dirs, int n_dirs, float *f2, double *power);
 /* FarFieldMonitor::evaluate + finishEvaluation (src/FarField.cpp:278-335) on the reference
  * direction set. directivity_dbi: n_dirs floats (may be NULL). cuts: 3 x 6 floats per cut
  * (peak dBi, peak angle, beamwidth, front/back, unused, unused) (may be NULL). Returns

@@ -789,6 +789,21 @@ void or_ntff_accumulate(const or_grid *g, const float *E, const float *B, const 
     }
 }
 
+/* The Huygens-node samples that ntff_accumulate.comp forms (shaders/ntff_accumulate.comp:30-40),
+ * without the exponential means: out[6 i + c] = E at point i, out[6 i + 3 + c] = H. The running
+ * DFTs of these samples (a library extension with no reference counterpart) are summed by the
+ * caller in FP64. */
+void or_ntff_samples(const or_grid *g, const float *E, const float *B, const int bmin[3], const int bmax[3],
+                     const int fo[7], int n_points, float *out) {
+    for (int i = 0; i < n_points; ++i) {
+        int node[3];
+        float n[3], w;
+        or_ntff_point(bmin, bmax, fo, i, node, n, &w);
+        or_e_at_node(g, E, node[0], node[1], node[2], out + 6 * (size_t)i);
+        or_b_at_node(g, B, node[0], node[1], node[2], out + 6 * (size_t)i + 3);
+    }
+}
+
 static inline void crossf(const float a[3], const float b[3], float o[3]) {
     o[0] = a[1] * b[2] - b[1] * a[2];
     o[1] = a[2] * b[0] - b[2] * a[0];

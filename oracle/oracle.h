@@ -144,6 +144,8 @@ void or_ntff_eval_f64(const int box_min[3], const int box_max[3], const int face
 int or_build_directions(const float axis[3], float *dirs, int cap, int *map_index, float map_frame[9]);
 
 /* field_sampling.glsl */
+void or_ntff_samples(const or_grid *g, const float *E, const float *B, const int box_min[3], const int box_max[3],
+                     const int face_offsets[7], int n_points, float *out);
 void or_e_at_node(const or_grid *g, const float *E, int x, int y, int z, float out[3]);
 void or_b_at_node(const or_grid *g, const float *B, int x, int y, int z, float out[3]);
 

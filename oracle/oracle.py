@@ -71,6 +71,7 @@ _SIG = {
     "or_ntff_eval_f32": (None, [_P, _P, _P, _I, _P, _P, _I, _F, _P, _P]),
     "or_ntff_eval_f64": (None, [_P, _P, _P, _I, _P, _P, _I, _D, _P, _P, C.POINTER(_D)]),
     "or_build_directions": (_I, [_P, _P, _I, _P, _P]),
+    "or_ntff_samples": (None, [C.POINTER(OrGrid), _P, _P, _P, _P, _P, _I, _P]),
     "or_e_at_node": (None, [C.POINTER(OrGrid), _P, _I, _I, _I, _P]),
     "or_b_at_node": (None, [C.POINTER(OrGrid), _P, _I, _I, _I, _P]),
     "or_plane_flux": (_D, [C.POINTER(OrGrid), _P, _P, _I, _I, _I, _I, _I, _I]),
@@ -215,6 +216,22 @@ class OracleSim:
         self.ntff_extra.append(extra)
         return extra["phasors"]
 
+    def configure_ntff_dft(self, wavelengths):
+        """FP64 running DFTs of the Huygens-node samples at these wavelengths (the library's opt-in
+        far-field DFT): per wavelength (n_points, 3) complex128 sums for E at t = n dt and H at
+        (n - 1/2) dt, n being the step count after the step. The DFT is summed here in numpy."""
+        N = self.ntff
+        self.ntff_dft = dict(wavelengths=[float(w) for w in wavelengths],
+                             E=np.zeros((len(wavelengths), N["n"], 3), np.complex128),
+                             H=np.zeros((len(wavelengths), N["n"], 3), np.complex128))
+
+    def ntff_samples(self):
+        N = self.ntff
+        out = np.zeros((N["n"], 6), np.float32)
+        lib().or_ntff_samples(C.byref(self.g), ptr(self.E), ptr(self.B), ptr(N["bmin"]), ptr(N["bmax"]), ptr(N["fo"]),
+                              N["n"], ptr(out))
+        return out
+
     def step(self, count):
         L = lib()
         if count <= 0:
@@ -255,6 +272,16 @@ class OracleSim:
                                        C.byref(blend))
                 L.or_ntff_accumulate(C.byref(self.g), ptr(self.E), ptr(self.B), ptr(N["bmin"]), ptr(N["bmax"]),
                                      ptr(N["fo"]), N["n"], ptr(rot), blend.value, ptr(X["phasors"]))
+        D = getattr(self, "ntff_dft", None)
+        if D is not None:
+            smp = self.ntff_samples().astype(np.float64)
+            dt = float(np.float32(self.g.dt))
+            tE = float(self.step_count) * dt
+            tH = tE - 0.5 * dt
+            for q, wl in enumerate(D["wavelengths"]):
+                w = 2.0 * np.pi / wl
+                D["E"][q] += smp[:, :3] * complex(np.cos(w * tE), -np.sin(w * tE))
+                D["H"][q] += smp[:, 3:] * complex(np.cos(w * tH), -np.sin(w * tH))
         if getattr(self, "lines", None):
             Q = self.lines
             rot = np.zeros(4, np.float32)
