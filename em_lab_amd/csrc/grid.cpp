@@ -350,6 +350,8 @@ int emfd_create(int device, emfd_ctx **out) {
     if (e != hipSuccess || count == 0) return fail(EMFD_ERR_HIP, "emfd_create: no HIP device available");
     if (device < 0 || device >= count) return fail(EMFD_ERR_INVALID, "emfd_create: device index out of range");
     EMFD_HIP(hipSetDevice(device));
+    // the no-allocation entry points' device array: its address on this device, once
+    if (int rc = flux_scratch_init(device)) return rc;
     auto *ctx = new emfd_ctx();
     ctx->device = device;
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);

@@ -250,6 +250,30 @@ struct FluxPlane {
 };
 hipError_t launch_flux(const LatticeArgs &a, const FieldPtrs &f, const FluxPlane *planes, int n_planes,
                        int max_rows, float *partials, hipStream_t s);
+// A plane's surface and outward sign (+1 / -1).
+struct FluxRef {
+    int surface;
+    float sign;
+};
+
+// The no-allocation flux measurement (noalloc.hip): a file-scope device array per GPU holds the
+// plane list, the row partials and the surface sums. Limits: 64 surfaces, 262144 rows in all.
+constexpr int kFluxScratchSurfaces = 64;
+constexpr int kFluxScratchRows = 1 << 18;
+struct FluxScratch {
+    FluxPlane *planes;  // 6 * kFluxScratchSurfaces
+    FluxRef *refs;      // 6 * kFluxScratchSurfaces
+    double *sums;       // kFluxScratchSurfaces
+    float *partials;    // kFluxScratchRows
+};
+// Looks up the array's device address on `device` (once per device; call with it current).
+int flux_scratch_init(int device);
+// The regions of `device`'s array (flux_scratch_init must have succeeded there).
+int flux_scratch(int device, FluxScratch *out);
+// Per surface, in plane order: sums[s] = sum over its planes of sign * (FP64 sum of the rows'
+// partials), written (not accumulated) for surfaces 0 .. n_surfaces - 1.
+hipError_t launch_flux_surfaces(const FluxPlane *planes, const FluxRef *refs, int n_planes, const float *partials,
+                                int n_surfaces, double *sums, hipStream_t s);
 hipError_t launch_probe_gather(const LatticeArgs &a, const FieldPtrs &f, const int *origins, int n_probes,
                                const float *sigma_cell, const float *eps_cell, float *blocks, hipStream_t s);
 

@@ -907,17 +907,17 @@ int emfd_ntff_evaluate(emfd_ctx *ctx, int *valid, double *radiated_power, float 
 // ============================================================================================
 // FieldAnalyzer
 // ============================================================================================
-int emfd_flux_measure(emfd_ctx *ctx, const emfd_flux_surface *surfaces, int n, double *out) {
-    if (!ctx || (n > 0 && (!surfaces || !out))) return fail(EMFD_ERR_INVALID, "null argument");
-    if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
-    EMFD_HIP(hipSetDevice(ctx->device));
-    // plane list with outward signs (src/FieldAnalyzer.cpp:305-338, 491-505)
-    struct Ref { int surface; double sign; };
-    std::vector<FluxPlane> planes;
-    std::vector<Ref> refs;
+}  // extern "C"
+
+namespace {
+// The plane list of n flux surfaces with outward signs (src/FieldAnalyzer.cpp:305-338, 491-505)
+// into caller-sized arrays (at most 6 planes per surface): planes clipped to the lattice, each with
+// one partial per row of its rectangle.
+void flux_planes(const emfd_ctx *ctx, const emfd_flux_surface *surfaces, int n, FluxPlane *planes,
+                 FluxRef *refs, int *n_planes, int *total_rows, int *max_rows) {
     const int size[3] = {ctx->cfg.size[0], ctx->cfg.size[1], ctx->cfg.size[2]};
-    int total = 0, max_groups = 0;
-    auto add_plane = [&](int surf, double sign, int axis, int index, int mnu, int mnv, int mxu, int mxv) {
+    int np = 0, total = 0, max_groups = 0;
+    auto add_plane = [&](int surf, float sign, int axis, int index, int mnu, int mnv, int mxu, int mxv) {
         if (index < 0 || index >= size[axis]) return;
         const int u = axis == 0 ? 1 : 0, v = axis == 2 ? 1 : 2;
         mnu = std::max(mnu, 0);
@@ -933,16 +933,16 @@ int emfd_flux_measure(emfd_ctx *ctx, const emfd_flux_surface *surfaces, int n, d
         p.rsize[0] = mxu - mnu;
         p.rsize[1] = mxv - mnv;
         p.first_group = total;
-        const int groups = p.rsize[1];  // one partial per row of the rectangle
-        total += groups;
-        max_groups = std::max(max_groups, groups);
-        planes.push_back(p);
-        refs.push_back({surf, sign});
+        total += p.rsize[1];
+        max_groups = std::max(max_groups, p.rsize[1]);
+        planes[np] = p;
+        refs[np] = FluxRef{surf, sign};
+        ++np;
     };
     for (int s = 0; s < n; ++s) {
         const emfd_flux_surface &fs = surfaces[s];
         if (fs.kind == 0) {
-            add_plane(s, 1.0, std::clamp(fs.axis, 0, 2), fs.index, fs.rect_min[0], fs.rect_min[1], fs.rect_max[0],
+            add_plane(s, 1.0f, std::clamp(fs.axis, 0, 2), fs.index, fs.rect_min[0], fs.rect_min[1], fs.rect_max[0],
                       fs.rect_max[1]);
             continue;
         }
@@ -953,28 +953,76 @@ int emfd_flux_measure(emfd_ctx *ctx, const emfd_flux_surface *surfaces, int n, d
         }
         for (int axis = 0; axis < 3; ++axis) {
             const int u = axis == 0 ? 1 : 0, v = axis == 2 ? 1 : 2;
-            add_plane(s, -1.0, axis, mn[axis], mn[u], mn[v], mx[u] + 1, mx[v] + 1);
-            add_plane(s, +1.0, axis, mx[axis], mn[u], mn[v], mx[u] + 1, mx[v] + 1);
+            add_plane(s, -1.0f, axis, mn[axis], mn[u], mn[v], mx[u] + 1, mx[v] + 1);
+            add_plane(s, +1.0f, axis, mx[axis], mn[u], mn[v], mx[u] + 1, mx[v] + 1);
         }
     }
+    *n_planes = np;
+    *total_rows = total;
+    *max_rows = max_groups;
+}
+}  // namespace
+
+extern "C" {
+
+int emfd_flux_measure(emfd_ctx *ctx, const emfd_flux_surface *surfaces, int n, double *out) {
+    if (!ctx || (n > 0 && (!surfaces || !out))) return fail(EMFD_ERR_INVALID, "null argument");
+    if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    std::vector<FluxPlane> planes(6 * static_cast<size_t>(std::max(n, 0)));
+    std::vector<FluxRef> refs(planes.size());
+    int np = 0, total = 0, max_rows = 0;
+    flux_planes(ctx, surfaces, n, planes.data(), refs.data(), &np, &total, &max_rows);
     for (int s = 0; s < n; ++s) out[s] = 0.0;
-    if (planes.empty()) return EMFD_OK;
-    if (int rc = ctx->scratch_misc.ensure(planes.size() * sizeof(FluxPlane))) return rc;
+    if (np == 0) return EMFD_OK;
+    if (int rc = ctx->scratch_misc.ensure(static_cast<size_t>(np) * sizeof(FluxPlane))) return rc;
     if (int rc = ctx->scratch_partials.ensure(static_cast<size_t>(total) * sizeof(float))) return rc;
     EMFD_HIP(hipStreamSynchronize(ctx->stream));
-    EMFD_HIP(hipMemcpy(ctx->scratch_misc.ptr, planes.data(), planes.size() * sizeof(FluxPlane), hipMemcpyHostToDevice));
-    EMFD_HIP(launch_flux(ctx->lattice(), ctx->fields(ctx->cur), ctx->scratch_misc.as<FluxPlane>(),
-                         static_cast<int>(planes.size()), max_groups, ctx->scratch_partials.as<float>(), ctx->stream));
+    EMFD_HIP(hipMemcpy(ctx->scratch_misc.ptr, planes.data(), static_cast<size_t>(np) * sizeof(FluxPlane),
+                       hipMemcpyHostToDevice));
+    EMFD_HIP(launch_flux(ctx->lattice(), ctx->fields(ctx->cur), ctx->scratch_misc.as<FluxPlane>(), np, max_rows,
+                         ctx->scratch_partials.as<float>(), ctx->stream));
     EMFD_HIP(hipStreamSynchronize(ctx->stream));
     std::vector<float> partials(static_cast<size_t>(total));
     EMFD_HIP(hipMemcpy(partials.data(), ctx->scratch_partials.ptr, partials.size() * sizeof(float),
                        hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < planes.size(); ++i) {
-        const int groups = planes[i].rsize[1];
+    // per plane an FP64 sum over its rows, then signed into its surface, in plane order
+    for (int i = 0; i < np; ++i) {
         double sum = 0.0;
-        for (int k = 0; k < groups; ++k) sum += static_cast<double>(partials[static_cast<size_t>(planes[i].first_group + k)]);
-        out[refs[i].surface] += refs[i].sign * sum;
+        for (int k = 0; k < planes[i].rsize[1]; ++k)
+            sum += static_cast<double>(partials[static_cast<size_t>(planes[i].first_group + k)]);
+        out[refs[i].surface] += static_cast<double>(refs[i].sign) * sum;
     }
+    return EMFD_OK;
+}
+
+int emfd_flux_measure_noalloc(emfd_ctx *ctx, const emfd_flux_surface *surfaces, int n, double *out) {
+    if (!ctx || (n > 0 && (!surfaces || !out))) return fail(EMFD_ERR_INVALID, "null argument");
+    if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
+    if (n < 0 || n > kFluxScratchSurfaces)
+        return fail(EMFD_ERR_INVALID, "emfd_flux_measure_noalloc: 0 to 64 surfaces");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    FluxScratch sc{};
+    if (int rc = flux_scratch(ctx->device, &sc)) return rc;
+    FluxPlane planes[6 * kFluxScratchSurfaces];
+    FluxRef refs[6 * kFluxScratchSurfaces];
+    int np = 0, total = 0, max_rows = 0;
+    flux_planes(ctx, surfaces, n, planes, refs, &np, &total, &max_rows);
+    if (total > kFluxScratchRows)
+        return fail(EMFD_ERR_INVALID, "emfd_flux_measure_noalloc: more than 262144 plane rows in all");
+    for (int s = 0; s < n; ++s) out[s] = 0.0;
+    if (np == 0) return EMFD_OK;
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    EMFD_HIP(hipMemcpyAsync(sc.planes, planes, static_cast<size_t>(np) * sizeof(FluxPlane), hipMemcpyHostToDevice,
+                            ctx->stream));
+    EMFD_HIP(hipMemcpyAsync(sc.refs, refs, static_cast<size_t>(np) * sizeof(FluxRef), hipMemcpyHostToDevice,
+                            ctx->stream));
+    EMFD_HIP(launch_flux(ctx->lattice(), ctx->fields(ctx->cur), sc.planes, np, max_rows, sc.partials, ctx->stream));
+    EMFD_HIP(launch_flux_surfaces(sc.planes, sc.refs, np, sc.partials, n, sc.sums, ctx->stream));
+    EMFD_HIP(hipMemcpyAsync(out, sc.sums, static_cast<size_t>(n) * sizeof(double), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    // the host arrays above stay alive until the copies have completed
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
     return EMFD_OK;
 }
 

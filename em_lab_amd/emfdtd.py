@@ -19,6 +19,9 @@ _LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libemfdtd.so"
 _lib = None
 
 EMFD_OK = 0
+EMFD_ERR_INVALID = 1
+EMFD_ERR_HIP = 2
+EMFD_ERR_STATE = 3
 LAYOUT_TEXEL = 0
 LAYOUT_SOA = 1
 KERNEL_AUTO = 0
@@ -165,6 +168,7 @@ _SIGNATURES = {
     "emfd_ntff_evaluate": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_float), _P, _P,
                                      _P]),
     "emfd_flux_measure": (C.c_int, [_P, _P, C.c_int, _P]),
+    "emfd_flux_measure_noalloc": (C.c_int, [_P, _P, C.c_int, _P]),
     "emfd_probe_measure": (C.c_int, [_P, _P, C.c_int, _P]),
     "emfd_copy_probe": (C.c_int, [_P, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     "emfd_stream_probe": (C.c_int, [_P, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_double)]),

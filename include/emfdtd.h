@@ -362,6 +362,14 @@ int emfd_ntff_evaluate(emfd_ctx *ctx, int *valid, double *radiated_power, float 
  * --------------------------------------------------------------------------------------- */
 /* FieldAnalyzer::measureFlux (src/FieldAnalyzer.cpp:317-375) for n surfaces. */
 int emfd_flux_measure(emfd_ctx *ctx, const emfd_flux_surface *surfaces, int n, double *out);
+/* emfd_flux_measure without any device allocation. No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The plane list, row partials and sums live in a fixed device array per
+ * GPU (1 MiB + 24 KiB, set up by emfd_create), shared by every context on that GPU: calls that
+ * overlap on one GPU race. Limits: n <= 64 surfaces and at most 262144 plane rows in all (a closed
+ * box has the rows of its six faces), else EMFD_ERR_INVALID. Same results as emfd_flux_measure,
+ * bit for bit. */
+int emfd_flux_measure_noalloc(emfd_ctx *ctx, const emfd_flux_surface *surfaces, int n, double *out);
 /* FieldAnalyzer::probe / submit+collect (src/FieldAnalyzer.cpp:126-272,274-315):
  * positions: 3 floats each. */
 int emfd_probe_measure(emfd_ctx *ctx, const float *positions, int n, emfd_probe_sample *out);
