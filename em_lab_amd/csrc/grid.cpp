@@ -634,9 +634,10 @@ int emfd_grid_set_storage(emfd_ctx *ctx, int storage, const int *scale_log2) {
 int emfd_grid_set_kernel(emfd_ctx *ctx, int variant) {
     if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
     const int v = variant & 0xff;
-    if (v < 0 || v > 8) return fail(EMFD_ERR_INVALID, "unknown kernel variant");
+    if (v != EMFD_KERNEL_AUTO && v != EMFD_KERNEL_TWO_PASS && v != EMFD_KERNEL_TWO_PASS4 && v != EMFD_KERNEL_MARCH)
+        return fail(EMFD_ERR_INVALID, "unknown kernel variant");
     ctx->kernel_variant = v;
-    ctx->tile_cfg = variant >> 8;  // optional tile shape / z-chunk selector for the fused kernel
+    ctx->tile_cfg = variant >> 8;  // the march's z-chunk (0: automatic); the half kernels' cells per thread
     return EMFD_OK;
 }
 
@@ -735,7 +736,7 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
         // columns as they go (the 16-bit ones the stored, rounded values)
         const int v = ctx->kernel_variant;
         const bool tp4 = yee_two_pass4_supported(ctx->lattice()) &&
-                         (v == EMFD_KERNEL_AUTO || v == EMFD_KERNEL_TWO_PASS4 || v == EMFD_KERNEL_SLABS);
+                         (v == EMFD_KERNEL_AUTO || v == EMFD_KERNEL_TWO_PASS4);
         const bool half = (ctx->storage == kStorageF16 || ctx->storage == kStorageBF16) &&
                           yee_half_supported(ctx->lattice());
         const bool march = (v == EMFD_KERNEL_MARCH && yee_march_supported(ctx->lattice())) ||
@@ -784,10 +785,6 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
                                       : yee_two_pass4_supported(lat) ? EMFD_KERNEL_TWO_PASS4 : EMFD_KERNEL_TWO_PASS;
         if (variant == EMFD_KERNEL_TWO_PASS4 && !yee_two_pass4_supported(lat))
             return fail(EMFD_ERR_INVALID, "the vectorised two-pass kernel needs nx % 4 == 0");
-        if (variant == EMFD_KERNEL_ZMARCH && !yee_zmarch_supported(lat))
-            return fail(EMFD_ERR_INVALID, "the z-march kernel supports nx <= 256 with nx % 4 == 0");
-        if (variant == EMFD_KERNEL_ROWS && !yee_rows_supported(lat))
-            return fail(EMFD_ERR_INVALID, "the row-wavefront kernel supports nx <= 256");
         if (variant == EMFD_KERNEL_MARCH && !yee_march_supported(lat))
             return fail(EMFD_ERR_INVALID, "the one-pass march kernel needs nx % 4 == 0 and planes of at most 2 GB");
         if (variant == -1) {
@@ -795,31 +792,10 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
         } else if (variant == EMFD_KERNEL_TWO_PASS) {
             EMFD_HIP(launch_yee_two_pass(lat, ctx->fields(ctx->cur), ctx->stream));
         } else if (variant == EMFD_KERNEL_TWO_PASS4) {
-            // streaming level / block order: explicit bits 8..10 of the variant, else 2 (measured fastest)
-            const int st = ctx->kernel_variant == EMFD_KERNEL_AUTO ? 2 : (ctx->tile_cfg & 7);
-            EMFD_HIP(launch_yee_two_pass4(lat, ctx->fields(ctx->cur), st, ctx->stream));
-        } else if (variant == EMFD_KERNEL_SLABS) {
-            if (!yee_two_pass4_supported(lat))
-                return fail(EMFD_ERR_INVALID, "the slab-interleaved kernel needs nx % 4 == 0");
-            EMFD_HIP(launch_yee_two_pass4_slabs(lat, ctx->fields(ctx->cur), ctx->tile_cfg > 0 ? ctx->tile_cfg : 32,
-                                                ctx->stream));
-        } else if (variant == EMFD_KERNEL_POINT) {
-            EMFD_HIP(launch_yee_point(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->stream));
-            ctx->cur ^= 1;
-        } else if (variant == EMFD_KERNEL_ZMARCH) {
-            EMFD_HIP(launch_yee_zmarch(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->tile_cfg,
-                                       ctx->stream));
-            ctx->cur ^= 1;
-        } else if (variant == EMFD_KERNEL_MARCH) {
+            EMFD_HIP(launch_yee_two_pass4(lat, ctx->fields(ctx->cur), ctx->stream));
+        } else {
             EMFD_HIP(launch_yee_march(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->tile_cfg,
                                       ctx->fix_b.as<uint32_t>(), ctx->n_fix_b, ctx->fix_e.as<uint32_t>(), ctx->n_fix_e,
-                                      ctx->stream));
-            ctx->cur ^= 1;
-        } else if (variant == EMFD_KERNEL_ROWS) {
-            EMFD_HIP(launch_yee_rows(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->tile_cfg, ctx->stream));
-            ctx->cur ^= 1;
-        } else {
-            EMFD_HIP(launch_yee_fused(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->tile_cfg,
                                       ctx->stream));
             ctx->cur ^= 1;
         }

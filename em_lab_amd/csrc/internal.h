@@ -118,13 +118,6 @@ struct StepConsts {
 
 // ---- launchers (kernels.hip) ----
 hipError_t launch_build_table(const uint16_t *keys, int classes, float dt, float *table, hipStream_t s);
-hipError_t launch_yee_fused(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int tile_cfg,
-                            hipStream_t s);
-hipError_t launch_yee_rows(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int zchunk, hipStream_t s);
-int yee_rows_supported(const LatticeArgs &a);
-hipError_t launch_yee_zmarch(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int zchunk,
-                             hipStream_t s);
-int yee_zmarch_supported(const LatticeArgs &a);
 // Fused one-pass z-march (yee_march.hip): src -> dst, then the sparse recomputation of the texels
 // in fix_b (B, thin-wire classes) and fix_e (E, non-vacuum classes and thin-wire neighbours).
 // cfg: z-chunk (0 = automatic).
@@ -132,10 +125,8 @@ hipError_t launch_yee_march(const LatticeArgs &a, const FieldPtrs &src, const Fi
                             const uint32_t *fix_b, int n_fix_b, const uint32_t *fix_e, int n_fix_e,
                             hipStream_t s);
 int yee_march_supported(const LatticeArgs &a);
-hipError_t launch_yee_point(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, hipStream_t s);
 hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
-hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, int streaming, hipStream_t s);
-hipError_t launch_yee_two_pass4_slabs(const LatticeArgs &a, const FieldPtrs &f, int slab, hipStream_t s);
+hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
 int yee_two_pass4_supported(const LatticeArgs &a);
 hipError_t launch_yee_half(const LatticeArgs &a, const FieldPtrs &f, int cells_per_thread, hipStream_t s);
 hipError_t launch_yee_f64(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);

@@ -19,7 +19,6 @@
 namespace emfd {
 namespace {
 
-constexpr int kXcds = 8;  // MI355X: 8 accelerator dies, each with its own L2
 
 // ---------------------------------------------------------------------------------------------
 // Coefficient table from binary16 keys: 12 halves per class
@@ -183,244 +182,6 @@ __global__ void __launch_bounds__(256) k_update_e(LatticeArgs a, FieldPtrs f) {
     for (int c = 0; c < 3; ++c) f.e[c][p] = ca[c] * f.e[c][p] + cb[c] * curl[c];
 }
 
-// ---------------------------------------------------------------------------------------------
-// Fused leapfrog: B^{n+1/2} and E^{n+1} in one pass, ping-pong buffers.
-//
-// A workgroup owns a TX x (TY*RY) tile of (x, y) columns and marches up a z-chunk. Per plane z it
-//   - holds E^n(z) of the tile plus a one-cell halo in LDS (double-buffered: E^n(z+1) is staged
-//     while plane z is consumed),
-//   - computes B^{n+1/2}(z) on the tile and on the low-x column / low-y row halo (recomputed, not
-//     exchanged), keeps it in LDS for the -x/-y neighbours of the E update, and carries the own
-//     texel's B(z) in registers as the -z neighbour of plane z+1,
-//   - updates E^{n+1}(z) from LDS and registers.
-// HBM traffic per texel is E and B read once and written once (48 B) plus the class id.
-// A z-chunk that does not start at 0 recomputes B at z0-1 first (one extra plane).
-// ---------------------------------------------------------------------------------------------
-template <int TX, int TY, int RY>
-struct FusedTile {
-    static constexpr int RYT = TY * RY;            // rows per tile
-    static constexpr int EW = TX + 2, EH = RYT + 2;  // E plane with halo on every side
-    static constexpr int BW = TX + 1, BH = RYT + 1;  // B plane with low-side halo
-    static constexpr int NT = TX * TY;
-    static constexpr int EHALO = 2 * TX + 2 * EH;  // top/bottom rows + left/right columns
-    static constexpr int BHALO = TX + RYT;         // low-y row + low-x column
-    static constexpr int EH_SLOTS = (EHALO + NT - 1) / NT;
-    static constexpr int BH_SLOTS = (BHALO + NT - 1) / NT;
-};
-
-template <int TX, int TY, int RY>
-__global__ void __launch_bounds__(TX *TY) k_yee_fused(LatticeArgs a, FieldPtrs src, FieldPtrs dst, int tiles_x,
-                                                       int tiles_y, int zchunk, int n_tiles) {
-    using T = FusedTile<TX, TY, RY>;
-    constexpr int RYT = T::RYT, EW = T::EW, EH = T::EH, BW = T::BW, NT = T::NT;
-    __shared__ float sE[2][3][EH][EW];
-    __shared__ float sB[3][T::BH][BW];
-
-    // XCD-aware tile order: consecutive blocks land on different XCDs, so give each XCD a
-    // contiguous run of tiles (neighbours then share that XCD's L2 for the halo reads).
-    const int per = (n_tiles + kXcds - 1) / kXcds;
-    const int tile = (static_cast<int>(blockIdx.x) % kXcds) * per + static_cast<int>(blockIdx.x) / kXcds;
-    if (tile >= n_tiles) return;
-    const int ti_x = tile % tiles_x;
-    const int ti_y = (tile / tiles_x) % tiles_y;
-    const int ti_z = tile / (tiles_x * tiles_y);
-    const int x0 = ti_x * TX, y0 = ti_y * RYT;
-    const int z0 = ti_z * zchunk;
-    const int z1 = min(z0 + zchunk, a.nz);
-    const int zs = z0 > 0 ? z0 - 1 : 0;
-
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int tid = tx + TX * ty;
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const size_t sy = static_cast<size_t>(nx), sz = static_cast<size_t>(nx) * ny;
-
-    // ---- E-region entry coordinates (own rows + halo) ----
-    // own: (ex, ey) = (tx+1, ty + TY r + 1); halo entries enumerated by h.
-    int ehx[T::EH_SLOTS], ehy[T::EH_SLOTS];
-#pragma unroll
-    for (int s = 0; s < T::EH_SLOTS; ++s) {
-        const int h = tid + s * NT;
-        int ex = -1, ey = -1;
-        if (h < TX) { ex = h + 1; ey = 0; }
-        else if (h < 2 * TX) { ex = h - TX + 1; ey = EH - 1; }
-        else if (h < T::EHALO) {
-            const int hh = h - 2 * TX;
-            ex = hh < EH ? 0 : EW - 1;
-            ey = hh % EH;
-        }
-        ehx[s] = ex;
-        ehy[s] = ey;
-    }
-    // ---- B halo entries: low-y row (bx in 1..TX, by 0), low-x column (bx 0, by 1..RYT) ----
-    int bhx[T::BH_SLOTS], bhy[T::BH_SLOTS];
-#pragma unroll
-    for (int s = 0; s < T::BH_SLOTS; ++s) {
-        const int h = tid + s * NT;
-        int bx = -1, by = -1;
-        if (h < TX) { bx = h + 1; by = 0; }
-        else if (h < T::BHALO) { bx = 0; by = h - TX + 1; }
-        bhx[s] = bx;
-        bhy[s] = by;
-    }
-
-    auto ld_e = [&](int c, int gx, int gy, int gz) -> float {
-        return (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz < nz) ? src.e[c][gz * sz + gy * sy + gx] : 0.0f;
-    };
-    auto ld_b = [&](int c, int gx, int gy, int gz) -> float {
-        return (gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? src.b[c][gz * sz + gy * sy + gx] : 0.0f;
-    };
-    auto ld_id = [&](int gx, int gy, int gz) -> uint32_t {
-        return (a.mat_id && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? load_id(a, gz * sz + gy * sy + gx) : 0u;
-    };
-
-    // registers: own E at z (current) and staged E at z+1, own B_old(z), halo E staged
-    float eo[RY][3], en[RY][3];
-    float bprev[RY][2];  // B^{n+1/2} at (own, z-1): x, y components (the -z neighbour)
-
-    // prologue: E plane zs into LDS buffer (zs & 1) and registers
-#pragma unroll
-    for (int r = 0; r < RY; ++r) {
-        const int gy = y0 + ty + TY * r;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            eo[r][c] = ld_e(c, x0 + tx, gy, zs);
-            sE[zs & 1][c][ty + TY * r + 1][tx + 1] = eo[r][c];
-        }
-        bprev[r][0] = bprev[r][1] = 0.0f;
-    }
-#pragma unroll
-    for (int s = 0; s < T::EH_SLOTS; ++s)
-        if (ehx[s] >= 0)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) sE[zs & 1][c][ehy[s]][ehx[s]] = ld_e(c, x0 + ehx[s] - 1, y0 + ehy[s] - 1, zs);
-
-    for (int z = zs; z < z1; ++z) {
-        const int cur = z & 1, nxt = cur ^ 1;
-        // (a) stage E(z+1) into LDS (zero beyond the top face)
-#pragma unroll
-        for (int r = 0; r < RY; ++r) {
-            const int gy = y0 + ty + TY * r;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                en[r][c] = ld_e(c, x0 + tx, gy, z + 1);
-                sE[nxt][c][ty + TY * r + 1][tx + 1] = en[r][c];
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < T::EH_SLOTS; ++s)
-            if (ehx[s] >= 0)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) sE[nxt][c][ehy[s]][ehx[s]] = ld_e(c, x0 + ehx[s] - 1, y0 + ehy[s] - 1, z + 1);
-        // B_old(z) for own texels and B-halo texels, class ids
-        float bo[RY][3];
-        uint32_t ido[RY];
-#pragma unroll
-        for (int r = 0; r < RY; ++r) {
-            const int gy = y0 + ty + TY * r;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) bo[r][c] = ld_b(c, x0 + tx, gy, z);
-            ido[r] = ld_id(x0 + tx, gy, z);
-        }
-        float bho[T::BH_SLOTS][3];
-        uint32_t idh[T::BH_SLOTS];
-#pragma unroll
-        for (int s = 0; s < T::BH_SLOTS; ++s) {
-            const int gx = x0 + bhx[s] - 1, gy = y0 + bhy[s] - 1;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) bho[s][c] = bhx[s] >= 0 ? ld_b(c, gx, gy, z) : 0.0f;
-            idh[s] = bhx[s] >= 0 ? ld_id(gx, gy, z) : 0u;
-        }
-        __syncthreads();  // #1: E(z+1) staged
-
-        // (d) B^{n+1/2}(z): own texels
-#pragma unroll
-        for (int r = 0; r < RY; ++r) {
-            const int ey = ty + TY * r + 1, ex = tx + 1;
-            const int gx = x0 + tx, gy = y0 + ty + TY * r;
-            float b[3] = {bo[r][0], bo[r][1], bo[r][2]};
-            if (gx < nx && gy < ny) {
-                float e0[3], exn[3], eyn[3], ezn[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    e0[c] = sE[cur][c][ey][ex];
-                    exn[c] = sE[cur][c][ey][ex + 1];
-                    eyn[c] = sE[cur][c][ey + 1][ex];
-                    ezn[c] = en[r][c];
-                }
-                faraday(a, gx, gy, z, e0, exn, eyn, ezn, ido[r], b);
-                if (z >= z0) {
-                    const size_t p = z * sz + gy * sy + gx;
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) dst.b[c][p] = b[c];
-                }
-            } else {
-                b[0] = b[1] = b[2] = 0.0f;
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) sB[c][ey][ex] = b[c];
-            bo[r][0] = b[0];
-            bo[r][1] = b[1];
-            bo[r][2] = b[2];
-        }
-        // halo texels (recomputed; zero outside the domain = "B below the low faces is 0")
-#pragma unroll
-        for (int s = 0; s < T::BH_SLOTS; ++s) {
-            if (bhx[s] < 0) continue;
-            const int bx = bhx[s], by = bhy[s];
-            const int gx = x0 + bx - 1, gy = y0 + by - 1;
-            float b[3] = {bho[s][0], bho[s][1], bho[s][2]};
-            if (gx >= 0 && gy >= 0 && gx < nx && gy < ny) {
-                float e0[3], exn[3], eyn[3], ezn[3];
-                // E index of texel (gx, gy) is (bx, by) in the E region
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    e0[c] = sE[cur][c][by][bx];
-                    exn[c] = sE[cur][c][by][bx + 1];
-                    eyn[c] = sE[cur][c][by + 1][bx];
-                    ezn[c] = sE[nxt][c][by][bx];
-                }
-                faraday(a, gx, gy, z, e0, exn, eyn, ezn, idh[s], b);
-            } else {
-                b[0] = b[1] = b[2] = 0.0f;
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) sB[c][by][bx] = b[c];
-        }
-        __syncthreads();  // #2: B(z) visible
-
-        // (f) E^{n+1}(z) for own texels
-        if (z >= z0) {
-#pragma unroll
-            for (int r = 0; r < RY; ++r) {
-                const int gx = x0 + tx, gy = y0 + ty + TY * r;
-                if (gx < nx && gy < ny) {
-                    const int by = ty + TY * r + 1, bx = tx + 1;
-                    const float b0x = bo[r][0], b0y = bo[r][1], b0z = bo[r][2];
-                    const float bxm_y = sB[1][by][bx - 1], bxm_z = sB[2][by][bx - 1];
-                    const float bym_x = sB[0][by - 1][bx], bym_z = sB[2][by - 1][bx];
-                    const float bzm_x = bprev[r][0], bzm_y = bprev[r][1];
-                    const float curl[3] = {(b0z - bym_z) - (b0y - bzm_y), (b0x - bzm_x) - (b0z - bxm_z),
-                                           (b0y - bxm_y) - (b0x - bym_x)};
-                    float ca[3], cb[3];
-                    edge_coefficients(a, gx, gy, z, ido[r], ca, cb);
-                    const size_t p = z * sz + gy * sy + gx;
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) dst.e[c][p] = ca[c] * eo[r][c] + cb[c] * curl[c];
-                }
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < RY; ++r) {
-            bprev[r][0] = bo[r][0];
-            bprev[r][1] = bo[r][1];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) eo[r][c] = en[r][c];
-        }
-        // next iteration writes sE[cur] (read in (d) above, before barrier #2) and sB (read in
-        // (f) above, before the next barrier #1): no third barrier needed.
-    }
-}
-
 // Per-texel class masks, bit x % 32 of word (z ny + y) wpr + x / 32: wire_mask marks texels whose
 // class carries a thin-wire factor other than 1 (only those need their class in the Faraday
 // update: a factor of exactly 1 leaves every product unchanged); class_mask marks texels whose
@@ -531,53 +292,6 @@ hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStre
     if (e != hipSuccess) return e;
     k_update_e<<<grid, block, 0, s>>>(a, f);
     return hipGetLastError();
-}
-
-namespace {
-template <int TX, int TY, int RY>
-hipError_t launch_fused_cfg(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int zchunk,
-                            hipStream_t s) {
-    const int tiles_x = (a.nx + TX - 1) / TX;
-    const int tiles_y = (a.ny + TY * RY - 1) / (TY * RY);
-    const int tiles_z = (a.nz + zchunk - 1) / zchunk;
-    const int n_tiles = tiles_x * tiles_y * tiles_z;
-    const int blocks = (n_tiles + kXcds - 1) / kXcds * kXcds;
-    dim3 block(TX, TY, 1);
-    k_yee_fused<TX, TY, RY><<<blocks, block, 0, s>>>(a, src, dst, tiles_x, tiles_y, zchunk, n_tiles);
-    return hipGetLastError();
-}
-}  // namespace
-
-// tile_cfg: low byte = tile shape id, upper bits = z-chunk (0 = choose automatically)
-hipError_t launch_yee_fused(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int tile_cfg,
-                            hipStream_t s) {
-    const int shape = tile_cfg & 0xff;
-    int zchunk = tile_cfg >> 8;
-    auto auto_chunk = [&](int tx, int ty) {
-        // aim for >= 4 workgroups per CU over 256 CUs
-        const long cols = static_cast<long>((a.nx + tx - 1) / tx) * ((a.ny + ty - 1) / ty);
-        long chunks = (2048 + cols - 1) / cols;
-        if (chunks < 1) chunks = 1;
-        int zc = static_cast<int>((a.nz + chunks - 1) / chunks);
-        return std::max(zc, 8);
-    };
-    switch (shape) {
-    case 1:
-        if (!zchunk) zchunk = auto_chunk(64, 8);
-        return launch_fused_cfg<64, 4, 2>(a, src, dst, zchunk, s);
-    case 2:
-        if (!zchunk) zchunk = auto_chunk(64, 16);
-        return launch_fused_cfg<64, 4, 4>(a, src, dst, zchunk, s);
-    case 3:
-        if (!zchunk) zchunk = auto_chunk(64, 8);
-        return launch_fused_cfg<64, 8, 1>(a, src, dst, zchunk, s);
-    case 4:
-        if (!zchunk) zchunk = auto_chunk(128, 4);
-        return launch_fused_cfg<128, 2, 2>(a, src, dst, zchunk, s);
-    default:
-        if (!zchunk) zchunk = auto_chunk(64, 8);
-        return launch_fused_cfg<64, 4, 2>(a, src, dst, zchunk, s);
-    }
 }
 
 hipError_t launch_apply_sources(const LatticeArgs &a, const FieldPtrs &f, const int *cells, int n_cells,

@@ -25,13 +25,8 @@ EMFD_ERR_STATE = 3
 LAYOUT_TEXEL = 0
 LAYOUT_SOA = 1
 KERNEL_AUTO = 0
-KERNEL_FUSED = 1
 KERNEL_TWO_PASS = 2
-KERNEL_ROWS = 3
-KERNEL_POINT = 4
 KERNEL_TWO_PASS4 = 5
-KERNEL_SLABS = 6
-KERNEL_ZMARCH = 7
 KERNEL_MARCH = 8
 STORAGE_F32 = 0
 STORAGE_F16 = 1
@@ -494,9 +489,7 @@ class SimulationGrid:
             return "k_yee_march (+k_march_fix_b, k_march_fix_e)"
         if v in (KERNEL_AUTO, KERNEL_TWO_PASS4) and self.size[0] % 4 == 0:
             return "k_update_b4+k_update_e4"
-        return {KERNEL_FUSED: "k_yee_fused", KERNEL_TWO_PASS: "k_update_b+k_update_e", KERNEL_ROWS: "k_yee_rows",
-                KERNEL_POINT: "k_yee_point", KERNEL_SLABS: "k_update_b4+k_update_e4 (z-slabs)",
-                KERNEL_ZMARCH: "k_yee_zmarch"}.get(v, "k_update_b+k_update_e")
+        return "k_update_b+k_update_e"
 
     @property
     def config(self) -> GridConfig:

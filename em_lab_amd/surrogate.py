@@ -378,7 +378,8 @@ def screening_run(n_candidates: int = 65536, top: int = 256, plain_designs: int 
                inference_s=round(t3 - t2, 3), verify_s=round(t4 - t3, 3),
                designs_per_s=round(n_candidates / screen_s, 2), plain_fdtd_designs_per_s=round(plain, 3),
                speedup=round(n_candidates / screen_s / plain, 1),
-               accepted=int(np.sum(err <= 0.5)), s11_err_median_db=round(float(np.median(err)), 3),
+               pick_precision=round(float(np.mean(err <= 0.5)), 4), picks_within_0p5db=int(np.sum(err <= 0.5)),
+               s11_err_median_db=round(float(np.median(err)), 3),
                s11_err_max_db=round(float(np.max(err)), 3),
                verified_bw_mean=round(float(np.mean(bw_fdtd)), 4), predicted_bw_mean=round(float(np.mean(bw_pred[idx])), 4),
                surrogate_members=members, forward_ms_65536=round(ms_fwd, 4),

@@ -43,18 +43,9 @@ extern "C" {
 
 #define EMFD_KERNEL_AUTO 0     /* fastest variant that supports the grid (default): EMFD_KERNEL_MARCH for FP32
                                    lattices of at least 8e6 cells, else the vectorised two-pass kernel */
-#define EMFD_KERNEL_FUSED 1    /* fused B+E, LDS-tiled z-march (ping-pong buffers) */
 #define EMFD_KERNEL_TWO_PASS 2 /* separate update_B / update_E passes, in place */
-#define EMFD_KERNEL_ROWS 3     /* fused B+E, one wavefront per x-row z-march (nx <= 256) */
-#define EMFD_KERNEL_POINT 4    /* fused B+E, one thread per texel, neighbour B rebuilt */
-#define EMFD_KERNEL_TWO_PASS4 5 /* two passes, four cells per thread with float4 streams (nx % 4 == 0); bits
-                                   8..10 = 1: non-temporal stores, 2: also non-temporal loads of the
-                                   plane each pass reads once (AUTO uses 2), 3: also class ids;
-                                   4 / 6: 0 / 2 with XCD-contiguous block order */
-#define EMFD_KERNEL_SLABS 6     /* TWO_PASS4 interleaved per z-slab (B, E on slab 0, then slab 1, ...); bits 8..
-                                   of the variant = slab thickness in planes (default 32) */
-#define EMFD_KERNEL_ZMARCH 7    /* fused B+E, z-march of 4-row groups with LDS row exchange and a helper
-                                   wave (nx <= 256, nx % 4 == 0); bits 8.. = z-chunk (0 = automatic) */
+#define EMFD_KERNEL_TWO_PASS4 5 /* two passes, four cells per thread with float4 streams and non-temporal
+                                   hints (nx % 4 == 0) */
 #define EMFD_KERNEL_MARCH 8     /* fused B+E in one pass (ping-pong buffers, nx % 4 == 0, planes of at most
                                    2 GB): z-march of 256-cell x segments of 7 rows with one workgroup barrier
                                    per plane, vacuum and absorber coefficients in the march, then sparse
@@ -233,7 +224,8 @@ int emfd_grid_upload_fields(emfd_ctx *ctx, const float *E, const float *B, int l
 /* Device bytes held by the grid (fields, coefficients, source lists). */
 int64_t emfd_grid_memory_bytes(emfd_ctx *ctx);
 /* Choose the Yee kernel variant: EMFD_KERNEL_* in the low byte; bits 8.. carry a variant
- * parameter (FUSED: tile shape | z-chunk << 8; ROWS: z-chunk), 0 = automatic. */
+ * parameter (MARCH: z-chunk; 16-bit storage: cells per thread), 0 = automatic. Other values:
+ * EMFD_ERR_INVALID. */
 int emfd_grid_set_kernel(emfd_ctx *ctx, int variant);
 /* Field storage (no counterpart in the reference, which keeps RGBA32F textures): EMFD_STORAGE_*
  * with scale_log2[6] = k per component (E x y z, B x y z; NULL = all 0), the stored value being

@@ -130,7 +130,7 @@ def test_storage_roundtrip_and_errors():
     scale = np.array([2.0, 1.0, 0.5], np.float32)[:, None]
     assert np.array_equal(e1, (e0 / scale).astype(np.float16).astype(np.float32) * scale)
     with pytest.raises(E.EmfdError):
-        g.set_kernel(E.KERNEL_ROWS)
+        g.set_kernel(E.KERNEL_MARCH)
         g.step(1)
     g.set_kernel(E.KERNEL_AUTO)
     g.step(2)
