@@ -86,6 +86,26 @@ def build_oracle() -> Path:
     return root / "oracle" / "_build" / "liboracle.so"
 
 
+INTEGRATION_SRC = PKG / "integration" / "simulation_grid.cpp"
+INTEGRATION_BIN = LIB_DIR / "simulation_grid"
+
+
+def build_integration() -> Path:
+    """Compile the C++ binding example (a host program calling the C ABI) against
+    include/emfdtd.h and libemfdtd.so; it finds the library next to itself."""
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if not cxx:
+        raise RuntimeError("no host C++ compiler for the binding example")
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", str(PKG.parent / "include"), str(INTEGRATION_SRC),
+           "-o", str(INTEGRATION_BIN), "-L", str(LIB_DIR), "-lemfdtd", "-Wl,-rpath,$ORIGIN",
+           "-Wl,--allow-shlib-undefined"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("binding example build failed:\n" + r.stdout)
+    return INTEGRATION_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
     print(build_oracle())
+    print(build_integration())

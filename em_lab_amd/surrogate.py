@@ -415,7 +415,25 @@ def main(argv=None) -> int:
     sc.add_argument("--candidates", type=int, default=65536)
     sc.add_argument("--top", type=int, default=256)
     sc.add_argument("--weights", type=Path, default=WEIGHTS)
+    fw = sub.add_parser("forward", help="inference only: the surrogate on seeded candidates, timed")
+    fw.add_argument("--candidates", type=int, default=65536)
+    fw.add_argument("--iterations", type=int, default=10)
+    fw.add_argument("--weights", type=Path, default=WEIGHTS)
     args = ap.parse_args(argv)
+    if args.cmd == "forward":
+        t = time.time()
+        _, _, X = candidates(args.candidates)
+        print(f"{args.candidates} candidates in {time.time() - t:.1f} s", flush=True)
+        m = load_surrogate(args.weights, max_batch=args.candidates)
+        members = len(m.members) if isinstance(m, Ensemble) else 1
+        flop = members * 2 * args.candidates * sum(i * o for i, o in zip(DIMS[:-1], DIMS[1:]))
+        for it in range(args.iterations):
+            t = time.time()
+            m.forward(X)
+            dt = time.time() - t
+            print(f"forward {it}: {dt * 1e3:.2f} ms ({flop / dt / 1e12:.1f} TFLOP/s with the host copies)", flush=True)
+        m.close()
+        return 0
     if args.cmd == "train":
         X, Z, _ = load_data(args.data)
         if args.members > 1:

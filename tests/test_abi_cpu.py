@@ -38,3 +38,16 @@ def test_library_exports_every_declared_symbol():
 def test_python_binding_signatures_cover_header():
     from em_lab_amd import emfdtd
     assert set(declared_functions()) == set(emfdtd._SIGNATURES)
+
+
+def test_binding_example_builds_and_links():
+    """The integration guide's C++ binding compiles against the header, links the library and
+    runs up to its argument check (no device call)."""
+    import subprocess
+    from em_lab_amd import build
+    lib_path = ROOT / "em_lab_amd" / "_lib" / "libemfdtd.so"
+    if not lib_path.exists():
+        build.build()
+    exe = build.build_integration()
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr

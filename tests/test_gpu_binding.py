@@ -1,0 +1,74 @@
+"""The C++ binding of the integration guide (em_lab_amd/integration/simulation_grid.cpp: the
+SimulationGrid, PortMonitor and FarFieldMonitor bodies a maintainer would add to EM_Lab, compiled
+against include/emfdtd.h) run as a separate program on a 32^3 scene with a thin-wire dipole, a
+dielectric sphere, ports with a sweep and the far field; everything it writes equals what the
+Python (ctypes) path computes for the same scene, bit for bit."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from em_lab_amd import build as B
+from em_lab_amd import emfdtd as E
+from em_lab_amd.scene import spheres_dipole
+
+from gpu_helpers import gpu_grid
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 300
+
+
+def write_bundle(sc, d: Path, steps: int) -> None:
+    """The scene as the program reads it: the ABI structs and arrays, raw."""
+    d.mkdir(parents=True, exist_ok=True)
+    (d / "grid.bin").write_bytes(bytes(sc.grid_config().to_c()))
+    s, e = sc.flat_materials()
+    (d / "sigma.f32").write_bytes(np.ascontiguousarray(s, np.float32).tobytes())
+    (d / "eps.f32").write_bytes(np.ascontiguousarray(e, np.float32).tobytes())
+    (d / "edges.bin").write_bytes(sc.subcell.edge_array().tobytes())
+    (d / "wires.bin").write_bytes(sc.subcell.wire_array().tobytes())
+    (d / "sources.bin").write_bytes(b"".join(bytes(x.to_c()) for x in sc.sources))
+    (d / "ports.bin").write_bytes(b"".join(bytes(p.to_c()) for p in sc.ports))
+    if sc.sweep is not None:
+        sw = sc.sweep
+        (d / "sweep.bin").write_bytes(bytes(E.SweepSetupC(sw.wavelength_min, sw.wavelength_max, sw.points)))
+    (d / "farfield.bin").write_bytes(bytes(sc.far_field.to_c()))
+    (d / "run.txt").write_text(f"{steps} {sc.far_field.averaging_periods!r} {sc.cell_size!r}\n")
+
+
+def test_cpp_binding_matches_ctypes(tmp_path):
+    print("case: C++ binding program, 32^3 spheres + dipole, 300 steps", flush=True)
+    exe = B.INTEGRATION_BIN
+    assert exe.exists(), "build the binding example first (em_lab_amd.build.build_integration)"
+    sc = spheres_dipole(n=32, pml=4, n_spheres=2, wavelength=12.0, radius_range=(2.0, 3.0), sweep_points=16)
+    write_bundle(sc, tmp_path / "in", STEPS)
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([str(exe), str(tmp_path / "in"), str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+
+    g, pm, ff = gpu_grid(sc)
+    g.step(STEPS // 2)
+    g.step(STEPS - STEPS // 2)
+    Eg, Bg = g.fields(E.LAYOUT_SOA)
+    assert np.abs(Eg).max() > 0
+    assert np.array_equal(np.fromfile(out / "E.f32", np.float32), Eg.ravel())
+    assert np.array_equal(np.fromfile(out / "B.f32", np.float32), Bg.ravel())
+    assert np.array_equal(np.fromfile(out / "port_phasors.f32", np.float32), pm.phasors().ravel())
+    assert np.array_equal(np.fromfile(out / "port_series.f32", np.float32), pm.read_series().ravel())
+    rd = pm.evaluate()[0]
+    want = [float(rd.valid), rd.z_ohm.real, rd.z_ohm.imag, rd.gamma.real, rd.gamma.imag, rd.vswr,
+            rd.return_loss_db, rd.accepted_power, rd.available_power]
+    assert np.array_equal(np.fromfile(out / "readings.f64", np.float64), np.array(want))
+    sw = pm.evaluate_sweep(g.dt)
+    assert np.array_equal(np.fromfile(out / "sweep.f64", np.float64).reshape(-1, 5), sw["ports"][0]["points"])
+    assert np.array_equal(np.fromfile(out / "ntff_phasors.f32", np.float32), ff.phasors().ravel())
+    res = ff.evaluate()
+    valid, power, dbi = (out / "pattern.txt").read_text().split()
+    assert int(valid) == int(res["valid"]) == 1
+    assert float(power) == res["radiated_power"]
+    assert np.float32(float(dbi)) == np.float32(res["max_directivity_dbi"])
+    g.close()
