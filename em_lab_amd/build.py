@@ -16,7 +16,7 @@ LIB_DIR = PKG / "_lib"
 LIB = LIB_DIR / "libemfdtd.so"
 OBJ_DIR = PKG / "_build"
 
-SOURCES = ["kernels.hip", "yee_two_pass4.hip", "yee_march.hip", "yee_half.hip", "yee_f64.hip", "monitors.hip", "farfield.hip", "dft.hip", "noalloc.hip", "batch.hip", "mlp.hip",
+SOURCES = ["kernels.hip", "yee_two_pass4.hip", "yee_march.hip", "yee_half.hip", "yee_f64.hip", "monitors.hip", "farfield.hip", "dft.hip", "noalloc.hip", "cpml.hip", "batch.hip", "mlp.hip",
            "grid.cpp", "monitors.cpp", "batch.cpp", "mlp.cpp"]
 
 # -ffp-contract=off: every expression rounds as the reference shader writes it (no fused

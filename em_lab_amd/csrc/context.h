@@ -119,6 +119,11 @@ struct emfd_ctx {
     // reads a thin-wire texel's B (texel itself, +x, +y, +z)
     emfd::DevBuf fix_b, fix_e;
     int n_fix_b = 0, n_fix_e = 0;
+    // absorber: EMFD_ABSORBER_GRADED (the reference's) or EMFD_ABSORBER_CPML, with the latter's
+    // coefficient tables and auxiliary fields
+    int absorber = 0;
+    float cpml_sigma_max = 0.0f, cpml_grading = 0.0f, cpml_alpha_max = 0.0f;
+    emfd::DevBuf d_cpml_coef, d_psi;
     int prof_stride = 0;
 
     // sources

@@ -64,6 +64,13 @@ uint16_t f32_to_f16(float f) {
 // SourceDesc::signal (src/SimulationGrid.cpp:99-130), double precision
 double source_signal(const emfd_source_desc &s, double tAbs) { return source_signal_d(s, tAbs); }
 
+// The convolutional PML's auxiliary fields back to 0 (no-op with the graded absorber).
+int clear_psi(emfd_ctx *ctx) {
+    if (ctx->absorber != EMFD_ABSORBER_CPML || !ctx->d_psi.ptr) return EMFD_OK;
+    EMFD_HIP(launch_fill(ctx->d_psi.as<float>(), ctx->d_psi.bytes / sizeof(float), 0.0f, ctx->stream));
+    return EMFD_OK;
+}
+
 emfd_grid_config clamp_config(const emfd_grid_config &requested) {
     emfd_grid_config c = requested;
     c.pml_thickness = std::clamp(c.pml_thickness, 0, 64);
@@ -374,7 +381,8 @@ void emfd_destroy(emfd_ctx *ctx) {
                       &ctx->ntff.d_out, &ctx->ntff.d_power, &ctx->scratch_partials, &ctx->scratch_misc,
                       &ctx->lines.d_quant, &ctx->lines.d_terms, &ctx->lines.d_weights, &ctx->lines.d_phasors,
                       &ctx->lines.d_now, &ctx->ntff.d_ahi, &ctx->ntff.d_alo, &ctx->ntff.d_absmax, &ctx->fix_b,
-                      &ctx->fix_e, &ctx->ports.d_dft, &ctx->ntff.d_dft, &ctx->ntff.d_dft_ph, &ctx->d_consts64};
+                      &ctx->fix_e, &ctx->ports.d_dft, &ctx->ntff.d_dft, &ctx->ntff.d_dft_ph, &ctx->d_consts64,
+                      &ctx->d_cpml_coef, &ctx->d_psi};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t ev : ctx->ev_pool) (void)hipEventDestroy(ev);
     for (int i = 0; i < 2; ++i) {
@@ -411,6 +419,7 @@ int emfd_grid_configure(emfd_ctx *ctx, const emfd_grid_config *requested, emfd_g
     ctx->configured = true;
     ctx->cur = 0;
 
+    ctx->absorber = EMFD_ABSORBER_GRADED;
     int stride = 0;
     const std::vector<float> prof = absorber_profiles(c, &stride);
     ctx->prof_stride = stride;
@@ -552,6 +561,7 @@ int emfd_grid_upload_materials(emfd_ctx *ctx, const float *sigma, const float *e
     // uploadMaterials clears E and B and rewinds time (src/SimulationGrid.cpp:216-222,294)
     ctx->cur = 0;
     EMFD_HIP(hipMemsetAsync(ctx->field_buf[0].ptr, 0, ctx->field_bytes(), ctx->stream));
+    if (int rc = clear_psi(ctx)) return rc;
     ctx->step_count = 0;
     EMFD_HIP(hipStreamSynchronize(ctx->stream));
     return EMFD_OK;
@@ -597,6 +607,7 @@ int emfd_grid_reset_fields(emfd_ctx *ctx) {
     EMFD_HIP(hipSetDevice(ctx->device));
     ctx->cur = 0;
     EMFD_HIP(launch_fill(ctx->field_buf[0].as<float>(), ctx->field_bytes() / 4, 0.0f, ctx->stream));
+    if (int rc = clear_psi(ctx)) return rc;
     ctx->step_count = 0;
     return EMFD_OK;
 }
@@ -609,6 +620,8 @@ int emfd_grid_set_storage(emfd_ctx *ctx, int storage, const int *scale_log2) {
         return fail(EMFD_ERR_INVALID, "unknown field storage");
     if ((storage == EMFD_STORAGE_F16 || storage == EMFD_STORAGE_BF16) && ctx->cfg.size[0] % 8 != 0)
         return fail(EMFD_ERR_INVALID, "16-bit field storage needs nx % 8 == 0");
+    if (storage != EMFD_STORAGE_F32 && ctx->absorber == EMFD_ABSORBER_CPML)
+        return fail(EMFD_ERR_STATE, "the convolutional PML runs with FP32 field storage only");
     int k[6] = {0, 0, 0, 0, 0, 0};
     if (scale_log2)
         for (int c = 0; c < 6; ++c) {
@@ -626,6 +639,68 @@ int emfd_grid_set_storage(emfd_ctx *ctx, int storage, const int *scale_log2) {
     // as uploadMaterials: fields cleared, time rewound
     ctx->cur = 0;
     EMFD_HIP(hipMemsetAsync(ctx->field_buf[0].ptr, 0, ctx->field_bytes(), ctx->stream));
+    if (int rc = clear_psi(ctx)) return rc;
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->step_count = 0;
+    return EMFD_OK;
+}
+
+int emfd_grid_set_absorber(emfd_ctx *ctx, int kind, float sigma_max, float grading, float alpha_max) {
+    if (!ctx) return fail(EMFD_ERR_INVALID, "null context");
+    if (!ctx->configured) return fail(EMFD_ERR_STATE, "grid not configured");
+    if (kind != EMFD_ABSORBER_GRADED && kind != EMFD_ABSORBER_CPML) return fail(EMFD_ERR_INVALID, "unknown absorber");
+    if (kind == EMFD_ABSORBER_CPML && ctx->storage != kStorageF32)
+        return fail(EMFD_ERR_STATE, "the convolutional PML runs with FP32 field storage only");
+    if (!(sigma_max == sigma_max) || !(grading == grading) || !(alpha_max == alpha_max))
+        return fail(EMFD_ERR_INVALID, "NaN absorber parameter");
+    EMFD_HIP(hipSetDevice(ctx->device));
+    EMFD_HIP(hipStreamSynchronize(ctx->stream));
+    // the graded absorber's profiles: the configured ones, or none under the convolutional PML
+    emfd_grid_config c = ctx->cfg;
+    if (kind == EMFD_ABSORBER_CPML) c.pml_sigma_max = 0.0f;
+    int stride = 0;
+    const std::vector<float> prof = absorber_profiles(c, &stride);
+    EMFD_HIP(hipMemcpy(ctx->prof.ptr, prof.data(), prof.size() * sizeof(float), hipMemcpyHostToDevice));
+    ctx->absorber = kind;
+    EMFD_HIP(launch_build_pml_table(ctx->lattice(), ctx->pml_tab.as<float>(), ctx->stream));
+    ctx->src_recs_dirty = true;  // the sources' Cb in the absorber follow its conductivity
+    if (kind == EMFD_ABSORBER_CPML) {
+        const int L = ctx->cfg.pml_thickness;
+        const double m = grading > 0.0f ? grading : 3.0;
+        const double smax = sigma_max > 0.0f ? sigma_max : 0.8 * (m + 1.0);
+        const double amax = alpha_max > 0.0f ? alpha_max : 0.0;
+        ctx->cpml_sigma_max = static_cast<float>(smax);
+        ctx->cpml_grading = static_cast<float>(m);
+        ctx->cpml_alpha_max = static_cast<float>(amax);
+        // [axis][E position i / B position i + 1/2][i][b, a], as the auxiliary-field recursion reads it
+        const int n[3] = {ctx->cfg.size[0], ctx->cfg.size[1], ctx->cfg.size[2]};
+        const double dt = static_cast<double>(ctx->cfg.courant);
+        std::vector<float> coef(static_cast<size_t>(3) * 2 * stride * 2);
+        for (int u = 0; u < 3; ++u)
+            for (int kind2 = 0; kind2 < 2; ++kind2)
+                for (int i = 0; i < stride; ++i) {
+                    float *e = &coef[(static_cast<size_t>(u * 2 + kind2) * stride + i) * 2];
+                    e[0] = 1.0f;
+                    e[1] = 0.0f;
+                    if (i >= n[u] || L <= 0) continue;
+                    const double x = i + (kind2 ? 0.5 : 0.0), hi = (n[u] - 1.0) - L;
+                    double d = std::max(L - x, x - hi) / L;
+                    if (d <= 0.0) continue;
+                    d = std::min(d, 1.0);
+                    const double sg = smax * std::pow(d, m), al = amax * (1.0 - d);
+                    const double b = std::exp(-(sg + al) * dt);
+                    e[0] = static_cast<float>(b);
+                    e[1] = sg + al > 0.0 ? static_cast<float>(sg / (sg + al) * (b - 1.0)) : 0.0f;
+                }
+        if (int rc = ctx->d_cpml_coef.ensure(coef.size() * sizeof(float))) return rc;
+        EMFD_HIP(hipMemcpy(ctx->d_cpml_coef.ptr, coef.data(), coef.size() * sizeof(float), hipMemcpyHostToDevice));
+        const size_t nf = cpml_psi_floats(n[0], n[1], n[2], L);
+        if (int rc = ctx->d_psi.ensure(std::max<size_t>(nf, 1) * sizeof(float))) return rc;
+    }
+    // as uploadMaterials: fields (and auxiliary fields) cleared, time rewound
+    ctx->cur = 0;
+    EMFD_HIP(launch_fill(ctx->field_buf[0].as<float>(), ctx->field_bytes() / 4, 0.0f, ctx->stream));
+    if (int rc = clear_psi(ctx)) return rc;
     EMFD_HIP(hipStreamSynchronize(ctx->stream));
     ctx->step_count = 0;
     return EMFD_OK;
@@ -684,7 +759,8 @@ namespace {
 // 0.159 against 0.174 ms per Yee step for the two passes, at 768^3 3.83 against 5.76 ms); below
 // that its few long workgroups leave the GPU part idle (128^3: 0.046 against 0.036 ms).
 bool march_auto(const emfd_ctx *ctx) {
-    return ctx->storage == kStorageF32 && ctx->n >= 8000000u && yee_march_supported(ctx->lattice());
+    return ctx->storage == kStorageF32 && ctx->absorber == EMFD_ABSORBER_GRADED && ctx->n >= 8000000u &&
+           yee_march_supported(ctx->lattice());
 }
 }  // namespace
 
@@ -787,8 +863,22 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
             return fail(EMFD_ERR_INVALID, "the vectorised two-pass kernel needs nx % 4 == 0");
         if (variant == EMFD_KERNEL_MARCH && !yee_march_supported(lat))
             return fail(EMFD_ERR_INVALID, "the one-pass march kernel needs nx % 4 == 0 and planes of at most 2 GB");
+        if (variant == EMFD_KERNEL_MARCH && ctx->absorber == EMFD_ABSORBER_CPML)
+            return fail(EMFD_ERR_INVALID, "the convolutional PML runs with the two-pass kernels");
         if (variant == -1) {
             // launched above
+        } else if (ctx->absorber == EMFD_ABSORBER_CPML) {
+            // Faraday pass, its auxiliary fields, Ampere pass, its auxiliary fields
+            const FieldPtrs f = ctx->fields(ctx->cur);
+            const float *coef = ctx->d_cpml_coef.as<float>();
+            float *psi = ctx->d_psi.as<float>();
+            for (int ph = 0; ph < 2; ++ph) {
+                if (variant == EMFD_KERNEL_TWO_PASS4)
+                    EMFD_HIP(launch_yee_two_pass4(lat, f, ctx->stream, 1 << ph));
+                else
+                    EMFD_HIP(launch_yee_two_pass(lat, f, ctx->stream, 1 << ph));
+                EMFD_HIP(launch_cpml(lat, f, coef, psi, ph == 1, ctx->stream));
+            }
         } else if (variant == EMFD_KERNEL_TWO_PASS) {
             EMFD_HIP(launch_yee_two_pass(lat, ctx->fields(ctx->cur), ctx->stream));
         } else if (variant == EMFD_KERNEL_TWO_PASS4) {

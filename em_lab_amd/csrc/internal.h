@@ -125,8 +125,16 @@ hipError_t launch_yee_march(const LatticeArgs &a, const FieldPtrs &src, const Fi
                             const uint32_t *fix_b, int n_fix_b, const uint32_t *fix_e, int n_fix_e,
                             hipStream_t s);
 int yee_march_supported(const LatticeArgs &a);
-hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
-hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);
+// passes: bit 0 the Faraday (B) pass, bit 1 the Ampere (E) pass
+hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s, int passes = 3);
+hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s, int passes = 3);
+// Convolutional PML (cpml.hip): the auxiliary fields' update and correction of one phase (B after
+// the Faraday pass, E after the Ampere pass) in the absorber slabs of every axis, x then y then z.
+// coef: [axis][E position i / B position i + 1/2][i][b, a] over prof_stride entries; psi: per axis
+// 4 components (B_v, B_w, E_v, E_w) over the slab of 2 pml + 1 planes, x fastest.
+hipError_t launch_cpml(const LatticeArgs &a, const FieldPtrs &f, const float *coef, float *psi, bool e_phase,
+                       hipStream_t s);
+size_t cpml_psi_floats(int nx, int ny, int nz, int pml);
 int yee_two_pass4_supported(const LatticeArgs &a);
 hipError_t launch_yee_half(const LatticeArgs &a, const FieldPtrs &f, int cells_per_thread, hipStream_t s);
 hipError_t launch_yee_f64(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);

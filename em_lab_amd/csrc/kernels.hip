@@ -285,12 +285,14 @@ hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s) {
+hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s, int passes) {
     dim3 block(64, 4, 1), grid((a.nx + 63) / 64, (a.ny + 3) / 4, a.nz);
-    k_update_b<<<grid, block, 0, s>>>(a, f);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    k_update_e<<<grid, block, 0, s>>>(a, f);
+    if (passes & 1) {
+        k_update_b<<<grid, block, 0, s>>>(a, f);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (passes & 2) k_update_e<<<grid, block, 0, s>>>(a, f);
     return hipGetLastError();
 }
 

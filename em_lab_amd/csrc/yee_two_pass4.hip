@@ -26,12 +26,15 @@ __global__ void __launch_bounds__(256) k_update_e4(LatticeArgs a, FieldPtrs f, i
 }
 
 template <int ST>
-hipError_t launch_st(const LatticeArgs &a, const FieldPtrs &f, int z0, int nzs, hipStream_t s) {
+hipError_t launch_st(const LatticeArgs &a, const FieldPtrs &f, int z0, int nzs, hipStream_t s, int passes) {
     dim3 block(64, 4, 1), grid((a.nx / 4 + 63) / 64, (a.ny + 3) / 4, nzs);
     const size_t lds = static_cast<size_t>(4 * coef_entries(a, 1) * kCoefStride) * sizeof(float);
-    k_update_b4<ST><<<grid, block, lds, s>>>(a, f, z0);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (passes & 1) {
+        k_update_b4<ST><<<grid, block, lds, s>>>(a, f, z0);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (!(passes & 2)) return hipSuccess;
     if (a.pml_materials)
         k_update_e4<ST, true><<<grid, block, lds, s>>>(a, f, z0);
     else
@@ -45,9 +48,9 @@ int yee_two_pass4_supported(const LatticeArgs &a) { return a.nx % 4 == 0 ? 1 : 0
 
 // Non-temporal hints on the stores and on the loads of the plane each pass reads once (ST = 2 of
 // yee_quad.h's streaming levels, the fastest measured).
-hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s) {
+hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s, int passes) {
     if (!yee_two_pass4_supported(a)) return hipErrorInvalidValue;
-    return launch_st<2>(a, f, 0, a.nz, s);
+    return launch_st<2>(a, f, 0, a.nz, s, passes);
 }
 
 }  // namespace emfd

@@ -28,6 +28,8 @@ KERNEL_AUTO = 0
 KERNEL_TWO_PASS = 2
 KERNEL_TWO_PASS4 = 5
 KERNEL_MARCH = 8
+ABSORBER_GRADED = 0
+ABSORBER_CPML = 1
 STORAGE_F32 = 0
 STORAGE_F16 = 1
 STORAGE_BF16 = 2
@@ -131,6 +133,7 @@ _SIGNATURES = {
     "emfd_grid_memory_bytes": (C.c_int64, [_P]),
     "emfd_grid_set_kernel": (C.c_int, [_P, C.c_int]),
     "emfd_grid_material_classes": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "emfd_grid_set_absorber": (C.c_int, [_P, C.c_int, C.c_float, C.c_float, C.c_float]),
     "emfd_grid_set_storage": (C.c_int, [_P, C.c_int, _P]),
     "emfd_grid_download_fields_f64": (C.c_int, [_P, _P, _P]),
     "emfd_grid_kernel_timing": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
@@ -407,6 +410,7 @@ class SimulationGrid:
         applied = GridConfigC()
         _check(self._lib.emfd_grid_configure(self._h, C.byref(config.to_c()), C.byref(applied)))
         self._config = GridConfig.from_c(applied)
+        self._absorber = ABSORBER_GRADED
         return self._config
 
     def upload_materials(self, sigma: np.ndarray, eps_r: np.ndarray, subcell: Optional[SubcellModel] = None) -> None:
@@ -437,6 +441,13 @@ class SimulationGrid:
     def set_kernel(self, variant: int) -> None:
         _check(self._lib.emfd_grid_set_kernel(self._h, int(variant)))
         self._variant = int(variant)
+
+    def set_absorber(self, kind: int, sigma_max: float = 0.0, grading: float = 0.0, alpha_max: float = 0.0) -> None:
+        """ABSORBER_GRADED (the reference's) or ABSORBER_CPML (convolutional PML in the same slabs;
+        0 picks a parameter's default). Clears the fields and rewinds time."""
+        _check(self._lib.emfd_grid_set_absorber(self._h, int(kind), float(sigma_max), float(grading),
+                                                float(alpha_max)))
+        self._absorber = int(kind)
 
     def set_storage(self, storage: int, scale_log2=None) -> None:
         """Field storage STORAGE_F32 / STORAGE_F16 / STORAGE_BF16 with a power-of-two scale per
@@ -485,7 +496,8 @@ class SimulationGrid:
             return "k_update_bh+k_update_eh"
         nx, ny, nz = self.size
         if v == KERNEL_MARCH or (v == KERNEL_AUTO and nx * ny * nz >= 8_000_000 and nx % 4 == 0
-                                 and nx * ny * nz * 4 <= 0x80000000):
+                                 and nx * ny * nz * 4 <= 0x80000000
+                                 and getattr(self, "_absorber", ABSORBER_GRADED) == ABSORBER_GRADED):
             return "k_yee_march (+k_march_fix_b, k_march_fix_e)"
         if v in (KERNEL_AUTO, KERNEL_TWO_PASS4) and self.size[0] % 4 == 0:
             return "k_update_b4+k_update_e4"

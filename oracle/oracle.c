@@ -602,6 +602,163 @@ void or_step(const or_grid *g, float *E, float *B, const uint32_t *edge_mat, con
 }
 
 /* ------------------------------------------------------------------------------------------
+ * Convolutional PML (a library extension with no reference counterpart; the reference has the
+ * graded absorber only). The textbook recursive-convolution form: J. A. Roden and S. D. Gedney,
+ * Microwave Opt. Technol. Lett. 27(5), 334-339 (2000), with the complex-frequency-shifted
+ * stretch s = kappa + sigma / (alpha + j w) (M. Kuzuoglu and R. Mittra, IEEE MGWL 6(12), 1996),
+ * kappa = 1. In units where eps0 = mu0 = 1 and the cell is 1:
+ *   b = exp(-(sigma + alpha) dt),  a = sigma / (sigma + alpha) (b - 1),
+ *   psi^{n+1} = b psi^n + a (d_u F),  and the update's d_u F gains + psi.
+ * Profiles over the normalised depth d in (0, 1] of the graded absorber's slabs:
+ * sigma = sigma_max d^m, alpha = alpha_max (1 - d); 0 (b = 1, a = 0) outside.
+ * Each axis u holds psi for the S = 2L + 1 lattice planes i = 0 .. L-1 and n-1-L .. n-1 (the
+ * half-integer B positions of plane n-1-L are inside the absorber): 4 components (B_v, B_w, E_v,
+ * E_w with v = u+1, w = u+2 mod 3) over the slab's cells, x fastest.
+ * ------------------------------------------------------------------------------------------ */
+static int cpml_dims(const or_grid *g, int u, int dims[3]) {
+    const int n[3] = {g->nx, g->ny, g->nz};
+    for (int a = 0; a < 3; ++a) dims[a] = n[a];
+    dims[u] = 2 * g->pml + 1;
+    return dims[0] * dims[1] * dims[2];
+}
+
+static int cpml_plane(const or_grid *g, int u, int s) {
+    const int n[3] = {g->nx, g->ny, g->nz};
+    return s < g->pml ? s : n[u] - 1 - 2 * g->pml + s;
+}
+
+/* coef[((u * 2 + kind) * stride + i) * 2 + (0: b, 1: a)], kind 0 = E position i, 1 = B position
+ * i + 1/2; stride = max(nx, ny, nz). */
+void or_cpml_coefficients(const or_grid *g, double sigma_max, double grading, double alpha_max, float *coef) {
+    const int n[3] = {g->nx, g->ny, g->nz};
+    const int stride = n[0] > n[1] ? (n[0] > n[2] ? n[0] : n[2]) : (n[1] > n[2] ? n[1] : n[2]);
+    const double L = (double)g->pml, dt = (double)g->dt;
+    for (int u = 0; u < 3; ++u)
+        for (int kind = 0; kind < 2; ++kind)
+            for (int i = 0; i < stride; ++i) {
+                float *c = &coef[((size_t)(u * 2 + kind) * stride + i) * 2];
+                c[0] = 1.0f;
+                c[1] = 0.0f;
+                if (i >= n[u] || g->pml <= 0) continue;
+                const double x = (double)i + (kind ? 0.5 : 0.0);
+                const double hi = (double)(n[u] - 1) - L;
+                double d = (L - x > x - hi ? L - x : x - hi) / L;
+                if (d <= 0.0) continue;
+                if (d > 1.0) d = 1.0;
+                const double sigma = sigma_max * pow(d, grading), alpha = alpha_max * (1.0 - d);
+                const double bb = exp(-(sigma + alpha) * dt);
+                c[0] = (float)bb;
+                c[1] = sigma + alpha > 0.0 ? (float)(sigma / (sigma + alpha) * (bb - 1.0)) : 0.0f;
+            }
+}
+
+/* After the Faraday update: B_v += dt psi(d_u E_w), B_w -= dt psi(d_u E_v), forward
+ * differences (E beyond the last plane is 0); axis x, then y, then z. */
+void or_cpml_b(const or_grid *g, float *B, const float *E, const float *coef, float *psi) {
+    const int n[3] = {g->nx, g->ny, g->nz};
+    const int stride = n[0] > n[1] ? (n[0] > n[2] ? n[0] : n[2]) : (n[1] > n[2] ? n[1] : n[2]);
+    const float dt = g->dt;
+    size_t off = 0;
+    for (int u = 0; u < 3; ++u) {
+        const int v = (u + 1) % 3, w = (u + 2) % 3;
+        int dims[3];
+        const int cells = cpml_dims(g, u, dims);
+        float *pv = psi + off, *pw = psi + off + cells;
+#pragma omp parallel for schedule(static)
+        for (int t = 0; t < cells; ++t) {
+            int q[3] = {t % dims[0], (t / dims[0]) % dims[1], t / (dims[0] * dims[1])};
+            q[u] = cpml_plane(g, u, q[u]);
+            int r[3] = {q[0], q[1], q[2]};
+            r[u] += 1;
+            const float *e0 = &E[4 * IDX(g, q[0], q[1], q[2])];
+            const int in = r[u] < n[u];
+            const float ew1 = in ? E[4 * IDX(g, r[0], r[1], r[2]) + w] : 0.0f;
+            const float ev1 = in ? E[4 * IDX(g, r[0], r[1], r[2]) + v] : 0.0f;
+            const float *c = &coef[((size_t)(u * 2 + 1) * stride + q[u]) * 2];
+            pv[t] = c[0] * pv[t] + c[1] * (ew1 - e0[w]);
+            pw[t] = c[0] * pw[t] + c[1] * (ev1 - e0[v]);
+            float *b = &B[4 * IDX(g, q[0], q[1], q[2])];
+            b[v] = b[v] + dt * pv[t];
+            b[w] = b[w] - dt * pw[t];
+        }
+        off += 4 * (size_t)cells;
+    }
+}
+
+/* After the Ampere update: E_v -= Cb_v psi(d_u B_w), E_w += Cb_w psi(d_u B_v), backward
+ * differences (B before the first plane is 0), Cb of the edge as update_E.comp forms it. */
+void or_cpml_e(const or_grid *g, float *E, const float *B, const uint32_t *edge_mat, const float *coef, float *psi) {
+    const int n[3] = {g->nx, g->ny, g->nz};
+    const int stride = n[0] > n[1] ? (n[0] > n[2] ? n[0] : n[2]) : (n[1] > n[2] ? n[1] : n[2]);
+    const float dt = g->dt;
+    size_t off = 0;
+    for (int u = 0; u < 3; ++u) {
+        const int v = (u + 1) % 3, w = (u + 2) % 3;
+        int dims[3];
+        const int cells = cpml_dims(g, u, dims);
+        float *pv = psi + off + 2 * (size_t)cells, *pw = psi + off + 3 * (size_t)cells;
+#pragma omp parallel for schedule(static)
+        for (int t = 0; t < cells; ++t) {
+            int q[3] = {t % dims[0], (t / dims[0]) % dims[1], t / (dims[0] * dims[1])};
+            q[u] = cpml_plane(g, u, q[u]);
+            int r[3] = {q[0], q[1], q[2]};
+            r[u] -= 1;
+            const float *b0 = &B[4 * IDX(g, q[0], q[1], q[2])];
+            const int in = r[u] >= 0;
+            const float bw1 = in ? B[4 * IDX(g, r[0], r[1], r[2]) + w] : 0.0f;
+            const float bv1 = in ? B[4 * IDX(g, r[0], r[1], r[2]) + v] : 0.0f;
+            const float *c = &coef[((size_t)(u * 2 + 0) * stride + q[u]) * 2];
+            pv[t] = c[0] * pv[t] + c[1] * (b0[w] - bw1);
+            pw[t] = c[0] * pw[t] + c[1] * (b0[v] - bv1);
+            float sigma[3], epsr[3];
+            edge_sigma_eps(g, edge_mat, q[0], q[1], q[2], sigma, epsr);
+            float cb[3];
+            for (int k = 0; k < 3; ++k) {
+                const float dtEps = dt / epsr[k];
+                cb[k] = dtEps * or_phi(sigma[k] * dtEps);
+            }
+            float *e = &E[4 * IDX(g, q[0], q[1], q[2])];
+            e[v] = e[v] - cb[v] * pv[t];
+            e[w] = e[w] + cb[w] * pw[t];
+        }
+        off += 4 * (size_t)cells;
+    }
+}
+
+/* or_step_storage in FP32 with the convolutional PML's corrections after each update. */
+void or_step_cpml(const or_grid *g, float *E, float *B, const uint32_t *edge_mat, const uint32_t *wire_tex,
+                  int thin_wires, const or_source_desc *src, int n_src, int count, uint64_t *step_count,
+                  const float *coef, float *psi) {
+    if (count <= 0) return;
+    float packed[256 * 16];
+    int kept_index[256];
+    int n_cells = 0, n_list = 0;
+    int kept = or_set_sources(g, src, n_src, NULL, NULL, NULL, &n_cells, NULL, &n_list);
+    int *cells = (int *)malloc(sizeof(int) * 4 * (size_t)(n_cells + 1));
+    int *lists = (int *)malloc(sizeof(int) * (size_t)(n_list > 0 ? n_list : 1));
+    kept = or_set_sources(g, src, n_src, packed, kept_index, cells, &n_cells, lists, &n_list);
+    float signals[256];
+    for (int k = 0; k < count; ++k) {
+        const double t = ((double)(*step_count) + 0.5) * (double)g->dt;
+        for (int s = 0; s < kept; ++s) signals[s] = (float)or_signal(&src[kept_index[s]], t);
+        or_update_B(g, B, E, wire_tex, thin_wires);
+        or_cpml_b(g, B, E, coef, psi);
+        or_update_E(g, E, B, edge_mat);
+        or_cpml_e(g, E, B, edge_mat, coef, psi);
+        if (n_cells > 0) or_apply_sources(g, E, edge_mat, packed, signals, 0, cells, n_cells, lists);
+        ++(*step_count);
+    }
+    free(cells);
+    free(lists);
+}
+
+int or_cpml_psi_floats(const or_grid *g) {
+    int total = 0, dims[3];
+    for (int u = 0; u < 3; ++u) total += 4 * cpml_dims(g, u, dims);
+    return total;
+}
+
+/* ------------------------------------------------------------------------------------------
  * Monitors
  * ------------------------------------------------------------------------------------------ */
 static inline const float *fetch_clamped(const or_grid *g, const float *F, int x, int y, int z) {

@@ -144,6 +144,13 @@ void or_ntff_eval_f64(const int box_min[3], const int box_max[3], const int face
 int or_build_directions(const float axis[3], float *dirs, int cap, int *map_index, float map_frame[9]);
 
 /* field_sampling.glsl */
+void or_cpml_coefficients(const or_grid *g, double sigma_max, double grading, double alpha_max, float *coef);
+void or_cpml_b(const or_grid *g, float *B, const float *E, const float *coef, float *psi);
+void or_cpml_e(const or_grid *g, float *E, const float *B, const uint32_t *edge_mat, const float *coef, float *psi);
+void or_step_cpml(const or_grid *g, float *E, float *B, const uint32_t *edge_mat, const uint32_t *wire_tex,
+                  int thin_wires, const or_source_desc *src, int n_src, int count, uint64_t *step_count,
+                  const float *coef, float *psi);
+int or_cpml_psi_floats(const or_grid *g);
 void or_ntff_samples(const or_grid *g, const float *E, const float *B, const int box_min[3], const int box_max[3],
                      const int face_offsets[7], int n_points, float *out);
 void or_e_at_node(const or_grid *g, const float *E, int x, int y, int z, float out[3]);

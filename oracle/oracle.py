@@ -72,6 +72,11 @@ _SIG = {
     "or_ntff_eval_f64": (None, [_P, _P, _P, _I, _P, _P, _I, _D, _P, _P, C.POINTER(_D)]),
     "or_build_directions": (_I, [_P, _P, _I, _P, _P]),
     "or_ntff_samples": (None, [C.POINTER(OrGrid), _P, _P, _P, _P, _P, _I, _P]),
+    "or_cpml_coefficients": (None, [C.POINTER(OrGrid), _D, _D, _D, _P]),
+    "or_cpml_b": (None, [C.POINTER(OrGrid), _P, _P, _P, _P]),
+    "or_cpml_e": (None, [C.POINTER(OrGrid), _P, _P, _P, _P, _P]),
+    "or_step_cpml": (None, [C.POINTER(OrGrid), _P, _P, _P, _P, _I, _P, _I, _I, C.POINTER(C.c_uint64), _P, _P]),
+    "or_cpml_psi_floats": (_I, [C.POINTER(OrGrid)]),
     "or_e_at_node": (None, [C.POINTER(OrGrid), _P, _I, _I, _I, _P]),
     "or_b_at_node": (None, [C.POINTER(OrGrid), _P, _I, _I, _I, _P]),
     "or_plane_flux": (_D, [C.POINTER(OrGrid), _P, _P, _I, _I, _I, _I, _I, _I]),
@@ -168,6 +173,22 @@ class OracleSim:
         self.B[:, :3] = 0.0
         self.step_count = 0
 
+    def set_cpml(self, sigma_max, grading, alpha_max):
+        """The library's opt-in convolutional PML in place of the graded absorber (same slabs):
+        the graded conductivity is switched off and the auxiliary fields start at 0; clears the
+        fields and rewinds time, as emfd_grid_set_absorber."""
+        L = lib()
+        self.g.sigma_max = 0.0
+        stride = max(self.g.nx, self.g.ny, self.g.nz)
+        self.cpml_coef = np.zeros(3 * 2 * stride * 2, np.float32)
+        # the library takes the parameters as floats
+        f32 = lambda v: float(np.float32(v))  # noqa: E731
+        L.or_cpml_coefficients(C.byref(self.g), f32(sigma_max), f32(grading), f32(alpha_max), ptr(self.cpml_coef))
+        self.cpml_psi = np.zeros(L.or_cpml_psi_floats(C.byref(self.g)), np.float32)
+        self.E[:, :3] = 0.0
+        self.B[:, :3] = 0.0
+        self.step_count = 0
+
     @property
     def size(self):
         return (self.g.nx, self.g.ny, self.g.nz)
@@ -239,9 +260,14 @@ class OracleSim:
         # one call per step keeps the per-step monitors in the reference's order
         for _ in range(count):
             sc = C.c_uint64(self.step_count)
-            L.or_step_storage(C.byref(self.g), ptr(self.E), ptr(self.B), ptr(self.edge), ptr(self.wire), self.thin,
-                              C.cast(self.src_c, C.c_void_p) if self.src_c else None, len(self.sources), 1,
-                              C.byref(sc), self.storage, ptr(self.scale_log2))
+            if getattr(self, "cpml_psi", None) is not None:
+                L.or_step_cpml(C.byref(self.g), ptr(self.E), ptr(self.B), ptr(self.edge), ptr(self.wire), self.thin,
+                               C.cast(self.src_c, C.c_void_p) if self.src_c else None, len(self.sources), 1,
+                               C.byref(sc), ptr(self.cpml_coef), ptr(self.cpml_psi))
+            else:
+                L.or_step_storage(C.byref(self.g), ptr(self.E), ptr(self.B), ptr(self.edge), ptr(self.wire),
+                                  self.thin, C.cast(self.src_c, C.c_void_p) if self.src_c else None,
+                                  len(self.sources), 1, C.byref(sc), self.storage, ptr(self.scale_log2))
             self.step_count = sc.value
             self._monitors()
 
