@@ -105,7 +105,12 @@ __host__ __device__ constexpr size_t march_xch_floats() {
     return static_cast<size_t>(2) * 2 * NW * 2 * kSeg;
 }
 
-template <int NW>
+// ZC > 0: the z-chunk is ZC planes and the march is unrolled over all ZC + 1 of its planes (the
+// leading one included; the first chunk leads with a virtual plane -1 of zero fields, a partial
+// last chunk runs on past the lattice without storing), so that the in-flight load slots never
+// meet at a loop back-edge, where the compiler would copy them and wait for every load in
+// flight. ZC = 0: a loop over any z-chunk.
+template <int NW, int ZC>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16 / NW, 16 / NW))) k_yee_march(LatticeArgs a, FieldPtrs src, FieldPtrs dst, int zchunk,
                                                         int nseg, int nyt, int n_tiles) {
     extern __shared__ float lds[];
@@ -123,7 +128,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
     const int y = yt * (NW - 1) - 1 + w;
     const int z0 = zc * zchunk;
     const int z1 = min(z0 + zchunk, nz);
-    const int zs = z0 > 0 ? z0 - 1 : 0;
+    const int zs = ZC > 0 ? z0 - 1 : (z0 > 0 ? z0 - 1 : 0);
     const int xs = seg * kSeg;
     const int x0 = xs + 4 * lane;
     const bool row_ok = y >= 0 && y < ny;
@@ -149,8 +154,8 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
     // compiler's wait counts stay exact (a skipped load, or a register reused while its load is
     // in flight, makes it wait for every load in flight).
     auto plane_of = [&](const float *p, int z, int zmax) {
-        const int zc = z <= zmax ? z : 0;
-        return plane_rsrc(p + static_cast<size_t>(zc) * nx * ny, z <= zmax ? plane : 0u);
+        const bool ok = z >= 0 && z <= zmax;
+        return plane_rsrc(p + static_cast<size_t>(ok ? z : 0) * nx * ny, ok ? plane : 0u);
     };
 
     float *de[3] = {dst.e[0], dst.e[1], dst.e[2]}, *db[3] = {dst.b[0], dst.b[1], dst.b[2]};
@@ -291,9 +296,10 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
             bn[1][j] = ba[1] * bc.b.v[1][j] - bb[1] * cyv;
             bn[2][j] = ba[2] * bc.b.v[2][j] - bb[2] * czv;
         }
-        if (!row_ok) {
-            // a row outside the lattice (the group below row 0): B there is the boundary's 0, which
-            // the first row's E update reads (shaders/update_E.comp:27-29)
+        if (!row_ok || z < 0) {
+            // a row outside the lattice (the group below row 0) or the virtual plane -1: B there is
+            // the boundary's 0, which the first row's or plane's E update reads
+            // (shaders/update_E.comp:27-29)
 #pragma unroll
             for (int j = 0; j < 4; ++j) bn[0][j] = bn[1][j] = bn[2][j] = 0.0f;
         }
@@ -320,7 +326,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
         __syncthreads();
 
         // ---- E^{n+1}(z) ----
-        if (out_row && z >= z0) {
+        if (out_row && z >= z0 && z < z1) {
             const float4 d0 = *reinterpret_cast<const float4 *>(xb_at(z, w - 1, 0) + 4 * lane);
             const float4 d1 = *reinterpret_cast<const float4 *>(xb_at(z, w - 1, 1) + 4 * lane);
             const float bxd[4] = {d0.x, d0.y, d0.z, d0.w}, bzd[4] = {d1.x, d1.y, d1.z, d1.w};
@@ -359,21 +365,40 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
         for (int c = 0; c < 3; ++c) { hlc[c] = hln[c]; hln[c] = hP[c]; }
         bc = Pb;
     };
-    for (int z = zs; z < z1; z += 2) {
-        plane_step(z, eP, eQ, hlP, hlQ, bP, bQ);
-        if (z + 1 < z1) plane_step(z + 1, eQ, eP, hlQ, hlP, bQ, bP);
+    if constexpr (ZC > 0) {
+#pragma unroll
+        for (int k = 0; k <= ZC; k += 2) {
+            plane_step(zs + k, eP, eQ, hlP, hlQ, bP, bQ);
+            if (k + 1 <= ZC) plane_step(zs + k + 1, eQ, eP, hlQ, hlP, bQ, bP);
+        }
+    } else {
+        for (int z = zs; z < z1; z += 2) {
+            plane_step(z, eP, eQ, hlP, hlQ, bP, bQ);
+            if (z + 1 < z1) plane_step(z + 1, eQ, eP, hlQ, hlP, bQ, bP);
+        }
     }
+}
+
+// List item of this thread. Workgroups are dealt to the 8 XCDs in turn; each XCD is given one
+// contiguous eighth of the (sorted) list instead, so that a texel's -y and -z neighbours, read a
+// row or a plane of the list earlier, were read through the same XCD's L2.
+__device__ __forceinline__ int fix_item(int n) {
+    const int blocks = static_cast<int>(gridDim.x), per = blocks / 8;  // the grid is a multiple of 8
+    const int b = static_cast<int>(blockIdx.x);
+    return ((b % 8) * per + b / 8) * 256 + static_cast<int>(threadIdx.x);
 }
 
 // B^{n+1/2} of the listed texels (thin-wire classes), from src as update_B.comp computes it
 // (faraday(): wire factors and absorber from the class and absorber tables).
 __global__ void __launch_bounds__(256) k_march_fix_b(LatticeArgs a, FieldPtrs src, FieldPtrs dst,
                                                      const uint32_t *list, int n) {
-    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    const int i = fix_item(n);
     if (i >= n) return;
-    const size_t p = list[i];
+    const uint32_t q = list[i], row = q / static_cast<uint32_t>(a.nx);  // at most 2^29 texels
+    const size_t p = q;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int x = static_cast<int>(p % nx), y = static_cast<int>((p / nx) % ny), z = static_cast<int>(p / (static_cast<size_t>(nx) * ny));
+    const int x = static_cast<int>(q - row * static_cast<uint32_t>(nx)), y = static_cast<int>(row % static_cast<uint32_t>(ny)),
+              z = static_cast<int>(row / static_cast<uint32_t>(ny));
     const size_t sy = nx, sz = static_cast<size_t>(nx) * ny;
     float e0[3], exn[3], eyn[3], ezn[3], b[3];
 #pragma unroll
@@ -402,11 +427,13 @@ __global__ void __launch_bounds__(256) k_march_fix_b(LatticeArgs a, FieldPtrs sr
 // thin-wire texel's B), from src E and dst B as update_E.comp computes it (edge_coefficients()).
 __global__ void __launch_bounds__(256) k_march_fix_e(LatticeArgs a, FieldPtrs src, FieldPtrs dst,
                                                      const uint32_t *list, int n) {
-    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    const int i = fix_item(n);
     if (i >= n) return;
-    const size_t p = list[i];
+    const uint32_t q = list[i], row = q / static_cast<uint32_t>(a.nx);
+    const size_t p = q;
     const int nx = a.nx, ny = a.ny;
-    const int x = static_cast<int>(p % nx), y = static_cast<int>((p / nx) % ny), z = static_cast<int>(p / (static_cast<size_t>(nx) * ny));
+    const int x = static_cast<int>(q - row * static_cast<uint32_t>(nx)), y = static_cast<int>(row % static_cast<uint32_t>(ny)),
+              z = static_cast<int>(row / static_cast<uint32_t>(ny));
     const size_t sy = nx, sz = static_cast<size_t>(nx) * ny;
     float b0[3], bX[3], bY[3], bZ[3];
 #pragma unroll
@@ -442,25 +469,35 @@ hipError_t launch_nw(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs
     const int nseg = (a.nx + kSeg - 1) / kSeg;
     const int nyt = (a.ny + (NW - 2)) / (NW - 1);
     if (zchunk <= 0) {
-        // thin z-chunks: the workgroups in flight then cover a thin slab, which streams best (at
-        // 768^3: chunks of 8 planes 3.83 ms per step, 12 planes 3.87, 16 planes 3.91, 48 planes
-        // 4.00, 96 planes 4.21); each chunk recomputes one leading plane of B
+        // thin z-chunks: the workgroups in flight then cover a thin slab, which streams best
+        // although each chunk re-reads a leading plane (768^3 vacuum, unrolled marches: chunks of
+        // 4 planes 3.89 ms per step, 6 planes 3.79, 8 planes 3.82, 12 planes 3.86, 16 planes 3.91;
+        // with a loop, 48 planes 4.00, 96 planes 4.21)
         zchunk = 8;
     }
     if (zchunk > a.nz) zchunk = a.nz;
     const int nzc = (a.nz + zchunk - 1) / zchunk;
     const int n_tiles = nseg * nyt * nzc;
     const size_t lds = (march_xch_floats<NW>() + static_cast<size_t>(NW) * 2 * a.pml_k * kCoefStride) * sizeof(float);
-    if (lds > 65536) {
-        static size_t raised = 0;  // dynamic LDS past 64 KB must be allowed per kernel
-        if (lds > raised) {
-            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_yee_march<NW>),
-                                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-            if (e != hipSuccess) return e;
-            raised = lds;
-        }
+    auto allow_lds = [&](const void *kernel, size_t &raised) -> hipError_t {
+        // dynamic LDS past 64 KB must be allowed per kernel
+        if (lds <= 65536 || lds <= raised) return hipSuccess;
+        const hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 static_cast<int>(lds));
+        if (e == hipSuccess) raised = lds;
+        return e;
+    };
+    if (zchunk == 8) {
+        static size_t raised = 0;
+        if (hipError_t e = allow_lds(reinterpret_cast<const void *>(&k_yee_march<NW, 8>), raised); e != hipSuccess)
+            return e;
+        k_yee_march<NW, 8><<<n_tiles, 64 * NW, lds, s>>>(a, src, dst, zchunk, nseg, nyt, n_tiles);
+    } else {
+        static size_t raised = 0;
+        if (hipError_t e = allow_lds(reinterpret_cast<const void *>(&k_yee_march<NW, 0>), raised); e != hipSuccess)
+            return e;
+        k_yee_march<NW, 0><<<n_tiles, 64 * NW, lds, s>>>(a, src, dst, zchunk, nseg, nyt, n_tiles);
     }
-    k_yee_march<NW><<<n_tiles, 64 * NW, lds, s>>>(a, src, dst, zchunk, nseg, nyt, n_tiles);
     return hipGetLastError();
 }
 
@@ -478,12 +515,13 @@ hipError_t launch_yee_march(const LatticeArgs &a, const FieldPtrs &src, const Fi
     const int zchunk = cfg;  // 0: automatic
     hipError_t e = launch_nw<8>(a, src, dst, zchunk, s);
     if (e != hipSuccess) return e;
+    auto grid8 = [](int n) { return ((n + 255) / 256 + 7) / 8 * 8; };
     if (n_fix_b > 0) {
-        k_march_fix_b<<<(n_fix_b + 255) / 256, 256, 0, s>>>(a, src, dst, fix_b, n_fix_b);
+        k_march_fix_b<<<grid8(n_fix_b), 256, 0, s>>>(a, src, dst, fix_b, n_fix_b);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (n_fix_e > 0) {
-        k_march_fix_e<<<(n_fix_e + 255) / 256, 256, 0, s>>>(a, src, dst, fix_e, n_fix_e);
+        k_march_fix_e<<<grid8(n_fix_e), 256, 0, s>>>(a, src, dst, fix_e, n_fix_e);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;
