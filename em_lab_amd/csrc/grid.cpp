@@ -406,8 +406,8 @@ int emfd_grid_configure(emfd_ctx *ctx, const emfd_grid_config *requested, emfd_g
 
     if (n != ctx->n || !ctx->field_buf[0].ptr) {
         ctx->n = n;
-        // the ping-pong buffer serves the fused FP32 kernels only
-        for (int k = 0; k < (ctx->storage == kStorageF32 ? 2 : 1); ++k) {
+        // the ping-pong buffer serves the fused kernels (FP32 and 16-bit storage)
+        for (int k = 0; k < (ctx->storage != kStorageF64 ? 2 : 1); ++k) {
             int rc = ctx->field_buf[k].ensure(ctx->field_bytes());
             if (rc) return rc;
         }
@@ -634,7 +634,7 @@ int emfd_grid_set_storage(emfd_ctx *ctx, int storage, const int *scale_log2) {
     std::copy(k, k + 6, ctx->scale_log2);
     ctx->field_buf[0].release();
     ctx->field_buf[1].release();
-    for (int b = 0; b < (storage == EMFD_STORAGE_F32 ? 2 : 1); ++b)
+    for (int b = 0; b < (storage != EMFD_STORAGE_F64 ? 2 : 1); ++b)
         if (int rc = ctx->field_buf[b].ensure(ctx->field_bytes())) return rc;
     // as uploadMaterials: fields cleared, time rewound
     ctx->cur = 0;
@@ -758,6 +758,10 @@ namespace {
 // EMFD_KERNEL_AUTO runs the one-pass march on FP32 lattices of at least 200^3 cells (8e6; at 256^3
 // 0.159 against 0.174 ms per Yee step for the two passes, at 768^3 3.83 against 5.76 ms); below
 // that its few long workgroups leave the GPU part idle (128^3: 0.046 against 0.036 ms).
+// The march is the automatic choice for FP32 fields from 8e6 texels. With 16-bit storage it halves
+// the field bytes but not its instructions, and measured slower than the two 16-bit passes
+// (768^3 precision scene 3.44 against 2.92 ms per Yee step, vacuum 2.84 against 2.76 ms): it runs
+// there only when asked for.
 bool march_auto(const emfd_ctx *ctx) {
     return ctx->storage == kStorageF32 && ctx->absorber == EMFD_ABSORBER_GRADED && ctx->n >= 8000000u &&
            yee_march_supported(ctx->lattice());
@@ -849,13 +853,20 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
         }
         int variant = ctx->kernel_variant;
         if (ctx->storage != kStorageF32) {
-            if (variant != EMFD_KERNEL_AUTO)
-                return fail(EMFD_ERR_INVALID, "16-bit and FP64 field storage run the automatic kernel only");
-            if (ctx->storage == kStorageF64)
+            // 16-bit storage: the two passes, or the march when asked for; FP64: its own two passes
+            const bool march16 = ctx->storage != kStorageF64 &&
+                                 (variant == EMFD_KERNEL_MARCH || (variant == EMFD_KERNEL_AUTO && march_auto(ctx)));
+            if (variant != EMFD_KERNEL_AUTO && !march16)
+                return fail(EMFD_ERR_INVALID, "16-bit and FP64 field storage run the automatic kernel or the march");
+            if (ctx->storage == kStorageF64) {
                 EMFD_HIP(launch_yee_f64(lat, ctx->fields(ctx->cur), ctx->stream));
-            else
+                variant = -1;
+            } else if (march16) {
+                variant = EMFD_KERNEL_MARCH;
+            } else {
                 EMFD_HIP(launch_yee_half(lat, ctx->fields(ctx->cur), ctx->tile_cfg & 0xf, ctx->stream));
-            variant = -1;
+                variant = -1;
+            }
         } else if (variant == EMFD_KERNEL_AUTO)
             variant = march_auto(ctx) ? EMFD_KERNEL_MARCH
                                       : yee_two_pass4_supported(lat) ? EMFD_KERNEL_TWO_PASS4 : EMFD_KERNEL_TWO_PASS;
@@ -884,7 +895,10 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
         } else if (variant == EMFD_KERNEL_TWO_PASS4) {
             EMFD_HIP(launch_yee_two_pass4(lat, ctx->fields(ctx->cur), ctx->stream));
         } else {
-            EMFD_HIP(launch_yee_march(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), ctx->tile_cfg,
+            // bits 8.. of the variant: the march's z-chunk, or under the automatic choice with 16-bit
+            // storage the two-pass kernels' cells per thread (the march then picks its z-chunk)
+            const int zc = ctx->storage != kStorageF32 && ctx->kernel_variant == EMFD_KERNEL_AUTO ? 0 : ctx->tile_cfg;
+            EMFD_HIP(launch_yee_march(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), zc,
                                       ctx->fix_b.as<uint32_t>(), ctx->n_fix_b, ctx->fix_e.as<uint32_t>(), ctx->n_fix_e,
                                       ctx->stream));
             ctx->cur ^= 1;

@@ -23,6 +23,7 @@
 // and wire edges) and the E texels next to a thin-wire texel are then recomputed from the same
 // src and dst planes by two sparse passes (k_march_fix_b, k_march_fix_e) with faraday() and
 // edge_coefficients(), so the result is bit-identical to the two-pass kernels.
+#include "port_device.h"
 #include "yee_quad.h"
 
 namespace emfd {
@@ -97,6 +98,48 @@ __device__ __forceinline__ void gst4(float *p, uint32_t off, const float (&v)[4]
     __builtin_nontemporal_store(d, reinterpret_cast<f32x4_t *>(reinterpret_cast<char *>(p) + off));
 }
 
+// Field storage of the planes: FP32, or binary16 / bfloat16 of value / scale (a power of two per
+// component; updates in FP32). ld4 / ld1 return values, st4 stores them, rnd rounds a value
+// through the storage (what a store and a load give back), as the 16-bit two-pass kernels do.
+template <class T>
+__device__ __forceinline__ void ld4(Rsrc r, uint32_t off, float s, float (&o)[4]) {
+    if constexpr (sizeof(T) == 4) {
+        bld4(r, off, o);
+    } else {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        const T *h = reinterpret_cast<const T *>(&v);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) o[j] = static_cast<float>(h[j]) * s;
+    }
+}
+template <class T>
+__device__ __forceinline__ float ld1(Rsrc r, uint32_t off, float s) {
+    if constexpr (sizeof(T) == 4) {
+        return bld1(r, off);
+    } else {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+        return static_cast<float>(*reinterpret_cast<const T *>(&v)) * s;
+    }
+}
+template <class T>
+__device__ __forceinline__ void st4(float *p, uint32_t off, float inv, const float (&v)[4]) {
+    if constexpr (sizeof(T) == 4) {
+        gst4(p, off, v);
+    } else {
+        typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+        u32x2_t raw;
+        T *h = reinterpret_cast<T *>(&raw);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) h[j] = static_cast<T>(v[j] * inv);
+        __builtin_nontemporal_store(raw, reinterpret_cast<u32x2_t *>(reinterpret_cast<char *>(p) + off));
+    }
+}
+template <class T>
+__device__ __forceinline__ float rnd(float v, float inv, float s) {
+    if constexpr (sizeof(T) == 4) return v;
+    else return static_cast<float>(static_cast<T>(v * inv)) * s;
+}
+
 // LDS layout (floats): xe[2][NW][2][kSeg] | xb[2][NW][2][kSeg] | coef[NW][2][K * kCoefStride]
 //   xe[p & 1][w]: E^n(row of wave w, plane p) z, x components
 //   xb[p & 1][w]: B^{n+1/2}(row of wave w, plane p) x, z components
@@ -110,7 +153,7 @@ __host__ __device__ constexpr size_t march_xch_floats() {
 // last chunk runs on past the lattice without storing), so that the in-flight load slots never
 // meet at a loop back-edge, where the compiler would copy them and wait for every load in
 // flight. ZC = 0: a loop over any z-chunk.
-template <int NW, int ZC>
+template <int NW, int ZC, class T>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16 / NW, 16 / NW))) k_yee_march(LatticeArgs a, FieldPtrs src, FieldPtrs dst, int zchunk,
                                                         int nseg, int nyt, int n_tiles) {
     extern __shared__ float lds[];
@@ -139,14 +182,19 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
     const bool left = xs > 0;
     const bool hl_ok = left && lane == 0 && row_ok;               // lane 0 rebuilds B^{n+1/2}(xs - 1)
     const bool hr_ok = xs + kSeg < nx && lane == 63 && row_ok;    // lane 63 reads E^n(xs + 256)
-    const uint32_t plane = static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny) * 4u;  // bytes per z plane
-    const uint32_t yrow = static_cast<uint32_t>(nx) * 4u;
+    constexpr uint32_t ES = sizeof(T);  // bytes per stored value
+    const uint32_t plane = static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny) * ES;  // bytes per z plane
+    const uint32_t yrow = static_cast<uint32_t>(nx) * ES;
     // byte offsets within a plane, kOob for an access outside the lattice
-    const uint32_t o_own = cell_ok ? (static_cast<uint32_t>(y) * nx + x0) * 4u : kOob;
-    const uint32_t o_hl = hl_ok ? (static_cast<uint32_t>(y) * nx + xs - 1) * 4u : kOob;
+    const uint32_t o_own = cell_ok ? (static_cast<uint32_t>(y) * nx + x0) * ES : kOob;
+    const uint32_t o_hl = hl_ok ? (static_cast<uint32_t>(y) * nx + xs - 1) * ES : kOob;
     const uint32_t o_hu = hl_ok && y + 1 < ny ? o_hl + yrow : kOob;
-    const uint32_t o_hr = hr_ok ? (static_cast<uint32_t>(y) * nx + xs + kSeg) * 4u : kOob;
-    const uint32_t o_up = up_ok && x0 < nx ? (static_cast<uint32_t>(y + 1) * nx + x0) * 4u : kOob;
+    const uint32_t o_hr = hr_ok ? (static_cast<uint32_t>(y) * nx + xs + kSeg) * ES : kOob;
+    const uint32_t o_up = up_ok && x0 < nx ? (static_cast<uint32_t>(y + 1) * nx + x0) * ES : kOob;
+    // storage scales of E x y z and B x y z and their inverses (1 for FP32)
+    const float sE[3] = {src.scale[0], src.scale[1], src.scale[2]}, sB[3] = {src.scale[3], src.scale[4], src.scale[5]};
+    const float iE[3] = {src.inv_scale[0], src.inv_scale[1], src.inv_scale[2]},
+                iB[3] = {src.inv_scale[3], src.inv_scale[4], src.inv_scale[5]};
     // The per-lane offsets above stay fixed for the whole march; each load's descriptor covers
     // the one z plane it reads (base + z plane, range one plane), a plane past the chunk's last
     // needed one or past the lattice has range 0 and reads zeros. Every load is issued
@@ -155,7 +203,8 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
     // in flight, makes it wait for every load in flight).
     auto plane_of = [&](const float *p, int z, int zmax) {
         const bool ok = z >= 0 && z <= zmax;
-        return plane_rsrc(p + static_cast<size_t>(ok ? z : 0) * nx * ny, ok ? plane : 0u);
+        const char *base = reinterpret_cast<const char *>(p) + static_cast<size_t>(ok ? z : 0) * plane;
+        return plane_rsrc(reinterpret_cast<const float *>(base), ok ? plane : 0u);
     };
 
     float *de[3] = {dst.e[0], dst.e[1], dst.e[2]}, *db[3] = {dst.b[0], dst.b[1], dst.b[2]};
@@ -183,8 +232,8 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const Rsrc q = plane_of(src.e[c], z, ze_max);
-            bld4(q, o_own, r.v[c]);
-            hl[c] = bld1(q, o_hl);
+            ld4<T>(q, o_own, sE[c], r.v[c]);
+            hl[c] = ld1<T>(q, o_hl, sE[c]);
         }
     };
     struct BSet {
@@ -199,16 +248,16 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
                    b2 = plane_of(src.b[2], z, zb_max);
         const Rsrc e0 = plane_of(src.e[0], z, zb_max), e1 = plane_of(src.e[1], z, zb_max),
                    e2 = plane_of(src.e[2], z, zb_max);
-        bld4(b0, o_own, s.b.v[0]);
-        bld4(b1, o_own, s.b.v[1]);
-        bld4(b2, o_own, s.b.v[2]);
-        s.hb[0] = bld1(b1, o_hl);
-        s.hb[1] = bld1(b2, o_hl);
-        s.hu = bld1(e0, o_hu);
-        s.hr[0] = bld1(e1, o_hr);
-        s.hr[1] = bld1(e2, o_hr);
-        bld4(e2, o_up, s.eu[0]);  // kOob but for the top wave
-        bld4(e0, o_up, s.eu[1]);
+        ld4<T>(b0, o_own, sB[0], s.b.v[0]);
+        ld4<T>(b1, o_own, sB[1], s.b.v[1]);
+        ld4<T>(b2, o_own, sB[2], s.b.v[2]);
+        s.hb[0] = ld1<T>(b1, o_hl, sB[1]);
+        s.hb[1] = ld1<T>(b2, o_hl, sB[2]);
+        s.hu = ld1<T>(e0, o_hu, sE[0]);
+        s.hr[0] = ld1<T>(e1, o_hr, sE[1]);
+        s.hr[1] = ld1<T>(e2, o_hr, sE[2]);
+        ld4<T>(e2, o_up, sE[2], s.eu[0]);  // kOob but for the top wave
+        ld4<T>(e0, o_up, sE[0], s.eu[1]);
     };
     // absorber coefficients of this wave's row at plane z: entry cx for (cx, class(y), class(z));
     // the wave issues the table loads, waits, and stores them (one entry per lane, more when the
@@ -303,6 +352,11 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
 #pragma unroll
             for (int j = 0; j < 4; ++j) bn[0][j] = bn[1][j] = bn[2][j] = 0.0f;
         }
+        // the E update reads B^{n+1/2} as stored (16-bit storage: rounded)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bn[c][j] = rnd<T>(bn[c][j], iB[c], sB[c]);
         // lane 0 of a segment past x = 0: B^{n+1/2}(xs - 1) y, z from its own loads
         float hby = 0.0f, hbz = 0.0f;
         if (left && lane == 0) {
@@ -310,8 +364,8 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
             read_coef(coefB + cxl * kCoefStride, ba, bb);
             const float t0y = hln[0] - hlc[0], t1y = ec.v[2][0] - hlc[2];
             const float t0z = ec.v[1][0] - hlc[1], t1z = bc.hu - hlc[0];
-            hby = ba[1] * bc.hb[0] - bb[1] * (t0y - t1y);
-            hbz = ba[2] * bc.hb[1] - bb[2] * (t0z - t1z);
+            hby = rnd<T>(ba[1] * bc.hb[0] - bb[1] * (t0y - t1y), iB[1], sB[1]);
+            hbz = rnd<T>(ba[2] * bc.hb[1] - bb[2] * (t0z - t1z), iB[2], sB[2]);
         }
         // publish B^{n+1/2}(z) x, z and E^n(z + 1) z, x
         *reinterpret_cast<float4 *>(xb_at(z, w, 0) + 4 * lane) = make_float4(bn[0][0], bn[0][1], bn[0][2], bn[0][3]);
@@ -340,16 +394,16 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
                 const float curl2 = (bn[1][j] - bxm_y) - (bn[0][j] - bxd[j]);
                 float ca[3], cb[3];
                 read_coef(coefE + cx[j] * kCoefStride, ca, cb);
-                e[0][j] = ca[0] * ec.v[0][j] + cb[0] * curl0;
-                e[1][j] = ca[1] * ec.v[1][j] + cb[1] * curl1;
-                e[2][j] = ca[2] * ec.v[2][j] + cb[2] * curl2;
+                e[0][j] = rnd<T>(ca[0] * ec.v[0][j] + cb[0] * curl0, iE[0], sE[0]);
+                e[1][j] = rnd<T>(ca[1] * ec.v[1][j] + cb[1] * curl1, iE[1], sE[1]);
+                e[2][j] = rnd<T>(ca[2] * ec.v[2][j] + cb[2] * curl2, iE[2], sE[2]);
             }
             const uint32_t o = o_own + static_cast<uint32_t>(z) * plane;
             if (cell_ok) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    gst4(db[c], o, bn[c]);
-                    gst4(de[c], o, e[c]);
+                    st4<T>(db[c], o, iB[c], bn[c]);
+                    st4<T>(de[c], o, iE[c], e[c]);
                 }
             }
             if (side_row && z >= a.side.z0 && z < a.side.z0 + a.side.nz && sbits != 0u) {
@@ -403,16 +457,18 @@ __global__ void __launch_bounds__(256) k_march_fix_b(LatticeArgs a, FieldPtrs sr
     float e0[3], exn[3], eyn[3], ezn[3], b[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const float *e = src.e[c];
-        e0[c] = e[p];
-        exn[c] = x + 1 < nx ? e[p + 1] : 0.0f;
-        eyn[c] = y + 1 < ny ? e[p + sy] : 0.0f;
-        ezn[c] = z + 1 < nz ? e[p + sz] : 0.0f;
-        b[c] = src.b[c][p];
+        e0[c] = field_load(src, c, p);
+        exn[c] = x + 1 < nx ? field_load(src, c, p + 1) : 0.0f;
+        eyn[c] = y + 1 < ny ? field_load(src, c, p + sy) : 0.0f;
+        ezn[c] = z + 1 < nz ? field_load(src, c, p + sz) : 0.0f;
+        b[c] = field_load(src, 3 + c, p);
     }
     faraday(a, x, y, z, e0, exn, eyn, ezn, load_id(a, p), b);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) dst.b[c][p] = b[c];
+    for (int c = 0; c < 3; ++c) {
+        field_store(dst, 3 + c, p, b[c]);
+        b[c] = storage_value(dst, 3 + c, b[c]);
+    }
     if (a.side.buf) {
         for (int face = 0; face < 2; ++face) {
             const long k = side_index(a.side, face, 3, x, y, z);
@@ -438,11 +494,10 @@ __global__ void __launch_bounds__(256) k_march_fix_e(LatticeArgs a, FieldPtrs sr
     float b0[3], bX[3], bY[3], bZ[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const float *bp = dst.b[c];
-        b0[c] = bp[p];
-        bX[c] = x > 0 ? bp[p - 1] : 0.0f;
-        bY[c] = y > 0 ? bp[p - sy] : 0.0f;
-        bZ[c] = z > 0 ? bp[p - sz] : 0.0f;
+        b0[c] = field_load(dst, 3 + c, p);
+        bX[c] = x > 0 ? field_load(dst, 3 + c, p - 1) : 0.0f;
+        bY[c] = y > 0 ? field_load(dst, 3 + c, p - sy) : 0.0f;
+        bZ[c] = z > 0 ? field_load(dst, 3 + c, p - sz) : 0.0f;
     }
     const float curl[3] = {(b0[2] - bY[2]) - (b0[1] - bZ[1]), (b0[0] - bZ[0]) - (b0[2] - bX[2]),
                            (b0[1] - bX[1]) - (b0[0] - bY[0])};
@@ -451,8 +506,9 @@ __global__ void __launch_bounds__(256) k_march_fix_e(LatticeArgs a, FieldPtrs sr
     float e[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        e[c] = ca[c] * src.e[c][p] + cb[c] * curl[c];
-        dst.e[c][p] = e[c];
+        e[c] = ca[c] * field_load(src, c, p) + cb[c] * curl[c];
+        field_store(dst, c, p, e[c]);
+        e[c] = storage_value(dst, c, e[c]);  // the far-field columns keep what the plane holds
     }
     if (a.side.buf) {
         for (int face = 0; face < 2; ++face) {
@@ -464,7 +520,7 @@ __global__ void __launch_bounds__(256) k_march_fix_e(LatticeArgs a, FieldPtrs sr
     }
 }
 
-template <int NW>
+template <int NW, class T>
 hipError_t launch_nw(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int zchunk, hipStream_t s) {
     const int nseg = (a.nx + kSeg - 1) / kSeg;
     const int nyt = (a.ny + (NW - 2)) / (NW - 1);
@@ -489,14 +545,14 @@ hipError_t launch_nw(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs
     };
     if (zchunk == 8) {
         static size_t raised = 0;
-        if (hipError_t e = allow_lds(reinterpret_cast<const void *>(&k_yee_march<NW, 8>), raised); e != hipSuccess)
+        if (hipError_t e = allow_lds(reinterpret_cast<const void *>(&k_yee_march<NW, 8, T>), raised); e != hipSuccess)
             return e;
-        k_yee_march<NW, 8><<<n_tiles, 64 * NW, lds, s>>>(a, src, dst, zchunk, nseg, nyt, n_tiles);
+        k_yee_march<NW, 8, T><<<n_tiles, 64 * NW, lds, s>>>(a, src, dst, zchunk, nseg, nyt, n_tiles);
     } else {
         static size_t raised = 0;
-        if (hipError_t e = allow_lds(reinterpret_cast<const void *>(&k_yee_march<NW, 0>), raised); e != hipSuccess)
+        if (hipError_t e = allow_lds(reinterpret_cast<const void *>(&k_yee_march<NW, 0, T>), raised); e != hipSuccess)
             return e;
-        k_yee_march<NW, 0><<<n_tiles, 64 * NW, lds, s>>>(a, src, dst, zchunk, nseg, nyt, n_tiles);
+        k_yee_march<NW, 0, T><<<n_tiles, 64 * NW, lds, s>>>(a, src, dst, zchunk, nseg, nyt, n_tiles);
     }
     return hipGetLastError();
 }
@@ -513,7 +569,10 @@ hipError_t launch_yee_march(const LatticeArgs &a, const FieldPtrs &src, const Fi
                             hipStream_t s) {
     if (!yee_march_supported(a)) return hipErrorInvalidValue;
     const int zchunk = cfg;  // 0: automatic
-    hipError_t e = launch_nw<8>(a, src, dst, zchunk, s);
+    hipError_t e = src.storage == kStorageF16    ? launch_nw<8, _Float16>(a, src, dst, zchunk, s)
+                   : src.storage == kStorageBF16 ? launch_nw<8, __bf16>(a, src, dst, zchunk, s)
+                   : src.storage == kStorageF32  ? launch_nw<8, float>(a, src, dst, zchunk, s)
+                                                 : hipErrorInvalidValue;
     if (e != hipSuccess) return e;
     auto grid8 = [](int n) { return ((n + 255) / 256 + 7) / 8 * 8; };
     if (n_fix_b > 0) {

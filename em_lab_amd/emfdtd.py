@@ -492,13 +492,14 @@ class SimulationGrid:
         v = self._variant & 0xFF
         if self._storage == STORAGE_F64:
             return "k_update_b_f64+k_update_e_f64"
-        if self._storage in (STORAGE_F16, STORAGE_BF16):
-            return "k_update_bh+k_update_eh"
         nx, ny, nz = self.size
-        if v == KERNEL_MARCH or (v == KERNEL_AUTO and nx * ny * nz >= 8_000_000 and nx % 4 == 0
+        half = self._storage in (STORAGE_F16, STORAGE_BF16)
+        if v == KERNEL_MARCH or (v == KERNEL_AUTO and not half and nx * ny * nz >= 8_000_000 and nx % 4 == 0
                                  and nx * ny * nz * 4 <= 0x80000000
                                  and getattr(self, "_absorber", ABSORBER_GRADED) == ABSORBER_GRADED):
             return "k_yee_march (+k_march_fix_b, k_march_fix_e)"
+        if half:
+            return "k_update_bh+k_update_eh"
         if v in (KERNEL_AUTO, KERNEL_TWO_PASS4) and self.size[0] % 4 == 0:
             return "k_update_b4+k_update_e4"
         return "k_update_b+k_update_e"

@@ -50,7 +50,8 @@ extern "C" {
                                    2 GB): z-march of 256-cell x segments of 7 rows with one workgroup barrier
                                    per plane, vacuum and absorber coefficients in the march, then sparse
                                    passes recompute the texels of non-vacuum classes and next to thin wires;
-                                   bits 8.. of the variant = z-chunk (0 = automatic) */
+                                   bits 8.. of the variant = z-chunk (0 = automatic); also runs 16-bit
+                                   storage (rounding B^{n+1/2} as stored), where it is not automatic */
 
 #define EMFD_ABSORBER_GRADED 0 /* the reference's graded absorber (default; bit-exact parity) */
 #define EMFD_ABSORBER_CPML 1   /* convolutional PML with auxiliary fields in the absorber slabs */

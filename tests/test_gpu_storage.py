@@ -1,6 +1,7 @@
 """16-bit field storage on the GPU against the oracle's storage emulation: every pass's output
 rounded to binary16 / bfloat16 of value / 2^k; all arithmetic FP32. Bit-exact, like the FP32
-path."""
+path, for the two-pass kernels (8 or 4 cells per thread) and for the one-pass march, which
+rounds B^{n+1/2} through the storage before the E update reads it."""
 import numpy as np
 import pytest
 
@@ -12,14 +13,18 @@ from gpu_helpers import absorber_material_scene, gpu_grid, oracle_sim, rel_l2
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("cells", [0, 4])
+KERNELS16 = [E.KERNEL_AUTO, E.KERNEL_AUTO | (4 << 8), E.KERNEL_MARCH, E.KERNEL_MARCH | (3 << 8)]
+
+
+@pytest.mark.parametrize("kernel", KERNELS16)
 @pytest.mark.parametrize("storage", [E.STORAGE_F16, E.STORAGE_BF16])
-def test_hertzian_storage_bit_exact(storage, cells):
-    """cells: 16-bit kernel cells per thread (bits 8.. of the automatic variant; 0 = 8)."""
-    print("case: hertzian 64^3 storage", storage, "cells per thread", cells, flush=True)
+def test_hertzian_storage_bit_exact(storage, kernel):
+    """kernel: the two-pass kernels at 8 or 4 cells per thread (bits 8.. of the automatic
+    variant), or the march (bits 8..: z-chunk, 0 = 8)."""
+    print("case: hertzian 64^3 storage", storage, "kernel", kernel, flush=True)
     sc = hertzian_vacuum(64, 8)
     k = [2, 2, 2, -3, -3, -3]
-    g, _, ff = gpu_grid(sc, E.KERNEL_AUTO | (cells << 8))
+    g, _, ff = gpu_grid(sc, kernel)
     g.set_storage(storage, k)
     sim = oracle_sim(sc, gpu_ntff_setup=ff.setup)
     sim.set_storage(storage, k)
@@ -32,18 +37,19 @@ def test_hertzian_storage_bit_exact(storage, cells):
     g.close()
 
 
+@pytest.mark.parametrize("kernel", [E.KERNEL_AUTO, E.KERNEL_MARCH])
 @pytest.mark.parametrize("storage", [E.STORAGE_F32, E.STORAGE_F16, E.STORAGE_BF16])
-def test_far_field_box_through_source(storage):
+def test_far_field_box_through_source(storage, kernel):
     """A far-field box whose upper x face (x = 32) runs through the Hertzian source at the centre
     of a 64^3 grid: the x-face side columns the Yee and source passes keep must hold exactly what
     the planes hold (the stored, rounded values for 16-bit storage), so the phasors stay equal to
     the oracle's bit for bit."""
-    print("case: far-field box through the source, storage", storage, flush=True)
+    print("case: far-field box through the source, storage", storage, "kernel", kernel, flush=True)
     sc = hertzian_vacuum(64, 8)
     ff0 = sc.far_field
     sc.far_field = E.FarFieldSetup((14, 13, 12), (32, 50, 49), ff0.wavelength, ff0.averaging_periods, ff0.map_axis)
     k = [2, 2, 2, -3, -3, -3]
-    g, _, ff = gpu_grid(sc)
+    g, _, ff = gpu_grid(sc, kernel)
     sim = oracle_sim(sc, gpu_ntff_setup=ff.setup)
     if storage != E.STORAGE_F32:
         g.set_storage(storage, k)
@@ -58,12 +64,13 @@ def test_far_field_box_through_source(storage):
     g.close()
 
 
+@pytest.mark.parametrize("kernel", [E.KERNEL_AUTO, E.KERNEL_MARCH])
 @pytest.mark.parametrize("storage", [E.STORAGE_F16, E.STORAGE_BF16])
-def test_material_in_absorber_storage(storage):
-    print("case: material in the absorber 48^3, storage", storage, flush=True)
+def test_material_in_absorber_storage(storage, kernel):
+    print("case: material in the absorber 48^3, storage", storage, "kernel", kernel, flush=True)
     sc = absorber_material_scene()
     k = [2, 2, 2, -3, -3, -3]
-    g, _, _ = gpu_grid(sc, ntff=False, ports=False)
+    g, _, _ = gpu_grid(sc, kernel, ntff=False, ports=False)
     g.set_storage(storage, k)
     sim = oracle_sim(sc, ntff=False, ports=False)
     sim.set_storage(storage, k)
@@ -75,15 +82,16 @@ def test_material_in_absorber_storage(storage):
     g.close()
 
 
+@pytest.mark.parametrize("kernel", [E.KERNEL_AUTO, E.KERNEL_MARCH])
 @pytest.mark.parametrize("storage", [E.STORAGE_F16, E.STORAGE_BF16])
-def test_thick_absorber_partial_rows_storage(storage):
+def test_thick_absorber_partial_rows_storage(storage, kernel):
     """16-bit kernels with a 36-cell absorber (two rows of 74 depth classes staged per wavefront:
     more than one pass over the lanes) on a 96-wide lattice (a row fills 12 of 32 lanes) and
     dielectric boxes reaching into the absorber: bit for bit against the oracle."""
-    print("case: 96^3, 36-cell absorber, storage", storage, flush=True)
+    print("case: 96^3, 36-cell absorber, storage", storage, "kernel", kernel, flush=True)
     sc = absorber_material_scene(n=96, pml=36)
     k = [2, 2, 2, -3, -3, -3]
-    g, _, _ = gpu_grid(sc, ntff=False, ports=False)
+    g, _, _ = gpu_grid(sc, kernel, ntff=False, ports=False)
     g.set_storage(storage, k)
     sim = oracle_sim(sc, ntff=False, ports=False)
     sim.set_storage(storage, k)
@@ -95,12 +103,12 @@ def test_thick_absorber_partial_rows_storage(storage):
     g.close()
 
 
-@pytest.mark.parametrize("cells", [0, 4])
-def test_spheres_dipole_f16_ports_bit_exact(cells):
-    print("case: spheres+dipole 48^3 f16 storage with ports, cells per thread", cells, flush=True)
+@pytest.mark.parametrize("kernel", KERNELS16[:3])
+def test_spheres_dipole_f16_ports_bit_exact(kernel):
+    print("case: spheres+dipole 48^3 f16 storage with ports, kernel", kernel, flush=True)
     sc = spheres_dipole(n=48, pml=8, n_spheres=4, radius_range=(3.0, 6.0), sweep_points=16)
     k = [0, 0, 0, -4, -4, -4]
-    g, pm, _ = gpu_grid(sc, E.KERNEL_AUTO | (cells << 8), ntff=False)
+    g, pm, _ = gpu_grid(sc, kernel, ntff=False)
     g.set_storage(E.STORAGE_F16, k)
     sim = oracle_sim(sc, ntff=False)
     sim.set_storage(1, k)
@@ -130,7 +138,7 @@ def test_storage_roundtrip_and_errors():
     scale = np.array([2.0, 1.0, 0.5], np.float32)[:, None]
     assert np.array_equal(e1, (e0 / scale).astype(np.float16).astype(np.float32) * scale)
     with pytest.raises(E.EmfdError):
-        g.set_kernel(E.KERNEL_MARCH)
+        g.set_kernel(E.KERNEL_TWO_PASS4)
         g.step(1)
     g.set_kernel(E.KERNEL_AUTO)
     g.step(2)
