@@ -113,7 +113,7 @@ def traffic_for(key: str):
 
 
 def run_grid(name: str, n: int, storage: str, kernel: int, steps: int, warmup: int, monitors: bool,
-             pattern: bool = False):
+             pattern: bool = False, absorber: str = "graded"):
     """Runs one workload; returns (result dict, whether the fields stayed finite)."""
     import numpy as np
     from em_lab_amd import emfdtd as E
@@ -126,6 +126,8 @@ def run_grid(name: str, n: int, storage: str, kernel: int, steps: int, warmup: i
         g.upload_materials(s, e, scene.subcell)
         g.set_sources(scene.sources)
         g.set_kernel(kernel)
+        if absorber == "cpml":
+            g.set_absorber(E.ABSORBER_CPML)  # default profile over the same 10-cell slabs
         st = {"f32": E.STORAGE_F32, "f16": E.STORAGE_F16, "bf16": E.STORAGE_BF16}[storage]
         if st != E.STORAGE_F32:
             # E near the feeds peaks at O(1); B at O(1/8): stored values stay far below 65504
@@ -160,13 +162,14 @@ def run_grid(name: str, n: int, storage: str, kernel: int, steps: int, warmup: i
         bpc = BYTES_PER_CELL if st == E.STORAGE_F32 else BYTES_PER_CELL // 2
         med_ms = statistics.median(samples) if samples else yee_ms / max(launches, 1)
         achieved = bpc * cells / (med_ms * 1e-3) / 1e9
-        key = f"{name}_{n}_{storage}_k{kernel}"
+        key = f"{name}_{n}_{storage}_k{kernel}" + ("_cpml" if absorber == "cpml" else "")
         res = {
             "value": round(cells * steps / wall / 1e6, 2),
             "ms_per_step": round(wall / steps * 1e3, 5),
-            "workload": workload_text(name, n, storage, monitors),
+            "workload": workload_text(name, n, storage, monitors)
+                        + (", convolutional PML in the absorber slabs" if absorber == "cpml" else ""),
             "grid": [nx, ny, nz], "courant": round(g.dt, 6), "material_classes": g.material_classes(),
-            "kernel": kernel, "field_storage": storage,
+            "kernel": kernel, "field_storage": storage, "absorber": absorber,
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(key),
                          "kernel": g.yee_kernel_name(), "median_launch_ms": round(med_ms, 5),
@@ -187,7 +190,9 @@ def run_grid(name: str, n: int, storage: str, kernel: int, steps: int, warmup: i
             pairs = 3 * dirs.shape[0] * ff.n_points
             res["ntff_pattern"] = {"directions": int(dirs.shape[0]), "points": int(ff.n_points), "wavelengths": 3,
                                    "seconds": round(dt_p, 4), "pairs_per_s": round(pairs / dt_p, 1),
-                                   "flop_per_pair": "sincos + 12 FMA"}
+                                   "flop_per_pair": "sincos + 12 FMA in the direct-sum form",
+                                   "method": "per-face phase-matrix products on v_mfma_f32_32x32x16_f16 "
+                                             "(fp16 hi/lo operand pairs, FP32 accumulation)"}
         return res, finite
     finally:
         g.close()
@@ -206,6 +211,8 @@ def main() -> int:
                          "scene, headline); spheres: spheres around one thin-wire dipole; vacuum: no materials")
     ap.add_argument("--storage", choices=["f32", "f16", "bf16"], default="f32",
                     help="field storage: FP32 planes, or 16-bit planes with FP32 updates")
+    ap.add_argument("--absorber", choices=["graded", "cpml"], default="graded",
+                    help="graded: the reference's conductivity absorber; cpml: the convolutional PML")
     ap.add_argument("--no-extras", action="store_true", help="skip the vacuum / 16-bit / 256^3 extra workloads")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true",
@@ -235,7 +242,7 @@ def main() -> int:
         return 0
 
     head, ok = run_grid(args.scene, args.size, args.storage, args.kernel, args.steps, args.warmup,
-                        not args.no_monitors, pattern=not args.no_monitors)
+                        not args.no_monitors, pattern=not args.no_monitors, absorber=args.absorber)
     if not ok:
         print("bench.py: non-finite fields", file=sys.stderr)
         return 2
@@ -255,7 +262,7 @@ def main() -> int:
         "dtype": "f32" if args.storage == "f32" else f"{args.storage} storage, f32 update",
         "data": "synthetic",
         "config": {k: head[k] for k in ("workload", "grid", "courant", "material_classes", "kernel",
-                                        "field_storage")},
+                                        "field_storage", "absorber")},
         "roofline": head["roofline"],
         "cpu_baseline": None,
     }

@@ -245,8 +245,9 @@ int emfd_grid_set_absorber(emfd_ctx *ctx, int kind, float sigma_max, float gradi
  * with scale_log2[6] = k per component (E x y z, B x y z; NULL = all 0), the stored value being
  * field / 2^k rounded to nearest even. Every update still computes in FP32; only the planes are
  * 16-bit, which halves the field traffic. 16-bit storage needs nx % 8 == 0 and runs the
- * two-pass kernel (EMFD_KERNEL_AUTO; bits 8..11 of the variant = cells per thread, 8 or 4,
- * 0 = 8). EMFD_STORAGE_F64 runs every update and source in double
+ * two-pass kernel under EMFD_KERNEL_AUTO (bits 8..11 of the variant = cells per thread, 8 or 4,
+ * 0 = 8), or the one-pass EMFD_KERNEL_MARCH when chosen explicitly; both give the same bits.
+ * EMFD_STORAGE_F64 runs every update and source in double
  * precision (the reference for accuracy studies); monitors read its fields as float. Clears the fields and rewinds time, like
  * emfd_grid_upload_materials. Downloads and uploads convert to and from FP32 values. */
 int emfd_grid_set_storage(emfd_ctx *ctx, int storage, const int *scale_log2);

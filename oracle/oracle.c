@@ -9,20 +9,35 @@
  * the reference's operand order; the file is compiled with -ffp-contract=off so that no
  * multiply-add is fused behind the reader's back.
  *
- * Pinning status (what checks each function; the tests live in tests/test_oracle_*.py):
- *   f16 packing          known answers (binary16 RNE) + exhaustive round trip of all halves
+ * Pinning status (what checks each function; the tests live in tests/test_oracle_*.py,
+ * tests/test_storage_cpu.py and tests/test_coax_cpu.py):
+ *   f16 packing          known answers (binary16 RNE) + numpy's RNE conversion on 69k values
  *   configure            clamp table from src/SimulationGrid.cpp:149-156
- *   update_B / update_E  cavity eigenfrequency, discrete energy invariant, lossy decay
- *                        closed form, brute-force 4^3 step, phi continuity
- *   apply_sources        sum J = I footprint closed form, cell-list brute force
+ *   upload_materials     linear media: every edge average equals the medium at the edge's
+ *                        midpoint; one isolated cell touches exactly the 12 edges its geometry
+ *                        implies (index-0 clamping at the lower faces)
+ *   update_B / update_E  exact discrete plane wave, discrete energy invariant, lossy decay
+ *                        closed form, phi continuity across its branch point
+ *   apply_sources        sum J = J0 x Poisson-summed footprint, cell-list brute force
  *   signal               direct evaluation of the formula at sample times
  *   port / ntff EMA      two-stage EMA closed form for a pure sinusoid
- *   ntff pattern         Hertzian dipole closed-form near field -> D = 1.5, HPBW 90 deg;
+ *   e_at_node / b_at_node / ntff_samples
+ *                        linear fields at each component's stagger return the analytic node
+ *                        value exactly (a half-cell offset fails)
+ *   ntff pattern         Hertzian dipole closed-form near field -> D = 1.5 sin^2, P = k^2/12pi;
  *                        f32 order vs f64 brute force
- *   flux / probe         brute force on small grids, plane-wave closed form
+ *   flux / probe         uniform and linear fields, closed-form plane and box flux
  *   sweep                resistor closed form (Z = R), recurrence vs direct polar
- * GLSL built-ins (exp, sin, cos, unpackHalf2x16) are driver-defined in the reference;
- * here they are the C library's. That boundary is parity unpinned at the ulp level.
+ *   line_accumulate      Laplace / gradient / Stokes identities of the term lists
+ *   step_storage         numpy float16 / torch bfloat16 conversions, incl. ties and subnormals
+ *   cpml_*               closed-form reflection of a graded stretched-coordinate layer, held to
+ *                        the lattice's sampling excess (10-20% in ln R at 8 cells); parity
+ *                        unpinned beyond that bound
+ * A planted-mistake run (tests/test_oracle_mutations.py) shows each pin rejecting the mistakes
+ * it is meant to catch.
+ * GLSL built-ins are driver-defined in the reference. exp is or_exp below (the library
+ * evaluates the same method); sin and cos in the pattern are the C library's, so that
+ * boundary is parity unpinned at the ulp level (patterns are compared in dB).
  */
 #include "oracle.h"
 
