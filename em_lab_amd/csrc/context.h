@@ -119,6 +119,22 @@ struct emfd_ctx {
     // reads a thin-wire texel's B (texel itself, +x, +y, +z)
     emfd::DevBuf fix_b, fix_e;
     int n_fix_b = 0, n_fix_e = 0;
+    // the march's own material coefficients (emfd::MarchMat, 8-plane chunks): per texel local
+    // class bytes, per tile (first entry, count), entries; and what the E recomputation still
+    // takes with them (fix_e_rest: thin-wire neighbours, materials in the absorber, overflow)
+    emfd::DevBuf march_lid, march_tiles, march_coef, fix_e_rest;
+    int n_fix_e_rest = 0;
+    bool march_tables = false;
+    std::vector<uint32_t> march_classes;  // class of each entry (host copy, until the table exists)
+    emfd::MarchMat march_mat() const {
+        emfd::MarchMat m;
+        if (march_tables) {
+            m.lid = march_lid.as<uint8_t>();
+            m.tiles = march_tiles.as<int2>();
+            m.coef = march_coef.as<float>();
+        }
+        return m;
+    }
     // absorber: EMFD_ABSORBER_GRADED (the reference's) or EMFD_ABSORBER_CPML, with the latter's
     // coefficient tables and auxiliary fields
     int absorber = 0;

@@ -433,7 +433,8 @@ int emfd_grid_configure(emfd_ctx *ctx, const emfd_grid_config *requested, emfd_g
     // vacuum everywhere: E = (0,0,0, sigma 0), B = (0,0,0, eps 1) (src/SimulationGrid.cpp:185-192)
     ctx->classes = 0;
     ctx->has_wire = 0;
-    ctx->n_fix_b = ctx->n_fix_e = 0;
+    ctx->n_fix_b = ctx->n_fix_e = ctx->n_fix_e_rest = 0;
+    ctx->march_tables = false;
     ctx->id_bytes = 0;
     ctx->masks_ok = false;
     ctx->pml_materials = 0;  // vacuum everywhere
@@ -471,19 +472,96 @@ int build_fix_lists(emfd_ctx *ctx, const std::vector<ClassKey> &keys, const uint
         fe.push_back(static_cast<uint32_t>(i));
         if (!wire_class[id]) continue;
         fb.push_back(static_cast<uint32_t>(i));
+        near.push_back(static_cast<uint32_t>(i));
         const size_t x = i % nx, y = (i / nx) % ny, z = i / (static_cast<size_t>(nx) * ny);
         if (x + 1 < static_cast<size_t>(nx)) near.push_back(static_cast<uint32_t>(i + 1));
         if (y + 1 < static_cast<size_t>(ny)) near.push_back(static_cast<uint32_t>(i + nx));
         if (z + 1 < static_cast<size_t>(nz)) near.push_back(static_cast<uint32_t>(i + static_cast<size_t>(nx) * ny));
     }
-    if (!near.empty()) {
-        std::sort(near.begin(), near.end());
+    std::sort(near.begin(), near.end());
+    near.erase(std::unique(near.begin(), near.end()), near.end());
+    auto merge_near = [&near](std::vector<uint32_t> &list) {
+        if (near.empty()) return;
         std::vector<uint32_t> merged;
-        merged.reserve(fe.size() + near.size());
-        std::merge(fe.begin(), fe.end(), near.begin(), near.end(), std::back_inserter(merged));
+        merged.reserve(list.size() + near.size());
+        std::merge(list.begin(), list.end(), near.begin(), near.end(), std::back_inserter(merged));
         merged.erase(std::unique(merged.begin(), merged.end()), merged.end());
-        fe.swap(merged);
+        list.swap(merged);
+    };
+
+    // The march's material tables (its automatic 8-plane chunks): each texel of a class other than
+    // vacuum outside the absorber gets its 1-based position in its tile's class list (up to 255
+    // classes per tile); the others (absorber, overflow) stay in the E recomputation list.
+    ctx->march_tables = false;
+    ctx->march_classes.clear();
+    ctx->n_fix_e_rest = 0;
+    if (!fe.empty() && nx % 4 == 0 && nz >= kMarchTileZ) {
+        const int L = ctx->cfg.pml_thickness;
+        const int nseg = (nx + kMarchTileX - 1) / kMarchTileX, nyt = (ny + kMarchTileRows - 1) / kMarchTileRows,
+                  nzc = (nz + kMarchTileZ - 1) / kMarchTileZ;
+        const size_t n_tiles = static_cast<size_t>(nseg) * nyt * nzc;
+        std::vector<std::vector<uint32_t>> tile_cls(n_tiles);
+        std::vector<uint8_t> lid(n, 0);
+        std::vector<uint32_t> rest;
+        size_t last_tile = SIZE_MAX;
+        uint32_t last_id = 0;
+        uint8_t last_l = 0;
+        for (const uint32_t q : fe) {
+            const size_t i = q;
+            const int x = static_cast<int>(i % nx), y = static_cast<int>((i / nx) % ny),
+                      z = static_cast<int>(i / (static_cast<size_t>(nx) * ny));
+            const bool absorber = x < L || y < L || z < L || x >= nx - (L + 1) || y >= ny - (L + 1) || z >= nz - (L + 1);
+            if (absorber) {
+                rest.push_back(q);
+                continue;
+            }
+            const size_t t = static_cast<size_t>(x / kMarchTileX) +
+                             nseg * (static_cast<size_t>(y / kMarchTileRows) + static_cast<size_t>(nyt) * (z / kMarchTileZ));
+            const uint32_t id = ids[i];
+            uint8_t l = 0;
+            if (t == last_tile && id == last_id) {
+                l = last_l;
+            } else {
+                std::vector<uint32_t> &cls = tile_cls[t];
+                const auto it = std::find(cls.begin(), cls.end(), id);
+                if (it != cls.end()) {
+                    l = static_cast<uint8_t>(it - cls.begin() + 1);
+                } else if (cls.size() < static_cast<size_t>(kMarchMaxLocal)) {
+                    cls.push_back(id);
+                    l = static_cast<uint8_t>(cls.size());
+                }
+                last_tile = t;
+                last_id = id;
+                last_l = l;
+            }
+            if (l == 0) rest.push_back(q);  // past the tile's 255 classes
+            lid[i] = l;
+        }
+        std::vector<int2> tiles(n_tiles);
+        size_t entries = 0;
+        for (size_t t = 0; t < n_tiles; ++t) {
+            tiles[t] = make_int2(static_cast<int>(entries), static_cast<int>(tile_cls[t].size()));
+            entries += tile_cls[t].size();
+        }
+        if (entries > 0 && entries < 0x7fffffffu) {
+            ctx->march_classes.reserve(entries);
+            for (const auto &cls : tile_cls) ctx->march_classes.insert(ctx->march_classes.end(), cls.begin(), cls.end());
+            if (int rc = ctx->march_lid.ensure(n)) return rc;
+            if (int rc = ctx->march_tiles.ensure(n_tiles * sizeof(int2))) return rc;
+            EMFD_HIP(hipMemcpy(ctx->march_lid.ptr, lid.data(), n, hipMemcpyHostToDevice));
+            EMFD_HIP(hipMemcpy(ctx->march_tiles.ptr, tiles.data(), n_tiles * sizeof(int2), hipMemcpyHostToDevice));
+            merge_near(rest);
+            if (rest.size() > 0x7fffffffu) return fail(EMFD_ERR_INVALID, "too many material texels");
+            if (!rest.empty()) {
+                if (int rc = ctx->fix_e_rest.ensure(rest.size() * sizeof(uint32_t))) return rc;
+                EMFD_HIP(hipMemcpy(ctx->fix_e_rest.ptr, rest.data(), rest.size() * sizeof(uint32_t),
+                                   hipMemcpyHostToDevice));
+            }
+            ctx->n_fix_e_rest = static_cast<int>(rest.size());
+            ctx->march_tables = true;  // once the entries hold the class coefficients (fill_march_coef)
+        }
     }
+    merge_near(fe);
     if (fb.size() > 0x7fffffffu || fe.size() > 0x7fffffffu) return fail(EMFD_ERR_INVALID, "too many material texels");
     if (!fb.empty()) {
         if (int rc = ctx->fix_b.ensure(fb.size() * sizeof(uint32_t))) return rc;
@@ -495,6 +573,28 @@ int build_fix_lists(emfd_ctx *ctx, const std::vector<ClassKey> &keys, const uint
     }
     ctx->n_fix_b = static_cast<int>(fb.size());
     ctx->n_fix_e = static_cast<int>(fe.size());
+    return EMFD_OK;
+}
+
+// Entries of the march's material tables: Ca x3, Cb x3 of each listed class, from the class table
+// the E kernels read (edge_coefficients: a class other than vacuum outside the absorber).
+int fill_march_coef(emfd_ctx *ctx) {
+    if (!ctx->march_tables) return EMFD_OK;
+    std::vector<float> table(static_cast<size_t>(ctx->classes) * kTableStride);
+    EMFD_HIP(hipMemcpy(table.data(), ctx->table.ptr, table.size() * sizeof(float), hipMemcpyDeviceToHost));
+    const size_t entries = ctx->march_classes.size();
+    std::vector<float> coef(entries * kMarchCoefStride, 0.0f);
+    for (size_t k = 0; k < entries; ++k) {
+        const float *t = table.data() + static_cast<size_t>(ctx->march_classes[k]) * kTableStride;
+        for (int c = 0; c < 3; ++c) {
+            coef[k * kMarchCoefStride + c] = t[kCa + c];
+            coef[k * kMarchCoefStride + 3 + c] = t[kCb + c];
+        }
+    }
+    if (int rc = ctx->march_coef.ensure(coef.size() * sizeof(float))) return rc;
+    EMFD_HIP(hipMemcpy(ctx->march_coef.ptr, coef.data(), coef.size() * sizeof(float), hipMemcpyHostToDevice));
+    ctx->march_classes.clear();
+    ctx->march_classes.shrink_to_fit();
     return EMFD_OK;
 }
 }  // namespace
@@ -530,6 +630,7 @@ int emfd_grid_upload_materials(emfd_ctx *ctx, const float *sigma, const float *e
     if (keys.size() == 1 && !thin) {
         ctx->classes = 0;  // uniform vacuum: kernels skip the class id stream
         ctx->id_bytes = 0;
+        ctx->march_tables = false;
     } else {
         ctx->classes = static_cast<int>(keys.size());
         ctx->id_bytes = keys.size() <= 65536 ? 2 : 4;
@@ -557,6 +658,7 @@ int emfd_grid_upload_materials(emfd_ctx *ctx, const float *sigma, const float *e
         }
         EMFD_HIP(hipStreamSynchronize(ctx->stream));
         d_keys.release();
+        if (int rc = fill_march_coef(ctx)) return rc;
     }
     // uploadMaterials clears E and B and rewinds time (src/SimulationGrid.cpp:216-222,294)
     ctx->cur = 0;
@@ -900,6 +1002,7 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
             const int zc = ctx->storage != kStorageF32 && ctx->kernel_variant == EMFD_KERNEL_AUTO ? 0 : ctx->tile_cfg;
             EMFD_HIP(launch_yee_march(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), zc,
                                       ctx->fix_b.as<uint32_t>(), ctx->n_fix_b, ctx->fix_e.as<uint32_t>(), ctx->n_fix_e,
+                                      ctx->march_mat(), ctx->fix_e_rest.as<uint32_t>(), ctx->n_fix_e_rest,
                                       ctx->stream));
             ctx->cur ^= 1;
         }

@@ -118,12 +118,27 @@ struct StepConsts {
 
 // ---- launchers (kernels.hip) ----
 hipError_t launch_build_table(const uint16_t *keys, int classes, float dt, float *table, hipStream_t s);
+// Material coefficients the one-pass march applies itself (march tiles of its default 8-plane
+// z-chunk: 256 x cells, 7 rows, 8 planes; tile = x / 256 + nseg * (y / 7 + nyt * (z / 8))). Each
+// tile lists up to 255 of the classes of its texels outside the absorber; lid holds a texel's
+// 1-based position in its tile's list (0: none, the march's vacuum / absorber coefficients).
+// Entries: kMarchCoefStride floats each (Ca x3, Cb x3 of the class, 2 unused).
+constexpr int kMarchTileRows = 7, kMarchTileZ = 8, kMarchTileX = 256, kMarchMaxLocal = 255;
+constexpr int kMarchCoefStride = 8;
+struct MarchMat {
+    const uint8_t *lid = nullptr;  // per texel, lattice order; null: no material tables
+    const int2 *tiles = nullptr;   // per tile: first entry, entry count
+    const float *coef = nullptr;   // entries x kMarchCoefStride floats
+};
+
 // Fused one-pass z-march (yee_march.hip): src -> dst, then the sparse recomputation of the texels
 // in fix_b (B, thin-wire classes) and fix_e (E, non-vacuum classes and thin-wire neighbours).
-// cfg: z-chunk (0 = automatic).
+// cfg: z-chunk (0 = automatic). With the default 8-plane chunk and material tables (mm.lid set),
+// the march applies the listed classes' coefficients and the E recomputation takes fix_e_rest
+// (thin-wire neighbours, materials in the absorber, classes past a tile's 255) instead of fix_e.
 hipError_t launch_yee_march(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int cfg,
                             const uint32_t *fix_b, int n_fix_b, const uint32_t *fix_e, int n_fix_e,
-                            hipStream_t s);
+                            const MarchMat &mm, const uint32_t *fix_e_rest, int n_fix_e_rest, hipStream_t s);
 int yee_march_supported(const LatticeArgs &a);
 // passes: bit 0 the Faraday (B) pass, bit 1 the Ampere (E) pass
 hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s, int passes = 3);

@@ -73,6 +73,39 @@ __device__ __forceinline__ void side_put4(const SideCols &s, uint32_t bits, int 
             }
 }
 
+// The same values into the wave's LDS record of the chunk (kz = plane within the chunk), at
+// [face][column][component][kz]; side_flush stores the record once the chunk is done, so that no
+// scattered store sits among the march's loads and stores (each would enter the wait counts of
+// the plane loads in flight).
+__device__ __forceinline__ void side_lds4(const SideCols &s, uint32_t bits, float *rec, int comp0, int x0, int kz,
+                                          const float (&v)[3][4]) {
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (bits & (1u << (f * 4 + j))) {
+                const int xo = x0 + j - (s.x[f] - 1);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) rec[((f * 2 + xo) * 6 + comp0 + c) * 8 + kz] = v[c][j];
+            }
+}
+__device__ __forceinline__ void side_flush(const SideCols &s, const float *rec, int lane, int xs, int nx, int y, int z0,
+                                           int z1) {
+    // the record was written by this wave's lanes: LDS operations of a wave complete in order,
+    // the fences keep the compiler from moving the reads above the writes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const long col = static_cast<long>(s.nz) * s.ny;
+    for (int i = lane; i < 2 * 2 * 6 * 8; i += 64) {
+        const int f = i / 96, xo = (i / 48) & 1, comp = (i / 8) % 6, kz = i & 7;
+        const int z = z0 + kz, x = s.x[f] - 1 + xo;
+        if (z < z1 && z >= s.z0 && z < s.z0 + s.nz && x >= xs && x < xs + 256 && x < nx)
+            s.buf[(static_cast<long>(f * 6 + comp) * 2 + xo) * col + static_cast<long>(z - s.z0) * s.ny + (y - s.y0)] =
+                rec[i];
+    }
+}
+
 // Plane reads go through buffer resources: a byte offset per lane against the plane's 32-bit
 // range, where an offset at or past the range (kOob) reads 0 without touching memory. That gives
 // the lattice's zero boundaries and idle lanes without a select the compiler could turn into a
@@ -140,13 +173,26 @@ __device__ __forceinline__ float rnd(float v, float inv, float s) {
     else return static_cast<float>(static_cast<T>(v * inv)) * s;
 }
 
-// LDS layout (floats): xe[2][NW][2][kSeg] | xb[2][NW][2][kSeg] | coef[NW][2][K * kCoefStride]
+// LDS layout (floats): xe[2][NW][2][kSeg] | xb[2][NW][2][kSeg] | mat[kMatFloats] |
+// side[NW][kSideFloats] | coef[NW][2][K * kCoefStride]
 //   xe[p & 1][w]: E^n(row of wave w, plane p) z, x components
 //   xb[p & 1][w]: B^{n+1/2}(row of wave w, plane p) x, z components
+//   mat: the tile's material classes, entry l (1-based local class) at l * kCoefStride
+//   side[w]: far-field side column values of wave w's row over the chunk (8-plane chunks)
 template <int NW>
 __host__ __device__ constexpr size_t march_xch_floats() {
     return static_cast<size_t>(2) * 2 * NW * 2 * kSeg;
 }
+// one float4 per thread of the 512 copies entries 1 .. 256 (2 per entry)
+constexpr int kMatFloats = kMarchCoefStride + 4 * 512;
+// [face][column x = X - 1, X][component E x y z, B x y z][plane of the chunk]
+constexpr int kSideFloats = 2 * 2 * 6 * kMarchTileZ;
+template <int NW>
+__host__ __device__ constexpr size_t march_lds_fixed_floats() {
+    return march_xch_floats<NW>() + kMatFloats + static_cast<size_t>(NW) * kSideFloats;
+}
+static_assert(kMarchCoefStride == kCoefStride, "march material entries use the absorber entries' layout");
+static_assert(2 * (kMarchMaxLocal + 1) <= 512 + 2, "one float4 copy per thread covers a tile's entries");
 
 // ZC > 0: the z-chunk is ZC planes and the march is unrolled over all ZC + 1 of its planes (the
 // leading one included; the first chunk leads with a virtual plane -1 of zero fields, a partial
@@ -155,7 +201,8 @@ __host__ __device__ constexpr size_t march_xch_floats() {
 // flight. ZC = 0: a loop over any z-chunk.
 template <int NW, int ZC, class T>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16 / NW, 16 / NW))) k_yee_march(LatticeArgs a, FieldPtrs src, FieldPtrs dst, int zchunk,
-                                                        int nseg, int nyt, int n_tiles) {
+                                                        int nseg, int nyt, int n_tiles, MarchMat mm) {
+    static_assert(ZC == 0 || ZC == kMarchTileZ, "material tables and side records are laid out for 8-plane chunks");
     extern __shared__ float lds[];
     const int lane = static_cast<int>(threadIdx.x) & 63;
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
@@ -211,8 +258,35 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
 
     float *xe = lds;
     float *xb = lds + 2 * NW * 2 * kSeg;
-    float *coef = lds + march_xch_floats<NW>() + static_cast<size_t>(w) * 2 * K * kCoefStride;
+    float *mat = lds + march_xch_floats<NW>();
+    float *side_l = mat + kMatFloats + w * kSideFloats;
+    float *coef = lds + march_lds_fixed_floats<NW>() + static_cast<size_t>(w) * 2 * K * kCoefStride;
     float *coefB = coef, *coefE = coef + K * kCoefStride;
+    // The tile's material classes (8-plane chunks with tables only): copied into LDS by one
+    // unconditional float4 load per thread (a range of 0 reads zeros), issued before the plane
+    // loads so that waiting for it waits for nothing else. Per texel local class ids (4 per lane,
+    // one byte each) come with the B loads of their plane, from a range of 0 where the tile has
+    // no listed class.
+    int mat_first = 0, mat_n = 0;
+    if (ZC > 0 && mm.tiles) {
+        const int2 ti = mm.tiles[tile];
+        mat_first = ti.x;
+        mat_n = ti.y;
+    }
+    float mat_v[4];
+    {
+        const float *cb = mat_n > 0 ? mm.coef + static_cast<size_t>(mat_first) * kMarchCoefStride : src.e[0];
+        bld4(plane_rsrc(cb, static_cast<uint32_t>(mat_n) * kMarchCoefStride * 4u), static_cast<uint32_t>(threadIdx.x) * 16u,
+             mat_v);
+    }
+    const uint32_t lid_plane = static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny);
+    const uint32_t o_lid = out_row && cell_ok ? static_cast<uint32_t>(y) * nx + x0 : kOob;
+    auto lid_of = [&](int z, int zmax) {
+        const bool ok = mat_n > 0 && z >= 0 && z <= zmax;
+        const char *base = ok ? reinterpret_cast<const char *>(mm.lid) + static_cast<size_t>(z) * lid_plane
+                              : reinterpret_cast<const char *>(src.e[0]);
+        return plane_rsrc(reinterpret_cast<const float *>(base), ok ? lid_plane : 0u);
+    };
     auto xe_at = [&](int p, int wave, int comp) { return xe + ((((p & 1) * NW + wave) * 2 + comp) * kSeg); };
     auto xb_at = [&](int p, int wave, int comp) { return xb + ((((p & 1) * NW + wave) * 2 + comp) * kSeg); };
 
@@ -242,6 +316,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
         float hu;        // lane 0: E^n x at (xs - 1, y + 1)
         float hr[2];     // lane 63: E^n y, z at xs + 256
         float eu[2][4];  // top wave: E^n z, x of row y + 1
+        uint32_t lid;    // local material classes of the lane's four cells (byte j: cell j)
     };
     auto load_b = [&](int z, BSet &s) {
         const Rsrc b0 = plane_of(src.b[0], z, zb_max), b1 = plane_of(src.b[1], z, zb_max),
@@ -258,6 +333,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
         s.hr[1] = ld1<T>(e2, o_hr, sE[2]);
         ld4<T>(e2, o_up, sE[2], s.eu[0]);  // kOob but for the top wave
         ld4<T>(e0, o_up, sE[0], s.eu[1]);
+        s.lid = __builtin_amdgcn_raw_buffer_load_b32(lid_of(z, zb_max), o_lid, 0, 0);
     };
     // absorber coefficients of this wave's row at plane z: entry cx for (cx, class(y), class(z));
     // the wave issues the table loads, waits, and stores them (one entry per lane, more when the
@@ -295,6 +371,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
     load_e(zs + 2, eP, hlP);
     load_b(zs, bc);
     load_b(zs + 1, bP);
+    *reinterpret_cast<float4 *>(mat + kMarchCoefStride + 4 * threadIdx.x) = make_float4(mat_v[0], mat_v[1], mat_v[2], mat_v[3]);
     // publish E^n(zs) z, x of the row for the wave below
     *reinterpret_cast<float4 *>(xe_at(zs, w, 0) + 4 * lane) = make_float4(ec.v[2][0], ec.v[2][1], ec.v[2][2], ec.v[2][3]);
     *reinterpret_cast<float4 *>(xe_at(zs, w, 1) + 4 * lane) = make_float4(ec.v[0][0], ec.v[0][1], ec.v[0][2], ec.v[0][3]);
@@ -393,7 +470,9 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
                 const float curl1 = (bn[0][j] - bm[0][j]) - (bn[2][j] - bxm_z);
                 const float curl2 = (bn[1][j] - bxm_y) - (bn[0][j] - bxd[j]);
                 float ca[3], cb[3];
-                read_coef(coefE + cx[j] * kCoefStride, ca, cb);
+                // a listed material class, else the vacuum edge's (absorber) coefficients
+                const uint32_t l = (bc.lid >> (8 * j)) & 0xffu;
+                read_coef(l != 0u ? mat + l * kMarchCoefStride : coefE + cx[j] * kCoefStride, ca, cb);
                 e[0][j] = rnd<T>(ca[0] * ec.v[0][j] + cb[0] * curl0, iE[0], sE[0]);
                 e[1][j] = rnd<T>(ca[1] * ec.v[1][j] + cb[1] * curl1, iE[1], sE[1]);
                 e[2][j] = rnd<T>(ca[2] * ec.v[2][j] + cb[2] * curl2, iE[2], sE[2]);
@@ -406,7 +485,13 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
                     st4<T>(de[c], o, iE[c], e[c]);
                 }
             }
-            if (side_row && z >= a.side.z0 && z < a.side.z0 + a.side.nz && sbits != 0u) {
+            if constexpr (ZC > 0) {
+                // into LDS; stored once the chunk is done
+                if (side_row && sbits != 0u) {
+                    side_lds4(a.side, sbits, side_l, 3, x0, z - z0, bn);
+                    side_lds4(a.side, sbits, side_l, 0, x0, z - z0, e);
+                }
+            } else if (side_row && z >= a.side.z0 && z < a.side.z0 + a.side.nz && sbits != 0u) {
                 side_put4(a.side, sbits, 3, x0, y, z, bn);
                 side_put4(a.side, sbits, 0, x0, y, z, e);
             }
@@ -425,6 +510,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
             plane_step(zs + k, eP, eQ, hlP, hlQ, bP, bQ);
             if (k + 1 <= ZC) plane_step(zs + k + 1, eQ, eP, hlQ, hlP, bQ, bP);
         }
+        if (side_row) side_flush(a.side, side_l, lane, xs, nx, y, z0, z1);
     } else {
         for (int z = zs; z < z1; z += 2) {
             plane_step(z, eP, eQ, hlP, hlQ, bP, bQ);
@@ -521,7 +607,8 @@ __global__ void __launch_bounds__(256) k_march_fix_e(LatticeArgs a, FieldPtrs sr
 }
 
 template <int NW, class T>
-hipError_t launch_nw(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int zchunk, hipStream_t s) {
+hipError_t launch_nw(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int zchunk, const MarchMat &mm,
+                     hipStream_t s) {
     const int nseg = (a.nx + kSeg - 1) / kSeg;
     const int nyt = (a.ny + (NW - 2)) / (NW - 1);
     if (zchunk <= 0) {
@@ -534,7 +621,7 @@ hipError_t launch_nw(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs
     if (zchunk > a.nz) zchunk = a.nz;
     const int nzc = (a.nz + zchunk - 1) / zchunk;
     const int n_tiles = nseg * nyt * nzc;
-    const size_t lds = (march_xch_floats<NW>() + static_cast<size_t>(NW) * 2 * a.pml_k * kCoefStride) * sizeof(float);
+    const size_t lds = (march_lds_fixed_floats<NW>() + static_cast<size_t>(NW) * 2 * a.pml_k * kCoefStride) * sizeof(float);
     auto allow_lds = [&](const void *kernel, size_t &raised) -> hipError_t {
         // dynamic LDS past 64 KB must be allowed per kernel
         if (lds <= 65536 || lds <= raised) return hipSuccess;
@@ -547,12 +634,12 @@ hipError_t launch_nw(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs
         static size_t raised = 0;
         if (hipError_t e = allow_lds(reinterpret_cast<const void *>(&k_yee_march<NW, 8, T>), raised); e != hipSuccess)
             return e;
-        k_yee_march<NW, 8, T><<<n_tiles, 64 * NW, lds, s>>>(a, src, dst, zchunk, nseg, nyt, n_tiles);
+        k_yee_march<NW, 8, T><<<n_tiles, 64 * NW, lds, s>>>(a, src, dst, zchunk, nseg, nyt, n_tiles, mm);
     } else {
         static size_t raised = 0;
         if (hipError_t e = allow_lds(reinterpret_cast<const void *>(&k_yee_march<NW, 0, T>), raised); e != hipSuccess)
             return e;
-        k_yee_march<NW, 0, T><<<n_tiles, 64 * NW, lds, s>>>(a, src, dst, zchunk, nseg, nyt, n_tiles);
+        k_yee_march<NW, 0, T><<<n_tiles, 64 * NW, lds, s>>>(a, src, dst, zchunk, nseg, nyt, n_tiles, MarchMat{});
     }
     return hipGetLastError();
 }
@@ -566,12 +653,19 @@ int yee_march_supported(const LatticeArgs &a) {
 
 hipError_t launch_yee_march(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int cfg,
                             const uint32_t *fix_b, int n_fix_b, const uint32_t *fix_e, int n_fix_e,
-                            hipStream_t s) {
+                            const MarchMat &mm, const uint32_t *fix_e_rest, int n_fix_e_rest, hipStream_t s) {
     if (!yee_march_supported(a)) return hipErrorInvalidValue;
     const int zchunk = cfg;  // 0: automatic
-    hipError_t e = src.storage == kStorageF16    ? launch_nw<8, _Float16>(a, src, dst, zchunk, s)
-                   : src.storage == kStorageBF16 ? launch_nw<8, __bf16>(a, src, dst, zchunk, s)
-                   : src.storage == kStorageF32  ? launch_nw<8, float>(a, src, dst, zchunk, s)
+    // the material tables are laid out for the automatic 8-plane chunks
+    const bool tables = mm.lid && mm.tiles && mm.coef && (zchunk == 0 || zchunk == kMarchTileZ) && a.nz >= kMarchTileZ;
+    const MarchMat use = tables ? mm : MarchMat{};
+    if (tables) {
+        fix_e = fix_e_rest;
+        n_fix_e = n_fix_e_rest;
+    }
+    hipError_t e = src.storage == kStorageF16    ? launch_nw<8, _Float16>(a, src, dst, zchunk, use, s)
+                   : src.storage == kStorageBF16 ? launch_nw<8, __bf16>(a, src, dst, zchunk, use, s)
+                   : src.storage == kStorageF32  ? launch_nw<8, float>(a, src, dst, zchunk, use, s)
                                                  : hipErrorInvalidValue;
     if (e != hipSuccess) return e;
     auto grid8 = [](int n) { return ((n + 255) / 256 + 7) / 8 * 8; };
