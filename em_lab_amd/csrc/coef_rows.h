@@ -37,7 +37,9 @@ __device__ __forceinline__ CoefLoad coef_load(const LatticeArgs &a, int i, int y
     const int r = (R == 2 && ii >= K) ? 1 : 0;
     const int cx = ii - r * K;
     const int y = y_first + r;
-    const int cy = y < a.ny ? pml_class(y, a.ny, L) : 0, cz = pml_class(z, a.nz, L);
+    // a row or plane outside the lattice (idle work, nothing stored) takes class 0: a depth class
+    // of its own would index past the table
+    const int cy = y >= 0 && y < a.ny ? pml_class(y, a.ny, L) : 0, cz = z >= 0 && z < a.nz ? pml_class(z, a.nz, L) : 0;
     c.interior = (cx | cy | cz) == 0;
     const float *t = a.pml_tab + static_cast<size_t>(cx + K * (cy + K * cz)) * kPmlStride + off;
 #pragma unroll

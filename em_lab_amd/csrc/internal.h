@@ -27,13 +27,23 @@ enum TableField : int {
 // x (X = box min / max x): the Yee passes (and the source pass) write each value there as they
 // store it, so the far-field accumulation reads those faces coalesced instead of gathering one or
 // two floats per cache line. Layout [face][component 0..5 (E, B)][x - (X - 1)][z - z0][y - y0].
+// Layout [face][component E x y z, B x y z][column x = X - 1, X][z - z0][yp]: y in groups of
+// kSideGroup rows (the one-pass march's row groups, y / 7) padded to 8 slots, yp = 8 (y / 7 -
+// gy0) + y % 7, so that the march stores a group's values of one plane as one aligned 32-byte
+// sector.
+constexpr int kSideGroup = 7;
 struct SideCols {
     float *buf;   // null: not maintained
     int x[2];     // X of face 0 and face 1
     int y0, ny;   // y range (box min y - 1 .. box max y)
     int z0, nz;   // z range
     int e_valid;  // the E components are maintained too (else only B: read E from the planes)
+    int gy0;      // first row group: y0 / 7
+    int nyp;      // padded slots per z: 8 x row groups
 };
+__host__ __device__ __forceinline__ int side_yp(const SideCols &s, int y) {
+    return (y / kSideGroup - s.gy0) * 8 + y % kSideGroup;
+}
 
 // Per-grid constants every lattice kernel needs.
 struct LatticeArgs {
@@ -63,8 +73,8 @@ struct LatticeArgs {
 template <int N, class F>
 __device__ __forceinline__ void side_write_cells(const SideCols &s, int comp0, int x0, int y, int z, F value) {
     if (!s.buf || y < s.y0 || y >= s.y0 + s.ny || z < s.z0 || z >= s.z0 + s.nz) return;
-    const long col = static_cast<long>(s.nz) * s.ny;  // one x column of one component
-    const long yz = static_cast<long>(z - s.z0) * s.ny + (y - s.y0);
+    const long col = static_cast<long>(s.nz) * s.nyp;  // one x column of one component
+    const long yz = static_cast<long>(z - s.z0) * s.nyp + side_yp(s, y);
 #pragma unroll
     for (int face = 0; face < 2; ++face) {
         const int xo0 = x0 - (s.x[face] - 1);  // column offset of cell j: xo0 + j, kept for 0 or 1
@@ -84,7 +94,7 @@ __device__ __forceinline__ void side_write_cells(const SideCols &s, int comp0, i
 __host__ __device__ inline long side_index(const SideCols &s, int face, int comp, int x, int y, int z) {
     const int xo = x - (s.x[face] - 1);
     if (xo < 0 || xo > 1 || y < s.y0 || y >= s.y0 + s.ny || z < s.z0 || z >= s.z0 + s.nz) return -1;
-    return ((static_cast<long>(face * 6 + comp) * 2 + xo) * s.nz + (z - s.z0)) * s.ny + (y - s.y0);
+    return ((static_cast<long>(face * 6 + comp) * 2 + xo) * s.nz + (z - s.z0)) * s.nyp + side_yp(s, y);
 }
 
 // Absorber coefficient table entry (vacuum edges): B: exp(-s dt), dt phi(s dt) per component;
@@ -124,6 +134,7 @@ hipError_t launch_build_table(const uint16_t *keys, int classes, float dt, float
 // 1-based position in its tile's list (0: none, the march's vacuum / absorber coefficients).
 // Entries: kMarchCoefStride floats each (Ca x3, Cb x3 of the class, 2 unused).
 constexpr int kMarchTileRows = 7, kMarchTileZ = 8, kMarchTileX = 256, kMarchMaxLocal = 255;
+static_assert(kMarchTileRows == kSideGroup, "side column row groups are the march's");
 constexpr int kMarchCoefStride = 8;
 struct MarchMat {
     const uint8_t *lid = nullptr;  // per texel, lattice order; null: no material tables

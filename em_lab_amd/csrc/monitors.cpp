@@ -662,7 +662,9 @@ int emfd_ntff_configure(emfd_ctx *ctx, const emfd_far_field_setup *setup, emfd_f
     N.side.ny = b.bmax[1] - b.bmin[1] + 2;
     N.side.z0 = b.bmin[2] - 1;
     N.side.nz = b.bmax[2] - b.bmin[2] + 2;
-    if (int rc = N.d_side.ensure(static_cast<size_t>(2 * 6 * 2) * N.side.ny * N.side.nz * sizeof(float))) return rc;
+    N.side.gy0 = N.side.y0 / kSideGroup;
+    N.side.nyp = ((N.side.y0 + N.side.ny - 1) / kSideGroup - N.side.gy0 + 1) * 8;
+    if (int rc = N.d_side.ensure(static_cast<size_t>(2 * 6 * 2) * N.side.nyp * N.side.nz * sizeof(float))) return rc;
     EMFD_HIP(hipMemset(N.d_phasors.ptr, 0, static_cast<size_t>(b.n_points) * 24 * sizeof(float)));
     N.box = b;
     N.ff = ff_geometry(b, &N.ff_halves);

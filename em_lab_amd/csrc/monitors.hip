@@ -59,10 +59,10 @@ __device__ __forceinline__ void ntff_fields(const LatticeArgs &a, const FieldPtr
         // which would put each load behind a branch of its own
         const SideCols &sc = a.side;
         const int face = n[0] > 0.0f ? 1 : 0;
-        const float *base = sc.buf + static_cast<size_t>(face) * 12 * sc.nz * sc.ny;
+        const float *base = sc.buf + static_cast<size_t>(face) * 12 * sc.nz * sc.nyp;
         const int x1 = sc.x[face] - 1;
         const auto side = [&](int c, int x, int y, int z) {
-            return base[((2 * c + (x - x1)) * sc.nz + (z - sc.z0)) * sc.ny + (y - sc.y0)];
+            return base[((2 * c + (x - x1)) * sc.nz + (z - sc.z0)) * sc.nyp + side_yp(sc, y)];
         };
         if (sc.e_valid) e_at_node_with(side, node[0], node[1], node[2], E);
         else e_at_node(a, f, node[0], node[1], node[2], E);

@@ -59,8 +59,8 @@ __device__ __forceinline__ uint32_t side_bits4(const SideCols &s, int x0, int nx
 }
 __device__ __forceinline__ void side_put4(const SideCols &s, uint32_t bits, int comp0, int x0, int y, int z,
                                           const float (&v)[3][4]) {
-    const long col = static_cast<long>(s.nz) * s.ny;
-    const long yz = static_cast<long>(z - s.z0) * s.ny + (y - s.y0);
+    const long col = static_cast<long>(s.nz) * s.nyp;
+    const long yz = static_cast<long>(z - s.z0) * s.nyp + side_yp(s, y);
 #pragma unroll
     for (int f = 0; f < 2; ++f)
 #pragma unroll
@@ -73,36 +73,39 @@ __device__ __forceinline__ void side_put4(const SideCols &s, uint32_t bits, int 
             }
 }
 
-// The same values into the wave's LDS record of the chunk (kz = plane within the chunk), at
-// [face][column][component][kz]; side_flush stores the record once the chunk is done, so that no
-// scattered store sits among the march's loads and stores (each would enter the wait counts of
-// the plane loads in flight).
-__device__ __forceinline__ void side_lds4(const SideCols &s, uint32_t bits, float *rec, int comp0, int x0, int kz,
-                                          const float (&v)[3][4]) {
+// 8-plane chunks: each wave keeps its row's side column values of the chunk in an LDS record
+// [column k = 2 face + (x - X + 1)][component E x y z, B x y z][plane of the chunk], and the
+// workgroup stores the records once the chunk is done (side_flush), so that no scattered store
+// sits among the march's loads and stores (stores enter the same wait count as the plane loads in
+// flight). The column's lane and cell are the same for the whole wave, so a plane's copy is one
+// branch on the wave-uniform cell index and six LDS writes on one lane.
+template <int J>
+__device__ __forceinline__ void side_rec_put(bool on, float *r, const float (&b)[3][4], const float (&e)[3][4]) {
+    if (on) {
 #pragma unroll
-    for (int f = 0; f < 2; ++f)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if (bits & (1u << (f * 4 + j))) {
-                const int xo = x0 + j - (s.x[f] - 1);
-#pragma unroll
-                for (int c = 0; c < 3; ++c) rec[((f * 2 + xo) * 6 + comp0 + c) * 8 + kz] = v[c][j];
-            }
+        for (int c = 0; c < 3; ++c) {
+            r[c * 8] = e[c][J];
+            r[(3 + c) * 8] = b[c][J];
+        }
+    }
 }
-__device__ __forceinline__ void side_flush(const SideCols &s, const float *rec, int lane, int xs, int nx, int y, int z0,
+// The workgroup's records as whole 32-byte sectors of the layout (a row group's 7 values and the
+// pad slot of one plane, column and component); rows outside the box's range store values the
+// gather never reads.
+template <int NW>
+__device__ __forceinline__ void side_flush(const SideCols &s, const float *recs, int yt, int xs, int xend, int z0,
                                            int z1) {
-    // the record was written by this wave's lanes: LDS operations of a wave complete in order,
-    // the fences keep the compiler from moving the reads above the writes
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const long col = static_cast<long>(s.nz) * s.ny;
-    for (int i = lane; i < 2 * 2 * 6 * 8; i += 64) {
-        const int f = i / 96, xo = (i / 48) & 1, comp = (i / 8) % 6, kz = i & 7;
-        const int z = z0 + kz, x = s.x[f] - 1 + xo;
-        if (z < z1 && z >= s.z0 && z < s.z0 + s.nz && x >= xs && x < xs + 256 && x < nx)
-            s.buf[(static_cast<long>(f * 6 + comp) * 2 + xo) * col + static_cast<long>(z - s.z0) * s.ny + (y - s.y0)] =
-                rec[i];
+    const long col = static_cast<long>(s.nz) * s.nyp;
+    const int yg = yt - s.gy0;
+    for (int k = static_cast<int>(threadIdx.x); k < 2 * 2 * 6 * 8 * 8; k += 64 * NW) {
+        const int sec = k >> 3, slot = k & 7;
+        const int cl = sec / 48, comp = (sec >> 3) % 6, kz = sec & 7;
+        const int z = z0 + kz, x = s.x[cl >> 1] - 1 + (cl & 1);
+        if (z < z1 && z >= s.z0 && z < s.z0 + s.nz && x >= xs && x < xend) {
+            const float v = slot < 7 ? recs[(slot + 1) * (2 * 2 * 6 * 8) + sec] : 0.0f;
+            s.buf[(static_cast<long>((cl >> 1) * 6 + comp) * 2 + (cl & 1)) * col + static_cast<long>(z - s.z0) * s.nyp +
+                  yg * 8 + slot] = v;
+        }
     }
 }
 
@@ -291,9 +294,25 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
     auto xb_at = [&](int p, int wave, int comp) { return xb + ((((p & 1) * NW + wave) * 2 + comp) * kSeg); };
 
     // far-field side columns this wave's output row keeps (lanes with column cells, rows in range)
-    const uint32_t sbits = out_row ? side_bits4(a.side, x0, nx) : 0u;
-    const bool side_row = out_row && a.side.buf && y >= a.side.y0 && y < a.side.y0 + a.side.ny &&
-                          __builtin_amdgcn_ballot_w64(sbits != 0u) != 0;
+    const uint32_t sbits = ZC == 0 && out_row ? side_bits4(a.side, x0, nx) : 0u;
+    const bool side_row = ZC == 0 ? out_row && a.side.buf && y >= a.side.y0 && y < a.side.y0 + a.side.ny &&
+                                        __builtin_amdgcn_ballot_w64(sbits != 0u) != 0
+                                  : out_row && a.side.buf && y >= a.side.y0 && y < a.side.y0 + a.side.ny;
+    const int xend = min(xs + kSeg, nx);
+    // 8-plane chunks: the workgroup stores side columns when one of the columns lies in its
+    // segment, its rows meet the box's range and the chunk its z range (uniform)
+    bool side_wg = false;
+    if (ZC > 0 && a.side.buf) {
+        const int ylo = yt * (NW - 1), yhi = ylo + NW - 2;
+        bool cols = false;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int xk = a.side.x[k >> 1] - 1 + (k & 1);
+            cols = cols || (xk >= xs && xk < xend);
+        }
+        side_wg = cols && yhi >= a.side.y0 && ylo < a.side.y0 + a.side.ny && z0 < a.side.z0 + a.side.nz &&
+                  z1 > a.side.z0;
+    }
     // depth class per cell of the lane (0 outside the lattice: any finite entry will do there)
     int cx[4];
 #pragma unroll
@@ -340,15 +359,19 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
     // absorber is thicker than 31 cells)
     auto stage = [&](int z) {
         const int yy = row_ok ? y : 0;
-        const CoefLoad cb = coef_load<1>(a, lane, yy, z, kPmlBa);
-        const CoefLoad ce = coef_load<1>(a, lane, yy, z, kPmlEa);
+        // planes outside the lattice (the virtual plane -1, the planes a partial last chunk runs
+        // on past nz - 1) store nothing: their entries come from the nearest lattice plane, since
+        // their own depth class would index past the absorber table
+        const int zz = min(max(z, 0), nz - 1);
+        const CoefLoad cb = coef_load<1>(a, lane, yy, zz, kPmlBa);
+        const CoefLoad ce = coef_load<1>(a, lane, yy, zz, kPmlEa);
         if (lane < K) {
             coef_store(a, coefB, lane, cb);
             coef_store(a, coefE, lane, ce);
         }
         for (int i = lane + 64; i < K; i += 64) {
-            coef_store(a, coefB, i, coef_load<1>(a, i, yy, z, kPmlBa));
-            coef_store(a, coefE, i, coef_load<1>(a, i, yy, z, kPmlEa));
+            coef_store(a, coefB, i, coef_load<1>(a, i, yy, zz, kPmlBa));
+            coef_store(a, coefE, i, coef_load<1>(a, i, yy, zz, kPmlEa));
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -379,7 +402,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
 
     int cz_prev = -1;
     auto plane_step = [&](int z, Row4 &P, Row4 &Q, float (&hP)[3], float (&hQ)[3], BSet &Pb, BSet &Qb) {
-        const int czz = pml_class(z, nz, L);
+        const int czz = pml_class(min(max(z, 0), nz - 1), nz, L);
         if (czz != cz_prev) {  // uniform over the workgroup
             stage(z);
             cz_prev = czz;
@@ -486,10 +509,22 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
                 }
             }
             if constexpr (ZC > 0) {
-                // into LDS; stored once the chunk is done
-                if (side_row && sbits != 0u) {
-                    side_lds4(a.side, sbits, side_l, 3, x0, z - z0, bn);
-                    side_lds4(a.side, sbits, side_l, 0, x0, z - z0, e);
+                // into the wave's LDS record; stored once the chunk is done
+                if (side_row) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int d = a.side.x[k >> 1] - 1 + (k & 1) - xs;  // uniform
+                        if (d >= 0 && xs + d < xend) {
+                            const bool on = lane == (d >> 2);
+                            float *r = side_l + k * 48 + (z - z0);
+                            switch (d & 3) {
+                                case 0: side_rec_put<0>(on, r, bn, e); break;
+                                case 1: side_rec_put<1>(on, r, bn, e); break;
+                                case 2: side_rec_put<2>(on, r, bn, e); break;
+                                default: side_rec_put<3>(on, r, bn, e); break;
+                            }
+                        }
+                    }
                 }
             } else if (side_row && z >= a.side.z0 && z < a.side.z0 + a.side.nz && sbits != 0u) {
                 side_put4(a.side, sbits, 3, x0, y, z, bn);
@@ -510,7 +545,10 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
             plane_step(zs + k, eP, eQ, hlP, hlQ, bP, bQ);
             if (k + 1 <= ZC) plane_step(zs + k + 1, eQ, eP, hlQ, hlP, bQ, bP);
         }
-        if (side_row) side_flush(a.side, side_l, lane, xs, nx, y, z0, z1);
+        if (side_wg) {
+            __syncthreads();
+            side_flush<NW>(a.side, mat + kMatFloats, yt, xs, xend, z0, z1);
+        }
     } else {
         for (int z = zs; z < z1; z += 2) {
             plane_step(z, eP, eQ, hlP, hlQ, bP, bQ);
@@ -559,7 +597,7 @@ __global__ void __launch_bounds__(256) k_march_fix_b(LatticeArgs a, FieldPtrs sr
         for (int face = 0; face < 2; ++face) {
             const long k = side_index(a.side, face, 3, x, y, z);
             if (k < 0) continue;
-            const long plane = 2L * a.side.nz * a.side.ny;
+            const long plane = 2L * a.side.nz * a.side.nyp;
             for (int c = 0; c < 3; ++c) a.side.buf[k + c * plane] = b[c];
         }
     }
@@ -600,7 +638,7 @@ __global__ void __launch_bounds__(256) k_march_fix_e(LatticeArgs a, FieldPtrs sr
         for (int face = 0; face < 2; ++face) {
             const long k = side_index(a.side, face, 0, x, y, z);
             if (k < 0) continue;
-            const long plane = 2L * a.side.nz * a.side.ny;
+            const long plane = 2L * a.side.nz * a.side.nyp;
             for (int c = 0; c < 3; ++c) a.side.buf[k + c * plane] = e[c];
         }
     }

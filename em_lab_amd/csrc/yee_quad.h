@@ -45,7 +45,7 @@ __device__ __forceinline__ void side_write4(const LatticeArgs &a, int comp0, int
         for (int face = 0; face < 2; ++face) {
             const long k = side_index(s, face, comp0, x0 + j, y, z);
             if (k < 0) continue;
-            const long plane = 2L * s.nz * s.ny;  // component stride
+            const long plane = 2L * s.nz * s.nyp;  // component stride
 #pragma unroll
             for (int c = 0; c < 3; ++c) s.buf[k + c * plane] = v[c][j];
         }
