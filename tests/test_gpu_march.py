@@ -90,6 +90,23 @@ def test_march_material_in_absorber():
     _same(_run(sc, E.KERNEL_MARCH, 120), _run(sc, E.KERNEL_AUTO, 120))
 
 
+def test_march_far_field_partial_chunk_and_group():
+    """Far-field side columns written by the march where the Huygens box reaches into the
+    lattice's partial last z-chunk (nz = 53: planes 48-52) and partial last row group (ny = 38:
+    rows 35-37), with dielectric boxes and a thin-wire dipole inside the box: fields and
+    far-field phasors bit-identical to the two-pass kernel."""
+    print("case: march side columns, 264 x 38 x 53", flush=True)
+    sc = Scene(size=(264, 38, 53), pml=1, courant=0.57)
+    sc.box((100, 12, 12), (140, 30, 40), eps_r=2.5, sigma=0.01)
+    sc.box((20, 8, 8), (60, 20, 20), eps_r=4.0)
+    sc.dipole_thin((132, 19, 26), axis=2, length=9.0, radius=0.1, impedance=50.0, amplitude=1.0, wavelength=16.0,
+                   waveform="pulse", pulse_width=1.0)
+    sc.emit_thin_wires()
+    ff = sc.set_far_field(margin=0, wavelength=16.0)
+    assert ff.box_max[1] >= 35 and ff.box_max[2] >= 48
+    _same(_run(sc, E.KERNEL_MARCH, 100), _run(sc, E.KERNEL_AUTO, 100))
+
+
 def test_march_far_field_box_through_x_seam():
     """A far-field box whose x faces are maintained as side columns by the march (and by the
     recomputation for material texels on them), on a lattice of two segments."""
