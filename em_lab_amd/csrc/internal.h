@@ -126,6 +126,11 @@ struct StepConsts {
     int ntff_off;
 };
 
+// Radar back-projection image (radar.hip): power [nr][nel][naz] on the device.
+hipError_t launch_radar_backproject(const emfd_radar_image_setup &s, const float *z, long m, int n_el,
+                                    const float *el_pos, const double *el_w, const float *tx_pos,
+                                    const double *tx_amp, int n_tx, double *power, hipStream_t st);
+
 // ---- launchers (kernels.hip) ----
 hipError_t launch_build_table(const uint16_t *keys, int classes, float dt, float *table, hipStream_t s);
 // Material coefficients the one-pass march applies itself (march tiles of its default 8-plane

@@ -100,6 +100,14 @@ class ProbeSampleC(C.Structure):
                 ("sigma", C.c_float), ("epsilon_r", C.c_float)]
 
 
+class RadarImageSetupC(C.Structure):
+    _fields_ = [("tx", C.c_float * 3), ("axes", C.c_float * 9), ("interior_min", C.c_float * 3),
+                ("interior_max", C.c_float * 3), ("wavelength", C.c_float), ("dt", C.c_double),
+                ("sample_dt", C.c_double), ("r0", C.c_float), ("dr", C.c_float), ("nr", C.c_int),
+                ("az0", C.c_float), ("daz", C.c_float), ("naz", C.c_int), ("el0", C.c_float), ("del_", C.c_float),
+                ("nel", C.c_int)]
+
+
 class BatchSceneC(C.Structure):
     _fields_ = [("config", GridConfigC), ("sigma", C.c_void_p), ("eps_r", C.c_void_p), ("n_cells", C.c_int64),
                 ("edges", C.c_void_p), ("n_edges", C.c_int64), ("wires", C.c_void_p), ("n_wires", C.c_int64),
@@ -203,6 +211,7 @@ _SIGNATURES = {
     "emfd_mlp_set_dataset": (C.c_int, [_P, _P, _P, C.c_int64]),
     "emfd_mlp_train": (C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, _P, C.c_int]),
     "emfd_mlp_time_forward": (C.c_int, [_P, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
+    "emfd_radar_backproject": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int, _P, _P, _P, _P, C.c_int, _P]),
 }
 
 
@@ -276,6 +285,7 @@ class SourceDesc:
     delay_periods: float = 0.0
     enabled: bool = True
     port: int = -1
+    emitter: int = -1  # emitter{} index (src/SimulationGrid.hpp:149), host-side only
 
     def to_c(self) -> SourceDescC:
         f3 = C.c_float * 3
@@ -607,6 +617,29 @@ class LineMonitor:
         now = np.zeros(max(self.n, 1), np.float32)
         _check(self._lib.emfd_lines_read(self.grid.handle, _ptr(ph), _ptr(now), ph.shape[0]))
         return ph[: self.n], now[: self.n]
+
+
+def radar_backproject(grid: "SimulationGrid", setup: dict, z: np.ndarray, element_pos, weights, tx_pos,
+                      tx_amp) -> np.ndarray:
+    """Back-projection image (emfd_radar_backproject) on the grid's device: power [nr][nel][naz]
+    (-1 outside the interior). setup keys: tx, axes (3 x 3 rows), interior_min, interior_max,
+    wavelength, dt, sample_dt, r0, dr, nr, az0, daz, naz, el0, del, nel. z: [m][n_el] complex."""
+    zz = np.ascontiguousarray(np.asarray(z, np.complex64)).view(np.float32)
+    m, n_el = np.asarray(z).shape
+    pos = np.ascontiguousarray(element_pos, np.float32).reshape(n_el, 3)
+    w = np.ascontiguousarray(weights, np.float64).reshape(n_el)
+    txp = np.ascontiguousarray(tx_pos, np.float32).reshape(-1, 3)
+    amp = np.ascontiguousarray(tx_amp, np.float64).reshape(-1)
+    f3 = C.c_float * 3
+    st = RadarImageSetupC(f3(*setup["tx"]), (C.c_float * 9)(*np.asarray(setup["axes"], np.float32).ravel()),
+                          f3(*setup["interior_min"]), f3(*setup["interior_max"]), setup["wavelength"], setup["dt"],
+                          setup["sample_dt"], setup["r0"], setup["dr"], int(setup["nr"]), setup["az0"], setup["daz"],
+                          int(setup["naz"]), setup["el0"], setup["del"], int(setup["nel"]))
+    out = np.zeros((int(setup["nr"]), int(setup["nel"]), int(setup["naz"])), np.float64)
+    _check(grid._lib.emfd_radar_backproject(grid.handle, C.byref(st), _ptr(zz), int(m), int(n_el), _ptr(pos), _ptr(w),
+                                            _ptr(txp) if txp.size else None, _ptr(amp) if amp.size else None,
+                                            int(amp.size), _ptr(out)))
+    return out
 
 
 def coax_geometry_factor(inner: int, outer: int) -> float:

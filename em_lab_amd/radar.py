@@ -1,0 +1,782 @@
+"""Pulsed phased-array radar on the recorded port series (RadarMonitor, src/Radar.hpp,
+src/Radar.cpp): every antenna feed transmits (the pulsed ones) and receives, its port voltage
+recorded at every step by a record-only port monitor together with three reference channels (Ex,
+Ey, Ez on boresight). A calibration pass without the targets is subtracted from the pass with
+them (clutter cancellation); the echoes are pulse-compressed with the measured two-way template
+T = REF^2 / S, decimated to complex baseband and back-projected onto a range x azimuth x
+elevation grid (the imaging on the GPU, emfd_radar_backproject); detections are the local maxima
+above the noise floor.
+
+Host processing is FP64 as the reference's. The FFT is numpy's (the reference's own radix-2
+transform computes the same sums in another order). The receiver noise uses the reference's
+generators (std::mt19937 seeded per element, libstdc++'s normal_distribution), restated in
+em_lab_amd.stdrandom.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import emfdtd as E
+from .stdrandom import Mt19937, NormalDistribution
+
+K_REFERENCE_CHANNELS = 3  # Ex, Ey, Ez at the timing reference point (src/Radar.cpp:17)
+K_MAX_DETECTIONS = 8
+TWO_PI = 2.0 * math.pi
+
+f32 = np.float32
+
+
+def to_db(p: float) -> float:
+    """toDb (src/Radar.cpp:20): float dB, -3000 for nothing."""
+    return float(f32(10.0 * math.log10(p))) if p > 1e-300 else -3000.0
+
+
+def vertex_offset(a: float, b: float, c: float) -> float:
+    """Parabola vertex through (-1, a), (0, b), (1, c), within [-0.5, 0.5] (src/Radar.cpp:23-27)."""
+    a, b, c = f32(a), f32(b), f32(c)
+    den = f32(f32(a - f32(2.0) * b) + c)
+    if abs(den) < f32(1e-12):
+        return 0.0
+    return float(min(max(f32(f32(0.5) * f32(a - c)) / den, f32(-0.5)), f32(0.5)))
+
+
+def boresight_frame(n):
+    """boresightFrame (src/FarField.hpp:29-37): (right, up) for boresight n."""
+    n = np.asarray(n, np.float32)
+    if abs(n[2]) > 0.5:
+        return np.array([1, 0, 0], np.float32), np.array([0, 1, 0], np.float32)
+    up = np.array([0, 0, 1], np.float32)
+    return np.cross(n, up).astype(np.float32), up
+
+
+def source_signal(s: E.SourceDesc, t: float) -> float:
+    """SourceDesc::signal (src/SimulationGrid.cpp:103-130), FP64."""
+    wl = max(float(f32(s.wavelength)), 1e-3)
+    t = t - max(float(f32(s.delay_periods)), 0.0) * wl
+    if not s.enabled or t < 0.0:
+        return 0.0
+    w = TWO_PI / wl
+    if s.waveform == E.WAVE_CW:
+        ramp_t = max(float(f32(s.ramp_periods)), 0.0) * wl
+        ramp = 1.0
+        if ramp_t > 0.0 and t < ramp_t:
+            r = t / ramp_t
+            ramp = r * r * r * (10.0 + r * (-15.0 + 6.0 * r))
+        return ramp * math.sin(w * t + float(f32(s.phase)))
+    tau = max(float(f32(s.ramp_periods)), 0.1) * wl
+    t0 = 3.0 * tau
+    tl = t
+    if s.repeat_periods > 0.0:
+        rep = max(float(f32(s.repeat_periods)) * wl, 2.0 * t0)
+        tl = math.fmod(t, rep)
+    u = (tl - t0) / tau
+    if abs(u) > 6.0:
+        return 0.0
+    return math.exp(-u * u) * math.sin(w * (tl - t0) + float(f32(s.phase)))
+
+
+# -------------------------------------------------------------------------------------------
+# Settings and results (src/Radar.hpp:39-102)
+# -------------------------------------------------------------------------------------------
+@dataclass
+class RadarSetup:
+    range_min: float = 0.0          # image range (cells from the transmit phase centre); 0: 1 wavelength
+    range_max: float = 0.0          # 0: to the absorbing layer along boresight
+    fov_az_deg: float = 40.0
+    fov_el_deg: float = 40.0
+    noise_db: float = -300.0        # receiver noise rms in band vs the transmit pulse peak at a port
+    threshold_db: float = 13.0
+    dynamic_range_db: float = 25.0
+    reference_distance: float = 0.0  # timing reference on boresight (cells); 0: 1.5 wavelengths
+    taper: int = 1                  # 0 uniform, 1 Hamming, 2 Hann
+    phase_center_offset: tuple = (0.0, 0.0, 0.0)
+    tx_center: Optional[tuple] = None
+    log_path: str = ""
+    # Boresight of the array (a unit axis vector). The reference takes it from the first feed's
+    # gap axis and sign (patch arrays fed by probes normal to the board); None keeps that rule.
+    boresight: Optional[tuple] = None
+
+
+@dataclass
+class TargetInfo:
+    name: str = ""
+    min: tuple = (0.0, 0.0, 0.0)
+    max: tuple = (0.0, 0.0, 0.0)
+    points: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.float32))
+
+
+@dataclass
+class RadarDetection:
+    position: np.ndarray = field(default_factory=lambda: np.zeros(3, np.float32))
+    range: float = 0.0
+    az_deg: float = 0.0
+    el_deg: float = 0.0
+    snr_db: float = 0.0
+    rel_db: float = 0.0
+    target: int = -1
+    target_range: float = 0.0
+    target_az_deg: float = 0.0
+    target_el_deg: float = 0.0
+
+
+@dataclass
+class RadarImage:
+    valid: bool = False
+    complete: bool = False
+    complete_range: float = 0.0
+    recorded_time: float = 0.0
+    nr: int = 0
+    naz: int = 0
+    nel: int = 0
+    r0: float = 0.0
+    dr: float = 1.0
+    az0: float = 0.0
+    daz: float = 1.0
+    el0: float = 0.0
+    del_: float = 1.0
+    power_db: np.ndarray = field(default_factory=lambda: np.zeros((0, 0, 0), np.float32))  # [r][el][az]
+    floor_db: float = -100.0
+    echo_db: float = 0.0
+    detections: list = field(default_factory=list)
+    bscope: np.ndarray = field(default_factory=lambda: np.zeros((0, 0), np.float32))  # [r][az]
+    cscope: np.ndarray = field(default_factory=lambda: np.zeros((0, 0), np.float32))  # [el][az]
+    ascope: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))       # [r]
+    cscope_range: float = 0.0
+    trace_time: list = field(default_factory=list)
+    trace_cal: list = field(default_factory=list)
+    trace_meas: list = field(default_factory=list)
+    trace_echo: list = field(default_factory=list)
+    system_delay: float = 0.0
+
+
+def target_info(name: str, inside, size) -> TargetInfo:
+    """targetInfo (src/ScriptEngine.cpp:800-827) for a shape given as a boolean mask over the
+    lattice cells [z][y][x] (cell centres inside the shape), subsampled with the same stride."""
+    idx = np.argwhere(inside)
+    if idx.size == 0:
+        return TargetInfo(name=name)
+    lo, hi = idx.min(axis=0), idx.max(axis=0)
+    span = (hi - lo + 1).astype(np.float32)
+    stride = max(1, int(math.ceil(float(f32(np.cbrt(f32(span[0] * span[1] * span[2] / f32(200000.0))))))))
+    zz, yy, xx = idx[:, 0], idx[:, 1], idx[:, 2]
+    keep = ((zz - lo[0]) % stride == 0) & ((yy - lo[1]) % stride == 0) & ((xx - lo[2]) % stride == 0)
+    pts = np.stack([xx[keep], yy[keep], zz[keep]], axis=1).astype(np.float32) + f32(0.5)
+    return TargetInfo(name=name, min=tuple((pts - 0.5).min(axis=0)), max=tuple((pts + 0.5).max(axis=0)), points=pts)
+
+
+class RadarMonitor:
+    """RadarMonitor (src/Radar.hpp:104-205) over a grid's record-only port monitor."""
+
+    OFF, CALIBRATING, MEASURING, DONE = "off", "calibrating", "measuring", "done"
+    CALIBRATE, MEASURE, REPROCESS = "calibrate", "measure", "reprocess"
+
+    def __init__(self, grid: E.SimulationGrid):
+        self.grid = grid
+        self.recorder = E.PortMonitor(grid)
+        self.phase = self.OFF
+        self.setup = RadarSetup()
+        self.image = RadarImage()
+        self.targets: list = []
+        self.target_nearest: list = []
+        self.elements: list = []
+        self.element_pos = np.zeros((0, 3), np.float32)
+        self.weights = np.zeros(0)
+        self.axes = np.eye(3, dtype=np.float32)
+        self.tx = np.zeros(3, np.float32)
+        self.reference = np.zeros(3, np.float32)
+        self.reference_distance = 0.0
+        self.wavelength = 16.0
+        self.cell_size = 1e-3
+        self.interior_min = np.zeros(3, np.float32)
+        self.interior_max = np.zeros(3, np.float32)
+        self.dt = 0.5
+        self.tx_time = 0.0
+        self.pulse_width = 8.0
+        self.steps = 0
+        self.have_calibration = False
+        self.have_measurement = False
+        self.cal = np.zeros((0, 0), np.float32)
+        self.meas = np.zeros((0, 0), np.float32)
+        self.tx_sources: list = []
+        self.tx_pos = np.zeros((0, 3), np.float32)
+        self.tx_amp = np.zeros(0)
+        self.tx_weight = 1.0
+        self.drive = np.zeros(0)
+        self.pulse_end = 0.0
+        self.extent = 0.0
+        self.provisional_steps = 0
+        self.cal_steps = 0
+        self.window_measured = False
+        self.last_window_check = 0
+        self.fft_size = 0
+        self.filter = np.zeros(0, np.complex128)
+        self.peak_tx = 0.0
+        self.system_delay = 0.0
+        self.template_end = 0.0
+
+    # ---------------------------------------------------------------------------------------
+    def direction(self, az_deg: float, el_deg: float) -> np.ndarray:
+        """direction (src/Radar.cpp:67-70), float."""
+        a = f32(f32(az_deg) * f32(0.017453292519943295))
+        e = f32(f32(el_deg) * f32(0.017453292519943295))
+        sa, ca, se, ce = np.sin(a), np.cos(a), np.sin(e), np.cos(e)
+        return (f32(sa * ce) * self.axes[0] + se * self.axes[1]) + f32(ca * ce) * self.axes[2]
+
+    def tx_array_factor(self, p, omega: float) -> complex:
+        """txArrayFactor (src/Radar.cpp:74-84)."""
+        v = np.asarray(p, np.float32) - self.tx_pos
+        d = np.maximum(np.sqrt((v * v).sum(axis=1, dtype=np.float32)).astype(np.float64), 0.5)
+        g = np.maximum((v @ self.axes[2]).astype(np.float64) / d, 0.0)
+        return complex(np.sum((self.tx_amp * g / d) * np.exp(-1j * omega * d)))
+
+    def tx_delay(self, p) -> float:
+        """txDelay (src/Radar.cpp:86-104)."""
+        w = TWO_PI / self.wavelength
+        v = np.asarray(p, np.float32) - self.tx_pos
+        d = np.maximum(np.sqrt((v * v).sum(axis=1, dtype=np.float32)).astype(np.float64), 0.5)
+        g = np.maximum((v @ self.axes[2]).astype(np.float64) / d, 0.0)
+        term = (self.tx_amp * g / d) * np.exp(-1j * w * d)
+        a, ad, mag = term.sum(), (d * term).sum(), float((self.tx_amp * g / d).sum())
+        if abs(a) < 1e-3 * mag:
+            return float(np.linalg.norm(np.asarray(p, np.float32) - self.tx))
+        return float((ad / a).real)
+
+    def angles_of(self, p):
+        """anglesOf (src/Radar.cpp:106-110): azimuth, elevation in degrees."""
+        u = np.asarray(p, np.float32) - self.tx
+        u = u / np.float32(np.linalg.norm(u))
+        return (math.degrees(math.atan2(float(u @ self.axes[0]), float(u @ self.axes[2]))),
+                math.degrees(math.asin(min(max(float(u @ self.axes[1]), -1.0), 1.0))))
+
+    def progress(self) -> float:
+        if self.phase == self.DONE:
+            return 1.0
+        if self.phase == self.OFF or self.steps == 0:
+            return 0.0
+        return min(1.0, self.recorder.recorded_steps() / self.steps)
+
+    def disable(self) -> None:
+        self.phase = self.OFF
+        self.have_calibration = self.have_measurement = False
+        self.cal = np.zeros((0, 0), np.float32)
+        self.meas = np.zeros((0, 0), np.float32)
+        self.image = RadarImage()
+        self.recorder.disable()
+
+    # ---------------------------------------------------------------------------------------
+    def configure(self, setup: RadarSetup, elements: Sequence[E.PortSetup], sources: Sequence[E.SourceDesc],
+                  wavelength: float, cell_size: float, targets: Sequence[TargetInfo], start: str = "calibrate",
+                  warnings: Optional[list] = None) -> None:
+        """RadarMonitor::configure (src/Radar.cpp:122-290)."""
+        warnings = warnings if warnings is not None else []
+        old_phase = self.phase
+        self.setup = RadarSetup(**setup.__dict__)
+        self.targets = list(targets)
+        self.wavelength = max(float(wavelength), 2.0)
+        self.cell_size = float(cell_size)
+        self.dt = float(self.grid.dt)
+        self.elements = list(elements)
+        if not self.elements:
+            warnings.append("radar: the scene has no antenna feeds to transmit and receive with")
+            self.disable()
+            return
+        p0 = self.elements[0]
+        n = np.zeros(3, np.float32)
+        if setup.boresight is not None:
+            n[:] = np.asarray(setup.boresight, np.float32)
+        else:
+            n[p0.axis] = 1.0 if p0.sign >= 0.0 else -1.0
+        right, up = boresight_frame(n)
+        self.axes = np.stack([right, up, n]).astype(np.float32)
+        off = np.asarray(setup.phase_center_offset, np.float32)
+        pos = []
+        for p in self.elements:
+            c = np.asarray(p.texel, np.float32).copy()
+            c[p.axis] += f32(0.5)
+            pos.append(c + off)
+        self.element_pos = np.asarray(pos, np.float32)
+
+        # receive taper: separable window over the element rows and columns
+        self.weights = np.ones(len(self.elements))
+        if setup.taper != 0:
+            a = 0.54 if setup.taper == 1 else 0.5
+            for axis_index in range(2):
+                coords = np.round((self.element_pos @ self.axes[axis_index]).astype(np.float32) * f32(4.0)) / f32(4.0)
+                uniq = np.unique(coords)
+                if uniq.size < 2:
+                    continue
+                count = float(uniq.size)
+                for i, cval in enumerate(coords):
+                    idx = float(np.searchsorted(uniq, cval))
+                    self.weights[i] *= a - (1.0 - a) * math.cos(TWO_PI * (idx + 1.0) / (count + 1.0))
+
+        # transmit phase centre and pulse timing from the pulsed feeds (drive-weighted)
+        wsum = tsum = tau_sum = t_end = 0.0
+        csum = np.zeros(3)
+        for s in sources:
+            if not s.enabled or s.waveform != E.WAVE_PULSE:
+                continue
+            wgt = abs(float(f32(s.amplitude)))
+            if wgt <= 0.0:
+                continue
+            period = max(float(f32(s.wavelength)), 1e-3)
+            tau = max(float(f32(s.ramp_periods)), 0.1) * period
+            t0 = max(float(f32(s.delay_periods)), 0.0) * period + 3.0 * tau
+            csum += wgt * np.asarray(s.center, np.float64)
+            tsum += wgt * t0
+            tau_sum += wgt * tau
+            wsum += wgt
+            t_end = max(t_end, t0 + 3.0 * tau)
+        if wsum <= 0.0:
+            warnings.append('radar: no pulsed transmitter; drive the antennas with waveform = "pulse"')
+            self.disable()
+            return
+        self.tx = (np.asarray(setup.tx_center, np.float32) if setup.tx_center is not None
+                   else (csum / wsum).astype(np.float32) + off)
+        self.tx_sources = [s for s in sources if s.enabled and s.waveform == E.WAVE_PULSE and s.amplitude != 0.0]
+        self.tx_weight = wsum
+        if setup.tx_center is not None:
+            self.tx_pos = np.asarray([setup.tx_center], np.float32)
+            self.tx_amp = np.ones(1)
+        else:
+            self.tx_pos = np.asarray([np.asarray(s.center, np.float32) + off for s in self.tx_sources], np.float32)
+            self.tx_amp = np.asarray([abs(float(f32(s.amplitude))) for s in self.tx_sources])
+        self.tx_time = tsum / wsum
+        self.target_nearest = []
+        for t in self.targets:
+            best = np.clip(self.tx, np.asarray(t.min, np.float32), np.asarray(t.max, np.float32))
+            if len(t.points):
+                dists = np.sqrt(((t.points - self.tx) ** 2).sum(axis=1))
+                best = t.points[int(np.argmin(dists))]
+            self.target_nearest.append(np.asarray(best, np.float32))
+        self.pulse_width = tau_sum / wsum
+
+        # image ranges: by default from one wavelength out to the absorbing layer along boresight
+        gs = np.asarray(self.grid.size, np.float32)
+        L = f32(self.grid.config.pml_thickness)
+        self.interior_min = np.full(3, L, np.float32)
+        self.interior_max = gs - L
+        ax = int(np.argmax(np.abs(n)))
+        to_edge = float(gs[ax] - L - f32(2.0) - self.tx[ax]) if n[ax] > 0.0 else float(self.tx[ax] - L - f32(1.0))
+        self.setup.range_max = setup.range_max if setup.range_max > 0.0 else max(to_edge, 2.0 * self.wavelength)
+        self.setup.range_min = setup.range_min if setup.range_min > 0.0 else self.wavelength
+        if self.setup.range_max <= self.setup.range_min:
+            self.setup.range_max = self.setup.range_min + self.wavelength
+        if self.setup.range_max > to_edge + 1.0:
+            warnings.append("radar: the image range reaches into the absorbing layer along boresight")
+        rd = setup.reference_distance if setup.reference_distance > 0.0 else 1.5 * self.wavelength
+        rd = min(rd, 0.8 * to_edge)
+        ref = (self.tx + n * f32(rd)).astype(np.float32)
+        self.reference = (np.sign(ref) * np.floor(np.abs(ref) + f32(0.5))).astype(np.float32)  # glm::round
+        self.reference_distance = float(np.linalg.norm(self.reference - self.tx))
+
+        # listening window
+        self.extent = float(max(np.linalg.norm(p - self.tx) for p in self.element_pos))
+        self.pulse_end = t_end
+        provisional = t_end + 2.0 * (self.setup.range_max + self.extent) + 24.0 * self.wavelength
+        self.provisional_steps = int(math.ceil(provisional / self.dt))
+        k = np.arange(self.provisional_steps)
+        drive = np.zeros(self.provisional_steps)
+        for s in self.tx_sources:
+            drive += float(f32(s.amplitude)) * np.array([source_signal(s, (kk + 0.5) * self.dt) for kk in k])
+        self.drive = drive / self.tx_weight
+
+        if start == self.REPROCESS and old_phase != self.OFF:
+            if self.have_measurement and self.phase == self.DONE:
+                self.image = self.compute_image(self.meas, self.meas.shape[0], True)
+                self.write_log()
+            return
+        if start != self.CALIBRATE and self.have_calibration and self.cal.size:
+            self.have_measurement = False
+            self.phase = self.MEASURING
+            self.steps = self.cal_steps
+            self.image = RadarImage()
+            self._start_recording()
+            return
+        self.have_calibration = self.have_measurement = False
+        self.cal = np.zeros((0, 0), np.float32)
+        self.meas = np.zeros((0, 0), np.float32)
+        self.image = RadarImage()
+        self.phase = self.CALIBRATING
+        self.steps = self.provisional_steps
+        self.window_measured = False
+        self.last_window_check = 0
+        self._start_recording()
+
+    def _start_recording(self) -> None:
+        """startRecording (src/Radar.cpp:292-306): the elements and 3 reference channels."""
+        channels = list(self.elements)
+        for k in range(K_REFERENCE_CHANNELS):
+            channels.append(E.PortSetup(name="reference", texel=tuple(int(v) for v in self.reference), axis=k, edges=1,
+                                        sign=1.0, wavelength=self.wavelength, voltage_feed=False, drive=0.0))
+        self.recorder.configure(channels, 4.0, None, self.cell_size, self.steps)
+
+    def _read_series(self) -> np.ndarray:
+        """V per [step][channel] (readSeries, src/Radar.cpp:350-355)."""
+        return self.recorder.read_series()[:, :, 0].astype(np.float32)
+
+    def calibration_complete(self) -> bool:
+        return self.phase == self.CALIBRATING and self.have_calibration
+
+    def begin_measurement(self) -> None:
+        """beginMeasurement (src/Radar.cpp:318-327): targets uploaded and fields reset."""
+        if not self.have_calibration:
+            return
+        self.have_measurement = False
+        self.phase = self.MEASURING
+        self.steps = self.cal_steps
+        self.image = RadarImage()
+        self._start_recording()
+
+    def restart(self) -> None:
+        """restart (src/Radar.cpp:329-348): fields were reset, record the current phase again."""
+        if self.phase == self.OFF:
+            return
+        if self.phase == self.CALIBRATING:
+            self.have_calibration = False
+            self.steps = self.provisional_steps
+            self.window_measured = False
+            self.last_window_check = 0
+        else:
+            self.phase = self.MEASURING
+            self.have_measurement = False
+            self.steps = self.cal_steps
+            self.image = RadarImage()
+        self._start_recording()
+
+    def update(self, partial: bool = False) -> None:
+        """update (src/Radar.cpp:370-407), processing in the calling thread; partial images of
+        a measurement in progress only when asked for (the reference refreshes one every 500 ms)."""
+        recorded = self.recorder.recorded_steps()
+        if self.phase == self.CALIBRATING and not self.have_calibration:
+            period = max(16, int(self.wavelength / self.dt))
+            if not self.window_measured and recorded >= self.last_window_check + period:
+                self.last_window_check = recorded
+                self.measure_window()
+            if recorded >= self.steps:
+                rec = self._read_series()
+                self.cal = rec[: min(rec.shape[0], self.steps)]
+                self.cal_steps = self.cal.shape[0]
+                self.build_template()
+                self.have_calibration = True
+        elif self.phase == self.MEASURING:
+            if recorded >= self.steps:
+                rec = self._read_series()
+                self.meas = rec[: min(rec.shape[0], self.steps)]
+                self.have_measurement = True
+                self.image = self.compute_image(self.meas, self.meas.shape[0], True)
+                self.phase = self.DONE
+                self.write_log()
+            elif partial and recorded > 0:
+                self.meas = self._read_series()
+                self.image = self.compute_image(self.meas, self.meas.shape[0], False)
+
+    # ---------------------------------------------------------------------------------------
+    def fidelity_level(self) -> float:
+        """fidelityLevel (src/Radar.cpp:412): 10 dB below the detection dynamic range."""
+        return 10.0 ** (-(self.setup.dynamic_range_db + 10.0) / 20.0)
+
+    def reference_channel(self, rec: np.ndarray, n: int) -> int:
+        """referenceChannel (src/Radar.cpp:449-464): the strongest reference channel."""
+        C = len(self.elements)
+        peaks = np.abs(rec[:n, C:C + K_REFERENCE_CHANNELS].astype(np.float64)).max(axis=0)
+        return int(np.argmax(peaks))
+
+    def measure_window(self) -> None:
+        """measureWindow (src/Radar.cpp:419-447)."""
+        rec = self._read_series()
+        C = len(self.elements)
+        n = rec.shape[0]
+        period = max(1, int(round(self.wavelength / self.dt)))
+        blocks = n // period
+        if blocks < 6:
+            return
+        best = self.reference_channel(rec, n)
+        ch = np.abs(rec[: blocks * period, C + best].astype(np.float64)).reshape(blocks, period)
+        block_max = ch.max(axis=1)
+        peak = float(block_max.max())
+        if not peak > 0.0:
+            return
+        if np.any(block_max[blocks - 3:] >= self.fidelity_level() * peak):
+            return
+        F = 1
+        while F < 2 * n:
+            F <<= 1
+        templ = self.template_spectrum(rec, n, F)
+        response = self.response_length(templ)
+        window = 2.0 * self.setup.range_max + self.extent + response + self.wavelength
+        self.steps = int(min(max(int(math.ceil(window / self.dt)), n), self.provisional_steps))
+        self.window_measured = True
+
+    def response_length(self, templ: np.ndarray) -> float:
+        """responseLength (src/Radar.cpp:467-477)."""
+        t = np.fft.ifft(templ)
+        half = t.size // 2
+        mag = np.abs(t[:half])
+        tmax = float(mag.max()) if half else 0.0
+        above = np.nonzero(mag > self.fidelity_level() * tmax)[0]
+        last = int(above[-1]) if above.size else 0
+        return (last + 1) * self.dt
+
+    def template_spectrum(self, rec: np.ndarray, n: int, F: int) -> np.ndarray:
+        """buildTemplateSpectrum (src/Radar.cpp:487-518): T = REF^2 / S on an F-point grid, with the
+        transmit array's near-field factor to the reference divided out (regularized)."""
+        C = len(self.elements)
+        dt, w = self.dt, TWO_PI / self.wavelength
+        dw = TWO_PI / (F * dt)
+        best = self.reference_channel(rec, n)
+        ref = np.zeros(F, np.complex128)
+        drive = np.zeros(F, np.complex128)
+        ref[:n] = rec[:n, C + best].astype(np.float64)
+        m = min(n, self.drive.size)
+        drive[:m] = self.drive[:m]
+        ref = np.fft.fft(ref)
+        drive = np.fft.fft(drive)
+        drive_max = float(np.abs(drive).max())
+        templ = np.zeros(F, np.complex128)
+        ks = np.arange(1, F // 2)
+        om = ks * dw
+        inband = np.abs(om - w) <= 0.6 * w
+        ref_af = np.zeros(F // 2, np.complex128)
+        for k, o in zip(ks[inband], om[inband]):
+            ref_af[k] = self.tx_array_factor(self.reference, float(o))
+        af_max = float(np.abs(ref_af).max()) if ref_af.size else 0.0
+        af = ref_af[ks]
+        ok = np.abs(af) >= 1e-12
+        r = ref[ks] * np.conj(af) / (np.abs(af) ** 2 + 0.05 * af_max * af_max)
+        t = r * r * np.conj(drive[ks]) / (np.abs(drive[ks]) ** 2 + 1e-8 * drive_max * drive_max)
+        templ[ks[ok]] = t[ok]
+        return templ
+
+    def build_template(self) -> None:
+        """buildTemplate (src/Radar.cpp:527-557): the pulse-compression filter and system delay."""
+        C = len(self.elements)
+        N = self.cal_steps
+        dt, w = self.dt, TWO_PI / self.wavelength
+        self.peak_tx = float(np.abs(self.cal[:N, :C].astype(np.float64)).max()) if N else 0.0
+        F = 1
+        while F < 2 * N:
+            F <<= 1
+        self.fft_size = F
+        dw = TWO_PI / (F * dt)
+        templ = self.template_spectrum(self.cal, N, F)
+        self.filter = np.zeros(F, np.complex128)
+        ks = np.arange(1, F // 2)
+        om = ks * dw
+        band = np.exp(-(((om - w) / (0.3 * w)) ** 2))
+        self.filter[ks] = np.conj(templ[ks]) * band
+        k0 = int(min(max(int(round(w / dw)), 2), F // 2 - 2))
+        slope = np.angle(templ[k0 + 1] * np.conj(templ[k0 - 1])) / (2.0 * dw)
+        self.system_delay = float(f32(-slope - self.tx_time))
+        self.template_end = self.response_length(templ)
+
+    # ---------------------------------------------------------------------------------------
+    def compressed_echoes(self, meas: np.ndarray, N: int):
+        """Echoes (measurement minus calibration) plus receiver noise, compressed and decimated to
+        complex baseband (src/Radar.cpp:570-597): z [M][C] at path delay m D dt - dt / 2."""
+        C = len(self.elements)
+        dt, w = self.dt, TWO_PI / self.wavelength
+        peak_tx = self.peak_tx
+        in_band = peak_tx * 10.0 ** (self.setup.noise_db / 20.0) if self.setup.noise_db > -299.0 else 0.0
+        sigma = in_band * math.sqrt((0.5 / dt) / (0.376 / self.wavelength))
+        F = self.fft_size
+        D = max(1, int(round(0.25 * self.wavelength / dt)))
+        M = (N + D - 1) // D
+        echo = meas[:N, :C].astype(np.float64) - self.cal[:N, :C].astype(np.float64)
+        if sigma > 0.0:
+            for i in range(C):
+                rng = Mt19937(20251105 + i)
+                gauss = NormalDistribution(0.0, 1.0)
+                echo[:, i] += sigma * np.array([gauss(rng) for _ in range(N)])
+        buf = np.zeros((F, C), np.complex128)
+        buf[:N] = echo
+        spec = np.fft.fft(buf, axis=0) * self.filter[:, None]
+        out = np.fft.ifft(spec, axis=0)
+        m = np.arange(M)
+        lag = (m * D) * dt - 0.5 * dt
+        z = (out[m * D] * np.exp(-1j * w * lag)[:, None]).astype(np.complex64)
+        return z, D, M
+
+    def compute_image(self, meas: np.ndarray, n_meas: int, final: bool) -> RadarImage:
+        """computeImage (src/Radar.cpp:559-760)."""
+        img = RadarImage()
+        C = len(self.elements)
+        N = min(n_meas, self.cal_steps, meas.shape[0])
+        if N < 32 or self.fft_size < 2 * N or not self.peak_tx > 0.0:
+            return img
+        dt = self.dt
+        peak_tx = self.peak_tx
+        img.system_delay = self.system_delay
+        img.recorded_time = N * dt
+        img.complete = final
+        img.complete_range = (self.setup.range_max if final else
+                              float(f32(min(max((img.recorded_time - self.template_end - self.extent) / 2.0, 0.0),
+                                            self.setup.range_max))))
+        peak_echo = float(np.abs(meas[:N, :C].astype(np.float64) - self.cal[:N, :C].astype(np.float64)).max())
+        img.echo_db = to_db((peak_echo / peak_tx) * (peak_echo / peak_tx))
+
+        z, D, M = self.compressed_echoes(meas, N)
+
+        # back-projection image over range x azimuth x elevation (on the GPU)
+        dr, d_ang = f32(f32(0.125) * f32(self.wavelength)), f32(1.5)
+        img.r0 = float(f32(self.setup.range_min))
+        img.dr = float(dr)
+        img.nr = max(2, int(f32(f32(self.setup.range_max) - f32(self.setup.range_min)) / dr) + 1)
+        img.naz = 2 * int(round(float(f32(self.setup.fov_az_deg) / d_ang))) + 1
+        img.nel = 2 * int(round(float(f32(self.setup.fov_el_deg) / d_ang))) + 1
+        img.daz = img.del_ = float(d_ang)
+        img.az0 = float(-d_ang * f32(img.naz // 2))
+        img.el0 = float(-d_ang * f32(img.nel // 2))
+        setup = dict(tx=self.tx, axes=self.axes, interior_min=self.interior_min, interior_max=self.interior_max,
+                     wavelength=self.wavelength, dt=dt, sample_dt=float(D) * dt, r0=img.r0, dr=img.dr, nr=img.nr,
+                     az0=img.az0, daz=img.daz, naz=img.naz, el0=img.el0, nel=img.nel)
+        setup["del"] = img.del_
+        power = E.radar_backproject(self.grid, setup, z, self.element_pos, self.weights, self.tx_pos, self.tx_amp)
+        return self.finish_image(img, power, meas, N, final)
+
+    def finish_image(self, img: RadarImage, power: np.ndarray, meas: np.ndarray, N: int, final: bool) -> RadarImage:
+        """Levels, floor, detections, truth and display slices (src/Radar.cpp:660-760)."""
+        C = len(self.elements)
+        dr = f32(img.dr)
+        nr, nel, naz = img.nr, img.nel, img.naz
+        if final:
+            final_bins = nr
+        else:
+            final_bins = int(min(max(math.floor(float(f32(f32(img.complete_range) - f32(img.r0)) / dr)) + 1, 0), nr))
+        flat = power.reshape(nr, nel * naz)
+        pmax_all = float(max(power.max(), 0.0))
+        pmax = float(max(flat[:final_bins].max(), 0.0)) if final_bins > 0 else 0.0
+        if not pmax > 0.0:
+            pmax = pmax_all
+        if not pmax > 0.0:
+            return img
+        with np.errstate(divide="ignore"):
+            rel = power / pmax
+            db = np.where(rel > 1e-300, (10.0 * np.log10(np.where(rel > 1e-300, rel, 1.0))).astype(np.float32), f32(-3000.0))
+        db = np.where(power < 0.0, f32(-3000.0), db).astype(np.float32)
+        img.power_db = db
+        inside = db[:final_bins][power[:final_bins] >= 0.0]
+        if inside.size >= 16:
+            img.floor_db = float(np.partition(inside, inside.size // 2)[inside.size // 2])
+
+        def at(ir, ie, ia):
+            return float(db[min(max(ir, 0), nr - 1), min(max(ie, 0), nel - 1), min(max(ia, 0), naz - 1)])
+
+        min_level = max(f32(f32(img.floor_db) + f32(self.setup.threshold_db)), f32(-self.setup.dynamic_range_db))
+        last_bin = min(nr - 1, final_bins - 1)
+        dets = []
+        if inside.size >= 16 and last_bin > 1:
+            # local maxima: no neighbour above, ties to the first index (src/Radar.cpp:690-704)
+            pad = np.full((nr + 2, nel + 2, naz + 2), -np.inf, np.float32)
+            pad[1:-1, 1:-1, 1:-1] = db
+            peak = np.ones((nr, nel, naz), bool)
+            for a in (-1, 0, 1):
+                for b in (-1, 0, 1):
+                    for c in (-1, 0, 1):
+                        if not (a or b or c):
+                            continue
+                        u = pad[1 + a:1 + a + nr, 1 + b:1 + b + nel, 1 + c:1 + c + naz]
+                        earlier = a < 0 or (a == 0 and (b < 0 or (b == 0 and c < 0)))
+                        peak &= ~((u > db) | ((u == db) if earlier else False))
+            cand = np.zeros_like(peak)
+            cand[1:last_bin] = True
+            peak &= cand & (db >= min_level)
+            for ir, ie, ia in np.argwhere(peak):
+                ir, ie, ia = int(ir), int(ie), int(ia)
+                v = float(db[ir, ie, ia])
+                d = RadarDetection()
+                fr = vertex_offset(at(ir - 1, ie, ia), v, at(ir + 1, ie, ia))
+                fe = vertex_offset(at(ir, ie - 1, ia), v, at(ir, ie + 1, ia))
+                fa = vertex_offset(at(ir, ie, ia - 1), v, at(ir, ie, ia + 1))
+                d.range = float(f32(f32(img.r0) + f32(f32(ir) + f32(fr)) * f32(img.dr)))
+                d.el_deg = float(f32(f32(img.el0) + f32(f32(ie) + f32(fe)) * f32(img.del_)))
+                d.az_deg = float(f32(f32(img.az0) + f32(f32(ia) + f32(fa)) * f32(img.daz)))
+                d.position = (self.tx + f32(d.range) * self.direction(d.az_deg, d.el_deg)).astype(np.float32)
+                d.rel_db = v
+                d.snr_db = float(f32(f32(v) - f32(img.floor_db)))
+                dets.append(d)
+        dets.sort(key=lambda d: -d.rel_db)
+        img.detections = dets[:K_MAX_DETECTIONS]
+
+        # truth: the nearest marked target and its point nearest the radar
+        for d in img.detections:
+            nearest = 1e30
+            for ti, t in enumerate(self.targets):
+                if len(t.points):
+                    dist = float(np.sqrt(((t.points - d.position) ** 2).sum(axis=1)).min())
+                else:
+                    q = np.clip(d.position, np.asarray(t.min, np.float32), np.asarray(t.max, np.float32))
+                    dist = float(np.linalg.norm(d.position - q))
+                if dist < nearest:
+                    nearest = dist
+                    d.target = ti
+            if d.target < 0:
+                continue
+            q = self.target_nearest[d.target]
+            d.target_range = float(np.linalg.norm(q - self.tx))
+            d.target_az_deg, d.target_el_deg = self.angles_of(q)
+
+        # display slices
+        img.bscope = db.max(axis=1)
+        br, be, ba = 0, nel // 2, naz // 2
+        if img.detections:
+            d = img.detections[0]
+            br = min(max(int(round((d.range - img.r0) / img.dr)), 0), nr - 1)
+            be = min(max(int(round((d.el_deg - img.el0) / img.del_)), 0), nel - 1)
+            ba = min(max(int(round((d.az_deg - img.az0) / img.daz)), 0), naz - 1)
+        img.cscope_range = float(f32(f32(img.r0) + f32(br) * f32(img.dr)))
+        img.cscope = db[max(br - 2, 0):min(br + 2, nr - 1) + 1].max(axis=0)
+        img.ascope = db[:, be, ba].copy()
+
+        # raw signals at the element nearest the transmit centre, relative to its transmit peak
+        ce = int(np.argmin(np.linalg.norm(self.element_pos - self.tx, axis=1)))
+        stride = max(1, N // 1500)
+        for k in range(0, N, stride):
+            cal, mv = float(self.cal[k, ce]), float(meas[k, ce])
+            img.trace_time.append(float(f32((k + 1) * self.dt / self.wavelength)))
+            img.trace_cal.append(float(f32(cal / self.peak_tx)))
+            img.trace_meas.append(float(f32(mv / self.peak_tx)))
+            img.trace_echo.append(float(f32((mv - cal) / self.peak_tx)))
+        img.valid = True
+        return img
+
+    def write_log(self) -> None:
+        """writeLog (src/Radar.cpp:762-774): the detection CSV."""
+        if not self.setup.log_path or not self.image.valid:
+            return
+        with open(self.setup.log_path, "w") as f:
+            f.write("detection,range_cells,range_m,azimuth_deg,elevation_deg,above_floor_db,relative_db,target,"
+                    "target_range_cells,target_azimuth_deg,target_elevation_deg\n")
+            for k, d in enumerate(self.image.detections):
+                name = self.targets[d.target].name if d.target >= 0 else ""
+                f.write(f"{k + 1},{_g(d.range)},{_g(d.range * self.cell_size)},{_g(d.az_deg)},{_g(d.el_deg)},"
+                        f"{_g(d.snr_db)},{_g(d.rel_db)},{name},{_g(d.target_range)},{_g(d.target_az_deg)},"
+                        f"{_g(d.target_el_deg)}\n")
+
+
+def _g(v: float) -> str:
+    """operator<< of a float at the default precision (6 significant digits)."""
+    return f"{float(f32(v)):.6g}"
+
+
+def run_radar(grid: E.SimulationGrid, monitor: RadarMonitor, with_targets, steps_per_frame: int = 4,
+              max_frames: int = 1_000_000) -> RadarImage:
+    """The application's radar schedule (src/Main.cpp:1066-1088), from a grid holding the
+    calibration scene (targets left out) with the monitor configured: steps_per_frame steps per
+    frame and update() after each; once the calibration pass is recorded, the materials with the
+    targets (sigma, eps, subcell) are uploaded, the fields reset and the measurement pass
+    recorded, until the image is final."""
+    frames = 0
+    while monitor.phase in (RadarMonitor.CALIBRATING, RadarMonitor.MEASURING) and frames < max_frames:
+        grid.step(steps_per_frame)
+        frames += 1
+        monitor.update()
+        if monitor.calibration_complete():
+            grid.upload_materials(*with_targets)  # clears the fields and rewinds time
+            grid.reset_fields()
+            monitor.begin_measurement()
+    return monitor.image

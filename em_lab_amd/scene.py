@@ -106,6 +106,10 @@ class Scene:
     fluxes: list = field(default_factory=list)
     warnings: list = field(default_factory=list)
     coaxes: list = field(default_factory=list)
+    # radar targets (`target = true` objects, src/ScriptEngine.cpp:830-838): their TargetInfo, and
+    # the materials without them (the radar's calibration scene), kept once a target is placed
+    targets: list = field(default_factory=list)
+    background: Optional[tuple] = None
 
     def __post_init__(self):
         nx, ny, nz = self.size
@@ -136,7 +140,30 @@ class Scene:
         self.sigma[z, y, x] = max(sig, f32(0.0))
         self.eps[z, y, x] = max(eps, f32(1.0))
 
-    def sphere(self, center, radius, eps_r=1.0, sigma=None, conductivity=None, loss_tangent=None) -> int:
+    def _paint(self, idx, sig, eps, target: bool, name: str) -> None:
+        """Fills cells (index arrays z, y, x); a radar target goes into the full scene only and
+        is recorded (targetInfo, src/ScriptEngine.cpp:800-838), other objects into both."""
+        if target:
+            from .radar import target_info
+            if self.background is None:
+                self.background = (self.sigma.copy(), self.eps.copy())
+            mask = np.zeros(self.sigma.shape, bool)
+            mask[idx] = True
+            self.targets.append(target_info(name, mask, self.size))
+        elif self.background is not None:
+            self.background[0][idx] = max(sig, f32(0.0))
+            self.background[1][idx] = max(eps, f32(1.0))
+        self.sigma[idx] = max(sig, f32(0.0))
+        self.eps[idx] = max(eps, f32(1.0))
+
+    def background_materials(self):
+        """(sigma, eps) flattened without the radar targets (the calibration scene)."""
+        if self.background is None:
+            return self.flat_materials()
+        return np.ascontiguousarray(self.background[0].ravel()), np.ascontiguousarray(self.background[1].ravel())
+
+    def sphere(self, center, radius, eps_r=1.0, sigma=None, conductivity=None, loss_tangent=None, target=False,
+               name="sphere") -> int:
         """EnvironmentBuilder::sphere -> fill(EllipsoidShape) (src/Environment.cpp:82-105)."""
         sig, eps = self.resolve(eps_r, sigma, conductivity, loss_tangent)
         c = np.asarray(center, np.float32)
@@ -150,11 +177,10 @@ class Scene:
         py = (yy.astype(np.float32) + f32(0.5) - c[1]) / r
         pz = (zz.astype(np.float32) + f32(0.5) - c[2]) / r
         inside = (px * px + py * py) + pz * pz <= f32(1.0)
-        self.sigma[zz[inside], yy[inside], xx[inside]] = max(sig, f32(0.0))
-        self.eps[zz[inside], yy[inside], xx[inside]] = max(eps, f32(1.0))
+        self._paint((zz[inside], yy[inside], xx[inside]), sig, eps, target, name)
         return int(inside.sum())
 
-    def box(self, mn, mx, eps_r=1.0, sigma=None, conductivity=None) -> int:
+    def box(self, mn, mx, eps_r=1.0, sigma=None, conductivity=None, target=False, name="box") -> int:
         """EnvironmentBuilder::box (src/Environment.cpp:99-101): cells with min <= centre < max."""
         sig, eps = self.resolve(eps_r, sigma, conductivity)
         mn = np.minimum(np.asarray(mn, np.float32), np.asarray(mx, np.float32))
@@ -167,8 +193,7 @@ class Scene:
             idx.append(np.arange(lo, hi)[(cc >= mn[a]) & (cc < mx2[a])])
         if any(len(i) == 0 for i in idx):
             return 0
-        self.sigma[np.ix_(idx[2], idx[1], idx[0])] = max(sig, f32(0.0))
-        self.eps[np.ix_(idx[2], idx[1], idx[0])] = max(eps, f32(1.0))
+        self._paint(np.ix_(idx[2], idx[1], idx[0]), sig, eps, target, name)
         return len(idx[0]) * len(idx[1]) * len(idx[2])
 
     # -- sources --

@@ -287,6 +287,35 @@ int emfd_lines_read(emfd_ctx *ctx, float *phasors, float *now, int n);
 double emfd_coax_geometry_factor(int inner, int outer);
 
 /* ---------------------------------------------------------------------------------------
+ * Radar back-projection (RadarMonitor::computeImage's imaging loop, src/Radar.cpp:622-658)
+ * --------------------------------------------------------------------------------------- */
+typedef struct emfd_radar_image_setup {
+    float tx[3];              /* transmit phase centre (cells) */
+    float axes[9];            /* azimuth axis, elevation axis, boresight (src/Radar.hpp:186) */
+    float interior_min[3];    /* simulated volume inside the absorbing layer */
+    float interior_max[3];
+    float wavelength;         /* carrier wavelength (cells) */
+    double dt;                /* solver time step */
+    double sample_dt;         /* spacing of the decimated echo samples (D dt) */
+    float r0, dr;             /* range bins */
+    int nr;
+    float az0, daz;           /* azimuth bins (degrees) */
+    int naz;
+    float el0, del;           /* elevation bins (degrees) */
+    int nel;
+} emfd_radar_image_setup;
+/* The image power [nr][nel][naz] (double; -1 where the point lies outside the interior) from the
+ * compressed, decimated echoes z ([m][n_el] complex float, interleaved re, im) of n_el receive
+ * elements at element_pos ([n_el][3]) with receive weights ([n_el]), and the transmit delay of
+ * n_tx transmitting elements at tx_pos ([n_tx][3]) with amplitudes tx_amp ([n_tx]): per point
+ * P = tx + r dir(az, el), t_i = txDelay(P) + |P - element_i| (src/Radar.cpp:86-104),
+ * power = |sum_i w_i z_i(t_i) e^{j w t_i}|^2 with z linearly interpolated. Runs on the
+ * context's device and stream (the grid need not be configured). */
+int emfd_radar_backproject(emfd_ctx *ctx, const emfd_radar_image_setup *setup, const float *z, int64_t m, int n_el,
+                           const float *element_pos, const double *weights, const float *tx_pos, const double *tx_amp,
+                           int n_tx, double *power);
+
+/* ---------------------------------------------------------------------------------------
  * PortMonitor (src/Ports.hpp:84-140)
  * --------------------------------------------------------------------------------------- */
 /* PortMonitor::configure (src/Ports.cpp:183-215): at most 256 ports; sweep may be NULL;
