@@ -110,6 +110,7 @@ class Scene:
     # the materials without them (the radar's calibration scene), kept once a target is placed
     targets: list = field(default_factory=list)
     background: Optional[tuple] = None
+    emitters: list = field(default_factory=list)  # emitter{} transmitters (EmitterSetup)
 
     def __post_init__(self):
         nx, ny, nz = self.size
@@ -405,6 +406,20 @@ class Scene:
         g = node[axis]
         self.add_thin_wire(node, axis, g - arm, g + 1 + arm, radius, g)
         self.add_feed(w, tuple(node), axis, sign, 1, impedance, amplitude, True, name)
+
+    def emitter(self, position, name=None, polarization="linear", toward=None, amplitude=1.0, **wave):
+        """emitter{} (src/ScriptEngine.cpp:1850-1892): a point transmitter of ideal dipoles,
+        linear (a direction) or circular ("rhcp" / "lhcp", toward the grid centre by default)."""
+        from .crpa import EmitterSetup, emitter_sources
+        w = self.waveform(**wave)
+        name = name or f"emitter {len(self.emitters) + 1}"
+        if any(e.name == name for e in self.emitters):
+            raise ValueError(f"an emitter named '{name}' already exists")
+        center = tuple(float(f32(v) * f32(0.5)) for v in self.size)
+        srcs = emitter_sources(len(self.emitters), position, w, polarization, toward, amplitude, center)
+        self.sources.extend(srcs)
+        pol = polarization.lower() if isinstance(polarization, str) else "linear"
+        self.emitters.append(EmitterSetup(name, tuple(float(v) for v in position), pol))
 
     def wire(self, p0, p1, radius=None):
         """wire{} (src/ScriptEngine.cpp:1380-1394)."""
