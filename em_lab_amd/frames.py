@@ -49,6 +49,14 @@ def build_scene(name: str, n: Optional[int] = None):
         return S.spheres_array(n=n or 192)
     if name.startswith("array:"):
         return S.draw_array_scene(int(name.split(":", 1)[1]))[1]
+    if name == "radar" or name.startswith("radar:"):
+        # a 2 x 4 dipole array as a pulsed radar (scenes/16_array_radar.lua's processing):
+        # "radar:sphere" (default) or "radar:plate"
+        return S.radar_dipole_array(target=name.split(":", 1)[1] if ":" in name else "sphere")
+    if name in ("crpa", "crpa:ideal", "crpa:blind"):
+        # a 2 x 2 crossed-dipole CRPA with a satellite and two jammers (scenes/17_crpa_gps.lua's
+        # processing), blind (default) or ideal adaptation
+        return S.crpa_crossed_dipoles(blind=not name.endswith(":ideal"))
     if name == "coax_line" or name.startswith("coax_line:"):
         # scenes/validation/coax_line.lua: a 150-cell 50 ohm line, lambda 48, terminated by a
         # matched load, an open, a short or 100 ohm ("coax_line:open", "coax_line:100")
@@ -115,10 +123,12 @@ class Instruments:
     err: Callable[[str], None] = lambda s: print(s, file=sys.stderr, flush=True)
 
     coax: object = None
+    radar: object = None
+    crpa: object = None
 
     def empty(self) -> bool:
         return (not self.probes and not self.fluxes and self.far_field is None and self.ports is None
-                and self.coax is None)
+                and self.coax is None and self.radar is None and self.crpa is None)
 
     # src/Instruments.cpp:393-410
     def coax_lines(self) -> list:
@@ -205,6 +215,11 @@ class Instruments:
             out.append("    flux  %-28s P=%+.4e  <P>=%+.4e  peak|P|=%.4e"
                        % (f.surface.label, f.instantaneous, f.time_averaged, f.peak_abs))
         out += self.port_lines(4)
+        if self.radar is not None:
+            from em_lab_amd.radar import report_lines as radar_lines
+            out += radar_lines(self.radar)
+        if self.crpa is not None:
+            out += self.crpa.report_lines()
         out += self.coax_lines()
         if self.far_field is not None:
             st = self.ff_setup
@@ -328,14 +343,33 @@ def run(argv=None, after_setup: Optional[Callable] = None, out=print) -> int:
     try:
         g.configure(sc.grid_config())
         s, e = sc.flat_materials()
-        g.upload_materials(s, e, sc.subcell)
+        radar = getattr(sc, "radar", None)
+        crpa = getattr(sc, "crpa", None)
+        if radar is not None:
+            # the calibration pass runs on the scene without its targets (src/Main.cpp:472-505)
+            bs, be = sc.background_materials()
+            g.upload_materials(bs, be, sc.subcell)
+        else:
+            g.upload_materials(s, e, sc.subcell)
         g.set_sources(sc.sources)
         size = g.size
         inst = Instruments(g, cell_size=sc.cell_size, averaging_time=3.0 * max(float(sc.wavelength), 2.0))
         for name, pos in probes:
             inst.probes.append(ProbeState(name, tuple(min(max(v, 0.0), size[k] - 1.0) for k, v in enumerate(pos))))
         inst.fluxes = [FluxState(f) for f in (sc.fluxes or [])]
-        if sc.ports:
+        if radar is not None or crpa is not None:
+            # one port monitor per context: the radar's / CRPA's recorder takes it
+            if radar is not None:
+                from em_lab_amd.radar import RadarMonitor
+                inst.radar = RadarMonitor(g)
+                inst.radar.configure(radar, sc.ports, sc.sources, sc.ports[0].wavelength if sc.ports else sc.wavelength,
+                                     sc.cell_size, sc.targets)
+            else:
+                from em_lab_amd.crpa import CrpaMonitor
+                inst.crpa = CrpaMonitor(g)
+                inst.crpa.configure(crpa, sc.emitters, sc.sources, sc.ports,
+                                    sc.ports[0].wavelength if sc.ports else sc.wavelength, sc.cell_size)
+        elif sc.ports:
             inst.ports = E.PortMonitor(g)
             avg = sc.far_field.averaging_periods if sc.far_field else 4.0
             inst.ports.configure(sc.ports, avg, sc.sweep, sc.cell_size)
@@ -353,11 +387,20 @@ def run(argv=None, after_setup: Optional[Callable] = None, out=print) -> int:
             out(NO_INSTRUMENTS_HINT)
         sim_ms = 0.0
         for frame in range(max(args.frames, 0)):
+            if inst.crpa is not None:
+                inst.crpa.update()  # switches the transmitting sources between passes (src/Main.cpp:1067)
             if steps > 0:
                 t = time.perf_counter()
                 g.step(steps)
                 g.synchronize()
                 sim_ms = (time.perf_counter() - t) * 1e3
+            if inst.radar is not None:
+                # swap in the targets once the calibration pass is recorded (src/Main.cpp:1079-1087)
+                inst.radar.update()
+                if inst.radar.calibration_complete():
+                    g.upload_materials(s, e, sc.subcell)
+                    g.reset_fields()
+                    inst.radar.begin_measurement()
             if frame % 2 == 0 or steps == 0:
                 inst.measure()
             if inst.coax is not None:

@@ -758,6 +758,36 @@ class RadarMonitor:
                         f"{_g(d.target_el_deg)}\n")
 
 
+def report_lines(mon: "RadarMonitor") -> list:
+    """Instruments::printRadar's lines (src/Instruments.cpp:412-442)."""
+    out = []
+    if mon.phase == RadarMonitor.OFF:
+        return out
+    img = mon.image
+    if mon.phase == RadarMonitor.CALIBRATING:
+        out.append("    radar: calibrating without the targets (%.0f%% of %d steps)" % (100.0 * mon.progress(), mon.steps))
+        return out
+    if mon.phase == RadarMonitor.MEASURING:
+        out.append("    radar: measuring (%.0f%% of %d steps), image final out to %.1f mm"
+                   % (100.0 * mon.progress(), mon.steps, img.complete_range * mon.cell_size * 1e3 if img.valid else 0.0))
+        if not img.valid:
+            return out
+    if not img.valid:
+        out.append("    radar: no image")
+        return out
+    cell = mon.cell_size * 1e3
+    out.append("    radar: strongest echo %.1f dB below the transmit pulse at an element, image floor %.1f dB, "
+               "%d detection(s)" % (-img.echo_db, img.floor_db, len(img.detections)))
+    for k, d in enumerate(img.detections):
+        line = ("      #%d range %6.1f mm (%5.2f lambda)  az %+5.1f  el %+5.1f deg  %+6.1f dB (%.1f dB above floor)"
+                % (k + 1, d.range * cell, d.range / mon.wavelength, d.az_deg, d.el_deg, d.rel_db, d.snr_db))
+        if d.target >= 0:
+            line += " | %s: front at %.1f mm, az %+.1f, el %+.1f" % (mon.targets[d.target].name, d.target_range * cell,
+                                                                    d.target_az_deg, d.target_el_deg)
+        out.append(line)
+    return out
+
+
 def _g(v: float) -> str:
     """operator<< of a float at the default precision (6 significant digits)."""
     return f"{float(f32(v)):.6g}"

@@ -111,6 +111,8 @@ class Scene:
     targets: list = field(default_factory=list)
     background: Optional[tuple] = None
     emitters: list = field(default_factory=list)  # emitter{} transmitters (EmitterSetup)
+    radar: Optional[object] = None  # radar{} (RadarSetup)
+    crpa: Optional[object] = None   # crpa{} (CrpaSetup)
 
     def __post_init__(self):
         nx, ny, nz = self.size
@@ -790,3 +792,59 @@ def draw_array_scene(index: int, seed: int = DEFAULT_SEED, lambda0: float = ARRA
     """(geometry, Scene) of design `index` (draw_array_design)."""
     geom, size = draw_array_design(index, seed, lambda0)
     return geom, array_scene(geom, size, lambda0)
+
+
+def radar_dipole_array(lam: float = 20.0, target: str = "sphere", range_wl: float = 4.0, az_deg: float = 10.0,
+                       el_deg: float = 5.0) -> Scene:
+    """A compact radar scene in the manner of scenes/16_array_radar.lua: a 2 x 4 array of z thin-wire
+    dipoles (0.45 lambda, 0.5 lambda apart in y, 0.7 lambda in z) looking down +x, the central 2 x 2
+    transmitting a pulse, every element receiving, and a metal target (sphere of radius
+    0.75 lambda, or a 2 x 2 lambda plate facing the array) `range_wl` wavelengths out."""
+    from .radar import RadarSetup
+    sc = Scene(size=(200, 112, 112), pml=10, courant=MAX_COURANT, cell_size=1.0e-3, wavelength=lam)
+    x0, yc, zc = 40, 56, 56
+    for row in (-1, 1):
+        for col in range(4):
+            y = yc + int(round((col - 1.5) * 10))
+            tx = col in (1, 2)
+            sc.dipole_thin((x0, y, zc + 7 * row), axis=2, length=9.0, radius=0.1, impedance=50.0,
+                           amplitude=1.0 if tx else 0.0, wavelength=lam, waveform="pulse", pulse_width=1.0,
+                           name=f"p[{row},{col}]")
+    sc.emit_thin_wires()
+    r = range_wl * lam
+    ra, re = math.radians(az_deg), math.radians(el_deg)
+    c = (x0 + r * math.cos(ra) * math.cos(re), yc - r * math.sin(ra) * math.cos(re), zc + r * math.sin(re))
+    if target == "sphere":
+        sc.sphere(c, 0.75 * lam, conductivity=5.8e7, target=True, name="sphere")
+    elif target == "plate":
+        sc.box((c[0] - 1, c[1] - lam, c[2] - lam), (c[0] + 1, c[1] + lam, c[2] + lam), conductivity=5.8e7,
+               target=True, name="plate")
+    else:
+        raise ValueError(f"unknown radar target '{target}'")
+    sc.radar = RadarSetup(fov_az_deg=30.0, fov_el_deg=30.0, boresight=(1.0, 0.0, 0.0))
+    return sc
+
+
+def crpa_crossed_dipoles(lam: float = 20.0, blind: bool = False) -> Scene:
+    """A CRPA scene in the manner of scenes/17_crpa_gps.lua: a 2 x 2 array (0.5 lambda apart) of
+    crossed thin-wire dipole pairs (X and Y probes, one cell apart in z so the wires never touch)
+    under an RHCP satellite near the zenith and two linearly polarized jammers near the horizon."""
+    from .crpa import CrpaElement, CrpaSetup
+    sc = Scene(size=(120, 120, 100), pml=10, courant=MAX_COURANT, cell_size=1.0e-3, wavelength=lam)
+    elements = []
+    for i, (cx, cy) in enumerate([(55, 55), (65, 55), (55, 65), (65, 65)]):
+        sc.dipole_thin((cx, cy, 30), axis=0, length=9.0, radius=0.1, impedance=50.0, amplitude=1.0, wavelength=lam,
+                       waveform="cw", ramp=3.0, name=f"x{i}")
+        sc.dipole_thin((cx, cy, 31), axis=1, length=9.0, radius=0.1, impedance=50.0, amplitude=1.0, wavelength=lam,
+                       waveform="cw", ramp=3.0, name=f"y{i}")
+        elements.append(CrpaElement(f"x{i}", f"y{i}", (cx + 0.5, cy + 0.5, 30.5)))
+    sc.emit_thin_wires()
+    sc.emitter((62.0, 58.0, 84.0), name="sat", polarization="rhcp", toward=(60.0, 60.0, 30.5), wavelength=lam,
+               waveform="cw", ramp=3.0)
+    sc.emitter((108.0, 60.0, 38.0), name="jam east", polarization=(0.0, 1.0, 0.3), wavelength=lam, waveform="cw",
+               ramp=3.0)
+    sc.emitter((14.0, 96.0, 42.0), name="jam north-west", polarization=(1.0, 1.0, 0.0), wavelength=lam,
+               waveform="cw", ramp=3.0)
+    sc.crpa = CrpaSetup(satellite="sat", elements=elements, blind=blind, pass_periods=20.0, snapshot_periods=16.0,
+                        jnr_db=40.0, snr_db=-20.0, noise_seed=1)
+    return sc

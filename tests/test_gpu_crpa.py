@@ -9,44 +9,22 @@ import pytest
 
 from em_lab_amd import crpa as CR
 from em_lab_amd import emfdtd as E
-from em_lab_amd.scene import MAX_COURANT, Scene
+from em_lab_amd.scene import crpa_crossed_dipoles
 
 pytestmark = pytest.mark.gpu
 
 LAM = 20.0
 
 
-def crpa_scene():
-    sc = Scene(size=(120, 120, 100), pml=10, courant=MAX_COURANT, cell_size=1.0e-3, wavelength=LAM)
-    elements = []
-    for i, (cx, cy) in enumerate([(55, 55), (65, 55), (55, 65), (65, 65)]):
-        # X probe at z = 30, Y probe one cell higher: the two wires never touch
-        sc.dipole_thin((cx, cy, 30), axis=0, length=9.0, radius=0.1, impedance=50.0, amplitude=1.0, wavelength=LAM,
-                       waveform="cw", ramp=3.0, name=f"x{i}")
-        sc.dipole_thin((cx, cy, 31), axis=1, length=9.0, radius=0.1, impedance=50.0, amplitude=1.0, wavelength=LAM,
-                       waveform="cw", ramp=3.0, name=f"y{i}")
-        elements.append(CR.CrpaElement(f"x{i}", f"y{i}", (cx + 0.5, cy + 0.5, 30.5)))
-    sc.emit_thin_wires()
-    sc.emitter((62.0, 58.0, 84.0), name="sat", polarization="rhcp", toward=(60.0, 60.0, 30.5), wavelength=LAM,
-               waveform="cw", ramp=3.0)
-    sc.emitter((108.0, 60.0, 38.0), name="jam east", polarization=(0.0, 1.0, 0.3), wavelength=LAM, waveform="cw",
-               ramp=3.0)
-    sc.emitter((14.0, 96.0, 42.0), name="jam north-west", polarization=(1.0, 1.0, 0.0), wavelength=LAM,
-               waveform="cw", ramp=3.0)
-    return sc, elements
-
-
 def _run(blind):
-    sc, elements = crpa_scene()
+    sc = crpa_crossed_dipoles(LAM, blind)
     g = E.SimulationGrid(0)
     try:
         g.configure(sc.grid_config())
         s, e = sc.flat_materials()
         g.upload_materials(s, e, sc.subcell)
         mon = CR.CrpaMonitor(g)
-        setup = CR.CrpaSetup(satellite="sat", elements=elements, blind=blind, pass_periods=20.0, snapshot_periods=16.0,
-                             jnr_db=40.0, snr_db=-20.0, noise_seed=1)
-        mon.configure(setup, sc.emitters, sc.sources, sc.ports, LAM, sc.cell_size)
+        mon.configure(sc.crpa, sc.emitters, sc.sources, sc.ports, LAM, sc.cell_size)
         assert mon.phase == (CR.CrpaMonitor.LISTENING if blind else CR.CrpaMonitor.MEASURING)
         res = CR.run_crpa(g, mon, steps_per_frame=8, adapted_steps=200)
         assert mon.phase == CR.CrpaMonitor.ADAPTED and res.valid

@@ -77,3 +77,33 @@ def test_frames_coax_line():
     waves = [ln for ln in lines if ln.startswith("      waves: Z0 measured ")]
     assert waves and float(waves[-1].split("Z0 measured ")[1].split(" ohm")[0]) == pytest.approx(50.0, abs=1.0)
     assert float(waves[-1].split("end Gamma ")[1].split(" @")[0]) < 0.08
+
+
+def test_frames_radar_scene_reports_detection():
+    """--scene radar: the calibration pass on the scene without its target, the target swapped in,
+    the measurement pass, and printRadar's lines (src/Instruments.cpp:412-442) with the detection
+    against the target's front."""
+    print("case: frames radar, 2x4 dipole array, sphere", flush=True)
+    lines = []
+    rc = F.run(["--scene", "radar", "--frames", "600", "--steps", "8"], out=lines.append)
+    assert rc == 0
+    summary = [ln for ln in lines if ln.startswith("    radar: strongest echo")]
+    assert summary, lines[-12:]
+    det = [ln for ln in lines if ln.startswith("      #1 range")]
+    assert det and "| sphere: front at" in det[-1]
+    rng = float(det[-1].split("range")[1].split("mm")[0])
+    front = float(det[-1].split("front at")[1].split("mm")[0])
+    assert abs(rng - front) < 2.0  # mm (1 mm cells)
+
+
+def test_frames_crpa_scene_reports_adaptation():
+    print("case: frames crpa ideal, 2x2 crossed dipoles", flush=True)
+    lines = []
+    rc = F.run(["--scene", "crpa:ideal", "--frames", "400", "--steps", "8"], out=lines.append)
+    assert rc == 0
+    sinr = [ln for ln in lines if ln.startswith("    crpa (MVDR weights): SINR")]
+    assert sinr, lines[-12:]
+    applied = float(sinr[-1].split("applied")[1].split("dB")[0])
+    steering = float(sinr[-1].split("steering")[1].split("dB")[0])
+    assert applied > steering + 20.0
+    assert sum(1 for ln in lines if ln.startswith("      jam ")) >= 2

@@ -9,7 +9,7 @@ import pytest
 
 from em_lab_amd import emfdtd as E
 from em_lab_amd import radar as R
-from em_lab_amd.scene import MAX_COURANT, Scene
+from em_lab_amd.scene import radar_dipole_array
 from oracle import radar as OR
 
 pytestmark = pytest.mark.gpu
@@ -64,32 +64,9 @@ def test_backproject_single_element_closed_form():
     assert np.all(near | (vals == 0.0)) and near.any() and (vals == 0.0).any() and (p < 0).any()
 
 
-def radar_scene(target=True):
-    """A 2 x 4 array of z thin-wire dipoles (0.45 lambda long, 0.5 / 0.7 lambda apart) looking
-    down +x, the central 2 x 2 transmitting a pulse, and a metal sphere (radius 0.75 lambda) about
-    4 wavelengths out, 10 degrees right and 5 degrees up."""
-    lam = 20.0
-    sc = Scene(size=(200, 112, 112), pml=10, courant=MAX_COURANT, cell_size=1.0e-3, wavelength=lam)
-    x0, yc, zc = 40, 56, 56
-    for row in (-1, 1):
-        for col in range(4):
-            y = yc + int(round((col - 1.5) * 10))
-            z = zc + 7 * row
-            tx = col in (1, 2)
-            sc.dipole_thin((x0, y, z), axis=2, length=9.0, radius=0.1, impedance=50.0, amplitude=1.0 if tx else 0.0,
-                           wavelength=lam, waveform="pulse", pulse_width=1.0, name=f"p[{row},{col}]")
-    sc.emit_thin_wires()
-    if target:
-        r, az, el = 4.0 * lam, 10.0, 5.0
-        ra, re = math.radians(az), math.radians(el)
-        c = (x0 + r * math.cos(ra) * math.cos(re), yc - r * math.sin(ra) * math.cos(re), zc + r * math.sin(re))
-        sc.sphere(c, 0.75 * lam, conductivity=5.8e7, target=True, name="sphere")
-    return sc
-
-
 def test_radar_run_detects_sphere():
     print("case: radar run, 2x4 dipole array, metal sphere", flush=True)
-    sc = radar_scene()
+    sc = radar_dipole_array()
     g = E.SimulationGrid(0)
     try:
         g.configure(sc.grid_config())
@@ -98,8 +75,7 @@ def test_radar_run_detects_sphere():
         g.set_sources(sc.sources)
         mon = R.RadarMonitor(g)
         warnings = []
-        mon.configure(R.RadarSetup(fov_az_deg=30.0, fov_el_deg=30.0, boresight=(1.0, 0.0, 0.0)), sc.ports,
-                      sc.sources, sc.wavelength, sc.cell_size, sc.targets, warnings=warnings)
+        mon.configure(sc.radar, sc.ports, sc.sources, sc.wavelength, sc.cell_size, sc.targets, warnings=warnings)
         assert mon.phase == R.RadarMonitor.CALIBRATING
         s, e = sc.flat_materials()
         img = R.run_radar(g, mon, (s, e, sc.subcell), steps_per_frame=8)
