@@ -119,7 +119,7 @@ struct emfd_ctx {
     // reads a thin-wire texel's B (texel itself, +x, +y, +z)
     emfd::DevBuf fix_b, fix_e;
     int n_fix_b = 0, n_fix_e = 0;
-    // the march's own material coefficients (emfd::MarchMat, 8-plane chunks): per texel local
+    // the march's own material coefficients (emfd::MarchMat, automatic z-chunks): per texel local
     // class bytes, per tile (first entry, count), entries; and what the E recomputation still
     // takes with them (fix_e_rest: thin-wire neighbours, materials in the absorber, overflow)
     emfd::DevBuf march_lid, march_tiles, march_coef, fix_e_rest;

@@ -133,8 +133,8 @@ hipError_t launch_radar_backproject(const emfd_radar_image_setup &s, const float
 
 // ---- launchers (kernels.hip) ----
 hipError_t launch_build_table(const uint16_t *keys, int classes, float dt, float *table, hipStream_t s);
-// Material coefficients the one-pass march applies itself (march tiles of its default 8-plane
-// z-chunk: 256 x cells, 7 rows, 8 planes; tile = x / 256 + nseg * (y / 7 + nyt * (z / 8))). Each
+// Material coefficients the one-pass march applies itself (march tiles of its automatic
+// z-chunk: 256 x cells, 7 rows, kMarchTileZ planes; tile = x / 256 + nseg * (y / 7 + nyt * (z / kMarchTileZ))). Each
 // tile lists up to 255 of the classes of its texels outside the absorber; lid holds a texel's
 // 1-based position in its tile's list (0: none, the march's vacuum / absorber coefficients).
 // Entries: kMarchCoefStride floats each (Ca x3, Cb x3 of the class, 2 unused).
@@ -149,7 +149,7 @@ struct MarchMat {
 
 // Fused one-pass z-march (yee_march.hip): src -> dst, then the sparse recomputation of the texels
 // in fix_b (B, thin-wire classes) and fix_e (E, non-vacuum classes and thin-wire neighbours).
-// cfg: z-chunk (0 = automatic). With the default 8-plane chunk and material tables (mm.lid set),
+// cfg: z-chunk (0 = automatic). With the automatic chunk and material tables (mm.lid set),
 // the march applies the listed classes' coefficients and the E recomputation takes fix_e_rest
 // (thin-wire neighbours, materials in the absorber, classes past a tile's 255) instead of fix_e.
 hipError_t launch_yee_march(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int cfg,

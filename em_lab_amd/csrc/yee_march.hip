@@ -73,7 +73,7 @@ __device__ __forceinline__ void side_put4(const SideCols &s, uint32_t bits, int 
             }
 }
 
-// 8-plane chunks: each wave keeps its row's side column values of the chunk in an LDS record
+// Automatic chunks (kMarchTileZ planes): each wave keeps its row's side column values of the chunk in an LDS record
 // [column k = 2 face + (x - X + 1)][component E x y z, B x y z][plane of the chunk], and the
 // workgroup stores the records once the chunk is done (side_flush), so that no scattered store
 // sits among the march's loads and stores (stores enter the same wait count as the plane loads in
@@ -84,8 +84,8 @@ __device__ __forceinline__ void side_rec_put(bool on, float *r, const float (&b)
     if (on) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            r[c * 8] = e[c][J];
-            r[(3 + c) * 8] = b[c][J];
+            r[c * kMarchTileZ] = e[c][J];
+            r[(3 + c) * kMarchTileZ] = b[c][J];
         }
     }
 }
@@ -97,12 +97,13 @@ __device__ __forceinline__ void side_flush(const SideCols &s, const float *recs,
                                            int z1) {
     const long col = static_cast<long>(s.nz) * s.nyp;
     const int yg = yt - s.gy0;
-    for (int k = static_cast<int>(threadIdx.x); k < 2 * 2 * 6 * 8 * 8; k += 64 * NW) {
+    // record index sec = (column * 6 + component) * kMarchTileZ + kz; 8 row slots per sector
+    for (int k = static_cast<int>(threadIdx.x); k < 2 * 2 * 6 * kMarchTileZ * 8; k += 64 * NW) {
         const int sec = k >> 3, slot = k & 7;
-        const int cl = sec / 48, comp = (sec >> 3) % 6, kz = sec & 7;
+        const int cl = sec / (6 * kMarchTileZ), comp = (sec / kMarchTileZ) % 6, kz = sec % kMarchTileZ;
         const int z = z0 + kz, x = s.x[cl >> 1] - 1 + (cl & 1);
         if (z < z1 && z >= s.z0 && z < s.z0 + s.nz && x >= xs && x < xend) {
-            const float v = slot < 7 ? recs[(slot + 1) * (2 * 2 * 6 * 8) + sec] : 0.0f;
+            const float v = slot < 7 ? recs[(slot + 1) * (2 * 2 * 6 * kMarchTileZ) + sec] : 0.0f;
             s.buf[(static_cast<long>((cl >> 1) * 6 + comp) * 2 + (cl & 1)) * col + static_cast<long>(z - s.z0) * s.nyp +
                   yg * 8 + slot] = v;
         }
@@ -181,7 +182,7 @@ __device__ __forceinline__ float rnd(float v, float inv, float s) {
 //   xe[p & 1][w]: E^n(row of wave w, plane p) z, x components
 //   xb[p & 1][w]: B^{n+1/2}(row of wave w, plane p) x, z components
 //   mat: the tile's material classes, entry l (1-based local class) at l * kCoefStride
-//   side[w]: far-field side column values of wave w's row over the chunk (8-plane chunks)
+//   side[w]: far-field side column values of wave w's row over the chunk (automatic chunks)
 template <int NW>
 __host__ __device__ constexpr size_t march_xch_floats() {
     return static_cast<size_t>(2) * 2 * NW * 2 * kSeg;
@@ -205,7 +206,7 @@ static_assert(2 * (kMarchMaxLocal + 1) <= 512 + 2, "one float4 copy per thread c
 template <int NW, int ZC, class T>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16 / NW, 16 / NW))) k_yee_march(LatticeArgs a, FieldPtrs src, FieldPtrs dst, int zchunk,
                                                         int nseg, int nyt, int n_tiles, MarchMat mm) {
-    static_assert(ZC == 0 || ZC == kMarchTileZ, "material tables and side records are laid out for 8-plane chunks");
+    static_assert(ZC == 0 || ZC == kMarchTileZ, "material tables and side records are laid out for the automatic chunks");
     extern __shared__ float lds[];
     const int lane = static_cast<int>(threadIdx.x) & 63;
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
@@ -265,7 +266,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
     float *side_l = mat + kMatFloats + w * kSideFloats;
     float *coef = lds + march_lds_fixed_floats<NW>() + static_cast<size_t>(w) * 2 * K * kCoefStride;
     float *coefB = coef, *coefE = coef + K * kCoefStride;
-    // The tile's material classes (8-plane chunks with tables only): copied into LDS by one
+    // The tile's material classes (automatic chunks with tables only): copied into LDS by one
     // unconditional float4 load per thread (a range of 0 reads zeros), issued before the plane
     // loads so that waiting for it waits for nothing else. Per texel local class ids (4 per lane,
     // one byte each) come with the B loads of their plane, from a range of 0 where the tile has
@@ -299,7 +300,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
                                         __builtin_amdgcn_ballot_w64(sbits != 0u) != 0
                                   : out_row && a.side.buf && y >= a.side.y0 && y < a.side.y0 + a.side.ny;
     const int xend = min(xs + kSeg, nx);
-    // 8-plane chunks: the workgroup stores side columns when one of the columns lies in its
+    // automatic chunks: the workgroup stores side columns when one of the columns lies in its
     // segment, its rows meet the box's range and the chunk its z range (uniform)
     bool side_wg = false;
     if (ZC > 0 && a.side.buf) {
@@ -516,7 +517,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
                         const int d = a.side.x[k >> 1] - 1 + (k & 1) - xs;  // uniform
                         if (d >= 0 && xs + d < xend) {
                             const bool on = lane == (d >> 2);
-                            float *r = side_l + k * 48 + (z - z0);
+                            float *r = side_l + k * 6 * kMarchTileZ + (z - z0);
                             switch (d & 3) {
                                 case 0: side_rec_put<0>(on, r, bn, e); break;
                                 case 1: side_rec_put<1>(on, r, bn, e); break;
@@ -653,8 +654,10 @@ hipError_t launch_nw(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs
         // thin z-chunks: the workgroups in flight then cover a thin slab, which streams best
         // although each chunk re-reads a leading plane (768^3 vacuum, unrolled marches: chunks of
         // 4 planes 3.89 ms per step, 6 planes 3.79, 8 planes 3.82, 12 planes 3.86, 16 planes 3.91;
-        // with a loop, 48 planes 4.00, 96 planes 4.21)
-        zchunk = 8;
+        // with a loop, 48 planes 4.00, 96 planes 4.21; with the material tables and side records,
+        // 6 planes measured 3.82 against 3.84 ms in vacuum but 3.97 against 3.94 ms in the 768^3
+        // precision scene, whose per-chunk table loads and side stores then come more often)
+        zchunk = kMarchTileZ;
     }
     if (zchunk > a.nz) zchunk = a.nz;
     const int nzc = (a.nz + zchunk - 1) / zchunk;
@@ -668,11 +671,11 @@ hipError_t launch_nw(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs
         if (e == hipSuccess) raised = lds;
         return e;
     };
-    if (zchunk == 8) {
+    if (zchunk == kMarchTileZ) {
         static size_t raised = 0;
-        if (hipError_t e = allow_lds(reinterpret_cast<const void *>(&k_yee_march<NW, 8, T>), raised); e != hipSuccess)
+        if (hipError_t e = allow_lds(reinterpret_cast<const void *>(&k_yee_march<NW, kMarchTileZ, T>), raised); e != hipSuccess)
             return e;
-        k_yee_march<NW, 8, T><<<n_tiles, 64 * NW, lds, s>>>(a, src, dst, zchunk, nseg, nyt, n_tiles, mm);
+        k_yee_march<NW, kMarchTileZ, T><<<n_tiles, 64 * NW, lds, s>>>(a, src, dst, zchunk, nseg, nyt, n_tiles, mm);
     } else {
         static size_t raised = 0;
         if (hipError_t e = allow_lds(reinterpret_cast<const void *>(&k_yee_march<NW, 0, T>), raised); e != hipSuccess)
@@ -694,7 +697,7 @@ hipError_t launch_yee_march(const LatticeArgs &a, const FieldPtrs &src, const Fi
                             const MarchMat &mm, const uint32_t *fix_e_rest, int n_fix_e_rest, hipStream_t s) {
     if (!yee_march_supported(a)) return hipErrorInvalidValue;
     const int zchunk = cfg;  // 0: automatic
-    // the material tables are laid out for the automatic 8-plane chunks
+    // the material tables are laid out for the automatic chunks
     const bool tables = mm.lid && mm.tiles && mm.coef && (zchunk == 0 || zchunk == kMarchTileZ) && a.nz >= kMarchTileZ;
     const MarchMat use = tables ? mm : MarchMat{};
     if (tables) {
