@@ -212,7 +212,9 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
     // tiles in dispatch order: x segment fastest, then the row group, then the z-chunk, so the
     // workgroups in flight cover one thin z-slab of the whole lattice at a time (handing each XCD
-    // a contiguous run of tiles instead measured 4-8% slower at 768^3)
+    // a contiguous run of tiles instead measured 4-8% slower at 768^3, and dealing runs of 3 to 48
+    // consecutive tiles to one XCD, so that vertically adjacent row groups would share its L2,
+    // 1-6% slower)
     const int tile = static_cast<int>(blockIdx.x);
     if (tile >= n_tiles) return;  // uniform over the workgroup, before any barrier
     const int nx = a.nx, ny = a.ny, nz = a.nz, L = a.pml, K = a.pml_k;
@@ -252,8 +254,8 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
     // unconditionally and no address is computed per plane in vector registers, so that the
     // compiler's wait counts stay exact (a skipped load, or a register reused while its load is
     // in flight, makes it wait for every load in flight).
-    auto plane_of = [&](const float *p, int z, int zmax) {
-        const bool ok = z >= 0 && z <= zmax;
+    auto plane_of = [&](const float *p, int z, int zmax, int zmin = 0) {
+        const bool ok = z >= zmin && z <= zmax;
         const char *base = reinterpret_cast<const char *>(p) + static_cast<size_t>(ok ? z : 0) * plane;
         return plane_rsrc(reinterpret_cast<const float *>(base), ok ? plane : 0u);
     };
@@ -322,10 +324,13 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
 
     // E^n of the row at plane z, and lane 0's E^n x, y, z at xs - 1
     const int ze_max = min(z1, nz - 1), zb_max = z1 - 1;  // last E and B^n planes the chunk reads
+    // Values only B^{n+1/2} z depends on are not read on the leading plane (its B z is never
+    // used: no E is updated there), nor is E^n z on the plane past the chunk (only B x, y of the
+    // last plane read that plane): those loads get a range of 0 and move no bytes.
     auto load_e = [&](int z, Row4 &r, float (&hl)[3]) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const Rsrc q = plane_of(src.e[c], z, ze_max);
+            const Rsrc q = plane_of(src.e[c], z, c == 2 ? zb_max : ze_max);
             ld4<T>(q, o_own, sE[c], r.v[c]);
             hl[c] = ld1<T>(q, o_hl, sE[c]);
         }
@@ -340,8 +345,10 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
     };
     auto load_b = [&](int z, BSet &s) {
         const Rsrc b0 = plane_of(src.b[0], z, zb_max), b1 = plane_of(src.b[1], z, zb_max),
-                   b2 = plane_of(src.b[2], z, zb_max);
-        const Rsrc e0 = plane_of(src.e[0], z, zb_max), e1 = plane_of(src.e[1], z, zb_max),
+                   b2 = plane_of(src.b[2], z, zb_max, z0);
+        // e0 (lane 0's E x at y + 1, the top wave's E x of row y + 1) and e1 (lane 63's E y past
+        // the segment) feed B z only
+        const Rsrc e0 = plane_of(src.e[0], z, zb_max, z0), e1 = plane_of(src.e[1], z, zb_max, z0),
                    e2 = plane_of(src.e[2], z, zb_max);
         ld4<T>(b0, o_own, sB[0], s.b.v[0]);
         ld4<T>(b1, o_own, sB[1], s.b.v[1]);
