@@ -863,10 +863,12 @@ namespace {
 // The march is the automatic choice for FP32 fields from 8e6 texels. With 16-bit storage it halves
 // the field bytes but not its instructions, and measured slower than the two 16-bit passes
 // (768^3 precision scene 3.44 against 2.92 ms per Yee step, vacuum 2.84 against 2.76 ms): it runs
-// there only when asked for.
+// there only when asked for. With the convolutional PML the march runs too (its slabs corrected
+// after it) when every dimension exceeds the widened slabs, 2 pml + 2 cells.
 bool march_auto(const emfd_ctx *ctx) {
-    return ctx->storage == kStorageF32 && ctx->absorber == EMFD_ABSORBER_GRADED && ctx->n >= 8000000u &&
-           yee_march_supported(ctx->lattice());
+    const LatticeArgs lat = ctx->lattice();
+    return ctx->storage == kStorageF32 && ctx->n >= 8000000u && yee_march_supported(lat) &&
+           (ctx->absorber == EMFD_ABSORBER_GRADED || cpml_march_supported(lat));
 }
 }  // namespace
 
@@ -976,11 +978,11 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
             return fail(EMFD_ERR_INVALID, "the vectorised two-pass kernel needs nx % 4 == 0");
         if (variant == EMFD_KERNEL_MARCH && !yee_march_supported(lat))
             return fail(EMFD_ERR_INVALID, "the one-pass march kernel needs nx % 4 == 0 and planes of at most 2 GB");
-        if (variant == EMFD_KERNEL_MARCH && ctx->absorber == EMFD_ABSORBER_CPML)
-            return fail(EMFD_ERR_INVALID, "the convolutional PML runs with the two-pass kernels");
+        if (variant == EMFD_KERNEL_MARCH && ctx->absorber == EMFD_ABSORBER_CPML && !cpml_march_supported(lat))
+            return fail(EMFD_ERR_INVALID, "the convolutional PML with the march needs every dimension above 2 pml + 2");
         if (variant == -1) {
             // launched above
-        } else if (ctx->absorber == EMFD_ABSORBER_CPML) {
+        } else if (ctx->absorber == EMFD_ABSORBER_CPML && variant != EMFD_KERNEL_MARCH) {
             // Faraday pass, its auxiliary fields, Ampere pass, its auxiliary fields
             const FieldPtrs f = ctx->fields(ctx->cur);
             const float *coef = ctx->d_cpml_coef.as<float>();
@@ -1000,10 +1002,11 @@ int emfd_grid_step(emfd_ctx *ctx, int count) {
             // bits 8.. of the variant: the march's z-chunk, or under the automatic choice with 16-bit
             // storage the two-pass kernels' cells per thread (the march then picks its z-chunk)
             const int zc = ctx->storage != kStorageF32 && ctx->kernel_variant == EMFD_KERNEL_AUTO ? 0 : ctx->tile_cfg;
+            const MarchCpml cpml{ctx->d_cpml_coef.as<float>(), ctx->d_psi.as<float>()};
             EMFD_HIP(launch_yee_march(lat, ctx->fields(ctx->cur), ctx->fields(ctx->cur ^ 1), zc,
                                       ctx->fix_b.as<uint32_t>(), ctx->n_fix_b, ctx->fix_e.as<uint32_t>(), ctx->n_fix_e,
                                       ctx->march_mat(), ctx->fix_e_rest.as<uint32_t>(), ctx->n_fix_e_rest,
-                                      ctx->stream));
+                                      ctx->absorber == EMFD_ABSORBER_CPML ? &cpml : nullptr, ctx->stream));
             ctx->cur ^= 1;
         }
         if (ev1) EMFD_HIP(hipEventRecord(ev1, ctx->stream));

@@ -152,9 +152,16 @@ struct MarchMat {
 // cfg: z-chunk (0 = automatic). With the automatic chunk and material tables (mm.lid set),
 // the march applies the listed classes' coefficients and the E recomputation takes fix_e_rest
 // (thin-wire neighbours, materials in the absorber, classes past a tile's 255) instead of fix_e.
+// cpml (or null): the convolutional PML's tables; its B corrections follow the B recomputation,
+// its E recomputation and corrections follow the E one (launch_cpml_march).
+struct MarchCpml {
+    const float *coef;
+    float *psi;
+};
 hipError_t launch_yee_march(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int cfg,
                             const uint32_t *fix_b, int n_fix_b, const uint32_t *fix_e, int n_fix_e,
-                            const MarchMat &mm, const uint32_t *fix_e_rest, int n_fix_e_rest, hipStream_t s);
+                            const MarchMat &mm, const uint32_t *fix_e_rest, int n_fix_e_rest,
+                            const MarchCpml *cpml, hipStream_t s);
 int yee_march_supported(const LatticeArgs &a);
 // passes: bit 0 the Faraday (B) pass, bit 1 the Ampere (E) pass
 hipError_t launch_yee_two_pass(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s, int passes = 3);
@@ -166,6 +173,12 @@ hipError_t launch_yee_two_pass4(const LatticeArgs &a, const FieldPtrs &f, hipStr
 hipError_t launch_cpml(const LatticeArgs &a, const FieldPtrs &f, const float *coef, float *psi, bool e_phase,
                        hipStream_t s);
 size_t cpml_psi_floats(int nx, int ny, int nz, int pml);
+// The same absorber after the one-pass march (src: step n, dst: the march's output): e_phase
+// false, the B corrections of the slabs; true, E recomputed from the corrected B wherever a curl
+// reads one, then the E corrections. Needs every dimension above 2 pml + 2.
+hipError_t launch_cpml_march(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, const float *coef,
+                             float *psi, bool e_phase, hipStream_t s);
+int cpml_march_supported(const LatticeArgs &a);
 int yee_two_pass4_supported(const LatticeArgs &a);
 hipError_t launch_yee_half(const LatticeArgs &a, const FieldPtrs &f, int cells_per_thread, hipStream_t s);
 hipError_t launch_yee_f64(const LatticeArgs &a, const FieldPtrs &f, hipStream_t s);

@@ -704,7 +704,8 @@ int yee_march_supported(const LatticeArgs &a) {
 
 hipError_t launch_yee_march(const LatticeArgs &a, const FieldPtrs &src, const FieldPtrs &dst, int cfg,
                             const uint32_t *fix_b, int n_fix_b, const uint32_t *fix_e, int n_fix_e,
-                            const MarchMat &mm, const uint32_t *fix_e_rest, int n_fix_e_rest, hipStream_t s) {
+                            const MarchMat &mm, const uint32_t *fix_e_rest, int n_fix_e_rest,
+                            const MarchCpml *cpml, hipStream_t s) {
     if (!yee_march_supported(a)) return hipErrorInvalidValue;
     const int zchunk = cfg;  // 0: automatic
     // the material tables are laid out for the automatic chunks
@@ -726,10 +727,12 @@ hipError_t launch_yee_march(const LatticeArgs &a, const FieldPtrs &src, const Fi
         k_march_fix_b<<<grid8(n_fix_b), 256, 0, s>>>(a, src, dst, fix_b, n_fix_b);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
+    if (cpml && (e = launch_cpml_march(a, src, dst, cpml->coef, cpml->psi, false, s)) != hipSuccess) return e;
     if (n_fix_e > 0) {
         k_march_fix_e<<<grid8(n_fix_e), 256, 0, s>>>(a, src, dst, fix_e, n_fix_e);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
+    if (cpml && (e = launch_cpml_march(a, src, dst, cpml->coef, cpml->psi, true, s)) != hipSuccess) return e;
     return hipSuccess;
 }
 
