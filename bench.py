@@ -14,8 +14,9 @@ roofline = the Yee update: algorithmic 48 B per cell-update (E and B read once, 
            when no profile of that workload exists
 cpu_baseline = the CPU oracle (scalar C, OpenMP over z-planes) on the 192^3 copy of the same
            scene: median of 3 runs
-extras   = the pure-vacuum 768^3 Yee update (the roofline target), the same scene with binary16
-           field storage, and the 256^3 sphere / thin-wire dipole scene
+extras   = the pure-vacuum 768^3 Yee update, the same scene with the convolutional PML in the
+           absorber slabs (its roofline counting the auxiliary fields' bytes too), the same scene
+           with binary16 field storage, and the 256^3 sphere / thin-wire dipole scene
 surrogate = the learned screening path: 65,536 seeded thin-wire antenna candidates through the
            trained MLP (BF16 MFMA), the best 256 by predicted -10 dB bandwidth verified by the
            batched FDTD stepper (4000 steps each); designs screened per second against
@@ -37,6 +38,9 @@ sys.path.insert(0, str(ROOT))
 METRIC = "Mcell-updates/s & %HBM peak; designs/s vs plain FDTD at |ΔS11|≤0.5 dB"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = 48  # FP32 E and B (6 components), each read once and written once
+# convolutional PML: per texel of an axis's slab (2 pml + 1 planes), its four auxiliary fields
+# (B_v, B_w after Faraday, E_v, E_w after Ampere) read once and written once
+CPML_AUX_BYTES = 4 * 4 * 2
 MIN_TIMED_LAUNCHES = 8
 
 
@@ -161,7 +165,11 @@ def run_grid(name: str, n: int, storage: str, kernel: int, steps: int, warmup: i
         del Eg
         bpc = BYTES_PER_CELL if st == E.STORAGE_F32 else BYTES_PER_CELL // 2
         med_ms = statistics.median(samples) if samples else yee_ms / max(launches, 1)
-        achieved = bpc * cells / (med_ms * 1e-3) / 1e9
+        aux = 0
+        if absorber == "cpml":
+            slab = 2 * scene.pml + 1
+            aux = CPML_AUX_BYTES * slab * (ny * nz + nx * nz + nx * ny)
+        achieved = (bpc * cells + aux) / (med_ms * 1e-3) / 1e9
         key = f"{name}_{n}_{storage}_k{kernel}" + ("_cpml" if absorber == "cpml" else "")
         res = {
             "value": round(cells * steps / wall / 1e6, 2),
@@ -177,6 +185,7 @@ def run_grid(name: str, n: int, storage: str, kernel: int, steps: int, warmup: i
                          "min_launch_ms": round(min(samples), 5) if samples else None,
                          "max_launch_ms": round(max(samples), 5) if samples else None,
                          "launches": launches, "timed_every": every, "bytes_per_cell": bpc,
+                         **({"cpml_aux_bytes": aux} if aux else {}),
                          "traffic_key": key},
         }
         if pattern and ff is not None:
@@ -213,7 +222,7 @@ def main() -> int:
                     help="field storage: FP32 planes, or 16-bit planes with FP32 updates")
     ap.add_argument("--absorber", choices=["graded", "cpml"], default="graded",
                     help="graded: the reference's conductivity absorber; cpml: the convolutional PML")
-    ap.add_argument("--no-extras", action="store_true", help="skip the vacuum / 16-bit / 256^3 extra workloads")
+    ap.add_argument("--no-extras", action="store_true", help="skip the vacuum / CPML / 16-bit / 256^3 extra workloads")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the learned screening measurement (designs/s against one-at-a-time FDTD)")
@@ -283,10 +292,12 @@ def main() -> int:
     if not args.no_extras:
         extras = {}
         n_ex = max(8, min(args.steps, 40))
-        for key, (scene, n, storage, mon) in {"vacuum_768": ("vacuum", args.size, "f32", False),
-                                              "f16_768": (args.scene, args.size, "f16", True),
-                                              "spheres_256": ("spheres", 256, "f32", True)}.items():
-            r, ok = run_grid(scene, n, storage, args.kernel, n_ex if n > 256 else 4 * n_ex, 5, mon)
+        for key, (scene, n, storage, mon, absorber) in {
+                "vacuum_768": ("vacuum", args.size, "f32", False, "graded"),
+                "cpml_768": (args.scene, args.size, "f32", True, "cpml"),
+                "f16_768": (args.scene, args.size, "f16", True, "graded"),
+                "spheres_256": ("spheres", 256, "f32", True, "graded")}.items():
+            r, ok = run_grid(scene, n, storage, args.kernel, n_ex if n > 256 else 4 * n_ex, 5, mon, absorber=absorber)
             if not ok:
                 print(f"bench.py: non-finite fields in {key}", file=sys.stderr)
                 return 2

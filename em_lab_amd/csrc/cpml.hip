@@ -203,13 +203,28 @@ __global__ void __launch_bounds__(256) k_cpml_march_b(LatticeArgs a, FieldPtrs s
     int q[3];
     if (!shell_texel(a, a.pml, a.pml + 1, static_cast<long>(blockIdx.x) * 256 + threadIdx.x, q)) return;
     const size_t p = (static_cast<size_t>(q[2]) * a.ny + q[1]) * a.nx + q[0];
-    float b[3] = {dst.b[0][p], dst.b[1][p], dst.b[2][p]};
+    // the components some slab of this texel corrects (slab u: components u + 1, u + 2)
+    const int L = a.pml, n[3] = {a.nx, a.ny, a.nz};
+    bool in[3], touch[3];
+#pragma unroll
+    for (int u = 0; u < 3; ++u) in[u] = q[u] < L || q[u] >= n[u] - 1 - L;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) touch[c] = in[(c + 1) % 3] || in[(c + 2) % 3];
+    float b[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) b[c] = touch[c] ? dst.b[c][p] : 0.0f;
     march_b_axis<0>(a, src, coef, psi, stride, q, p, b);
     march_b_axis<1>(a, src, coef, psi, stride, q, p, b);
     march_b_axis<2>(a, src, coef, psi, stride, q, p, b);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) dst.b[c][p] = b[c];
-    cpml_side(a, 3, q, b);
+    for (int c = 0; c < 3; ++c)
+        if (touch[c]) dst.b[c][p] = b[c];
+    if (a.side.buf) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (!touch[c]) b[c] = dst.b[c][p];
+        cpml_side(a, 3, q, b);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_cpml_march_e(LatticeArgs a, FieldPtrs src, FieldPtrs dst,
