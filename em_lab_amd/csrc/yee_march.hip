@@ -330,10 +330,15 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
     // Values only B^{n+1/2} z depends on are not read on the leading plane (its B z is never
     // used: no E is updated there), nor is E^n z on the plane past the chunk (only B x, y of the
     // last plane read that plane): those loads get a range of 0 and move no bytes.
+    // Wave 0 (the row below the group) only feeds B^{n+1/2} x, z of its row on the chunk's own
+    // planes to the first row's E update: it reads no B y, nothing on the leading plane, and of
+    // the plane past the chunk only E y (B x of the last plane).
+    const bool halo = w == 0;  // wave-uniform
+    const int zlo = halo ? z0 : 0, zno = -2;  // zno: a last plane below every plane, range 0
     auto load_e = [&](int z, Row4 &r, float (&hl)[3]) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const Rsrc q = plane_of(src.e[c], z, c == 2 ? zb_max : ze_max);
+            const Rsrc q = plane_of(src.e[c], z, c == 2 || (c == 0 && halo) ? zb_max : ze_max, zlo);
             ld4<T>(q, o_own, sE[c], r.v[c]);
             hl[c] = ld1<T>(q, o_hl, sE[c]);
         }
@@ -347,12 +352,13 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(16
         uint32_t lid;    // local material classes of the lane's four cells (byte j: cell j)
     };
     auto load_b = [&](int z, BSet &s) {
-        const Rsrc b0 = plane_of(src.b[0], z, zb_max), b1 = plane_of(src.b[1], z, zb_max),
+        const Rsrc b0 = plane_of(src.b[0], z, zb_max, zlo), b1 = plane_of(src.b[1], z, halo ? zno : zb_max),
                    b2 = plane_of(src.b[2], z, zb_max, z0);
         // e0 (lane 0's E x at y + 1, the top wave's E x of row y + 1) and e1 (lane 63's E y past
-        // the segment) feed B z only
-        const Rsrc e0 = plane_of(src.e[0], z, zb_max, z0), e1 = plane_of(src.e[1], z, zb_max, z0),
-                   e2 = plane_of(src.e[2], z, zb_max);
+        // the segment) feed B z only; wave 0 reads e1 alone (lane 0's values and the top wave's
+        // feed only the E update, which wave 0 does not make)
+        const Rsrc e0 = plane_of(src.e[0], z, halo ? zno : zb_max, z0), e1 = plane_of(src.e[1], z, zb_max, z0),
+                   e2 = plane_of(src.e[2], z, halo ? zno : zb_max);
         ld4<T>(b0, o_own, sB[0], s.b.v[0]);
         ld4<T>(b1, o_own, sB[1], s.b.v[1]);
         ld4<T>(b2, o_own, sB[2], s.b.v[2]);
