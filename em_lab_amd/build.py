@@ -16,8 +16,8 @@ LIB_DIR = PKG / "_lib"
 LIB = LIB_DIR / "libemfdtd.so"
 OBJ_DIR = PKG / "_build"
 
-SOURCES = ["kernels.hip", "yee_two_pass4.hip", "yee_march.hip", "yee_half.hip", "yee_f64.hip", "monitors.hip", "farfield.hip", "dft.hip", "noalloc.hip", "cpml.hip", "batch.hip", "mlp.hip", "radar.hip",
-           "grid.cpp", "monitors.cpp", "batch.cpp", "mlp.cpp", "radar.cpp"]
+SOURCES = ["kernels.hip", "yee_two_pass4.hip", "yee_march.hip", "yee_half.hip", "yee_f64.hip", "monitors.hip", "farfield.hip", "dft.hip", "noalloc.hip", "cpml.hip", "batch.hip", "mlp.hip", "radar.hip", "render.hip",
+           "grid.cpp", "monitors.cpp", "batch.cpp", "mlp.cpp", "radar.cpp", "render.cpp"]
 
 # -ffp-contract=off: every expression rounds as the reference shader writes it (no fused
 # multiply-add), so results can be compared against the CPU restatement to the last ulp

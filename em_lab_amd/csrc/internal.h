@@ -131,6 +131,21 @@ hipError_t launch_radar_backproject(const emfd_radar_image_setup &s, const float
                                     const float *el_pos, const double *el_w, const float *tx_pos,
                                     const double *tx_amp, int n_tx, double *power, hipStream_t st);
 
+// Volume view of the fields (render.hip): one frame of EM_Lab's emission / absorption raymarch.
+struct RenderArgs {
+    float inv_vp[16];  // inverse view-projection, column-major: element (row r, column c) at [4 c + r]
+    float cam[3];
+    float vol_min[3], vol_size[3];    // world box of the whole lattice
+    float clip_min[3], clip_max[3];   // world box marched
+    int n[3];                         // cells per axis
+    float step_voxels, gain_e, gain_b, density, emission, material_opacity, exposure;
+    int mode, signed_axis;
+    int width, height;
+};
+// out: width x height pixels, bottom row first; rgba8 != 0: 4 bytes per pixel (unorm), else 4 floats
+hipError_t launch_raymarch(const RenderArgs &r, const FieldPtrs &f, const float *sigma_cell, const float *eps_cell,
+                           void *out, int rgba8, hipStream_t s);
+
 // ---- launchers (kernels.hip) ----
 hipError_t launch_build_table(const uint16_t *keys, int classes, float dt, float *table, hipStream_t s);
 // Material coefficients the one-pass march applies itself (march tiles of its automatic

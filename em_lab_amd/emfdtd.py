@@ -108,6 +108,19 @@ class RadarImageSetupC(C.Structure):
                 ("nel", C.c_int)]
 
 
+class FieldViewC(C.Structure):
+    _fields_ = [("size", C.c_int * 3), ("storage", C.c_int), ("e", C.c_void_p * 3), ("b", C.c_void_p * 3),
+                ("scale", C.c_float * 6), ("sigma", C.c_void_p), ("eps_r", C.c_void_p), ("device", C.c_int),
+                ("step_count", C.c_uint64)]
+
+
+class RenderSettingsC(C.Structure):
+    _fields_ = [("mode", C.c_int), ("signed_axis", C.c_int), ("gain_e", C.c_float), ("gain_b", C.c_float),
+                ("density", C.c_float), ("emission", C.c_float), ("step_voxels", C.c_float),
+                ("material_opacity", C.c_float), ("exposure", C.c_float), ("clip_min", C.c_float * 3),
+                ("clip_max", C.c_float * 3)]
+
+
 class BatchSceneC(C.Structure):
     _fields_ = [("config", GridConfigC), ("sigma", C.c_void_p), ("eps_r", C.c_void_p), ("n_cells", C.c_int64),
                 ("edges", C.c_void_p), ("n_edges", C.c_int64), ("wires", C.c_void_p), ("n_wires", C.c_int64),
@@ -212,6 +225,9 @@ _SIGNATURES = {
     "emfd_mlp_train": (C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, _P, C.c_int]),
     "emfd_mlp_time_forward": (C.c_int, [_P, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     "emfd_radar_backproject": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int, _P, _P, _P, _P, C.c_int, _P]),
+    "emfd_grid_field_view": (C.c_int, [_P, C.POINTER(FieldViewC)]),
+    "emfd_render_default_settings": (None, [C.POINTER(RenderSettingsC)]),
+    "emfd_render_volume": (C.c_int, [_P, C.POINTER(RenderSettingsC), _P, _P, C.c_int, C.c_int, C.c_int, _P]),
 }
 
 
@@ -414,6 +430,14 @@ class SimulationGrid:
     @property
     def handle(self):
         return self._h
+
+    def field_view(self) -> dict:
+        """The grid's device planes as they are (emfd_grid_field_view): pointers valid until the
+        next step or set-up call."""
+        v = FieldViewC()
+        _check(self._lib.emfd_grid_field_view(self._h, C.byref(v)))
+        return dict(size=tuple(v.size), storage=v.storage, e=[v.e[i] for i in range(3)], b=[v.b[i] for i in range(3)],
+                    scale=tuple(v.scale), sigma=v.sigma, eps_r=v.eps_r, device=v.device, step_count=v.step_count)
 
     # -- SimulationGrid API --
     def configure(self, config: GridConfig) -> GridConfig:
@@ -904,6 +928,102 @@ class FieldAnalyzer:
         _check(self._lib.emfd_probe_measure(self.grid.handle, _ptr(p), p.shape[0], C.cast(arr, C.c_void_p)))
         return [dict(position=np.array(s.position), E=np.array(s.E), B=np.array(s.B), S=np.array(s.S),
                      sigma=s.sigma, epsilon_r=s.epsilon_r) for s in arr[: p.shape[0]]]
+
+
+class Camera:
+    """EM_Lab's orbit camera (src/Camera.cpp:29-36,54-68,102-110, defaults src/Camera.hpp:48-58):
+    eye = target + distance (cos p cos y, cos p sin y, sin p), looking at the target, z up;
+    glm's right-handed lookAt and perspective (clip z in -1 .. 1)."""
+
+    def __init__(self, target=(0.0, 0.0, 0.0), distance=4.0, yaw_deg=0.0, pitch_deg=0.0, fov_deg=50.0,
+                 aspect=16.0 / 9.0, near=0.01, far=100.0):
+        self.target = np.asarray(target, np.float64)
+        self.distance = float(np.clip(distance, 0.05, 50.0))
+        self.yaw = math.radians(yaw_deg)
+        self.pitch = float(np.clip(math.radians(pitch_deg), -math.radians(89.0), math.radians(89.0)))
+        self.fov = math.radians(float(np.clip(fov_deg, 10.0, 120.0)))
+        self.aspect = float(aspect)
+        self.near = max(float(near), 1e-4)
+        self.far = max(float(far), self.near * 2.0)
+
+    def position(self) -> np.ndarray:
+        cp = math.cos(self.pitch)
+        d = np.array([cp * math.cos(self.yaw), cp * math.sin(self.yaw), math.sin(self.pitch)])
+        return self.target + self.distance * d
+
+    def view(self) -> np.ndarray:
+        eye = self.position()
+        f = self.target - eye
+        f = f / np.linalg.norm(f)
+        s = np.cross(f, np.array([0.0, 0.0, 1.0]))
+        s = s / np.linalg.norm(s)
+        u = np.cross(s, f)
+        m = np.eye(4)
+        m[0, :3], m[1, :3], m[2, :3] = s, u, -f
+        m[0, 3], m[1, 3], m[2, 3] = -s @ eye, -u @ eye, f @ eye
+        return m
+
+    def projection(self) -> np.ndarray:
+        t = math.tan(self.fov / 2.0)
+        n, fa = self.near, self.far
+        m = np.zeros((4, 4))
+        m[0, 0] = 1.0 / (self.aspect * t)
+        m[1, 1] = 1.0 / t
+        m[2, 2] = -(fa + n) / (fa - n)
+        m[2, 3] = -2.0 * fa * n / (fa - n)
+        m[3, 2] = -1.0
+        return m
+
+    def inverse_view_projection(self) -> np.ndarray:
+        """16 floats, column-major (glm's layout): element (r, c) at 4 c + r."""
+        inv = np.linalg.inv(self.projection() @ self.view())
+        return np.ascontiguousarray(inv.T.ravel(), dtype=np.float32)
+
+
+RENDER_FIELDS, RENDER_E, RENDER_B, RENDER_POYNTING, RENDER_SIGNED_E = 0, 1, 2, 3, 4
+IMAGE_RGBA8, IMAGE_RGBA32F, IMAGE_DEVICE = 0, 1, 0x100
+
+
+class RenderSettings:
+    """RenderSettings (src/VolumeRenderer.hpp:21-39) with the reference's defaults."""
+
+    def __init__(self, **kw):
+        c = RenderSettingsC()
+        load_library().emfd_render_default_settings(C.byref(c))
+        self.mode, self.signed_axis = c.mode, c.signed_axis
+        self.gain_e, self.gain_b, self.density, self.emission = c.gain_e, c.gain_b, c.density, c.emission
+        self.step_voxels, self.material_opacity, self.exposure = c.step_voxels, c.material_opacity, c.exposure
+        self.clip_min, self.clip_max = tuple(c.clip_min), tuple(c.clip_max)
+        for k, v in kw.items():
+            if not hasattr(self, k):
+                raise TypeError(f"unknown render setting {k}")
+            setattr(self, k, v)
+
+    def to_c(self) -> RenderSettingsC:
+        c = RenderSettingsC()
+        c.mode, c.signed_axis = int(self.mode), int(self.signed_axis)
+        c.gain_e, c.gain_b, c.density, c.emission = self.gain_e, self.gain_b, self.density, self.emission
+        c.step_voxels, c.material_opacity, c.exposure = self.step_voxels, self.material_opacity, self.exposure
+        c.clip_min[:] = [float(v) for v in self.clip_min]
+        c.clip_max[:] = [float(v) for v in self.clip_max]
+        return c
+
+
+class VolumeRenderer:
+    """The viewer's field raymarch over a grid's device planes (VolumeRenderer::render,
+    src/VolumeRenderer.cpp:70-121; shaders/raymarch.frag)."""
+
+    def render(self, grid: "SimulationGrid", camera: Camera, settings: Optional[RenderSettings] = None,
+               width: int = 320, height: int = 180, float_image: bool = False) -> np.ndarray:
+        """(height, width, 4) image, bottom row first: uint8 RGBA, or the shader's float output."""
+        settings = settings or RenderSettings()
+        ivp = camera.inverse_view_projection()
+        eye = np.ascontiguousarray(camera.position(), dtype=np.float32)
+        out = np.zeros((height, width, 4), np.float32 if float_image else np.uint8)
+        sc = settings.to_c()
+        _check(grid._lib.emfd_render_volume(grid.handle, C.byref(sc), _ptr(ivp), _ptr(eye), int(width), int(height),
+                                            IMAGE_RGBA32F if float_image else IMAGE_RGBA8, _ptr(out)))
+        return out
 
 
 class Batch:

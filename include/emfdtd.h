@@ -237,8 +237,9 @@ int emfd_grid_set_kernel(emfd_ctx *ctx, int variant);
  * graded conductivity off and adds the convolutional PML (Roden-Gedney recursive convolution,
  * complex-frequency-shifted stretch, kappa = 1) in the same slabs: sigma = sigma_max d^grading,
  * alpha = alpha_max (1 - d) over the normalised depth d; sigma_max <= 0 -> 0.8 (grading + 1),
- * grading <= 0 -> 3, alpha_max <= 0 -> 0. It keeps 4 auxiliary planes per slab cell and runs
- * with FP32 storage and the two-pass kernels (EMFD_KERNEL_AUTO picks them). Clears the fields
+ * grading <= 0 -> 3, alpha_max <= 0 -> 0. It keeps 4 auxiliary planes per slab cell and needs
+ * FP32 storage; its corrections follow either Yee form (EMFD_KERNEL_AUTO: the one-pass march from
+ * 8e6 cells when every dimension exceeds 2 pml + 2 cells, else the two passes). Clears the fields
  * and rewinds time. */
 int emfd_grid_set_absorber(emfd_ctx *ctx, int kind, float sigma_max, float grading, float alpha_max);
 /* Field storage (no counterpart in the reference, which keeps RGBA32F textures): EMFD_STORAGE_*
@@ -409,6 +410,69 @@ int emfd_flux_measure_noalloc(emfd_ctx *ctx, const emfd_flux_surface *surfaces, 
 /* FieldAnalyzer::probe / submit+collect (src/FieldAnalyzer.cpp:126-272,274-315):
  * positions: 3 floats each. */
 int emfd_probe_measure(emfd_ctx *ctx, const float *positions, int n, emfd_probe_sample *out);
+
+/* ---------------------------------------------------------------------------------------
+ * Volume view (src/VolumeRenderer.hpp:21-39, src/VolumeRenderer.cpp:70-121,
+ * shaders/raymarch.frag): EM_Lab's viewer binds the grid's E and B textures (E with sigma, B with
+ * eps_r in alpha, src/SimulationGrid.cpp:167-173,209-222) to its raymarch shader. Here the grid's
+ * device planes are handed out as they are (no copy), and the same raymarch runs as a HIP kernel
+ * over them.
+ * --------------------------------------------------------------------------------------- */
+/* The grid state on the device, in place of eFieldTexture() / bFieldTexture()
+ * (src/SimulationGrid.hpp:181-182). Planes are in lattice order (x fastest, (z ny + y) nx + x),
+ * element type by storage: float (F32), _Float16 or bfloat16 of value / scale (F16, BF16), double
+ * (F64). The pointers stay valid until the next step, reset, configure, material upload or
+ * storage / absorber change (the fused kernel alternates two sets of planes), and reading them
+ * must be ordered after the context's stream (emfd_grid_synchronize, or work on that stream). */
+typedef struct {
+    int size[3];
+    int storage;          /* EMFD_STORAGE_* */
+    void *e[3];           /* E x, y, z planes */
+    void *b[3];           /* B x, y, z planes */
+    float scale[6];       /* value = stored * scale (E x y z, B x y z); 1 for F32 / F64 */
+    const float *sigma;   /* cell sigma (the E texture's alpha), n floats */
+    const float *eps_r;   /* cell eps_r (the B texture's alpha), n floats */
+    int device;
+    uint64_t step_count;  /* the step the planes hold */
+} emfd_field_view;
+int emfd_grid_field_view(emfd_ctx *ctx, emfd_field_view *out);
+
+#define EMFD_RENDER_FIELDS 0   /* E + B glow */
+#define EMFD_RENDER_E 1
+#define EMFD_RENDER_B 2
+#define EMFD_RENDER_POYNTING 3 /* |E x B| */
+#define EMFD_RENDER_SIGNED_E 4 /* one E component, signed palette */
+
+/* RenderSettings (src/VolumeRenderer.hpp:21-39); emfd_render_default_settings fills the
+ * reference's defaults. resolutionScale is the caller's: render at the scaled size. */
+typedef struct {
+    int mode;               /* EMFD_RENDER_* */
+    int signed_axis;        /* component for EMFD_RENDER_SIGNED_E (clamped to 0..2) */
+    float gain_e, gain_b;
+    float density;          /* field opacity multiplier */
+    float emission;         /* field brightness multiplier */
+    float step_voxels;      /* ray step in voxels (clamped to 0.1 .. 4) */
+    float material_opacity;
+    float exposure;
+    float clip_min[3];      /* cutaway box in normalised grid coordinates (clamped to 0 .. 1) */
+    float clip_max[3];
+} emfd_render_settings;
+void emfd_render_default_settings(emfd_render_settings *out);
+
+#define EMFD_IMAGE_RGBA8 0      /* 4 bytes per pixel, unorm (the framebuffer's format) */
+#define EMFD_IMAGE_RGBA32F 1    /* 4 floats per pixel: the fragment shader's output */
+#define EMFD_IMAGE_DEVICE 0x100 /* flag: `image` is device memory; the call only queues the kernel */
+
+/* VolumeRenderer::render (src/VolumeRenderer.cpp:70-121) with raymarch.frag: one frame of
+ * width x height pixels, bottom row first (the framebuffer's order), of the current fields.
+ * inv_view_proj: inverse of the camera's projection * view, 16 floats column-major (glm's
+ * layout, src/Camera.hpp:36); cam_pos: the eye (src/Camera.cpp:66-68). The world box is the
+ * reference's: the lattice centred on the origin, its longest axis 2 units
+ * (src/VolumeRenderer.cpp:44-54). With EMFD_IMAGE_DEVICE, image is device memory on the
+ * context's device and the kernel is queued on the context's stream; otherwise the call copies the
+ * image to host memory and returns when it is there. */
+int emfd_render_volume(emfd_ctx *ctx, const emfd_render_settings *settings, const float *inv_view_proj,
+                       const float *cam_pos, int width, int height, int format, void *image);
 
 /* ---------------------------------------------------------------------------------------
  * Batched stepper (no counterpart in the reference, which runs one grid per process,

@@ -172,6 +172,23 @@ void or_probe(const or_grid *g, const float *E, const float *B, const float pos[
 double or_sweep(const float *series, size_t n_steps, int n_ports, int port, double dt, double wl_min,
                 double wl_max, int points, double z0, double *out, double *band);
 
+/* Volume raymarch (render.c): shaders/raymarch.frag with the set-up of
+ * src/VolumeRenderer.cpp:44-54,70-121 already applied (world box, clamps). */
+typedef struct {
+    float inv_vp[16];  /* column-major */
+    float cam[3];
+    float vol_min[3], vol_size[3], clip_min[3], clip_max[3];
+    int n[3];
+    float step_voxels, gain_e, gain_b, density, emission, material_opacity, exposure;
+    int mode, signed_axis;
+    int width, height;
+} or_render_args;
+float or_gradient_noise(float px, float py);
+float or_texture(const float *tex, int nx, int ny, int nz, int ch, float u, float v, float w);
+void or_box_hit(const float *ro, const float *rd, const float *bmin, const float *bmax, float *t_enter, float *t_exit);
+void or_render_pixel(const or_render_args *r, const float *E, const float *B, int px, int py, float *rgba);
+void or_render_volume(const or_render_args *r, const float *E, const float *B, float *rgba);
+
 #ifdef __cplusplus
 }
 #endif

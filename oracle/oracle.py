@@ -40,8 +40,22 @@ _P = C.c_void_p
 _F = C.c_float
 _D = C.c_double
 _I = C.c_int
+class OrRenderArgs(C.Structure):
+    _fields_ = [("inv_vp", C.c_float * 16), ("cam", C.c_float * 3), ("vol_min", C.c_float * 3),
+                ("vol_size", C.c_float * 3), ("clip_min", C.c_float * 3), ("clip_max", C.c_float * 3),
+                ("n", C.c_int * 3), ("step_voxels", C.c_float), ("gain_e", C.c_float), ("gain_b", C.c_float),
+                ("density", C.c_float), ("emission", C.c_float), ("material_opacity", C.c_float),
+                ("exposure", C.c_float), ("mode", C.c_int), ("signed_axis", C.c_int), ("width", C.c_int),
+                ("height", C.c_int)]
+
+
 _SIG = {
     "or_exp": (_F, [_F]),
+    "or_gradient_noise": (_F, [_F, _F]),
+    "or_texture": (_F, [C.c_void_p, _I, _I, _I, _I, _F, _F, _F]),
+    "or_box_hit": (None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(_F), C.POINTER(_F)]),
+    "or_render_pixel": (None, [C.POINTER(OrRenderArgs), C.c_void_p, C.c_void_p, _I, _I, C.c_void_p]),
+    "or_render_volume": (None, [C.POINTER(OrRenderArgs), C.c_void_p, C.c_void_p, C.c_void_p]),
     "or_f32_to_f16": (C.c_uint16, [_F]),
     "or_f16_to_f32": (_F, [C.c_uint16]),
     "or_pack_half2": (C.c_uint32, [_F, _F]),
@@ -360,3 +374,39 @@ class OracleSim:
         p = np.array(pos, np.float32)
         lib().or_probe(C.byref(self.g), ptr(self.E), ptr(self.B), ptr(p), ptr(out))
         return out
+
+
+def render_args(size, settings, inv_vp, cam, width, height) -> OrRenderArgs:
+    """The raymarch's inputs after the viewer's set-up (src/VolumeRenderer.cpp:44-54,89-112):
+    world box (longest axis 2 units, centred), cutaway and step clamps. settings: the fields of
+    RenderSettings (mode, signed_axis, gain_e, ...)."""
+    r = OrRenderArgs()
+    n = [int(v) for v in size]
+    ws = np.float32(2.0) / np.float32(max(max(n), 1))
+    for a in range(3):
+        r.n[a] = n[a]
+        r.cam[a] = float(cam[a])
+        vmin = np.float32(-0.5) * np.float32(n[a]) * ws
+        vsz = np.float32(n[a]) * ws
+        lo = min(max(min(settings["clip_min"][a], settings["clip_max"][a]), 0.0), 1.0)
+        hi = min(max(max(settings["clip_min"][a], settings["clip_max"][a]), 0.0), 1.0)
+        r.vol_min[a], r.vol_size[a] = float(vmin), float(vsz)
+        r.clip_min[a] = float(vmin + np.float32(lo) * vsz)
+        r.clip_max[a] = float(vmin + np.float32(hi) * vsz)
+    r.inv_vp[:] = [float(v) for v in inv_vp]
+    r.step_voxels = min(max(settings["step_voxels"], 0.1), 4.0)
+    for k in ("gain_e", "gain_b", "density", "emission", "material_opacity", "exposure"):
+        setattr(r, k, settings[k])
+    r.mode = int(settings["mode"])
+    r.signed_axis = min(max(int(settings["signed_axis"]), 0), 2)
+    r.width, r.height = int(width), int(height)
+    return r
+
+
+def render_volume(args: OrRenderArgs, E: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """(height, width, 4) float image, bottom row first; E, B: RGBA texels (n, 4) float32."""
+    E = np.ascontiguousarray(E, np.float32)
+    B = np.ascontiguousarray(B, np.float32)
+    out = np.zeros((args.height, args.width, 4), np.float32)
+    lib().or_render_volume(C.byref(args), ptr(E), ptr(B), ptr(out))
+    return out

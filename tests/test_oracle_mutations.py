@@ -1,9 +1,9 @@
 """The oracle's pins reject plausible mistakes.
 
-Each case below plants one mistake in a copy of oracle/oracle.c (a transposed neighbour, a wrong
-component, a half-cell shift, a sign), compiles that copy into a temporary library with the same
-flags as oracle/Makefile, and runs the pins of tests/test_oracle_pins.py that cover the mutated
-function against it. At least one pin must fail; the unmodified oracle passes them all (the pin
+Each case below plants one mistake in a copy of the oracle's sources (oracle/oracle.c,
+oracle/render.c: a transposed neighbour, a wrong component, a half-cell shift, a sign), compiles
+the copies into a temporary library with the same flags as oracle/Makefile, and runs the pins of
+tests/test_oracle_pins.py and tests/test_render_cpu.py that cover the mutated function against it. At least one pin must fail; the unmodified oracle passes them all (the pin
 file itself). Every planted mistake keeps its memory accesses inside the arrays (clamped or
 transposed indices only), so a mutant fails by its numbers, not by a crash.
 """
@@ -17,7 +17,7 @@ from pathlib import Path
 import pytest
 
 ROOT = Path(__file__).resolve().parents[1]
-SRC = ROOT / "oracle" / "oracle.c"
+SOURCES = [ROOT / "oracle" / "oracle.c", ROOT / "oracle" / "render.c"]
 
 # (name, text in oracle.c, replacement, pins that must reject it)
 MUTANTS = [
@@ -62,6 +62,28 @@ MUTANTS = [
      "const double hr = cos(w * 0.5 * dt), hi = sin(w * 0.5 * dt);",
      "const double hr = cos(w * 0.5 * dt), hi = -sin(w * 0.5 * dt);",
      ["sweep"]),
+    # volume raymarch (oracle/render.c)
+    ("texture sampled half a texel off",
+     "float s = c[a] * (float)n[a] - 0.5f;", "float s = c[a] * (float)n[a];",
+     ["texture"]),
+    ("texture edge mirrored instead of clamped",
+     "lo[a] = i < 0 ? 0 :", "lo[a] = i < 0 ? 1 :",
+     ["texture"]),
+    ("compositing weights by opacity instead of transmittance",
+     "scl(surface, a_mat)), tr));", "scl(surface, a_mat)), alpha));",
+     ["uniform", "metal"]),
+    ("one raymarch step short",
+     "int steps = (int)((t1 - t) / ds) + 1;", "int steps = (int)((t1 - t) / ds);",
+     ["uniform", "metal"]),
+    ("no early ray termination",
+     "if (alpha > 0.995f) break;", "if (alpha > 2.0f) break;",
+     ["metal"]),
+    ("Poynting x component from the wrong pair",
+     "rv3 s = v3(e.y * b.z - e.z * b.y,", "rv3 s = v3(e.y * b.y - e.z * b.z,",
+     ["parallel"]),
+    ("background gradient over the ray's y",
+     "float h = clampf(rd.z * 0.5f + 0.5f, 0.0f, 1.0f);", "float h = clampf(rd.y * 0.5f + 0.5f, 0.0f, 1.0f);",
+     ["background"]),
 ]
 
 
@@ -69,7 +91,15 @@ def _pins():
     spec = importlib.util.spec_from_file_location("oracle_pins_for_mutants", ROOT / "tests" / "test_oracle_pins.py")
     mod = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mod)
+    spec_r = importlib.util.spec_from_file_location("render_pins_for_mutants", ROOT / "tests" / "test_render_cpu.py")
+    ren = importlib.util.module_from_spec(spec_r)
+    spec_r.loader.exec_module(ren)
     return {
+        "texture": [lambda O: ren.test_texture_linear_field_and_edge_clamp(O)],
+        "uniform": [lambda O, m=m: ren.test_uniform_field_closed_form(O, m) for m in (0.05, 0.3, 1.5)],
+        "metal": [lambda O: ren.test_metal_block_closed_form(O)],
+        "parallel": [lambda O: ren.test_poynting_of_parallel_fields_is_dark(O)],
+        "background": [lambda O: ren.test_empty_volume_is_background(O)],
         "edge_linear": [lambda O: mod.test_edge_average_linear_medium(O)],
         "edge_isolated": [lambda O, c=c: mod.test_edge_average_isolated_cell(O, c)
                           for c in [(4, 3, 5), (0, 3, 2), (3, 0, 0)]],
@@ -81,14 +111,19 @@ def _pins():
 
 
 def _build_mutant(tmp: Path, text: str, repl: str) -> Path:
-    src = SRC.read_text()
-    assert src.count(text) == 1, f"mutation site not unique or missing: {text!r}"
-    mut = tmp / "oracle_mut.c"
-    mut.write_text(src.replace(text, repl))
+    srcs = {p.name: p.read_text() for p in SOURCES}
+    sites = [n for n, t in srcs.items() if t.count(text) == 1]
+    assert len(sites) == 1 and sum(t.count(text) for t in srcs.values()) == 1, \
+        f"mutation site not unique or missing: {text!r}"
+    srcs[sites[0]] = srcs[sites[0]].replace(text, repl)
+    files = []
+    for name, t in srcs.items():
+        (tmp / name).write_text(t)
+        files.append(str(tmp / name))
     (tmp / "oracle.h").write_text((ROOT / "oracle" / "oracle.h").read_text())
     lib = tmp / "liboracle_mut.so"
     r = subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-ffp-contract=off", "-fno-fast-math",
-                        "-fopenmp", "-o", str(lib), str(mut), "-lm"], capture_output=True, text=True)
+                        "-fopenmp", "-o", str(lib), *files, "-lm"], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     return lib
 
