@@ -528,9 +528,13 @@ class SimulationGrid:
             return "k_update_b_f64+k_update_e_f64"
         nx, ny, nz = self.size
         half = self._storage in (STORAGE_F16, STORAGE_BF16)
+        cpml = getattr(self, "_absorber", ABSORBER_GRADED) == ABSORBER_CPML
+        pml = self._config.pml_thickness if self._config is not None else 0
         if v == KERNEL_MARCH or (v == KERNEL_AUTO and not half and nx * ny * nz >= 8_000_000 and nx % 4 == 0
                                  and nx * ny * nz * 4 <= 0x80000000
-                                 and getattr(self, "_absorber", ABSORBER_GRADED) == ABSORBER_GRADED):
+                                 and (not cpml or min(nx, ny, nz) > 2 * pml + 2)):
+            if cpml:
+                return "k_yee_march (+k_march_fix_b, k_cpml_march_b, k_march_fix_e, k_cpml_march_e)"
             return "k_yee_march (+k_march_fix_b, k_march_fix_e)"
         if half:
             return "k_update_bh+k_update_eh"
