@@ -140,6 +140,7 @@ struct emfd_ctx {
     int absorber = 0;
     float cpml_sigma_max = 0.0f, cpml_grading = 0.0f, cpml_alpha_max = 0.0f;
     emfd::DevBuf d_cpml_coef, d_psi;
+    emfd::DevBuf d_render;  // staging of emfd_render_volume's host-bound image (kept between frames)
     int prof_stride = 0;
 
     // sources

@@ -382,7 +382,8 @@ void emfd_destroy(emfd_ctx *ctx) {
                       &ctx->lines.d_quant, &ctx->lines.d_terms, &ctx->lines.d_weights, &ctx->lines.d_phasors,
                       &ctx->lines.d_now, &ctx->ntff.d_ahi, &ctx->ntff.d_alo, &ctx->ntff.d_absmax, &ctx->fix_b,
                       &ctx->fix_e, &ctx->ports.d_dft, &ctx->ntff.d_dft, &ctx->ntff.d_dft_ph, &ctx->d_consts64,
-                      &ctx->d_cpml_coef, &ctx->d_psi};
+                      &ctx->d_cpml_coef, &ctx->d_psi, &ctx->ntff.d_samples, &ctx->class_masks, &ctx->march_lid,
+                      &ctx->march_tiles, &ctx->march_coef, &ctx->fix_e_rest, &ctx->d_render};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t ev : ctx->ev_pool) (void)hipEventDestroy(ev);
     for (int i = 0; i < 2; ++i) {

@@ -99,16 +99,17 @@ int emfd_render_volume(emfd_ctx *ctx, const emfd_render_settings *settings, cons
     EMFD_HIP(hipSetDevice(ctx->device));
     const size_t bytes = static_cast<size_t>(width) * height * (kind == EMFD_IMAGE_RGBA8 ? 4 : 16);
     void *dst = image;
-    DevBuf tmp;
     if (!on_device) {
-        if (int rc = tmp.ensure(bytes)) return rc;
-        dst = tmp.ptr;
+        // the staging image stays allocated for the next frame (a viewer renders every frame);
+        // earlier work on the stream may still read a smaller one, so wait before it grows
+        if (ctx->d_render.bytes < bytes) EMFD_HIP(hipStreamSynchronize(ctx->stream));
+        if (int rc = ctx->d_render.ensure(bytes)) return rc;
+        dst = ctx->d_render.ptr;
     }
     hipError_t e = launch_raymarch(r, ctx->fields(ctx->cur), ctx->sigma_cell.as<float>(), ctx->eps_cell.as<float>(), dst,
                                    kind == EMFD_IMAGE_RGBA8 ? 1 : 0, ctx->stream);
     if (e == hipSuccess && !on_device) e = hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess && !on_device) e = hipStreamSynchronize(ctx->stream);
-    tmp.release();
     if (e != hipSuccess) return hip_fail(e, "emfd_render_volume");
     return EMFD_OK;
 }

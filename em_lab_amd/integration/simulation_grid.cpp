@@ -10,7 +10,8 @@
 //
 // bundle: grid.bin (emfd_grid_config), sigma.f32 / eps.f32 (per cell), edges.bin, wires.bin,
 // sources.bin, ports.bin (arrays of the ABI structs), sweep.bin (emfd_sweep_setup, optional),
-// farfield.bin (emfd_far_field_setup, optional), run.txt ("steps averaging_periods cell_size").
+// farfield.bin (emfd_far_field_setup, optional), run.txt ("steps averaging_periods cell_size"),
+// camera.f32 (optional: a frame of the volume view).
 // out: E.f32 / B.f32 (3 x n component planes), port_phasors.f32, port_series.f32, sweep.f64
 // (wavelength, R, X, S11 dB, VSWR per port and point), readings.f64, ntff_phasors.f32,
 // pattern.txt (valid, radiated power, max directivity).
@@ -223,6 +224,39 @@ private:
     int m_points = 0;
 };
 
+// ---- VolumeRenderer (src/VolumeRenderer.cpp:70-121): the volume pass as one call ----
+struct RenderSettings {  // src/VolumeRenderer.hpp:21-39 (the fields the volume pass reads)
+    int mode = 0, signedAxis = 2;
+    float gainE = 1.0f, gainB = 1.0f, density = 1.0f, emission = 1.0f, stepVoxels = 0.75f;
+    float materialOpacity = 0.45f, exposure = 1.25f;
+    float clipMin[3] = {0.0f, 0.0f, 0.0f}, clipMax[3] = {1.0f, 1.0f, 1.0f};
+};
+class VolumeRenderer {
+public:
+    // invViewProj: inverse(camera.viewProjection()), column-major; eye: camera.position()
+    std::vector<uint8_t> render(const SimulationGrid &grid, const float *invViewProj, const float *eye,
+                                const RenderSettings &rs, int width, int height) const {
+        emfd_render_settings s;
+        emfd_render_default_settings(&s);
+        s.mode = rs.mode;
+        s.signed_axis = rs.signedAxis;
+        s.gain_e = rs.gainE;
+        s.gain_b = rs.gainB;
+        s.density = rs.density;
+        s.emission = rs.emission;
+        s.step_voxels = rs.stepVoxels;
+        s.material_opacity = rs.materialOpacity;
+        s.exposure = rs.exposure;
+        for (int a = 0; a < 3; ++a) {
+            s.clip_min[a] = rs.clipMin[a];
+            s.clip_max[a] = rs.clipMax[a];
+        }
+        std::vector<uint8_t> rgba(static_cast<size_t>(width) * height * 4);
+        check(emfd_render_volume(grid.context(), &s, invViewProj, eye, width, height, EMFD_IMAGE_RGBA8, rgba.data()));
+        return rgba;
+    }
+};
+
 // ---- the program ----
 template <class T>
 static std::vector<T> read_array(const std::string &path, bool required) {
@@ -339,6 +373,15 @@ int main(int argc, char **argv) {
             char line[128];
             std::snprintf(line, sizeof line, "%d %.17g %.9g\n", valid, power, static_cast<double>(dbi));
             p << line;
+        }
+        // camera.f32 (optional): 16 floats of the inverse view-projection, 3 of the eye, then
+        // width, height, mode as floats; the frame goes to frame.rgba
+        const auto cam = read_array<float>(in + "/camera.f32", false);
+        if (cam.size() == 22) {
+            RenderSettings rs;
+            rs.mode = static_cast<int>(cam[21]);
+            write_array(out + "/frame.rgba", VolumeRenderer().render(grid, cam.data(), cam.data() + 16, rs,
+                                                                      static_cast<int>(cam[19]), static_cast<int>(cam[20])));
         }
         std::cout << "simulation_grid: " << emfd_version() << " | " << c.size.x << "x" << c.size.y << "x" << c.size.z
                   << ", " << steps << " steps, " << pm.size() << " ports" << std::endl;

@@ -46,6 +46,10 @@ def test_cpp_binding_matches_ctypes(tmp_path):
     write_bundle(sc, tmp_path / "in", STEPS)
     out = tmp_path / "out"
     out.mkdir()
+    cam = E.Camera(distance=3.0, yaw_deg=30.0, pitch_deg=20.0, aspect=80 / 45)
+    eye = np.asarray(cam.position(), np.float32)
+    np.concatenate([cam.inverse_view_projection(), eye, np.array([80, 45, 0], np.float32)]).astype(
+        np.float32).tofile(tmp_path / "in" / "camera.f32")
     r = subprocess.run([str(exe), str(tmp_path / "in"), str(out)], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
@@ -71,4 +75,7 @@ def test_cpp_binding_matches_ctypes(tmp_path):
     assert int(valid) == int(res["valid"]) == 1
     assert float(power) == res["radiated_power"]
     assert np.float32(float(dbi)) == np.float32(res["max_directivity_dbi"])
+    # the viewer's volume pass through the compiled VolumeRenderer body: the same frame
+    frame = E.VolumeRenderer().render(g, cam, E.RenderSettings(mode=0), 80, 45)
+    assert np.array_equal(np.fromfile(out / "frame.rgba", np.uint8).reshape(45, 80, 4), frame)
     g.close()
