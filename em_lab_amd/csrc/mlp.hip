@@ -256,12 +256,54 @@ __global__ void __launch_bounds__(256) k_to_bf16(const float *p, const int *bf16
 // LDS (64 x 256 BF16, rows padded to 264 to spread the banks); each of the 4 wavefronts computes
 // a quarter of the next layer's columns for all 64 rows (up to 2 x 2 MFMA tiles of 32 x 32,
 // K in steps of 16 in increasing order, exactly as k_gemm accumulates), weight fragments come
-// straight from global memory one k-step ahead (the whole network is 338 KB: L2-resident), and
+// straight from global memory four k-steps ahead (the whole network is 338 KB: L2-resident), and
 // the epilogue (bias,
 // GELU, BF16) writes the next layer's input back into LDS. Same arithmetic as the layered forward,
 // so the outputs are bit-identical to it.
 // ---------------------------------------------------------------------------------------------
 constexpr int kFuseRows = 64, kFuseTm = kFuseRows / 32, kFuseLd = 264;
+constexpr int kFusePf = 4;  // k-steps of weight fragments in flight per wavefront
+
+// One layer's K loop of a wavefront: kt = K / 16 k-steps, each one A fragment per row tile from
+// LDS and the column tiles' weight fragments from global memory, loaded kFusePf steps ahead. With
+// KC > 0 (the network's widths) the loop is unrolled whole, so no loop back-edge makes the
+// compiler wait for every fragment in flight; KC = 0 takes any K (a multiple of 16, at most 256).
+template <int KC>
+__device__ __forceinline__ void fused_k(const __bf16 (&h)[kFuseRows][kFuseLd], const __bf16 *const (&wrow)[2], int tn_n,
+                                        floatx16 (&acc)[kFuseTm][2], int kdyn = 0) {
+    const int lane = threadIdx.x & 63;
+    const int K = KC > 0 ? KC : kdyn;
+    const int kt = K / 16;
+    bf16x8 q[kFusePf][2];
+#pragma unroll
+    for (int s = 0; s < kFusePf; ++s)
+        if (s < kt) {
+            q[s][0] = *reinterpret_cast<const bf16x8 *>(wrow[0] + 16 * s);
+            q[s][1] = tn_n > 1 ? *reinterpret_cast<const bf16x8 *>(wrow[1] + 16 * s) : q[s][0];
+        }
+    auto step = [&](int i, int s) {  // k-step i, its fragments in slot s
+        const bf16x8 b0 = q[s][0], b1 = q[s][1];
+        if (i + kFusePf < kt) {
+            q[s][0] = *reinterpret_cast<const bf16x8 *>(wrow[0] + 16 * (i + kFusePf));
+            if (tn_n > 1) q[s][1] = *reinterpret_cast<const bf16x8 *>(wrow[1] + 16 * (i + kFusePf));
+        }
+#pragma unroll
+        for (int tm = 0; tm < kFuseTm; ++tm) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&h[32 * tm + (lane & 31)][16 * i + 8 * (lane >> 5)]);
+            acc[tm][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, acc[tm][0], 0, 0, 0);
+            if (tn_n > 1) acc[tm][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1, acc[tm][1], 0, 0, 0);
+        }
+    };
+    if constexpr (KC > 0) {
+#pragma unroll
+        for (int i = 0; i < KC / 16; ++i) step(i, i % kFusePf);
+    } else {
+        for (int i = 0; i < kt; i += kFusePf)
+#pragma unroll
+            for (int s = 0; s < kFusePf; ++s)
+                if (i + s < kt) step(i + s, s);
+    }
+}
 
 __global__ void __launch_bounds__(256) k_mlp_forward_fused(MlpFusedArgs g) {
     __shared__ __attribute__((aligned(16))) __bf16 h[kFuseRows][kFuseLd];
@@ -287,25 +329,12 @@ __global__ void __launch_bounds__(256) k_mlp_forward_fused(MlpFusedArgs g) {
         for (int tm = 0; tm < kFuseTm; ++tm)
 #pragma unroll
             for (int tn = 0; tn < 2; ++tn) acc[tm][tn] = floatx16{};
-        // weight fragments of column tile tn at k, one k-step ahead of their use
+        // weight fragments of column tile tn, kFusePf k-steps ahead of their use
         const __bf16 *wrow[2] = {W + static_cast<size_t>(c0 + (lane & 31)) * K + 8 * (lane >> 5),
                                  W + static_cast<size_t>(c0 + 32 + (lane & 31)) * K + 8 * (lane >> 5)};
-        bf16x8 bnext[2];
-        bnext[0] = *reinterpret_cast<const bf16x8 *>(wrow[0]);
-        bnext[1] = tn_n > 1 ? *reinterpret_cast<const bf16x8 *>(wrow[1]) : bnext[0];
-        for (int k = 0; k < K; k += 16) {
-            const bf16x8 b0 = bnext[0], b1 = bnext[1];
-            if (k + 16 < K) {
-                bnext[0] = *reinterpret_cast<const bf16x8 *>(wrow[0] + k + 16);
-                if (tn_n > 1) bnext[1] = *reinterpret_cast<const bf16x8 *>(wrow[1] + k + 16);
-            }
-#pragma unroll
-            for (int tm = 0; tm < kFuseTm; ++tm) {
-                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&h[32 * tm + (lane & 31)][k + 8 * (lane >> 5)]);
-                acc[tm][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, acc[tm][0], 0, 0, 0);
-                if (tn_n > 1) acc[tm][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1, acc[tm][1], 0, 0, 0);
-            }
-        }
+        if (K == 256) fused_k<256>(h, wrow, tn_n, acc);
+        else if (K == 32) fused_k<32>(h, wrow, tn_n, acc);
+        else fused_k<0>(h, wrow, tn_n, acc, K);
         __syncthreads();  // every wavefront is done reading this layer's input
         const bool last = l == g.layers - 1;
 #pragma unroll
