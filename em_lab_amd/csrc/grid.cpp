@@ -345,8 +345,9 @@ extern "C" {
 const char *emfd_last_error(void) { return g_error.c_str(); }
 
 const char *emfd_version(void) {
-    return "emfdtd 0.2 gfx950: streaming two-pass Yee (FP32 / binary16 / bfloat16 / FP64 storage), fused z-march "
-           "options, sources, ports, multi-wavelength far field, flux, probes, batched stepper, BF16 MFMA MLP";
+    return "emfdtd 0.3 gfx950: one-pass z-march and two-pass Yee (FP32 / binary16 / bfloat16 / FP64 storage), "
+           "graded absorber or convolutional PML, sources, ports, multi-wavelength far field with FP64 DFTs and MFMA "
+           "pattern, flux, probes, coax lines, radar image, volume view, batched stepper, BF16 MFMA MLP";
 }
 
 int emfd_create(int device, emfd_ctx **out) {
